@@ -1,0 +1,284 @@
+/*
+ * snkpt.h — C ABI of the MI355X-native path-trace pass.
+ *
+ * This library replaces the ray-tracing pass of orangefrog45/Snake-VK:
+ *   RT::Init / RT::RecordRenderCmdBuf      Core/headers/rendering/Raytracing.h:11-31,
+ *                                          Core/src/rendering/Raytracing.cpp:227-278
+ * and the driver acceleration structures it traces against
+ *   BLAS::GenerateFromMeshData, TLAS::BuildFromInstances  Core/src/resources/TLAS.cpp:10-200.
+ *
+ * Scene mesh / material / emitter buffers and a camera go in; an HDR radiance
+ * image comes out.  Everything is plain C: pointers, sizes and status codes.
+ * Host arrays are copied before a call returns.  Device pointers (rays, hits,
+ * output images) belong to the caller and must be HIP device memory on the
+ * context's device.  Work is enqueued on the caller's stream (a hipStream_t
+ * passed as `snkpt_stream`, NULL = the null stream).
+ *
+ * Threading: a context is externally synchronised (one host thread at a time).
+ * Several contexts may share a device.
+ */
+#ifndef SNKPT_H
+#define SNKPT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct snkpt_ctx snkpt_ctx;
+typedef void* snkpt_stream; /* hipStream_t */
+typedef int32_t snkpt_status;
+
+#define SNKPT_OK 0
+#define SNKPT_ERR_INVALID_ARG (-1)
+#define SNKPT_ERR_OOM (-2)
+#define SNKPT_ERR_HIP (-3)
+#define SNKPT_ERR_CAPACITY (-4) /* a queue / table / stack limit would be exceeded: nothing is truncated */
+#define SNKPT_ERR_NOT_BUILT (-5) /* trace/render before snkpt_build_accel */
+
+/* Capacities of the reference's buffers (RaytracingBufferSystem.cpp:60-63,
+ * TransformBufferSystem.h:70-76, ResourceBufferManagers.cpp:10,
+ * LightBuffers.glsl:41-49).  The same limits are enforced here. */
+#define SNKPT_MAX_INSTANCES 4096u
+#define SNKPT_MAX_TRANSFORMS 4096u
+#define SNKPT_MAX_MATERIALS 4096u
+#define SNKPT_MAX_POINTLIGHTS 1024u
+#define SNKPT_MAX_SPOTLIGHTS 1024u
+
+#define SNKPT_INVALID_INDEX 0xFFFFFFFFu
+#define SNKPT_MAT_FLAG_EMISSIVE 1u /* TexMatBuffers.glsl:3; emission = albedo.xyz */
+#define SNKPT_MAT_FLAG_LAMBERT 2u  /* extension: pure albedo/pi BRDF (no reference counterpart) */
+
+/* InstanceData — RaytracingInstances.glsl:7-13, RaytracingBufferSystem.h:14-28.
+ * One record per (entity, submesh).  The hit's instance id is the record index
+ * (gl_InstanceCustomIndexEXT, TLAS.cpp:193) and its primitive id is the
+ * submesh-local triangle index (gl_PrimitiveID).  Vertex i of triangle p is
+ *   idx[mesh_buffer_index_offset + 3p + i] + mesh_buffer_vertex_offset. */
+typedef struct snkpt_instance {
+    uint32_t transform_idx;
+    uint32_t mesh_buffer_index_offset; /* in indices, a multiple of 3 */
+    uint32_t mesh_buffer_vertex_offset;
+    uint32_t material_idx;
+    uint32_t num_mesh_indices;
+} snkpt_instance; /* 20 B */
+
+/* Material — TexMatBuffers.glsl:5-20, writer ResourceBufferManagers.cpp:64-101.
+ * Texture indices must be SNKPT_INVALID_INDEX (textures are not part of this path). */
+typedef struct snkpt_material {
+    uint32_t albedo_tex_idx, normal_tex_idx, roughness_tex_idx, metallic_tex_idx, ao_tex_idx;
+    float roughness, metallic, ao;
+    float albedo[4];
+    uint32_t flags, pad0, pad1, pad2;
+} snkpt_material; /* 64 B */
+
+/* CommonUBO — CommonUBO.glsl:1-13, SceneInfoBufferSystem.cpp:13-23,66-91.
+ * view = lookAt, proj = perspective(radians(fov/2), aspect, near, far) with the
+ * Vulkan Y flip (CameraComponent.h:15-19).  Column-major like glm.  Camera rays
+ * are cam_pos -> unproject(pixel centre, depth 1) (Util.glsl:9-13). */
+typedef struct snkpt_camera {
+    float view[16];
+    float proj[16];
+    float proj_view[16];
+    float cam_pos[4];
+    float cam_forward[4];
+    uint32_t frame_idx;
+    float time_elapsed; /* unused: randomness comes from (seed, frame) only */
+    float delta_time;   /* unused */
+    uint32_t prev_frame_invalid; /* disables temporal reuse (RayGen.rgen:422) */
+} snkpt_camera; /* 240 B */
+
+/* Light SSBO header — LightBuffers.glsl:1-63, LightBufferSystem.cpp:65-108.
+ * snkpt_set_lights accepts the block as laid out there: a 96 B directional
+ * light, {num_pointlights, num_spotlights, pad, pad}, then 1024 x 48 B point
+ * lights and 1024 x 64 B spot lights (the tail may be omitted). */
+typedef struct snkpt_pointlight {
+    float colour[4];
+    float pos[4];
+    float atten_exp, atten_linear, atten_constant, padding;
+} snkpt_pointlight; /* 48 B */
+
+#define SNKPT_MODE_PT 0u     /* unbiased path tracing, NEE + MIS */
+#define SNKPT_MODE_RESTIR 1u /* ReSTIR DI for direct light at the primary hit + PT for the rest */
+#define SNKPT_MODE_CACHE 2u  /* PT terminated into the MLP radiance cache, cache trained online */
+
+#define SNKPT_FLAG_ACCUMULATE 1u /* add this frame to the running mean and output the mean */
+#define SNKPT_FLAG_RESET 2u      /* clear the running mean first */
+#define SNKPT_FLAG_NO_TRAIN 4u   /* cache mode: query only, skip the training step */
+
+#define SNKPT_FORMAT_RGBA32F 0u
+#define SNKPT_FORMAT_RGBA16F 1u
+
+/* RtSettings (Raytracing.h:13-16) plus what the unbiased pass needs. */
+typedef struct snkpt_settings {
+    int32_t num_temporal_resamples; /* 0 disables temporal reuse */
+    int32_t num_spatial_resamples;  /* spatial neighbours (reference default 3, config 5) */
+    uint32_t mode;                  /* SNKPT_MODE_* */
+    uint32_t spp;                   /* samples per pixel per call (>= 1) */
+    uint32_t max_depth;             /* path vertices after the camera (1 = direct light only) */
+    uint32_t rr_start;              /* first bounce with Russian roulette; 0 = off */
+    uint32_t seed;                  /* RNG seed; streams are (seed, frame, pixel, sample) */
+    uint32_t frame;                 /* frame index for RNG streams */
+    uint32_t restir_candidates;     /* RIS candidates per pixel (LIGHT_RIS_ITERS, RayGen.rgen:95) */
+    uint32_t restir_m_cap;          /* temporal M cap (M_CAP, RayGen.rgen:97) */
+    float restir_radius;            /* spatial radius in pixels (RayGen.rgen:488) */
+    uint32_t cache_query_depth;     /* cache mode: bounce at which the cache terminates paths */
+    uint32_t cache_train_paths;     /* cache mode: training paths per frame */
+    float cache_lr;                 /* cache mode: Adam learning rate */
+    uint32_t flags;                 /* SNKPT_FLAG_* */
+    uint32_t reserved[5];
+} snkpt_settings;
+
+/* Ray and hit streams for snkpt_trace: structure-of-arrays, device pointers.
+ * org_tmin[i] = (ox, oy, oz, tmin), dir_tmax[i] = (dx, dy, dz, tmax), 16 B aligned. */
+typedef struct snkpt_rays {
+    const float* org_tmin;
+    const float* dir_tmax;
+} snkpt_rays;
+
+/* Closest hit: t, instance id, primitive id, barycentrics (u, v) with
+ * bary = (1-u-v, u, v) as in RayClosestHit.rchit:88.  Miss: t = -1 and
+ * inst = prim = SNKPT_INVALID_INDEX (RayMiss.rmiss:22).  Any-hit: t and ids of
+ * the first occluder found (any one), or the miss values.  u / v may be NULL. */
+typedef struct snkpt_hits {
+    float* t;
+    uint32_t* inst;
+    uint32_t* prim;
+    float* u;
+    float* v;
+} snkpt_hits;
+
+typedef struct snkpt_accel_info {
+    uint32_t num_triangles;   /* world-space triangles (all instances) */
+    uint32_t num_nodes;       /* BVH8 nodes, 80 B each */
+    uint32_t num_leaf_tris;   /* leaf triangle records, 48 B each (== num_triangles) */
+    uint32_t depth;           /* BVH8 depth (root = 1) */
+    uint32_t binary_depth;    /* LBVH depth before the collapse */
+    uint32_t num_emitters;    /* emissive triangles */
+    float scene_lo[3];
+    float scene_hi[3];
+    double build_ms;          /* device time of the last build */
+} snkpt_accel_info;
+
+/* ---- context ------------------------------------------------------------- */
+snkpt_status snkpt_create(int hip_device, snkpt_ctx** out);
+void snkpt_destroy(snkpt_ctx* ctx);
+const char* snkpt_last_error(const snkpt_ctx* ctx);
+
+/* ---- scene input ----------------------------------------------------------
+ * MeshBufferManager::LoadMeshFromData (MeshBufferManager.cpp:22-88): appends a
+ * mesh to the global SoA buffers.  pos/nrm/tan are vec3 (12 B stride), uv is
+ * vec2; nrm/uv/tan may be NULL (zeros).  idx are submesh-local u32.  Returns
+ * the vertex and index offsets of the mesh in the global buffers. */
+snkpt_status snkpt_upload_mesh(snkpt_ctx* ctx, const float* pos, const float* nrm, const float* uv,
+                               const float* tan, uint32_t n_verts, const uint32_t* idx, uint32_t n_idx,
+                               uint32_t* out_vertex_offset, uint32_t* out_index_offset);
+/* RaytracingInstanceBufferSystem::UpdateInstanceBuffer (RaytracingBufferSystem.cpp:69-103)
+ * + TransformBufferSystem::UpdateTransformBuffer (TransformBufferSystem.h:92-122):
+ * instance records and column-major mat4 transforms.  Replaces the previous set. */
+snkpt_status snkpt_set_instances(snkpt_ctx* ctx, const snkpt_instance* inst, uint32_t n,
+                                 const float* mat4s, uint32_t n_mat4);
+/* GlobalMaterialBufferManager::UpdateMaterialUBO (ResourceBufferManagers.cpp:64-101). */
+snkpt_status snkpt_set_materials(snkpt_ctx* ctx, const snkpt_material* m, uint32_t n);
+/* LightBufferSystem::UpdateLightSSBO (LightBufferSystem.cpp:65-108) and
+ * UpdateEmissiveIdxBuffer (RaytracingBufferSystem.cpp:105-130).  light_block may
+ * be NULL (no analytic lights).  emissive_instances lists the emissive instance
+ * records; NULL means "every instance whose material has the emissive flag". */
+snkpt_status snkpt_set_lights(snkpt_ctx* ctx, const void* light_block, size_t bytes,
+                              const uint32_t* emissive_instances, uint32_t n_emissive);
+
+/* BLAS::GenerateFromMeshData + TlasSystem::BuildTlas (TLAS.cpp:10-200,
+ * TlasSystem.cpp:27-45): flattens every instance to world space and builds one
+ * BVH8 on the device (Morton30 -> radix sort -> Karras -> SAH collapse ->
+ * 8-bit quantised children), plus the emitter table.  Synchronises `stream`. */
+snkpt_status snkpt_build_accel(snkpt_ctx* ctx, snkpt_stream stream);
+snkpt_status snkpt_get_accel_info(const snkpt_ctx* ctx, snkpt_accel_info* out);
+
+/* Copies one build array to host memory for bit-exact checks.  `which`:
+ *   0 sorted Morton keys (u32 / triangle)     1 sorted triangle ids (u32 / triangle)
+ *   2 binary nodes: left, right (u32 x 2 / internal node; leaves are tagged 0x80000000|leaf)
+ *   3 binary AABBs (float x 6 / node, internal nodes first then leaves)
+ *   4 BVH8 nodes (80 B / node)                5 leaf triangles (48 B / record)
+ *   6 world triangles in input order (float x 9 / triangle)
+ *   7 binary-node collapse flags (u32 / internal node: bit0 left-is-leaf-cluster, bit1 right)
+ * `bytes` must be at least the array's size; *out_bytes receives it. */
+snkpt_status snkpt_accel_export(snkpt_ctx* ctx, uint32_t which, void* host_dst, size_t bytes,
+                                size_t* out_bytes);
+
+/* ---- tracing --------------------------------------------------------------
+ * traceRayEXT with gl_RayFlagsOpaqueEXT (closest hit, RayGen.rgen:164-176) or
+ * TerminateOnFirstHit|Opaque|SkipClosestHit (any hit, RayGen.rgen:101-121).
+ * Both faces are hit (Q5: no cull flags).  Ray i's result is at hits[i]. */
+snkpt_status snkpt_trace(snkpt_ctx* ctx, const snkpt_rays* rays, uint32_t n, const snkpt_hits* hits,
+                         int any_hit, snkpt_stream stream);
+/* Same traversal, also counting BVH8 nodes and triangles visited per ray into
+ * node_visits / tri_visits (device u32 arrays, may be NULL). */
+snkpt_status snkpt_trace_counted(snkpt_ctx* ctx, const snkpt_rays* rays, uint32_t n,
+                                 const snkpt_hits* hits, int any_hit, uint32_t* node_visits,
+                                 uint32_t* tri_visits, snkpt_stream stream);
+
+/* ---- the pass -------------------------------------------------------------
+ * RT::RecordRenderCmdBuf (Raytracing.cpp:227-267): renders one frame of
+ * settings->spp samples per pixel into out_rgba (device, w*h RGBA, row-major,
+ * row 0 at the top).  With SNKPT_FLAG_ACCUMULATE the output is the running mean
+ * over all accumulated frames. */
+snkpt_status snkpt_render(snkpt_ctx* ctx, const snkpt_camera* cam, const snkpt_settings* settings,
+                          void* out_rgba, uint32_t w, uint32_t h, uint32_t out_format,
+                          snkpt_stream stream);
+
+/* Rays traced by the last snkpt_render, by kind. */
+typedef struct snkpt_frame_stats {
+    uint64_t camera_rays;
+    uint64_t extension_rays; /* closest-hit rays after the camera ray */
+    uint64_t shadow_rays;
+    uint64_t cache_queries;
+    uint64_t cache_train_samples;
+} snkpt_frame_stats;
+snkpt_status snkpt_get_frame_stats(snkpt_ctx* ctx, snkpt_frame_stats* out);
+
+/* Converged-image oracle: adds `spp` samples per pixel of the plain path
+ * tracer to accum_rgb (device, w*h*3 doubles, sums).  The only FP64 on the
+ * device; tracing and shading stay FP32. */
+snkpt_status snkpt_oracle_accumulate(snkpt_ctx* ctx, const snkpt_camera* cam,
+                                     const snkpt_settings* settings, double* accum_rgb, uint32_t w,
+                                     uint32_t h, uint32_t spp, snkpt_stream stream);
+
+/* ---- synthetic scenes (no reference counterpart: workload generators) ------
+ * config 1..5 as described in the scene generator; variant picks one of the
+ * scenes of a multi-scene config (config 5: 0..15) and is 0 otherwise.
+ * All arrays are host memory owned by the scene object. */
+typedef struct snkpt_scene {
+    float* pos;
+    float* nrm;
+    float* uv;
+    float* tan;
+    uint32_t n_verts;
+    uint32_t* idx;
+    uint32_t n_idx;
+    snkpt_instance* inst;
+    uint32_t n_inst;
+    float* mat4s;
+    uint32_t n_mat4;
+    snkpt_material* mats;
+    uint32_t n_mats;
+    uint32_t* emissive_instances;
+    uint32_t n_emissive;
+    snkpt_camera camera;
+    uint32_t width, height;
+    uint32_t max_depth, rr_start;
+    uint32_t seed;
+} snkpt_scene;
+snkpt_status snkpt_scene_generate(uint32_t config, uint32_t variant, snkpt_scene** out);
+void snkpt_scene_free(snkpt_scene* scene);
+/* Camera of frame f of the config's camera path (config 3: seeded orbit). */
+snkpt_status snkpt_scene_camera(uint32_t config, uint32_t variant, uint32_t frame, uint32_t w,
+                                uint32_t h, snkpt_camera* out);
+/* upload_mesh + set_instances + set_materials + set_lights for a generated scene. */
+snkpt_status snkpt_scene_load(snkpt_ctx* ctx, const snkpt_scene* scene);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SNKPT_H */

@@ -1,0 +1,139 @@
+"""numpy binding of the CPU oracle (liboracle.so).  TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg;
+never by the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
+    lib = C.CDLL(LIB_PATH)
+    VP, U32, I = C.c_void_p, C.c_uint32, C.c_int
+    lib.o_pcg32_fill.argtypes = [C.c_uint64, C.c_uint64, VP, U32]
+    lib.o_flatten.argtypes = [VP, U32, VP, U32, VP, U32, VP, U32, VP, U32]
+    lib.o_flatten.restype = I
+    lib.o_intersect.argtypes = [VP, VP, VP, VP, VP, VP, VP, VP]
+    lib.o_intersect.restype = I
+    lib.o_build.argtypes = [VP, U32, C.POINTER(OBvh)]
+    lib.o_build.restype = I
+    lib.o_bvh_free.argtypes = [C.POINTER(OBvh)]
+    lib.o_trace_bvh8.argtypes = [C.POINTER(OBvh), VP, U32, I, VP, VP, VP, VP, VP, VP, VP, I]
+    lib.o_trace_brute.argtypes = [VP, U32, VP, U32, I, VP, VP, VP, VP, VP, I]
+    _lib = lib
+    return lib
+
+
+class OBvh(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("num_nodes", C.c_uint32), ("depth", C.c_uint32),
+                ("binary_depth", C.c_uint32), ("keys", C.POINTER(C.c_uint32)), ("ids", C.POINTER(C.c_uint32)),
+                ("child", C.POINTER(C.c_uint32)), ("first", C.POINTER(C.c_uint32)),
+                ("count", C.POINTER(C.c_uint32)), ("aabb", C.POINTER(C.c_float)),
+                ("leafify", C.POINTER(C.c_uint32)), ("nodes", C.POINTER(C.c_uint32)),
+                ("leaf", C.POINTER(C.c_float))]
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def pcg32(initstate: int, initseq: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.uint32)
+    load().o_pcg32_fill(initstate, initseq, _p(out), n)
+    return out
+
+
+def flatten(pos, idx, inst5, mat4s) -> np.ndarray:
+    """World triangle records (n, 12) float32 view; columns 9..11 hold inst, prim, gid bits."""
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    inst5 = np.ascontiguousarray(inst5, np.uint32).reshape(-1)
+    mat4s = np.ascontiguousarray(mat4s, np.float32).reshape(-1)
+    ntri = int(sum(int(inst5[5 * i + 4]) // 3 for i in range(inst5.size // 5)))
+    out = np.zeros((max(ntri, 1), 12), np.float32)
+    r = load().o_flatten(_p(pos), pos.size // 3, _p(idx), idx.size, _p(inst5), inst5.size // 5, _p(mat4s),
+                         mat4s.size // 16, _p(out), ntri)
+    if r < 0:
+        raise ValueError("o_flatten: invalid scene")
+    return out[:r]
+
+
+def intersect(o, d, v0, v1, v2):
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (o, d, v0, v1, v2)]
+    t, u, v = C.c_float(), C.c_float(), C.c_float()
+    hit = load().o_intersect(*[_p(a) for a in arrs], C.byref(t), C.byref(u), C.byref(v))
+    return bool(hit), t.value, u.value, v.value
+
+
+class Bvh:
+    """CPU twin of the GPU build; arrays are numpy views into oracle memory."""
+
+    def __init__(self, world: np.ndarray):
+        self.world = np.ascontiguousarray(world, np.float32)
+        self.b = OBvh()
+        n = self.world.shape[0]
+        if load().o_build(_p(self.world), n, C.byref(self.b)) != 0:
+            raise RuntimeError("o_build failed")
+        b = self.b
+        as_u32 = lambda p, k: np.ctypeslib.as_array(p, (k,)).copy() if k else np.zeros(0, np.uint32)
+        as_f32 = lambda p, k: np.ctypeslib.as_array(p, (k,)).copy() if k else np.zeros(0, np.float32)
+        n_int = max(n - 1, 0)
+        self.n = n
+        self.num_nodes = b.num_nodes
+        self.depth = b.depth
+        self.binary_depth = b.binary_depth
+        self.keys = as_u32(b.keys, n)
+        self.ids = as_u32(b.ids, n)
+        self.child = as_u32(b.child, 2 * n_int)
+        self.first = as_u32(b.first, n_int)
+        self.count = as_u32(b.count, n_int)
+        self.aabb = as_f32(b.aabb, 6 * (n_int + n))
+        self.leafify = as_u32(b.leafify, n_int)
+        self.nodes = as_u32(b.nodes, 20 * b.num_nodes)
+        self.leaf = as_f32(b.leaf, 12 * n)
+
+    def trace(self, rays: np.ndarray, any_hit=False, threads=8):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        t = np.zeros(n, np.float32)
+        inst = np.zeros(n, np.uint32)
+        prim = np.zeros(n, np.uint32)
+        u = np.zeros(n, np.float32)
+        v = np.zeros(n, np.float32)
+        nv = np.zeros(n, np.uint32)
+        tv = np.zeros(n, np.uint32)
+        load().o_trace_bvh8(C.byref(self.b), _p(rays), n, int(any_hit), _p(t), _p(inst), _p(prim), _p(u), _p(v),
+                            _p(nv), _p(tv), threads)
+        return t, inst, prim, u, v, nv, tv
+
+    def __del__(self):
+        if _lib is not None and getattr(self, "b", None) is not None:
+            _lib.o_bvh_free(C.byref(self.b))
+            self.b = None
+
+
+def trace_brute(world: np.ndarray, rays: np.ndarray, any_hit=False, threads=8):
+    world = np.ascontiguousarray(world, np.float32)
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    n = rays.shape[0]
+    t = np.zeros(n, np.float32)
+    inst = np.zeros(n, np.uint32)
+    prim = np.zeros(n, np.uint32)
+    u = np.zeros(n, np.float32)
+    v = np.zeros(n, np.float32)
+    load().o_trace_brute(_p(world), world.shape[0], _p(rays), n, int(any_hit), _p(t), _p(inst), _p(prim), _p(u),
+                         _p(v), threads)
+    return t, inst, prim, u, v

@@ -1,0 +1,82 @@
+/*
+ * snkpt oracle — CPU restatement used ONLY as test infrastructure.
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg load
+ * this library, and only as the checker or the timed CPU baseline.  The
+ * product (snake-vk_amd/libsnkpt.so) never links, loads or calls it.
+ *
+ * What it restates and where each part is pinned (see the header comment of
+ * each source file for the per-function pins):
+ *   pcg32          O'Neill 2014 PCG32; pinned by the published pcg32 demo vectors.
+ *   flatten        object->world transform of instances (TLAS.cpp:188-200 semantics,
+ *                  RaytracingInstances.glsl:7-13 indexing); pinned by closed-form transforms.
+ *   intersect      Woop, Benthin & Wald 2013 watertight test, both faces (Q5);
+ *                  pinned by closed-form hits, double-precision t and shared-edge closure.
+ *   build          LBVH (Karras 2012) + SAH collapse to BVH8 (Ylitie et al. 2017);
+ *                  pinned by invariants (sortedness, range/split property, box containment,
+ *                  every triangle in exactly one leaf) and by traversal == brute force.
+ *   trace          BVH8 traversal and brute-force closest/any hit.
+ *   shading / PT / ReSTIR / MLP: see snkpt_oracle_pt.c and snkpt_oracle_mlp.c.
+ */
+#ifndef SNKPT_ORACLE_H
+#define SNKPT_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- PCG32 ---------------------------------------------------------------- */
+typedef struct {
+    uint64_t state, inc;
+} o_pcg32;
+void o_pcg32_seed(o_pcg32* r, uint64_t initstate, uint64_t initseq);
+uint32_t o_pcg32_next(o_pcg32* r);
+/* n outputs of pcg32(initstate, initseq) into out */
+void o_pcg32_fill(uint64_t initstate, uint64_t initseq, uint32_t* out, uint32_t n);
+
+/* ---- geometry ------------------------------------------------------------- */
+/* World-space triangle records, 12 x 32-bit per triangle:
+ * v0.xyz v1.xyz v2.xyz (float) then inst, prim, gid (u32). */
+int o_flatten(const float* pos, uint32_t n_verts, const uint32_t* idx, uint32_t n_idx, const uint32_t* inst5,
+              uint32_t n_inst, const float* mat4s, uint32_t n_mat4, float* out, uint32_t max_tris);
+
+/* Watertight intersection of one ray (o, d) with one triangle; returns 1 on a
+ * hit of the supporting triangle (no t range test) with t, u, v. */
+int o_intersect(const float o[3], const float d[3], const float v0[3], const float v1[3], const float v2[3],
+                float* t, float* u, float* v);
+
+/* ---- LBVH -> BVH8 build ----------------------------------------------------- */
+typedef struct {
+    uint32_t n;           /* triangles */
+    uint32_t num_nodes;   /* BVH8 nodes */
+    uint32_t depth;       /* BVH8 levels */
+    uint32_t binary_depth;
+    uint32_t* keys;       /* n, sorted Morton keys */
+    uint32_t* ids;        /* n, sorted triangle ids */
+    uint32_t* child;      /* 2*(n-1), tagged 0x80000000 | leaf */
+    uint32_t* first;      /* n-1 */
+    uint32_t* count;      /* n-1 */
+    float* aabb;          /* 6*(2n-1): internal nodes then leaves */
+    uint32_t* leafify;    /* n-1 */
+    uint32_t* nodes;      /* 20 u32 per BVH8 node */
+    float* leaf;          /* 12 x 32-bit per leaf record */
+} o_bvh;
+
+int o_build(const float* world, uint32_t n, o_bvh* out);
+void o_bvh_free(o_bvh* b);
+
+/* ---- tracing ---------------------------------------------------------------
+ * rays: 8 floats per ray (ox oy oz tmin dx dy dz tmax).  Outputs as the
+ * library's snkpt_hits (t = -1 and ids ~0 on a miss); u, v, nvis, tvis may be NULL. */
+void o_trace_bvh8(const o_bvh* b, const float* rays, uint32_t n, int any_hit, float* t, uint32_t* inst,
+                  uint32_t* prim, float* u, float* v, uint32_t* nvis, uint32_t* tvis, int threads);
+void o_trace_brute(const float* world, uint32_t ntris, const float* rays, uint32_t n, int any_hit, float* t,
+                   uint32_t* inst, uint32_t* prim, float* u, float* v, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

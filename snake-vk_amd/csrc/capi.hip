@@ -1,0 +1,309 @@
+// C ABI entry points (snkpt.h): context lifetime, scene input, dispatch.
+//
+// Mirrors the buffer-upload side of the reference pass: MeshBufferManager
+// (MeshBufferManager.cpp:22-88), RaytracingInstanceBufferSystem
+// (RaytracingBufferSystem.cpp:69-130), GlobalMaterialBufferManager
+// (ResourceBufferManagers.cpp:64-101) and LightBufferSystem
+// (LightBufferSystem.cpp:65-108).  Where the reference asserts and breaks into
+// the debugger (util.h:18-30) this library returns a status and keeps the
+// message for snkpt_last_error.
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "snkpt_host.h"
+
+namespace snkpt {
+
+snkpt_status fail(snkpt_ctx* ctx, snkpt_status code, const std::string& msg) {
+    if (ctx) ctx->last_error = msg;
+    return code;
+}
+
+snkpt_status hip_fail(snkpt_ctx* ctx, hipError_t e, const char* what) {
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+        return fail(ctx, SNKPT_ERR_OOM, std::string(what) + ": " + hipGetErrorString(e));
+    return fail(ctx, SNKPT_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+static snkpt_status upload(snkpt_ctx* ctx, DeviceBuffer& buf, const void* src, size_t bytes) {
+    hipError_t e = buf.reserve(bytes ? bytes : 4);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+    if (bytes) {
+        e = hipMemcpy(buf.ptr, src, bytes, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpy");
+    }
+    return SNKPT_OK;
+}
+
+// Per instance: object->world rows (3 x float4) and the normal matrix
+// transpose(inverse(mat3(M))) rows (3 x float4), as RayClosestHit.rchit:110-111.
+static void instance_xf(const float* m, float out[24]) {
+    // m column-major: element (row r, col c) = m[4c + r]
+    for (int r = 0; r < 3; ++r) {
+        out[4 * r + 0] = m[r];
+        out[4 * r + 1] = m[4 + r];
+        out[4 * r + 2] = m[8 + r];
+        out[4 * r + 3] = m[12 + r];
+    }
+    double a[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) a[r][c] = m[4 * c + r];
+    double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                 a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+    double inv_det = det != 0.0 ? 1.0 / det : 0.0;
+    // cofactor matrix / det = transpose(inverse(a))
+    double cof[3][3];
+    cof[0][0] = a[1][1] * a[2][2] - a[1][2] * a[2][1];
+    cof[0][1] = -(a[1][0] * a[2][2] - a[1][2] * a[2][0]);
+    cof[0][2] = a[1][0] * a[2][1] - a[1][1] * a[2][0];
+    cof[1][0] = -(a[0][1] * a[2][2] - a[0][2] * a[2][1]);
+    cof[1][1] = a[0][0] * a[2][2] - a[0][2] * a[2][0];
+    cof[1][2] = -(a[0][0] * a[2][1] - a[0][1] * a[2][0]);
+    cof[2][0] = a[0][1] * a[1][2] - a[0][2] * a[1][1];
+    cof[2][1] = -(a[0][0] * a[1][2] - a[0][2] * a[1][0]);
+    cof[2][2] = a[0][0] * a[1][1] - a[0][1] * a[1][0];
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) out[12 + 4 * r + c] = (float)(cof[r][c] * inv_det);
+        out[12 + 4 * r + 3] = 0.0f;
+    }
+}
+
+}  // namespace snkpt
+
+using namespace snkpt;
+
+extern "C" {
+
+snkpt_status snkpt_create(int hip_device, snkpt_ctx** out) {
+    if (!out) return SNKPT_ERR_INVALID_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || hip_device < 0 || hip_device >= n) return SNKPT_ERR_INVALID_ARG;
+    snkpt_ctx* ctx = new (std::nothrow) snkpt_ctx();
+    if (!ctx) return SNKPT_ERR_OOM;
+    ctx->device = hip_device;
+    if (hipSetDevice(hip_device) != hipSuccess || hipGetDeviceProperties(&ctx->props, hip_device) != hipSuccess) {
+        delete ctx;
+        return SNKPT_ERR_HIP;
+    }
+    *out = ctx;
+    return SNKPT_OK;
+}
+
+void snkpt_destroy(snkpt_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    render_state_free(ctx->render);
+    cache_state_free(ctx->cache);
+    delete ctx;
+}
+
+const char* snkpt_last_error(const snkpt_ctx* ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+snkpt_status snkpt_upload_mesh(snkpt_ctx* ctx, const float* pos, const float* nrm, const float* uv, const float* tan,
+                               uint32_t n_verts, const uint32_t* idx, uint32_t n_idx, uint32_t* out_vertex_offset,
+                               uint32_t* out_index_offset) {
+    if (!ctx) return SNKPT_ERR_INVALID_ARG;
+    if ((n_verts && !pos) || (n_idx && !idx) || n_idx % 3 != 0)
+        return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_upload_mesh: null arrays or index count not a multiple of 3");
+    for (uint32_t i = 0; i < n_idx; ++i)
+        if (idx[i] >= n_verts) return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_upload_mesh: index out of range");
+    uint64_t vo = ctx->pos.size() / 3, io = ctx->idx.size();
+    if (vo + n_verts >= (1ull << 32) || io + n_idx >= (1ull << 32))
+        return fail(ctx, SNKPT_ERR_CAPACITY, "snkpt_upload_mesh: global buffers exceed 2^32 entries");
+    ctx->pos.insert(ctx->pos.end(), pos, pos + 3 * (size_t)n_verts);
+    if (nrm)
+        ctx->nrm.insert(ctx->nrm.end(), nrm, nrm + 3 * (size_t)n_verts);
+    else
+        ctx->nrm.resize(ctx->nrm.size() + 3 * (size_t)n_verts, 0.0f);
+    if (uv)
+        ctx->uv.insert(ctx->uv.end(), uv, uv + 2 * (size_t)n_verts);
+    else
+        ctx->uv.resize(ctx->uv.size() + 2 * (size_t)n_verts, 0.0f);
+    if (tan)
+        ctx->tan.insert(ctx->tan.end(), tan, tan + 3 * (size_t)n_verts);
+    else
+        ctx->tan.resize(ctx->tan.size() + 3 * (size_t)n_verts, 0.0f);
+    ctx->idx.insert(ctx->idx.end(), idx, idx + n_idx);
+    if (out_vertex_offset) *out_vertex_offset = (uint32_t)vo;
+    if (out_index_offset) *out_index_offset = (uint32_t)io;
+    ctx->dirty = true;
+    ctx->accel.built = false;
+    return SNKPT_OK;
+}
+
+snkpt_status snkpt_set_instances(snkpt_ctx* ctx, const snkpt_instance* inst, uint32_t n, const float* mat4s,
+                                 uint32_t n_mat4) {
+    if (!ctx) return SNKPT_ERR_INVALID_ARG;
+    if ((n && !inst) || (n_mat4 && !mat4s)) return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_set_instances: null arrays");
+    if (n > SNKPT_MAX_INSTANCES) return fail(ctx, SNKPT_ERR_CAPACITY, "snkpt_set_instances: more than 4096 instances");
+    if (n_mat4 > SNKPT_MAX_TRANSFORMS)
+        return fail(ctx, SNKPT_ERR_CAPACITY, "snkpt_set_instances: more than 4096 transforms");
+    ctx->inst.assign(inst, inst + n);
+    ctx->mat4s.assign(mat4s, mat4s + 16 * (size_t)n_mat4);
+    ctx->dirty = true;
+    ctx->accel.built = false;
+    return SNKPT_OK;
+}
+
+snkpt_status snkpt_set_materials(snkpt_ctx* ctx, const snkpt_material* m, uint32_t n) {
+    if (!ctx) return SNKPT_ERR_INVALID_ARG;
+    if (n && !m) return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_set_materials: null array");
+    if (n > SNKPT_MAX_MATERIALS) return fail(ctx, SNKPT_ERR_CAPACITY, "snkpt_set_materials: more than 4096 materials");
+    for (uint32_t i = 0; i < n; ++i)
+        if (m[i].albedo_tex_idx != SNKPT_INVALID_INDEX)
+            return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_set_materials: textures are not supported on this path");
+    ctx->mats.assign(m, m + n);
+    ctx->dirty = true;
+    ctx->accel.built = false;
+    return SNKPT_OK;
+}
+
+snkpt_status snkpt_set_lights(snkpt_ctx* ctx, const void* light_block, size_t bytes,
+                              const uint32_t* emissive_instances, uint32_t n_emissive) {
+    if (!ctx) return SNKPT_ERR_INVALID_ARG;
+    if (light_block) {
+        if (bytes < 112) return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_set_lights: light block shorter than its header");
+        uint32_t np, ns;
+        std::memcpy(&np, (const uint8_t*)light_block + 96, 4);
+        std::memcpy(&ns, (const uint8_t*)light_block + 100, 4);
+        if (np > SNKPT_MAX_POINTLIGHTS || ns > SNKPT_MAX_SPOTLIGHTS)
+            return fail(ctx, SNKPT_ERR_CAPACITY, "snkpt_set_lights: light counts above 1024");
+        if (bytes < 112 + (size_t)np * 48)
+            return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_set_lights: block shorter than its point lights");
+        ctx->light_block.assign((const uint8_t*)light_block, (const uint8_t*)light_block + bytes);
+    } else {
+        ctx->light_block.clear();
+    }
+    if (emissive_instances) {
+        ctx->emissive_inst.assign(emissive_instances, emissive_instances + n_emissive);
+        ctx->emissive_auto = false;
+    } else {
+        ctx->emissive_inst.clear();
+        ctx->emissive_auto = true;
+    }
+    ctx->dirty = true;
+    ctx->accel.built = false;
+    return SNKPT_OK;
+}
+
+snkpt_status snkpt_build_accel(snkpt_ctx* ctx, snkpt_stream stream) {
+    if (!ctx) return SNKPT_ERR_INVALID_ARG;
+    if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, SNKPT_ERR_HIP, "hipSetDevice");
+    hipStream_t st = (hipStream_t)stream;
+    snkpt_status s;
+    if ((s = upload(ctx, ctx->d_pos, ctx->pos.data(), ctx->pos.size() * 4)) != SNKPT_OK) return s;
+    if ((s = upload(ctx, ctx->d_nrm, ctx->nrm.data(), ctx->nrm.size() * 4)) != SNKPT_OK) return s;
+    if ((s = upload(ctx, ctx->d_uv, ctx->uv.data(), ctx->uv.size() * 4)) != SNKPT_OK) return s;
+    if ((s = upload(ctx, ctx->d_idx, ctx->idx.data(), ctx->idx.size() * 4)) != SNKPT_OK) return s;
+    if ((s = upload(ctx, ctx->d_inst, ctx->inst.data(), ctx->inst.size() * sizeof(snkpt_instance))) != SNKPT_OK)
+        return s;
+    if ((s = upload(ctx, ctx->d_mat4, ctx->mat4s.data(), ctx->mat4s.size() * 4)) != SNKPT_OK) return s;
+    if ((s = upload(ctx, ctx->d_mats, ctx->mats.data(), ctx->mats.size() * sizeof(snkpt_material))) != SNKPT_OK)
+        return s;
+    std::vector<float> xf(24 * ctx->inst.size());
+    const uint32_t n_mat4 = (uint32_t)(ctx->mat4s.size() / 16);
+    for (size_t i = 0; i < ctx->inst.size(); ++i) {
+        if (ctx->inst[i].transform_idx >= n_mat4)
+            return fail(ctx, SNKPT_ERR_INVALID_ARG, "instance " + std::to_string(i) + ": transform_idx out of range");
+        instance_xf(&ctx->mat4s[16 * (size_t)ctx->inst[i].transform_idx], &xf[24 * i]);
+    }
+    if ((s = upload(ctx, ctx->d_inst_xf, xf.data(), xf.size() * 4)) != SNKPT_OK) return s;
+    ctx->dirty = false;
+    return build_accel(ctx, st);
+}
+
+snkpt_status snkpt_get_accel_info(const snkpt_ctx* ctx, snkpt_accel_info* out) {
+    if (!ctx || !out) return SNKPT_ERR_INVALID_ARG;
+    const Accel& A = ctx->accel;
+    if (!A.built) return SNKPT_ERR_NOT_BUILT;
+    std::memset(out, 0, sizeof(*out));
+    out->num_triangles = A.num_tris;
+    out->num_nodes = A.num_nodes;
+    out->num_leaf_tris = A.num_tris;
+    out->depth = A.depth;
+    out->binary_depth = A.binary_depth;
+    out->num_emitters = ctx->emit.count;
+    for (int a = 0; a < 3; ++a) {
+        out->scene_lo[a] = A.lo[a];
+        out->scene_hi[a] = A.hi[a];
+    }
+    out->build_ms = A.build_ms;
+    return SNKPT_OK;
+}
+
+snkpt_status snkpt_accel_export(snkpt_ctx* ctx, uint32_t which, void* host_dst, size_t bytes, size_t* out_bytes) {
+    if (!ctx) return SNKPT_ERR_INVALID_ARG;
+    return accel_export(ctx, which, host_dst, bytes, out_bytes);
+}
+
+snkpt_status snkpt_trace(snkpt_ctx* ctx, const snkpt_rays* rays, uint32_t n, const snkpt_hits* hits, int any_hit,
+                         snkpt_stream stream) {
+    if (!ctx) return SNKPT_ERR_INVALID_ARG;
+    if (!rays || !hits) return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_trace: null rays or hits");
+    return launch_trace(ctx, (const float4*)rays->org_tmin, (const float4*)rays->dir_tmax, n, *hits, any_hit != 0,
+                        nullptr, nullptr, nullptr, (hipStream_t)stream);
+}
+
+snkpt_status snkpt_trace_counted(snkpt_ctx* ctx, const snkpt_rays* rays, uint32_t n, const snkpt_hits* hits,
+                                 int any_hit, uint32_t* node_visits, uint32_t* tri_visits, snkpt_stream stream) {
+    if (!ctx) return SNKPT_ERR_INVALID_ARG;
+    if (!rays || !hits) return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_trace_counted: null rays or hits");
+    if (!node_visits && !tri_visits)
+        return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_trace_counted: no counter arrays given");
+    return launch_trace(ctx, (const float4*)rays->org_tmin, (const float4*)rays->dir_tmax, n, *hits, any_hit != 0,
+                        node_visits, tri_visits, nullptr, (hipStream_t)stream);
+}
+
+snkpt_status snkpt_render(snkpt_ctx* ctx, const snkpt_camera* cam, const snkpt_settings* settings, void* out_rgba,
+                          uint32_t w, uint32_t h, uint32_t out_format, snkpt_stream stream) {
+    if (!ctx) return SNKPT_ERR_INVALID_ARG;
+    if (!cam || !settings || !out_rgba || w == 0 || h == 0)
+        return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_render: null camera/settings/output or empty image");
+    if (out_format != SNKPT_FORMAT_RGBA32F && out_format != SNKPT_FORMAT_RGBA16F)
+        return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_render: unknown output format");
+    return render(ctx, cam, settings, out_rgba, w, h, out_format, (hipStream_t)stream);
+}
+
+snkpt_status snkpt_get_frame_stats(snkpt_ctx* ctx, snkpt_frame_stats* out) {
+    if (!ctx || !out) return SNKPT_ERR_INVALID_ARG;
+    *out = ctx->stats;
+    return SNKPT_OK;
+}
+
+snkpt_status snkpt_oracle_accumulate(snkpt_ctx* ctx, const snkpt_camera* cam, const snkpt_settings* settings,
+                                     double* accum_rgb, uint32_t w, uint32_t h, uint32_t spp, snkpt_stream stream) {
+    if (!ctx) return SNKPT_ERR_INVALID_ARG;
+    if (!cam || !settings || !accum_rgb || w == 0 || h == 0)
+        return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_oracle_accumulate: null argument or empty image");
+    return oracle_accumulate(ctx, cam, settings, accum_rgb, w, h, spp, (hipStream_t)stream);
+}
+
+snkpt_status snkpt_scene_generate(uint32_t config, uint32_t variant, snkpt_scene** out) {
+    return scene_generate(config, variant, out);
+}
+
+void snkpt_scene_free(snkpt_scene* scene) { scene_free(scene); }
+
+snkpt_status snkpt_scene_camera(uint32_t config, uint32_t variant, uint32_t frame, uint32_t w, uint32_t h,
+                                snkpt_camera* out) {
+    return scene_camera(config, variant, frame, w, h, out);
+}
+
+snkpt_status snkpt_scene_load(snkpt_ctx* ctx, const snkpt_scene* s) {
+    if (!ctx || !s) return SNKPT_ERR_INVALID_ARG;
+    ctx->pos.clear();
+    ctx->nrm.clear();
+    ctx->uv.clear();
+    ctx->tan.clear();
+    ctx->idx.clear();
+    snkpt_status st = snkpt_upload_mesh(ctx, s->pos, s->nrm, s->uv, s->tan, s->n_verts, s->idx, s->n_idx, nullptr,
+                                        nullptr);
+    if (st != SNKPT_OK) return st;
+    if ((st = snkpt_set_instances(ctx, s->inst, s->n_inst, s->mat4s, s->n_mat4)) != SNKPT_OK) return st;
+    if ((st = snkpt_set_materials(ctx, s->mats, s->n_mats)) != SNKPT_OK) return st;
+    return snkpt_set_lights(ctx, nullptr, 0, s->emissive_instances, s->n_emissive);
+}
+
+}  // extern "C"

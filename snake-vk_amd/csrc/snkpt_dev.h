@@ -1,0 +1,237 @@
+// Device-side layouts and inline routines shared by the snkpt kernels.
+//
+// BVH8 node (80 B, five 16 B words), after Ylitie, Karras & Laine 2017
+// ("Efficient incoherent ray traversal on GPUs through compressed wide BVHs"):
+//   w0 = p.x, p.y, p.z, (e.x | e.y << 8 | e.z << 16 | imask << 24)
+//   w1 = child_base, tri_base, meta[0..3], meta[4..7]
+//   w2 = qlo_x[0..3], qlo_x[4..7], qlo_y[0..3], qlo_y[4..7]
+//   w3 = qlo_z[0..3], qlo_z[4..7], qhi_x[0..3], qhi_x[4..7]
+//   w4 = qhi_y[0..3], qhi_y[4..7], qhi_z[0..3], qhi_z[4..7]
+// Child box of slot s on axis a: [p_a + qlo_a[s] * 2^(e_a-127), p_a + qhi_a[s] * 2^(e_a-127)].
+// meta[s] == 0: empty slot.  Internal child (imask bit s): meta = 0x20 | (24 + s);
+// its node index is child_base + popcount(imask & ((1 << s) - 1)).  Leaf child:
+// meta = (unary triangle count 1/3/7) << 5 | offset, its triangles are leaf
+// records tri_base + offset ... + count - 1.
+// Slot s holds the child lying towards octant s (bit a set = positive side of
+// axis a), so a ray visits slots in order k = 0, 1, ... with s = k ^ octant(ray).
+//
+// Leaf triangle record (48 B, three 16 B words):
+//   r0 = v0.x, v0.y, v0.z, v1.x   r1 = v1.y, v1.z, v2.x, v2.y   r2 = v2.z, inst, prim, gid
+// gid is the global triangle id (instances in record order, triangles in
+// submesh order); equal-t hits are resolved to the smaller gid so the closest
+// hit does not depend on traversal order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace snkpt {
+
+constexpr int kNodeWords = 5;  // uint4 per BVH8 node
+constexpr int kTriWords = 3;   // float4 per leaf triangle
+// 1 + 2^-16: box exits (and the current closest t they are clipped to) are
+// widened by this factor so rounding never culls a box holding a closer hit.
+constexpr float kBoxRobust = 1.0000152587890625f;
+
+struct Ray {
+    float ox, oy, oz, tmin;
+    float dx, dy, dz, tmax;
+};
+
+// Per-ray constants of the watertight test (Woop, Benthin & Wald 2013) and of
+// the box test.
+struct RayPre {
+    float ox, oy, oz;
+    float ix, iy, iz;  // 1/d with |d| clamped away from 0
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+    uint32_t oct;  // bit a set: the ray runs towards -a
+};
+
+__device__ __forceinline__ float sel3(float x, float y, float z, int k) { return k == 0 ? x : (k == 1 ? y : z); }
+
+__device__ __forceinline__ float safe_inv(float d) {
+    float a = fabsf(d) > 1e-20f ? d : copysignf(1e-20f, d);
+    return 1.0f / a;
+}
+
+__device__ __forceinline__ RayPre ray_setup(const Ray& r) {
+    RayPre p;
+    p.ox = r.ox;
+    p.oy = r.oy;
+    p.oz = r.oz;
+    p.ix = safe_inv(r.dx);
+    p.iy = safe_inv(r.dy);
+    p.iz = safe_inv(r.dz);
+    float ax = fabsf(r.dx), ay = fabsf(r.dy), az = fabsf(r.dz);
+    int kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    float dkz = sel3(r.dx, r.dy, r.dz, kz);
+    if (dkz < 0.0f) {
+        int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    p.kx = kx;
+    p.ky = ky;
+    p.kz = kz;
+    p.Sx = sel3(r.dx, r.dy, r.dz, kx) / dkz;
+    p.Sy = sel3(r.dx, r.dy, r.dz, ky) / dkz;
+    p.Sz = 1.0f / dkz;
+    p.oct = (p.ix < 0.0f ? 1u : 0u) | (p.iy < 0.0f ? 2u : 0u) | (p.iz < 0.0f ? 4u : 0u);
+    return p;
+}
+
+// Watertight ray/triangle test, both faces, FP32 only.  On a hit writes
+// t = T/det and returns true; U/V/W/det are kept for the barycentrics.
+// The operation order is fixed (the library is built without FP contraction)
+// so the CPU restatement reproduces every bit.
+__device__ __forceinline__ bool intersect_tri(const RayPre& p, float v0x, float v0y, float v0z, float v1x, float v1y,
+                                              float v1z, float v2x, float v2y, float v2z, float& t, float& U,
+                                              float& V, float& W, float& det) {
+    float Ax_ = v0x - p.ox, Ay_ = v0y - p.oy, Az_ = v0z - p.oz;
+    float Bx_ = v1x - p.ox, By_ = v1y - p.oy, Bz_ = v1z - p.oz;
+    float Cx_ = v2x - p.ox, Cy_ = v2y - p.oy, Cz_ = v2z - p.oz;
+    float Akz = sel3(Ax_, Ay_, Az_, p.kz), Bkz = sel3(Bx_, By_, Bz_, p.kz), Ckz = sel3(Cx_, Cy_, Cz_, p.kz);
+    float Ax = sel3(Ax_, Ay_, Az_, p.kx) - p.Sx * Akz;
+    float Ay = sel3(Ax_, Ay_, Az_, p.ky) - p.Sy * Akz;
+    float Bx = sel3(Bx_, By_, Bz_, p.kx) - p.Sx * Bkz;
+    float By = sel3(Bx_, By_, Bz_, p.ky) - p.Sy * Bkz;
+    float Cx = sel3(Cx_, Cy_, Cz_, p.kx) - p.Sx * Ckz;
+    float Cy = sel3(Cx_, Cy_, Cz_, p.ky) - p.Sy * Ckz;
+    U = Cx * By - Cy * Bx;
+    V = Ax * Cy - Ay * Cx;
+    W = Bx * Ay - By * Ax;
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    det = U + V + W;
+    if (det == 0.0f) return false;
+    float Az = p.Sz * Akz, Bz = p.Sz * Bkz, Cz = p.Sz * Ckz;
+    float T = U * Az + V * Bz + W * Cz;
+    t = T / det;
+    return true;
+}
+
+struct TraceResult {
+    float t;
+    uint32_t gid;
+    uint32_t rec;  // leaf record of the closest hit, ~0u if none
+};
+
+__device__ __forceinline__ float ubyte_f(uint32_t w, int b) { return (float)((w >> (8 * b)) & 0xFFu); }
+
+// BVH8 traversal for one ray.  `stk` points to this lane's first stack entry in
+// LDS; consecutive entries are `stride` uint2 apart.  Stack entries are node
+// groups: (child_base, hit bits << 24 | imask).  The stack never holds more
+// than depth-1 entries, so STACK >= depth-1 is checked by the launcher.
+template <bool ANY, bool COUNT, int STACK>
+__device__ __forceinline__ TraceResult traverse(const uint4* __restrict__ nodes, const float4* __restrict__ tris,
+                                                const Ray& ray, uint2* stk, int stride, uint32_t& nvis,
+                                                uint32_t& tvis) {
+    RayPre p = ray_setup(ray);
+    TraceResult res{ray.tmax, 0xFFFFFFFFu, 0xFFFFFFFFu};
+    const float tmin = ray.tmin;
+    const bool nx = p.ix < 0.0f, ny = p.iy < 0.0f, nz = p.iz < 0.0f;
+    int sp = 0;
+    // start with a node group holding the root in slot 0
+    uint32_t ng_base = 0, ng_hits = (1u << (24 + (0u ^ p.oct))) | 1u;
+    while (true) {
+        uint32_t tg_base = 0, tg_hits = 0;
+        if (ng_hits & 0xFF000000u) {
+            uint32_t k = __builtin_ctz(ng_hits >> 24);
+            ng_hits &= ~(1u << (24 + k));
+            uint32_t s = k ^ p.oct;
+            uint32_t node = ng_base + __builtin_popcount(ng_hits & 0xFFu & ((1u << s) - 1u));
+            if (ng_hits & 0xFF000000u) {
+                stk[sp * stride] = make_uint2(ng_base, ng_hits);
+                ++sp;
+            }
+            const uint4* np = nodes + (size_t)node * kNodeWords;
+            uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
+            if (COUNT) ++nvis;
+            float sx = __uint_as_float((w0.w & 0xFFu) << 23) * p.ix;
+            float sy = __uint_as_float(((w0.w >> 8) & 0xFFu) << 23) * p.iy;
+            float sz = __uint_as_float(((w0.w >> 16) & 0xFFu) << 23) * p.iz;
+            float ax = (__uint_as_float(w0.x) - p.ox) * p.ix;
+            float ay = (__uint_as_float(w0.y) - p.oy) * p.iy;
+            float az = (__uint_as_float(w0.z) - p.oz) * p.iz;
+            uint32_t imask = w0.w >> 24;
+            // near / far plane bytes per axis (qlo first for a positive direction)
+            uint32_t nx0 = nx ? w3.z : w2.x, nx1 = nx ? w3.w : w2.y;
+            uint32_t fx0 = nx ? w2.x : w3.z, fx1 = nx ? w2.y : w3.w;
+            uint32_t ny0 = ny ? w4.x : w2.z, ny1 = ny ? w4.y : w2.w;
+            uint32_t fy0 = ny ? w2.z : w4.x, fy1 = ny ? w2.w : w4.y;
+            uint32_t nz0 = nz ? w4.z : w3.x, nz1 = nz ? w4.w : w3.y;
+            uint32_t fz0 = nz ? w3.x : w4.z, fz1 = nz ? w3.y : w4.w;
+            uint32_t int_hits = 0, tri_hits = 0;
+#pragma unroll
+            for (int s2 = 0; s2 < 8; ++s2) {
+                const int b = s2 & 3;
+                uint32_t nxw = s2 < 4 ? nx0 : nx1, fxw = s2 < 4 ? fx0 : fx1;
+                uint32_t nyw = s2 < 4 ? ny0 : ny1, fyw = s2 < 4 ? fy0 : fy1;
+                uint32_t nzw = s2 < 4 ? nz0 : nz1, fzw = s2 < 4 ? fz0 : fz1;
+                uint32_t metaw = s2 < 4 ? w1.z : w1.w;
+                float tnx = fmaf(ubyte_f(nxw, b), sx, ax);
+                float tny = fmaf(ubyte_f(nyw, b), sy, ay);
+                float tnz = fmaf(ubyte_f(nzw, b), sz, az);
+                float tfx = fmaf(ubyte_f(fxw, b), sx, ax);
+                float tfy = fmaf(ubyte_f(fyw, b), sy, ay);
+                float tfz = fmaf(ubyte_f(fzw, b), sz, az);
+                float tn = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), tmin);
+                float tf = fminf(fminf(fminf(tfx, tfy), tfz), res.t) * kBoxRobust;
+                uint32_t meta = (metaw >> (8 * b)) & 0xFFu;
+                if (meta != 0u && tn <= tf) {
+                    if ((imask >> s2) & 1u)
+                        int_hits |= 1u << (24 + (s2 ^ p.oct));
+                    else
+                        tri_hits |= (meta >> 5) << (meta & 31u);
+                }
+            }
+            ng_base = w1.x;
+            ng_hits = int_hits | imask;
+            tg_base = w1.y;
+            tg_hits = tri_hits;
+        }
+        while (tg_hits) {
+            uint32_t j = __builtin_ctz(tg_hits);
+            tg_hits &= tg_hits - 1u;
+            uint32_t rec = tg_base + j;
+            const float4* tp = tris + (size_t)rec * kTriWords;
+            float4 r0 = tp[0], r1 = tp[1], r2 = tp[2];
+            if (COUNT) ++tvis;
+            float t, U, V, W, det;
+            if (intersect_tri(p, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, t, U, V, W, det)) {
+                uint32_t gid = __float_as_uint(r2.w);
+                if (t >= tmin && (t < res.t || (t == res.t && gid < res.gid))) {
+                    res.t = t;
+                    res.gid = gid;
+                    res.rec = rec;
+                    if (ANY) return res;
+                }
+            }
+        }
+        if (!(ng_hits & 0xFF000000u)) {
+            if (sp == 0) break;
+            --sp;
+            uint2 e = stk[sp * stride];
+            ng_base = e.x;
+            ng_hits = e.y;
+        }
+    }
+    return res;
+}
+
+// Barycentrics (u, v) of a leaf record hit; bary = (1-u-v, u, v).
+__device__ __forceinline__ void hit_uv(const float4* __restrict__ tris, uint32_t rec, const Ray& ray, float& u,
+                                       float& v) {
+    RayPre p = ray_setup(ray);
+    const float4* tp = tris + (size_t)rec * kTriWords;
+    float4 r0 = tp[0], r1 = tp[1], r2 = tp[2];
+    float t, U, V, W, det;
+    intersect_tri(p, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, t, U, V, W, det);
+    u = V / det;
+    v = W / det;
+}
+
+}  // namespace snkpt

@@ -1,0 +1,137 @@
+// Internal host-side declarations of the snkpt library (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "snkpt.h"
+
+namespace snkpt {
+
+// Owning device allocation; grows only.
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer() { release(); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    // Ensures at least `n` bytes; contents are not preserved.
+    hipError_t reserve(size_t n) {
+        if (n <= bytes) return hipSuccess;
+        release();
+        size_t alloc = n < 256 ? 256 : n;
+        hipError_t e = hipMalloc(&ptr, alloc);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            return e;
+        }
+        bytes = alloc;
+        return hipSuccess;
+    }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(ptr);
+    }
+};
+
+// Acceleration structure built by build_accel().
+struct Accel {
+    bool built = false;
+    uint32_t num_tris = 0;
+    uint32_t num_nodes = 0;
+    uint32_t depth = 0;
+    uint32_t binary_depth = 0;
+    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    double build_ms = 0;
+    DeviceBuffer world_tris;  // 48 B records in input (global triangle id) order
+    DeviceBuffer leaf_tris;   // 48 B records in BVH8 leaf order
+    DeviceBuffer nodes;       // 80 B BVH8 nodes, BFS order, root = 0
+    DeviceBuffer keys, ids;   // sorted Morton keys / triangle ids
+    DeviceBuffer bin_child;   // internal binary nodes: left, right (tagged)
+    DeviceBuffer bin_aabb;    // internal then leaf AABBs, 6 floats each
+    DeviceBuffer bin_leafify; // u32 per internal node
+    DeviceBuffer inst_tri_offset;  // per instance: first global triangle id (n_inst + 1 entries)
+};
+
+// Emitter table: one entry per emissive triangle, sampled proportionally to power.
+struct Emitters {
+    uint32_t count = 0;
+    double total_power = 0;
+    DeviceBuffer tri;    // float4 x 3 per emitter: v0 | v1 | v2 (w unused), world space
+    DeviceBuffer info;   // float4: emission rgb, area
+    DeviceBuffer alias;  // alias table: float prob, u32 alias
+    DeviceBuffer pdf;    // per emitter: selection probability
+    DeviceBuffer gid_to_emitter;  // per global triangle: emitter index or ~0u
+};
+
+struct RenderState;  // defined in render.hip
+struct CacheState;   // defined in cache.hip
+
+}  // namespace snkpt
+
+struct snkpt_ctx {
+    int device = 0;
+    std::string last_error;
+    hipDeviceProp_t props{};
+
+    // host copies of the scene input
+    std::vector<float> pos, nrm, uv, tan;
+    std::vector<uint32_t> idx;
+    std::vector<snkpt_instance> inst;
+    std::vector<float> mat4s;
+    std::vector<snkpt_material> mats;
+    std::vector<uint8_t> light_block;
+    std::vector<uint32_t> emissive_inst;
+    bool emissive_auto = true;
+
+    // device copies
+    snkpt::DeviceBuffer d_pos, d_nrm, d_uv, d_idx, d_inst, d_mat4, d_mats;
+    snkpt::DeviceBuffer d_inst_xf;  // per instance: 3x4 object->world + 3x3 normal matrix (float x 24)
+    bool dirty = true;
+
+    snkpt::Accel accel;
+    snkpt::Emitters emit;
+    snkpt::RenderState* render = nullptr;
+    snkpt::CacheState* cache = nullptr;
+    snkpt_frame_stats stats{};
+};
+
+namespace snkpt {
+
+snkpt_status fail(snkpt_ctx* ctx, snkpt_status code, const std::string& msg);
+snkpt_status hip_fail(snkpt_ctx* ctx, hipError_t e, const char* what);
+
+snkpt_status build_accel(snkpt_ctx* ctx, hipStream_t stream);
+snkpt_status accel_export(snkpt_ctx* ctx, uint32_t which, void* dst, size_t bytes, size_t* out_bytes);
+snkpt_status build_emitters(snkpt_ctx* ctx, hipStream_t stream);
+
+// trace.hip
+snkpt_status launch_trace(snkpt_ctx* ctx, const float4* org_tmin, const float4* dir_tmax, uint32_t n,
+                          const snkpt_hits& hits, bool any_hit, uint32_t* node_visits, uint32_t* tri_visits,
+                          const uint32_t* count_dev, hipStream_t stream);
+
+// render.hip
+snkpt_status render(snkpt_ctx* ctx, const snkpt_camera* cam, const snkpt_settings* s, void* out, uint32_t w,
+                    uint32_t h, uint32_t fmt, hipStream_t stream);
+snkpt_status oracle_accumulate(snkpt_ctx* ctx, const snkpt_camera* cam, const snkpt_settings* s, double* accum,
+                               uint32_t w, uint32_t h, uint32_t spp, hipStream_t stream);
+void render_state_free(RenderState* r);
+void cache_state_free(CacheState* c);
+
+// scenegen.cpp
+snkpt_status scene_generate(uint32_t config, uint32_t variant, snkpt_scene** out);
+void scene_free(snkpt_scene* s);
+snkpt_status scene_camera(uint32_t config, uint32_t variant, uint32_t frame, uint32_t w, uint32_t h,
+                          snkpt_camera* out);
+uint32_t c5_view_side(uint32_t view);
+
+}  // namespace snkpt

@@ -1,0 +1,138 @@
+// BVH8 closest-hit and any-hit tracing of a ray stream.
+//
+// Replaces traceRayEXT with gl_RayFlagsOpaqueEXT (RayGen.rgen:164-176,
+// RayClosestHit.rchit:87-91) and with TerminateOnFirstHit|Opaque|SkipClosestHit
+// (ShadowTest, RayGen.rgen:101-121).  One lane per ray; the traversal stack
+// lives in LDS (STACK entries of 8 B per lane, interleaved across the block so
+// lanes at equal depth hit distinct banks).
+#include "snkpt_dev.h"
+#include "snkpt_host.h"
+
+namespace snkpt {
+namespace {
+
+constexpr int kTraceBlock = 128;
+
+// Spreads consecutive logical blocks over few XCDs: hardware block b runs on
+// XCD b % 8, so logical blocks [x*q + min(x, r), ...) all land on XCD x and
+// neighbouring rays share that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+    uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u, j = b >> 3;
+    return x * q + (x < r ? x : r) + j;
+}
+
+template <bool ANY, bool COUNT, int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_trace(const float4* __restrict__ org_tmin,
+                                                       const float4* __restrict__ dir_tmax, uint32_t n,
+                                                       const uint32_t* __restrict__ count_dev,
+                                                       const uint4* __restrict__ nodes,
+                                                       const float4* __restrict__ tris, float* __restrict__ t_out,
+                                                       uint32_t* __restrict__ inst_out,
+                                                       uint32_t* __restrict__ prim_out, float* __restrict__ u_out,
+                                                       float* __restrict__ v_out, uint32_t* __restrict__ nvis_out,
+                                                       uint32_t* __restrict__ tvis_out) {
+    __shared__ uint2 stk[STACK * kTraceBlock];
+    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t i = lb * kTraceBlock + threadIdx.x;
+    const uint32_t nn = count_dev ? *count_dev : n;
+    if (i >= nn) return;
+    float4 a = org_tmin[i], d = dir_tmax[i];
+    Ray r{a.x, a.y, a.z, a.w, d.x, d.y, d.z, d.w};
+    uint32_t nv = 0, tv = 0;
+    TraceResult res = traverse<ANY, COUNT, STACK>(nodes, tris, r, &stk[threadIdx.x], kTraceBlock, nv, tv);
+    if (res.rec != 0xFFFFFFFFu) {
+        float4 r2 = tris[(size_t)res.rec * kTriWords + 2];
+        t_out[i] = res.t;
+        if (inst_out) inst_out[i] = __float_as_uint(r2.y);
+        if (prim_out) prim_out[i] = __float_as_uint(r2.z);
+        if (u_out || v_out) {
+            float u, v;
+            hit_uv(tris, res.rec, r, u, v);
+            if (u_out) u_out[i] = u;
+            if (v_out) v_out[i] = v;
+        }
+    } else {
+        t_out[i] = -1.0f;
+        if (inst_out) inst_out[i] = 0xFFFFFFFFu;
+        if (prim_out) prim_out[i] = 0xFFFFFFFFu;
+        if (u_out) u_out[i] = 0.0f;
+        if (v_out) v_out[i] = 0.0f;
+    }
+    if (COUNT) {
+        if (nvis_out) nvis_out[i] = nv;
+        if (tvis_out) tvis_out[i] = tv;
+    }
+}
+
+__global__ void k_trace_empty(uint32_t n, const uint32_t* __restrict__ count_dev, float* __restrict__ t_out,
+                              uint32_t* __restrict__ inst_out, uint32_t* __restrict__ prim_out,
+                              float* __restrict__ u_out, float* __restrict__ v_out, uint32_t* __restrict__ nvis_out,
+                              uint32_t* __restrict__ tvis_out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t nn = count_dev ? *count_dev : n;
+    if (i >= nn) return;
+    t_out[i] = -1.0f;
+    if (inst_out) inst_out[i] = 0xFFFFFFFFu;
+    if (prim_out) prim_out[i] = 0xFFFFFFFFu;
+    if (u_out) u_out[i] = 0.0f;
+    if (v_out) v_out[i] = 0.0f;
+    if (nvis_out) nvis_out[i] = 0;
+    if (tvis_out) tvis_out[i] = 0;
+}
+
+template <bool ANY, bool COUNT>
+void launch_stack(uint32_t stack, uint32_t grid, hipStream_t st, const float4* o, const float4* d, uint32_t n,
+                  const uint32_t* cnt, const uint4* nodes, const float4* tris, const snkpt_hits& h, uint32_t* nv,
+                  uint32_t* tv) {
+    if (stack <= 16)
+        k_trace<ANY, COUNT, 16><<<grid, kTraceBlock, 0, st>>>(o, d, n, cnt, nodes, tris, h.t, h.inst, h.prim, h.u,
+                                                              h.v, nv, tv);
+    else if (stack <= 32)
+        k_trace<ANY, COUNT, 32><<<grid, kTraceBlock, 0, st>>>(o, d, n, cnt, nodes, tris, h.t, h.inst, h.prim, h.u,
+                                                              h.v, nv, tv);
+    else
+        k_trace<ANY, COUNT, 64><<<grid, kTraceBlock, 0, st>>>(o, d, n, cnt, nodes, tris, h.t, h.inst, h.prim, h.u,
+                                                              h.v, nv, tv);
+}
+
+}  // namespace
+
+snkpt_status launch_trace(snkpt_ctx* ctx, const float4* org_tmin, const float4* dir_tmax, uint32_t n,
+                          const snkpt_hits& hits, bool any_hit, uint32_t* node_visits, uint32_t* tri_visits,
+                          const uint32_t* count_dev, hipStream_t st) {
+    const Accel& A = ctx->accel;
+    if (!A.built) return fail(ctx, SNKPT_ERR_NOT_BUILT, "trace: acceleration structure not built");
+    if (n == 0) return SNKPT_OK;
+    if (!org_tmin || !dir_tmax || !hits.t) return fail(ctx, SNKPT_ERR_INVALID_ARG, "trace: null ray or hit stream");
+    uint32_t grid = (n + kTraceBlock - 1) / kTraceBlock;
+    if (A.num_tris == 0) {
+        k_trace_empty<<<(n + 255) / 256, 256, 0, st>>>(n, count_dev, hits.t, hits.inst, hits.prim, hits.u, hits.v,
+                                                       node_visits, tri_visits);
+    } else {
+        uint32_t need = A.depth > 0 ? A.depth - 1 : 0;
+        if (need > 64) return fail(ctx, SNKPT_ERR_CAPACITY, "trace: BVH8 deeper than the 64-entry traversal stack");
+        bool count = node_visits || tri_visits;
+        const uint4* nodes = A.nodes.as<uint4>();
+        const float4* tris = A.leaf_tris.as<float4>();
+        if (any_hit) {
+            if (count)
+                launch_stack<true, true>(need, grid, st, org_tmin, dir_tmax, n, count_dev, nodes, tris, hits,
+                                         node_visits, tri_visits);
+            else
+                launch_stack<true, false>(need, grid, st, org_tmin, dir_tmax, n, count_dev, nodes, tris, hits,
+                                          nullptr, nullptr);
+        } else {
+            if (count)
+                launch_stack<false, true>(need, grid, st, org_tmin, dir_tmax, n, count_dev, nodes, tris, hits,
+                                          node_visits, tri_visits);
+            else
+                launch_stack<false, false>(need, grid, st, org_tmin, dir_tmax, n, count_dev, nodes, tris, hits,
+                                           nullptr, nullptr);
+        }
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(ctx, e, "trace launch");
+    return SNKPT_OK;
+}
+
+}  // namespace snkpt
