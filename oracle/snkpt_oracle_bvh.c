@@ -116,10 +116,9 @@ typedef struct {
     uint32_t oct;
 } o_raypre;
 
-static float safe_inv(float d) {
-    float a = fabsf(d) > 1e-20f ? d : copysignf(1e-20f, d);
-    return 1.0f / a;
-}
+/* 1/d with |d| < 1e-20 taken as exactly parallel (+-inf): the slab terms are
+ * then +-inf or NaN and fminf/fmaxf drop the NaN, so the axis never culls. */
+static float safe_inv(float d) { return fabsf(d) < 1e-20f ? copysignf(INFINITY, d) : 1.0f / d; }
 
 static void ray_pre(const float o[3], const float d[3], o_raypre* p) {
     for (int a = 0; a < 3; ++a) {

@@ -43,7 +43,7 @@ struct Ray {
 // the box test.
 struct RayPre {
     float ox, oy, oz;
-    float ix, iy, iz;  // 1/d with |d| clamped away from 0
+    float ix, iy, iz;  // 1/d, +-inf on a parallel axis
     int kx, ky, kz;
     float Sx, Sy, Sz;
     uint32_t oct;  // bit a set: the ray runs towards -a
@@ -51,9 +51,12 @@ struct RayPre {
 
 __device__ __forceinline__ float sel3(float x, float y, float z, int k) { return k == 0 ? x : (k == 1 ? y : z); }
 
+// 1/d, with |d| < 1e-20 treated as an exactly parallel axis (+-inf).  With an
+// infinite inverse the box-slab terms below become +-inf or NaN, and NaN is
+// dropped by fminf/fmaxf, so a parallel axis never culls a box wrongly; a huge
+// finite inverse would instead cancel catastrophically in the fused form.
 __device__ __forceinline__ float safe_inv(float d) {
-    float a = fabsf(d) > 1e-20f ? d : copysignf(1e-20f, d);
-    return 1.0f / a;
+    return fabsf(d) < 1e-20f ? copysignf(__builtin_inff(), d) : 1.0f / d;
 }
 
 __device__ __forceinline__ RayPre ray_setup(const Ray& r) {

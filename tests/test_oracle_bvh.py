@@ -258,3 +258,23 @@ def test_bvh8_traversal_equals_brute_force_room():
     ga = b.trace(rays, any_hit=True)
     wa = O.trace_brute(world, rays, any_hit=True)
     assert np.array_equal(ga[0] >= 0, wa[0] >= 0)
+
+
+def test_axis_parallel_rays_in_wall_planes_cpu():
+    """Rays with zero direction components, many lying exactly in wall planes:
+    BVH8 traversal still equals brute force (parallel axes never cull)."""
+    _, world = _room()
+    b = O.Bvh(world)
+    rng = np.random.default_rng(4)
+    n = 6000
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3] = rng.uniform(-1.2, 1.2, (n, 3)) + np.array([0, 1, 0])
+    axis = rng.integers(0, 3, n)
+    rays[np.arange(n), 4 + axis] = rng.choice([-1.0, 1.0], n)
+    rays[: n // 3, 1] = 0.0
+    rays[n // 3: 2 * n // 3, 0] = 1.0
+    rays[:, 7] = 1e30
+    got = b.trace(rays)
+    want = O.trace_brute(world, rays)
+    for a, c in zip(got[:5], want):
+        assert np.array_equal(a.view(np.uint32), c.view(np.uint32))
