@@ -113,12 +113,14 @@ typedef struct {
     float o[3], inv[3];
     int kx, ky, kz;
     float Sx, Sy, Sz;
-    uint32_t oct;
+    uint32_t oct, par;
 } o_raypre;
 
-/* 1/d with |d| < 1e-20 taken as exactly parallel (+-inf): the slab terms are
- * then +-inf or NaN and fminf/fmaxf drop the NaN, so the axis never culls. */
-static float safe_inv(float d) { return fabsf(d) < 1e-20f ? copysignf(INFINITY, d) : 1.0f / d; }
+/* 1/d; |d| < 1e-20 marks a parallel axis: its inverse is NaN (the fused slab
+ * terms become NaN and fminf/fmaxf drop them) and the axis is tested exactly
+ * instead: the origin must lie inside the child's decoded [lo, hi]. */
+#define PARALLEL_EPS 1e-20f
+static float safe_inv(float d) { return fabsf(d) < PARALLEL_EPS ? NAN : 1.0f / d; }
 
 static void ray_pre(const float o[3], const float d[3], o_raypre* p) {
     for (int a = 0; a < 3; ++a) {
@@ -146,6 +148,9 @@ static void ray_pre(const float o[3], const float d[3], o_raypre* p) {
     p->Sy = d[ky] / d[kz];
     p->Sz = 1.0f / d[kz];
     p->oct = (p->inv[0] < 0.0f ? 1u : 0u) | (p->inv[1] < 0.0f ? 2u : 0u) | (p->inv[2] < 0.0f ? 4u : 0u);
+    p->par = 0;
+    for (int a = 0; a < 3; ++a)
+        if (fabsf(d[a]) < PARALLEL_EPS) p->par |= 1u << a;
 }
 
 /* returns 1 if the ray's line crosses the triangle (edge functions agree) */
@@ -552,11 +557,21 @@ typedef struct {
     uint32_t gid, rec;
 } o_res;
 
+/* finite origin and direction, non-zero direction, tmin <= tmax */
+static int ray_valid(const float* r) {
+    for (int k = 0; k < 8; ++k)
+        if (k != 7 && !isfinite(r[k])) return 0;
+    if (isnan(r[7])) return 0;
+    if (r[4] == 0.0f && r[5] == 0.0f && r[6] == 0.0f) return 0;
+    return r[3] <= r[7];
+}
+
 static o_res trace_one(const o_bvh* B, const float* ray, int any, uint32_t* nvis, uint32_t* tvis) {
+    o_res res = {ray[7], NONE, NONE};
+    if (!ray_valid(ray)) return res;
     o_raypre p;
     float o[3] = {ray[0], ray[1], ray[2]}, d[3] = {ray[4], ray[5], ray[6]};
     ray_pre(o, d, &p);
-    o_res res = {ray[7], NONE, NONE};
     const float tmin = ray[3];
     const int nx = p.inv[0] < 0.0f, ny = p.inv[1] < 0.0f, nz = p.inv[2] < 0.0f;
     uint32_t stk[2 * 128];
@@ -590,6 +605,16 @@ static o_res trace_one(const o_bvh* B, const float* ray, int any, uint32_t* nvis
             const uint32_t* fyw = ny ? w + 10 : w + 16;
             const uint32_t* nzw = nz ? w + 18 : w + 12;
             const uint32_t* fzw = nz ? w + 12 : w + 18;
+            uint32_t pmiss = 0;
+            for (int a = 0; a < 3 && p.par; ++a) {
+                if (!((p.par >> a) & 1u)) continue;
+                float sc = u2f(((w[3] >> (8 * a)) & 0xFFu) << 23), pa = u2f(w[a]);
+                for (int s = 0; s < 8; ++s) {
+                    float lo = fmaf((float)((w[8 + 2 * a + (s >> 2)] >> (8 * (s & 3))) & 0xFFu), sc, pa);
+                    float hi = fmaf((float)((w[14 + 2 * a + (s >> 2)] >> (8 * (s & 3))) & 0xFFu), sc, pa);
+                    if (!(lo <= p.o[a] && p.o[a] <= hi)) pmiss |= 1u << s;
+                }
+            }
             uint32_t int_hits = 0, tri_hits = 0;
             for (int s2 = 0; s2 < 8; ++s2) {
                 int wi = s2 >> 2, sh = 8 * (s2 & 3);
@@ -602,7 +627,7 @@ static o_res trace_one(const o_bvh* B, const float* ray, int any, uint32_t* nvis
                 float tn = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), tmin);
                 float tf = fminf(fminf(fminf(tfx, tfy), tfz), res.t) * BOX_ROBUST;
                 uint32_t meta = (w[6 + wi] >> sh) & 0xFFu;
-                if (meta != 0u && tn <= tf) {
+                if (meta != 0u && tn <= tf && !((pmiss >> s2) & 1u)) {
                     if ((imask >> s2) & 1u)
                         int_hits |= 1u << (24 + ((uint32_t)s2 ^ p.oct));
                     else
@@ -698,7 +723,8 @@ static void* brute_worker(void* arg) {
         float best = ray[7];
         uint32_t best_gid = NONE;
         const float* best_rec = NULL;
-        for (uint32_t g = 0; g < j->ntris; ++g) {
+        const uint32_t ntris = ray_valid(ray) ? j->ntris : 0;
+        for (uint32_t g = 0; g < ntris; ++g) {
             const float* r = j->world + 12 * (size_t)g;
             float t, U, V, W, det;
             if (!tri_test(&p, r, r + 3, r + 6, &t, &U, &V, &W, &det)) continue;

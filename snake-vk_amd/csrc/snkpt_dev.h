@@ -43,20 +43,24 @@ struct Ray {
 // the box test.
 struct RayPre {
     float ox, oy, oz;
-    float ix, iy, iz;  // 1/d, +-inf on a parallel axis
+    float ix, iy, iz;  // 1/d, NaN on a parallel axis
     int kx, ky, kz;
     float Sx, Sy, Sz;
     uint32_t oct;  // bit a set: the ray runs towards -a
+    uint32_t par;  // bit a set: axis a is parallel
 };
 
 __device__ __forceinline__ float sel3(float x, float y, float z, int k) { return k == 0 ? x : (k == 1 ? y : z); }
+__device__ __forceinline__ float ubyte_f(uint32_t w, int b) { return (float)((w >> (8 * b)) & 0xFFu); }
 
-// 1/d, with |d| < 1e-20 treated as an exactly parallel axis (+-inf).  With an
-// infinite inverse the box-slab terms below become +-inf or NaN, and NaN is
-// dropped by fminf/fmaxf, so a parallel axis never culls a box wrongly; a huge
-// finite inverse would instead cancel catastrophically in the fused form.
+// 1/d; an axis with |d| < 1e-20 is "parallel" and gets NaN: every fused slab
+// term of that axis is then NaN, which fminf/fmaxf drop, and the axis is
+// instead tested exactly (origin inside the child's decoded [lo, hi]).  A huge
+// or infinite inverse would make the fused terms cancel or turn into NaN in
+// the wrong places (a vertical ray would then visit most of the tree).
+constexpr float kParallelEps = 1e-20f;
 __device__ __forceinline__ float safe_inv(float d) {
-    return fabsf(d) < 1e-20f ? copysignf(__builtin_inff(), d) : 1.0f / d;
+    return fabsf(d) < kParallelEps ? __builtin_nanf("") : 1.0f / d;
 }
 
 __device__ __forceinline__ RayPre ray_setup(const Ray& r) {
@@ -84,6 +88,8 @@ __device__ __forceinline__ RayPre ray_setup(const Ray& r) {
     p.Sy = sel3(r.dx, r.dy, r.dz, ky) / dkz;
     p.Sz = 1.0f / dkz;
     p.oct = (p.ix < 0.0f ? 1u : 0u) | (p.iy < 0.0f ? 2u : 0u) | (p.iz < 0.0f ? 4u : 0u);
+    p.par = (fabsf(r.dx) < kParallelEps ? 1u : 0u) | (fabsf(r.dy) < kParallelEps ? 2u : 0u) |
+            (fabsf(r.dz) < kParallelEps ? 4u : 0u);
     return p;
 }
 
@@ -122,7 +128,31 @@ struct TraceResult {
     uint32_t rec;  // leaf record of the closest hit, ~0u if none
 };
 
-__device__ __forceinline__ float ubyte_f(uint32_t w, int b) { return (float)((w >> (8 * b)) & 0xFFu); }
+// Children (bit s) whose decoded box does not contain the ray origin on some
+// parallel axis.  fmaf rounds once and monotonically, so the decoded planes
+// still enclose the child.
+__device__ __forceinline__ uint32_t parallel_miss_axis(float o, float pa, uint32_t ebyte, uint32_t lo0, uint32_t lo1,
+                                                       uint32_t hi0, uint32_t hi1) {
+    float sc = __uint_as_float(ebyte << 23);
+    uint32_t miss = 0;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        float lo = fmaf(ubyte_f(s < 4 ? lo0 : lo1, s & 3), sc, pa);
+        float hi = fmaf(ubyte_f(s < 4 ? hi0 : hi1, s & 3), sc, pa);
+        if (!(lo <= o && o <= hi)) miss |= 1u << s;
+    }
+    return miss;
+}
+__device__ __forceinline__ uint32_t parallel_miss(const RayPre& p, uint4 w0, uint4 w2, uint4 w3, uint4 w4) {
+    uint32_t miss = 0;
+    if (p.par & 1u)
+        miss |= parallel_miss_axis(p.ox, __uint_as_float(w0.x), w0.w & 0xFFu, w2.x, w2.y, w3.z, w3.w);
+    if (p.par & 2u)
+        miss |= parallel_miss_axis(p.oy, __uint_as_float(w0.y), (w0.w >> 8) & 0xFFu, w2.z, w2.w, w4.x, w4.y);
+    if (p.par & 4u)
+        miss |= parallel_miss_axis(p.oz, __uint_as_float(w0.z), (w0.w >> 16) & 0xFFu, w3.x, w3.y, w4.z, w4.w);
+    return miss;
+}
 
 // BVH8 traversal for one ray.  `stk` points to this lane's first stack entry in
 // LDS; consecutive entries are `stride` uint2 apart.  Stack entries are node
@@ -132,8 +162,15 @@ template <bool ANY, bool COUNT, int STACK>
 __device__ __forceinline__ TraceResult traverse(const uint4* __restrict__ nodes, const float4* __restrict__ tris,
                                                 const Ray& ray, uint2* stk, int stride, uint32_t& nvis,
                                                 uint32_t& tvis) {
-    RayPre p = ray_setup(ray);
     TraceResult res{ray.tmax, 0xFFFFFFFFu, 0xFFFFFFFFu};
+    // Non-finite rays, a zero direction or an empty [tmin, tmax] miss at once:
+    // a NaN would otherwise pass every box test and walk the whole tree.
+    const bool valid = __builtin_isfinite(ray.ox) && __builtin_isfinite(ray.oy) && __builtin_isfinite(ray.oz) &&
+                       __builtin_isfinite(ray.dx) && __builtin_isfinite(ray.dy) && __builtin_isfinite(ray.dz) &&
+                       (ray.dx != 0.0f || ray.dy != 0.0f || ray.dz != 0.0f) && ray.tmin <= ray.tmax &&
+                       !__builtin_isnan(ray.tmin) && !__builtin_isnan(ray.tmax);
+    if (!valid) return res;
+    RayPre p = ray_setup(ray);
     const float tmin = ray.tmin;
     const bool nx = p.ix < 0.0f, ny = p.iy < 0.0f, nz = p.iz < 0.0f;
     int sp = 0;
@@ -167,6 +204,9 @@ __device__ __forceinline__ TraceResult traverse(const uint4* __restrict__ nodes,
             uint32_t fy0 = ny ? w2.z : w4.x, fy1 = ny ? w2.w : w4.y;
             uint32_t nz0 = nz ? w4.z : w3.x, nz1 = nz ? w4.w : w3.y;
             uint32_t fz0 = nz ? w3.x : w4.z, fz1 = nz ? w3.y : w4.w;
+            // parallel axes: exact origin-in-slab test on the decoded planes
+            uint32_t pmiss = 0;
+            if (p.par) pmiss = parallel_miss(p, w0, w2, w3, w4);
             uint32_t int_hits = 0, tri_hits = 0;
 #pragma unroll
             for (int s2 = 0; s2 < 8; ++s2) {
@@ -184,7 +224,7 @@ __device__ __forceinline__ TraceResult traverse(const uint4* __restrict__ nodes,
                 float tn = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), tmin);
                 float tf = fminf(fminf(fminf(tfx, tfy), tfz), res.t) * kBoxRobust;
                 uint32_t meta = (metaw >> (8 * b)) & 0xFFu;
-                if (meta != 0u && tn <= tf) {
+                if (meta != 0u && tn <= tf && !((pmiss >> s2) & 1u)) {
                     if ((imask >> s2) & 1u)
                         int_hits |= 1u << (24 + (s2 ^ p.oct));
                     else

@@ -93,6 +93,8 @@ def secondary_rays_torch(org, dirs, t, inst, prim, world_tris, inst_tri_offset, 
     btg = torch.linalg.cross(n, tng)
     nd = tng * lx[:, None] + btg * ly[:, None] + n * lz[:, None]
     nd = nd / nd.norm(dim=1, keepdim=True)
+    ok = torch.isfinite(p).all(1) & torch.isfinite(nd).all(1)
+    p, nd = p[ok], nd[ok]
     m = p.shape[0]
     out_o = torch.zeros(m, 4, device=org.device)
     out_o[:, :3] = p

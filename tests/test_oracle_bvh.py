@@ -278,3 +278,42 @@ def test_axis_parallel_rays_in_wall_planes_cpu():
     want = O.trace_brute(world, rays)
     for a, c in zip(got[:5], want):
         assert np.array_equal(a.view(np.uint32), c.view(np.uint32))
+
+
+def test_invalid_rays_miss_cpu():
+    """NaN / inf components, zero directions and tmin > tmax are misses."""
+    _, world = _room()
+    b = O.Bvh(world)
+    rays = np.zeros((6, 8), np.float32)
+    rays[:, 1] = 1.0
+    rays[:, 4] = 1.0
+    rays[:, 7] = 1e30
+    rays[0, 4] = np.nan
+    rays[1, 0] = np.inf
+    rays[2, 4:7] = 0.0
+    rays[3, 3], rays[3, 7] = 2.0, 1.0
+    rays[4, 7] = np.nan
+    t, inst, prim, u, v, nv, tv = b.trace(rays)
+    assert (t[:5] == -1).all() and (nv[:5] == 0).all()
+    assert t[5] > 0  # the control ray hits the right wall
+    bt = O.trace_brute(world, rays)[0]
+    assert (bt[:5] == -1).all() and bt[5] == t[5]
+
+
+def test_vertical_ray_from_ground_visits_few_nodes_cpu():
+    """Regression: a ray along +y from the ground of the 1.02M-triangle scene
+    (two parallel axes) culls normally instead of walking the tree."""
+    import snkpt
+    scene = snkpt.SceneData(2)
+    pos, idx, inst, m4, mats = scene.arrays()
+    world = O.flatten(pos, idx, inst, m4)
+    b = O.Bvh(world)
+    rays = np.array([[-13.745107650756836, 0.010000953450798988, 41.78258514404297, 0.0, 0.0, 1.0, 0.0, 1e30],
+                     [5.0, 0.01, 5.0, 0.0, 0.0, 1.0, 0.0, 1e30],
+                     [5.0, 30.0, 5.0, 0.0, 0.0, -1.0, 0.0, 1e30],
+                     [-100.0, 1.0, 3.0, 0.0, 1.0, 0.0, 0.0, 1e30]], np.float32)
+    t, ii, pp, u, v, nv, tv = b.trace(rays)
+    assert nv.max() < 200 and tv.max() < 200, (nv, tv)
+    w = O.trace_brute(world, rays)
+    for a, c in zip((t, ii, pp, u, v), w):
+        assert np.array_equal(a.view(np.uint32), c.view(np.uint32))
