@@ -33,6 +33,16 @@ def load():
     lib.o_bvh_free.argtypes = [C.POINTER(OBvh)]
     lib.o_trace_bvh8.argtypes = [C.POINTER(OBvh), VP, U32, I, VP, VP, VP, VP, VP, VP, VP, I]
     lib.o_trace_brute.argtypes = [VP, U32, VP, U32, I, VP, VP, VP, VP, VP, I]
+    lib.o_pass_state.argtypes = [U32, U32, U32]
+    lib.o_pass_state.restype = C.c_uint64
+    lib.o_camera_matrix.argtypes = [VP, VP, VP]
+    F = C.c_float
+    lib.o_brdf_eval.argtypes = [VP, F, F, U32, VP, VP, VP, VP, VP]
+    lib.o_brdf_sample.argtypes = [VP, F, F, U32, VP, VP, F, F, F, VP]
+    lib.o_brdf_sample.restype = I
+    lib.o_pt_render.argtypes = [VP, U32, C.POINTER(OBvh), VP, U32, VP, VP, VP, VP, VP, U32, VP, VP, VP, U32, U32,
+                                U32, U32, U32, U32, U32, U32, VP, I]
+    lib.o_pt_render.restype = I
     _lib = lib
     return lib
 
@@ -137,3 +147,55 @@ def trace_brute(world: np.ndarray, rays: np.ndarray, any_hit=False, threads=8):
     load().o_trace_brute(_p(world), world.shape[0], _p(rays), n, int(any_hit), _p(t), _p(inst), _p(prim), _p(u),
                          _p(v), threads)
     return t, inst, prim, u, v
+
+
+def brdf_eval(albedo, roughness, metallic, flags, n, wo, wi):
+    f = np.zeros(3, np.float32)
+    pdf = C.c_float()
+    a = [np.ascontiguousarray(x, np.float32) for x in (albedo, n, wo, wi)]
+    load().o_brdf_eval(_p(a[0]), roughness, metallic, flags, _p(a[1]), _p(a[2]), _p(a[3]), _p(f), C.byref(pdf))
+    return f, pdf.value
+
+
+def brdf_sample(albedo, roughness, metallic, flags, n, wo, ul, u1, u2):
+    wi = np.zeros(3, np.float32)
+    a = [np.ascontiguousarray(x, np.float32) for x in (albedo, n, wo)]
+    ok = load().o_brdf_sample(_p(a[0]), roughness, metallic, flags, _p(a[1]), _p(a[2]), ul, u1, u2, _p(wi))
+    return bool(ok), wi
+
+
+def camera_matrix(cam):
+    v = np.array(cam.view, np.float32)
+    p = np.array(cam.proj, np.float32)
+    out = np.zeros(16, np.float32)
+    load().o_camera_matrix(_p(v), _p(p), _p(out))
+    return out
+
+
+def pt_render(arrs, bvh: "Bvh", cam, w, h, spp, max_depth, rr_start, seed, frame=0, seed_xor=0, threads=8):
+    """CPU path tracer.  arrs = (pos, idx, inst5, mat4s, mats16, nrm, emissive_instances);
+    returns (h, w, 3) float32 mean radiance."""
+    pos, idx, inst, m4, mats, nrm, em = arrs
+    inst = np.ascontiguousarray(inst, np.uint32)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    m4 = np.ascontiguousarray(m4, np.float32)
+    mats = np.ascontiguousarray(mats, np.uint32)
+    nrm = np.ascontiguousarray(nrm, np.float32)
+    em = np.ascontiguousarray(em, np.uint32)
+    view = np.array(cam.view, np.float32)
+    proj = np.array(cam.proj, np.float32)
+    cpos = np.array(cam.cam_pos[:3], np.float32)
+    out = np.zeros((h, w, 3), np.float32)
+    load().o_pt_render(_p(bvh.world), bvh.n, C.byref(bvh.b), _p(inst), inst.shape[0], _p(idx), _p(nrm), _p(m4),
+                       _p(mats), _p(em), em.size, _p(view), _p(proj), _p(cpos), w, h, spp, max_depth, rr_start,
+                       seed, frame, seed_xor, _p(out), threads)
+    return out
+
+
+def scene_arrays(scene):
+    """The pt_render input tuple of a generated snkpt.SceneData."""
+    pos, idx, inst, m4, mats = scene.arrays()
+    s = scene.s
+    nrm = np.ctypeslib.as_array(s.nrm, (s.n_verts * 3,)).copy()
+    em = np.ctypeslib.as_array(s.emissive_instances, (s.n_emissive,)).copy().astype(np.uint32)
+    return pos, idx, inst, m4, mats, nrm, em

@@ -76,6 +76,23 @@ void o_trace_bvh8(const o_bvh* b, const float* rays, uint32_t n, int any_hit, fl
 void o_trace_brute(const float* world, uint32_t ntris, const float* rays, uint32_t n, int any_hit, float* t,
                    uint32_t* inst, uint32_t* prim, float* u, float* v, int threads);
 
+void o_trace_one(const o_bvh* b, const float* ray, int any_hit, float* t, uint32_t* inst, uint32_t* prim, float* u,
+                 float* v);
+
+/* ---- shading and path tracing (snkpt_oracle_pt.c) -------------------------- */
+uint64_t o_pass_state(uint32_t seed, uint32_t frame, uint32_t pass);
+void o_camera_matrix(const float* view16, const float* proj16, float out16[16]);
+void o_brdf_eval(const float albedo[3], float roughness, float metallic, uint32_t flags, const float n[3],
+                 const float wo[3], const float wi[3], float out_f[3], float* out_pdf);
+int o_brdf_sample(const float albedo[3], float roughness, float metallic, uint32_t flags, const float n[3],
+                  const float wo[3], float ul, float u1, float u2, float out_wi[3]);
+/* Mean of `spp` samples per pixel (pass k uses o_pass_state(seed ^ seed_xor, frame, k)). */
+int o_pt_render(const float* world, uint32_t ntris, const o_bvh* bvh, const uint32_t* inst5, uint32_t n_inst,
+                const uint32_t* idx, const float* nrm, const float* mat4s, const uint32_t* mats16,
+                const uint32_t* emissive, uint32_t n_emissive, const float* view16, const float* proj16,
+                const float* cam_pos, uint32_t w, uint32_t h, uint32_t spp, uint32_t max_depth, uint32_t rr_start,
+                uint32_t seed, uint32_t frame, uint32_t seed_xor, float* out_rgb, int threads);
+
 #ifdef __cplusplus
 }
 #endif

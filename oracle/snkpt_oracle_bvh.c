@@ -788,3 +788,16 @@ void o_trace_brute(const float* world, uint32_t ntris, const float* rays, uint32
     o_job j = {NULL, world, ntris, rays, n, 0, 0, any_hit, t, u, v, inst, prim, NULL, NULL};
     run_jobs(&j, threads, brute_worker);
 }
+
+void o_trace_one(const o_bvh* b, const float* ray, int any_hit, float* t, uint32_t* inst, uint32_t* prim, float* u,
+                 float* v) {
+    o_job j = {0};
+    j.t = t, j.inst = inst, j.prim = prim, j.u = u, j.v = v;
+    if (b->n == 0) {
+        finish_hit(&j, 0, ray, NULL, 0.0f);
+        return;
+    }
+    uint32_t nv = 0, tv = 0;
+    o_res r = trace_one(b, ray, any_hit, &nv, &tv);
+    finish_hit(&j, 0, ray, r.rec != NONE ? b->leaf + 12 * (size_t)r.rec : NULL, r.t);
+}

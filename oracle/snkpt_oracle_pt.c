@@ -1,0 +1,621 @@
+/*
+ * snkpt oracle: the unbiased path tracer, restated on the CPU.  TEST
+ * INFRASTRUCTURE ONLY (checker and CPU baseline, never the product).
+ *
+ * Follows the reference pass where it is defined:
+ *   - camera ray through the pixel centre: inverse(view) * inverse(proj) at depth 1,
+ *     normalize(p - cam_pos) (Util.glsl:1-13);
+ *   - hit interpolation: bary = (1-u-v, u, v), normal through
+ *     transpose(inverse(mat3(M))) (RayClosestHit.rchit:88-111);
+ *   - emission = material albedo when flags & 1 (TexMatBuffers.glsl:3,
+ *     RayClosestHit.rchit:119-123), both faces hit and shade (Q5);
+ *   - BRDF: Cook-Torrance with Schlick Fresnel, GGX D, Schlick-GGX G with
+ *     k = (r+1)^2/8, spec = DGF / (4 n.v n.l + 1e-4), diffuse (1-F)(1-metallic)
+ *     albedo / PI, PI = 3.1415926, f0 = mix(0.04, albedo, metallic)
+ *     (LightBuffers.glsl:65-157, RayGen.rgen:552-553).
+ * and replaces the reference's biased BounceDiffuse (RayGen.rgen:139-213) with
+ * the estimator this build specifies: NEE to power-sampled emissive triangles +
+ * cosine/GGX BRDF sampling, power-heuristic MIS, Russian roulette.  Random
+ * numbers come from PCG32 (initstate from (seed, frame, pass), stream = pixel)
+ * drawn in the same order as the GPU, so most pixels reproduce the GPU's
+ * values; sinf/cosf of the platform library may differ in the last ulp.
+ * Pinned by: closed forms of the BRDF terms, pdf normalisation, the furnace
+ * test (tests/test_oracle_pt.py), and agreement with the GPU in expectation.
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "snkpt_oracle.h"
+
+#define REF_PI 3.1415926f
+#define PI_F 3.14159265358979323846f
+
+typedef struct {
+    float x, y, z;
+} v3;
+static inline v3 V(float x, float y, float z) {
+    v3 r = {x, y, z};
+    return r;
+}
+static inline v3 add(v3 a, v3 b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 sub(v3 a, v3 b) { return V(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 scl(v3 a, float s) { return V(a.x * s, a.y * s, a.z * s); }
+static inline v3 mulv(v3 a, v3 b) { return V(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline float dotv(v3 a, v3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+static inline v3 crossv(v3 a, v3 b) { return V(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static inline v3 nrmz(v3 a) {
+    float l = sqrtf(dotv(a, a));
+    return l > 0.0f ? V(a.x / l, a.y / l, a.z / l) : a;
+}
+static inline float u2f(uint32_t u) {
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+static inline uint32_t f2u(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return u;
+}
+
+/* ------------------------------------------------------------------ rng --- */
+typedef struct {
+    uint64_t state, inc;
+} prng;
+static inline uint32_t rnext(prng* r) {
+    uint64_t old = r->state;
+    r->state = old * 6364136223846793005ULL + r->inc;
+    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((-rot) & 31u));
+}
+static inline float runi(prng* r) { return (float)(rnext(r) >> 8) * (1.0f / 16777216.0f); }
+static prng rseed(uint64_t initstate, uint64_t initseq) {
+    prng r = {0u, (initseq << 1u) | 1u};
+    rnext(&r);
+    r.state += initstate;
+    rnext(&r);
+    return r;
+}
+uint64_t o_pass_state(uint32_t seed, uint32_t frame, uint32_t pass) {
+    uint64_t x = ((uint64_t)seed << 32) ^ ((uint64_t)frame * 0x9E3779B97F4A7C15ULL) ^
+                 ((uint64_t)pass * 0xBF58476D1CE4E5B9ULL);
+    x ^= x >> 31;
+    x *= 0x94D049BB133111EBULL;
+    x ^= x >> 29;
+    return x;
+}
+
+/* ----------------------------------------------------------------- BRDF --- */
+typedef struct {
+    v3 albedo;
+    float roughness, metallic;
+    uint32_t flags;
+} omat;
+
+static float ggx_d_ref(float ndh, float a2) {
+    float d = ndh * ndh * (a2 - 1.0f) + 1.0f;
+    return a2 / (REF_PI * d * d);
+}
+static float g1s(float x, float k) { return x / (x * (1.0f - k) + k); }
+
+static v3 brdf_f(const omat* m, v3 n, v3 wo, v3 wi) {
+    if (m->flags & 2u) return scl(m->albedo, 1.0f / REF_PI);
+    v3 f0 = V(0.04f + (m->albedo.x - 0.04f) * m->metallic, 0.04f + (m->albedo.y - 0.04f) * m->metallic,
+              0.04f + (m->albedo.z - 0.04f) * m->metallic);
+    v3 h = nrmz(add(wo, wi));
+    float hv = fmaxf(dotv(h, wo), 0.0f);
+    float c1 = fminf(fmaxf(1.0f - hv, 0.0f), 1.0f), c2 = c1 * c1;
+    float fw = (c2 * c2) * c1;
+    v3 F = V(f0.x + (1.0f - f0.x) * fw, f0.y + (1.0f - f0.y) * fw, f0.z + (1.0f - f0.z) * fw);
+    float a = m->roughness * m->roughness;
+    float D = ggx_d_ref(fmaxf(dotv(n, h), 0.0f), a * a);
+    float r = m->roughness + 1.0f;
+    float k = (r * r) / 8.0f;
+    float nv = fmaxf(dotv(n, wo), 0.0f), nl = fmaxf(dotv(n, wi), 0.0f);
+    float G = g1s(nv, k) * g1s(nl, k);
+    float denom = 4.0f * nv * nl + 0.0001f;
+    v3 spec = scl(F, D * G / denom);
+    float kdm = 1.0f - m->metallic;
+    v3 diff = scl(V((1.0f - F.x) * kdm * m->albedo.x, (1.0f - F.y) * kdm * m->albedo.y,
+                    (1.0f - F.z) * kdm * m->albedo.z),
+                  1.0f / REF_PI);
+    return add(diff, spec);
+}
+
+static float spec_p(const omat* m) { return (m->flags & 2u) ? 0.0f : 0.5f + 0.5f * m->metallic; }
+
+static float brdf_pdf_c(const omat* m, v3 n, v3 wo, v3 wi) {
+    float ci = dotv(n, wi);
+    if (ci <= 0.0f) return 0.0f;
+    float ps = spec_p(m);
+    float pdf = (1.0f - ps) * ci * (1.0f / PI_F);
+    if (ps > 0.0f) {
+        v3 h = nrmz(add(wo, wi));
+        float a = m->roughness * m->roughness;
+        float a2 = a * a;
+        float nh = fmaxf(dotv(n, h), 0.0f);
+        float d = nh * nh * (a2 - 1.0f) + 1.0f;
+        float D = a2 / (PI_F * d * d);
+        float wh = fabsf(dotv(wo, h));
+        if (wh > 0.0f) pdf += ps * D * nh / (4.0f * wh);
+    }
+    return pdf;
+}
+
+static void onb_c(v3 n, v3* t, v3* b) {
+    float s = copysignf(1.0f, n.z);
+    float a = -1.0f / (s + n.z);
+    float bb = n.x * n.y * a;
+    *t = V(1.0f + s * n.x * n.x * a, s * bb, -s * n.x);
+    *b = V(bb, s + n.y * n.y * a, -n.y);
+}
+
+static int brdf_sample_c(const omat* m, v3 n, v3 wo, float ul, float u1, float u2, v3* wi) {
+    v3 t, b;
+    onb_c(n, &t, &b);
+    float ps = spec_p(m);
+    if (ul < ps) {
+        float a = m->roughness * m->roughness;
+        float a2 = a * a;
+        float c2 = (1.0f - u1) / (1.0f + (a2 - 1.0f) * u1);
+        float ct = sqrtf(fmaxf(c2, 0.0f)), st = sqrtf(fmaxf(1.0f - c2, 0.0f));
+        float ph = 2.0f * PI_F * u2;
+        v3 h = add(add(scl(t, st * cosf(ph)), scl(b, st * sinf(ph))), scl(n, ct));
+        float wh = dotv(wo, h);
+        *wi = sub(scl(h, 2.0f * wh), wo);
+    } else {
+        float r = sqrtf(u1), ph = 2.0f * PI_F * u2;
+        float z = sqrtf(fmaxf(1.0f - u1, 0.0f));
+        *wi = add(add(scl(t, r * cosf(ph)), scl(b, r * sinf(ph))), scl(n, z));
+    }
+    return dotv(n, *wi) > 0.0f;
+}
+
+static float power_h(float a, float b) {
+    float a2 = a * a, b2 = b * b;
+    return a2 + b2 > 0.0f ? a2 / (a2 + b2) : 0.0f;
+}
+
+/* exported for the closed-form tests: f (rgb) and pdf for one configuration */
+void o_brdf_eval(const float albedo[3], float roughness, float metallic, uint32_t flags, const float n[3],
+                 const float wo[3], const float wi[3], float out_f[3], float* out_pdf) {
+    omat m = {V(albedo[0], albedo[1], albedo[2]), fminf(fmaxf(roughness, 0.02f), 1.0f), metallic, flags};
+    v3 f = brdf_f(&m, V(n[0], n[1], n[2]), V(wo[0], wo[1], wo[2]), V(wi[0], wi[1], wi[2]));
+    out_f[0] = f.x;
+    out_f[1] = f.y;
+    out_f[2] = f.z;
+    *out_pdf = brdf_pdf_c(&m, V(n[0], n[1], n[2]), V(wo[0], wo[1], wo[2]), V(wi[0], wi[1], wi[2]));
+}
+int o_brdf_sample(const float albedo[3], float roughness, float metallic, uint32_t flags, const float n[3],
+                  const float wo[3], float ul, float u1, float u2, float out_wi[3]) {
+    omat m = {V(albedo[0], albedo[1], albedo[2]), fminf(fmaxf(roughness, 0.02f), 1.0f), metallic, flags};
+    v3 wi;
+    int ok = brdf_sample_c(&m, V(n[0], n[1], n[2]), V(wo[0], wo[1], wo[2]), ul, u1, u2, &wi);
+    out_wi[0] = wi.x;
+    out_wi[1] = wi.y;
+    out_wi[2] = wi.z;
+    return ok;
+}
+
+/* --------------------------------------------------------------- scene --- */
+typedef struct {
+    const float* world;
+    uint32_t ntris;
+    const o_bvh* bvh;
+    const uint32_t* inst5;
+    uint32_t n_inst;
+    const uint32_t* idx;
+    const float* nrm;
+    const uint32_t* mats16;
+    uint32_t* tri_off;     /* n_inst + 1 */
+    float* nmat;           /* 9 per instance: transpose(inverse(mat3(M))) rows */
+    /* emitters */
+    uint32_t n_emit;
+    float* e_tri;          /* 9 per emitter */
+    float* e_info;         /* rgb, area */
+    float* e_prob;
+    uint32_t* e_alias;
+    float* e_pdf;
+    uint32_t* gid2e;
+} oscene;
+
+static void normal_matrix(const float* m, float out[9]) {
+    double a[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) a[r][c] = m[4 * c + r];
+    double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                 a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+    double inv_det = det != 0.0 ? 1.0 / det : 0.0;
+    double cof[3][3];
+    cof[0][0] = a[1][1] * a[2][2] - a[1][2] * a[2][1];
+    cof[0][1] = -(a[1][0] * a[2][2] - a[1][2] * a[2][0]);
+    cof[0][2] = a[1][0] * a[2][1] - a[1][1] * a[2][0];
+    cof[1][0] = -(a[0][1] * a[2][2] - a[0][2] * a[2][1]);
+    cof[1][1] = a[0][0] * a[2][2] - a[0][2] * a[2][0];
+    cof[1][2] = -(a[0][0] * a[2][1] - a[0][1] * a[2][0]);
+    cof[2][0] = a[0][1] * a[1][2] - a[0][2] * a[1][1];
+    cof[2][1] = -(a[0][0] * a[1][2] - a[0][2] * a[1][0]);
+    cof[2][2] = a[0][0] * a[1][1] - a[0][1] * a[1][0];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) out[3 * r + c] = (float)(cof[r][c] * inv_det);
+}
+
+/* Power-proportional alias table (Vose), in the order of the emissive instance list. */
+static void build_emitters_c(oscene* S, const uint32_t* em_list, uint32_t n_em) {
+    S->n_emit = 0;
+    uint32_t cap = 0;
+    for (uint32_t k = 0; k < n_em; ++k) cap += S->inst5[5 * em_list[k] + 4] / 3;
+    S->e_tri = (float*)calloc(9 * (size_t)(cap + 1), sizeof(float));
+    S->e_info = (float*)calloc(4 * (size_t)(cap + 1), sizeof(float));
+    S->e_prob = (float*)calloc(cap + 1, sizeof(float));
+    S->e_alias = (uint32_t*)calloc(cap + 1, sizeof(uint32_t));
+    S->e_pdf = (float*)calloc(cap + 1, sizeof(float));
+    S->gid2e = (uint32_t*)malloc(sizeof(uint32_t) * (S->ntris + 1));
+    for (uint32_t g = 0; g <= S->ntris; ++g) S->gid2e[g] = 0xFFFFFFFFu;
+    double* power = (double*)calloc(cap + 1, sizeof(double));
+    for (uint32_t k = 0; k < n_em; ++k) {
+        uint32_t i = em_list[k];
+        const uint32_t* mt = S->mats16 + 16 * (size_t)S->inst5[5 * i + 3];
+        float al[3] = {u2f(mt[8]), u2f(mt[9]), u2f(mt[10])};
+        double lum = 0.2126 * al[0] + 0.7152 * al[1] + 0.0722 * al[2];
+        for (uint32_t p = 0; p < S->inst5[5 * i + 4] / 3; ++p) {
+            uint32_t g = S->tri_off[i] + p;
+            const float* w = S->world + 12 * (size_t)g;
+            double e1[3], e2[3];
+            for (int a = 0; a < 3; ++a) {
+                e1[a] = (double)w[3 + a] - w[a];
+                e2[a] = (double)w[6 + a] - w[a];
+            }
+            double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+                   cz = e1[0] * e2[1] - e1[1] * e2[0];
+            double area = 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
+            double pw = lum * area;
+            if (!(pw > 0.0)) continue;
+            uint32_t e = S->n_emit++;
+            memcpy(S->e_tri + 9 * (size_t)e, w, 36);
+            S->e_info[4 * e] = al[0];
+            S->e_info[4 * e + 1] = al[1];
+            S->e_info[4 * e + 2] = al[2];
+            S->e_info[4 * e + 3] = (float)area;
+            power[e] = pw;
+            S->gid2e[g] = e;
+        }
+    }
+    uint32_t ne = S->n_emit;
+    double total = 0;
+    for (uint32_t e = 0; e < ne; ++e) total += power[e];
+    if (ne) {
+        double* scaled = (double*)malloc(sizeof(double) * ne);
+        uint32_t *small = (uint32_t*)malloc(sizeof(uint32_t) * ne), *large = (uint32_t*)malloc(sizeof(uint32_t) * ne);
+        uint32_t ns = 0, nl = 0;
+        for (uint32_t e = 0; e < ne; ++e) {
+            scaled[e] = power[e] / total * ne;
+            S->e_pdf[e] = (float)(power[e] / total);
+            S->e_prob[e] = 1.0f;
+            if (scaled[e] < 1.0)
+                small[ns++] = e;
+            else
+                large[nl++] = e;
+        }
+        while (ns && nl) {
+            uint32_t s = small[--ns], l = large[nl - 1];
+            S->e_prob[s] = (float)scaled[s];
+            S->e_alias[s] = l;
+            scaled[l] = (scaled[l] + scaled[s]) - 1.0;
+            if (scaled[l] < 1.0) {
+                --nl;
+                small[ns++] = l;
+            }
+        }
+        while (nl) {
+            uint32_t l = large[--nl];
+            S->e_prob[l] = 1.0f;
+            S->e_alias[l] = l;
+        }
+        while (ns) {
+            uint32_t s = small[--ns];
+            S->e_prob[s] = 1.0f;
+            S->e_alias[s] = s;
+        }
+        free(scaled);
+        free(small);
+        free(large);
+    }
+    free(power);
+}
+
+typedef struct {
+    v3 p, ng, ns;
+    omat m;
+    v3 le;
+    int emissive;
+    uint32_t gid;
+    float eps;
+} osurf;
+
+static osurf surface(const oscene* S, v3 dir, uint32_t inst, uint32_t prim, float u, float v) {
+    osurf s;
+    const uint32_t* in = S->inst5 + 5 * inst;
+    s.gid = S->tri_off[inst] + prim;
+    const float* w = S->world + 12 * (size_t)s.gid;
+    v3 v0 = V(w[0], w[1], w[2]), v1 = V(w[3], w[4], w[5]), v2 = V(w[6], w[7], w[8]);
+    float b0 = 1.0f - u - v;
+    s.p = add(add(scl(v0, b0), scl(v1, u)), scl(v2, v));
+    v3 ng = nrmz(crossv(sub(v1, v0), sub(v2, v0)));
+    const uint32_t* ix = S->idx + in[1] + 3 * prim;
+    const float* n0p = S->nrm + 3 * (size_t)(ix[0] + in[2]);
+    const float* n1p = S->nrm + 3 * (size_t)(ix[1] + in[2]);
+    const float* n2p = S->nrm + 3 * (size_t)(ix[2] + in[2]);
+    v3 nl = add(add(scl(V(n0p[0], n0p[1], n0p[2]), b0), scl(V(n1p[0], n1p[1], n1p[2]), u)),
+                scl(V(n2p[0], n2p[1], n2p[2]), v));
+    const float* nm = S->nmat + 9 * (size_t)inst;
+    v3 ns = V((nm[0] * nl.x + nm[1] * nl.y) + nm[2] * nl.z, (nm[3] * nl.x + nm[4] * nl.y) + nm[5] * nl.z,
+              (nm[6] * nl.x + nm[7] * nl.y) + nm[8] * nl.z);
+    float l2 = dotv(ns, ns);
+    ns = l2 > 1e-20f ? scl(ns, 1.0f / sqrtf(l2)) : ng;
+    if (dotv(ng, dir) > 0.0f) ng = scl(ng, -1.0f);
+    if (dotv(ns, ng) < 0.0f) ns = scl(ns, -1.0f);
+    s.ng = ng;
+    s.ns = ns;
+    const uint32_t* mt = S->mats16 + 16 * (size_t)in[3];
+    s.m.albedo = V(u2f(mt[8]), u2f(mt[9]), u2f(mt[10]));
+    s.m.roughness = fminf(fmaxf(u2f(mt[5]), 0.02f), 1.0f);
+    s.m.metallic = fminf(fmaxf(u2f(mt[6]), 0.0f), 1.0f);
+    s.m.flags = mt[12];
+    s.emissive = (mt[12] & 1u) != 0u;
+    s.le = s.m.albedo;
+    float am = fmaxf(fmaxf(fabsf(s.p.x), fabsf(s.p.y)), fabsf(s.p.z));
+    s.eps = 1e-4f * (1.0f + am);
+    return s;
+}
+
+typedef struct {
+    v3 wi, le;
+    float dist, pdf;
+    int ok;
+} olight;
+
+static olight sample_light_c(const oscene* S, v3 p, float u0, float u1, float u2, float u3) {
+    olight ls;
+    memset(&ls, 0, sizeof(ls));
+    if (S->n_emit == 0) return ls;
+    uint32_t e = (uint32_t)(u0 * (float)S->n_emit);
+    if (e >= S->n_emit) e = S->n_emit - 1;
+    if (u1 >= S->e_prob[e]) e = S->e_alias[e];
+    const float* t = S->e_tri + 9 * (size_t)e;
+    float su = sqrtf(u2);
+    float w1 = su * (1.0f - u3), w2 = su * u3, w0 = 1.0f - su;
+    v3 A = V(t[0], t[1], t[2]), B = V(t[3], t[4], t[5]), C = V(t[6], t[7], t[8]);
+    v3 q = add(add(scl(A, w0), scl(B, w1)), scl(C, w2));
+    v3 d = sub(q, p);
+    float d2 = dotv(d, d);
+    if (!(d2 > 0.0f)) return ls;
+    float dist = sqrtf(d2);
+    ls.wi = scl(d, 1.0f / dist);
+    v3 nl = crossv(sub(B, A), sub(C, A));
+    float nll = sqrtf(dotv(nl, nl));
+    if (!(nll > 0.0f)) return ls;
+    float cosl = fabsf(dotv(nl, ls.wi)) / nll;
+    if (cosl <= 0.0f) return ls;
+    const float* info = S->e_info + 4 * (size_t)e;
+    ls.le = V(info[0], info[1], info[2]);
+    ls.pdf = S->e_pdf[e] * d2 / (info[3] * cosl);
+    ls.dist = dist;
+    ls.ok = ls.pdf > 0.0f;
+    return ls;
+}
+
+static float light_pdf_c(const oscene* S, uint32_t gid, v3 ng, v3 dir, float t) {
+    if (S->n_emit == 0) return 0.0f;
+    uint32_t e = S->gid2e[gid];
+    if (e == 0xFFFFFFFFu) return 0.0f;
+    float cosl = fabsf(dotv(ng, dir));
+    if (cosl <= 0.0f) return 0.0f;
+    return S->e_pdf[e] * t * t / (S->e_info[4 * (size_t)e + 3] * cosl);
+}
+
+/* --------------------------------------------------------------- camera --- */
+static int inv4(const double m[16], double inv[16]) {
+    /* Gauss-Jordan with partial pivoting */
+    double a[4][8];
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 8; ++c) a[r][c] = c < 4 ? m[4 * r + c] : (c - 4 == r ? 1.0 : 0.0);
+    for (int c = 0; c < 4; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < 4; ++r)
+            if (fabs(a[r][c]) > fabs(a[piv][c])) piv = r;
+        if (a[piv][c] == 0.0) return 0;
+        for (int k = 0; k < 8; ++k) {
+            double t = a[c][k];
+            a[c][k] = a[piv][k];
+            a[piv][k] = t;
+        }
+        double d = a[c][c];
+        for (int k = 0; k < 8; ++k) a[c][k] /= d;
+        for (int r = 0; r < 4; ++r) {
+            if (r == c) continue;
+            double f = a[r][c];
+            for (int k = 0; k < 8; ++k) a[r][k] -= f * a[c][k];
+        }
+    }
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) inv[4 * r + c] = a[r][4 + c];
+    return 1;
+}
+
+void o_camera_matrix(const float* view16, const float* proj16, float out16[16]) {
+    double v[16], p[16], vi[16], pi[16];
+    for (int r = 0; r < 4; ++r)
+        for (int k = 0; k < 4; ++k) {
+            v[4 * r + k] = view16[4 * k + r];
+            p[4 * r + k] = proj16[4 * k + r];
+        }
+    inv4(v, vi);
+    inv4(p, pi);
+    for (int r = 0; r < 4; ++r)
+        for (int k = 0; k < 4; ++k) {
+            double acc = 0;
+            for (int j = 0; j < 4; ++j) acc += vi[4 * r + j] * pi[4 * j + k];
+            out16[4 * r + k] = (float)acc;
+        }
+}
+
+static v3 cam_dir(const float* m, v3 pos, uint32_t x, uint32_t y, uint32_t w, uint32_t h) {
+    float u = ((float)x + 0.5f) / (float)w * 2.0f - 1.0f;
+    float v = ((float)y + 0.5f) / (float)h * 2.0f - 1.0f;
+    float wp[4];
+    for (int r = 0; r < 4; ++r) wp[r] = ((m[4 * r] * u + m[4 * r + 1] * v) + m[4 * r + 2]) + m[4 * r + 3];
+    float iw = 1.0f / fmaxf(wp[3], 1e-6f);
+    v3 p = V(wp[0] * iw, wp[1] * iw, wp[2] * iw);
+    return nrmz(sub(p, pos));
+}
+
+/* ----------------------------------------------------------------- path --- */
+void o_trace_one(const o_bvh* b, const float* ray, int any_hit, float* t, uint32_t* inst, uint32_t* prim,
+                 float* u, float* v);
+
+typedef struct {
+    const oscene* S;
+    const float* cam;
+    v3 pos;
+    uint32_t w, h, spp, max_depth, rr_start, seed, frame, seed_xor;
+    float* out;
+    uint32_t begin, end;
+} pt_job;
+
+static void* pt_worker(void* arg) {
+    pt_job* J = (pt_job*)arg;
+    const oscene* S = J->S;
+    for (uint32_t px = J->begin; px < J->end; ++px) {
+        float rad[3] = {0, 0, 0};
+        for (uint32_t k = 0; k < J->spp; ++k) {
+            prng rng = rseed(o_pass_state(J->seed ^ J->seed_xor, J->frame, k), px);
+            v3 o = J->pos;
+            v3 d = cam_dir(J->cam, J->pos, px % J->w, px / J->w, J->w, J->h);
+            v3 thr = V(1, 1, 1);
+            float last_pdf = 0.0f;
+            for (uint32_t bounce = 0; bounce < J->max_depth; ++bounce) {
+                float ray[8] = {o.x, o.y, o.z, 0.0f, d.x, d.y, d.z, 1e30f};
+                float t, hu, hv;
+                uint32_t hi, hp;
+                o_trace_one(S->bvh, ray, 0, &t, &hi, &hp, &hu, &hv);
+                if (!(t >= 0.0f)) break;
+                osurf s = surface(S, d, hi, hp, hu, hv);
+                if (s.emissive) {
+                    float w = 1.0f;
+                    if (bounce > 0) w = power_h(last_pdf, light_pdf_c(S, s.gid, s.ng, d, t));
+                    v3 c = scl(mulv(thr, s.le), w);
+                    rad[0] = rad[0] + c.x;
+                    rad[1] = rad[1] + c.y;
+                    rad[2] = rad[2] + c.z;
+                    break;
+                }
+                v3 wo = scl(d, -1.0f);
+                float u0 = runi(&rng), u1 = runi(&rng), u2 = runi(&rng), u3 = runi(&rng);
+                float ul = runi(&rng), us1 = runi(&rng), us2 = runi(&rng), urr = runi(&rng);
+                int last = bounce + 1 >= J->max_depth;
+                olight ls = sample_light_c(S, s.p, u0, u1, u2, u3);
+                if (ls.ok) {
+                    float cs = dotv(s.ns, ls.wi);
+                    if (cs > 0.0f && dotv(s.ng, ls.wi) > 0.0f) {
+                        v3 f = brdf_f(&s.m, s.ns, wo, ls.wi);
+                        float w = last ? 1.0f : power_h(ls.pdf, brdf_pdf_c(&s.m, s.ns, wo, ls.wi));
+                        v3 c = scl(mulv(mulv(thr, f), ls.le), cs * w / ls.pdf);
+                        if (c.x > 0.0f || c.y > 0.0f || c.z > 0.0f) {
+                            v3 so = add(s.p, scl(s.ng, s.eps));
+                            float sray[8] = {so.x, so.y, so.z, 0.0f, ls.wi.x, ls.wi.y, ls.wi.z,
+                                             ls.dist * (1.0f - 1e-4f) - s.eps};
+                            float st, su_, sv_;
+                            uint32_t si, sp2;
+                            o_trace_one(S->bvh, sray, 1, &st, &si, &sp2, &su_, &sv_);
+                            if (!(st >= 0.0f)) {
+                                rad[0] = rad[0] + c.x;
+                                rad[1] = rad[1] + c.y;
+                                rad[2] = rad[2] + c.z;
+                            }
+                        }
+                    }
+                }
+                v3 wi;
+                if (last || !brdf_sample_c(&s.m, s.ns, wo, ul, us1, us2, &wi) || !(dotv(s.ng, wi) > 0.0f)) break;
+                float pdf = brdf_pdf_c(&s.m, s.ns, wo, wi);
+                if (!(pdf > 0.0f)) break;
+                v3 f = brdf_f(&s.m, s.ns, wo, wi);
+                v3 nt = scl(mulv(thr, f), dotv(s.ns, wi) / pdf);
+                int alive = nt.x > 0.0f || nt.y > 0.0f || nt.z > 0.0f;
+                if (alive && J->rr_start > 0 && bounce + 1 >= J->rr_start) {
+                    float q = fminf(fmaxf(fmaxf(nt.x, nt.y), nt.z), 0.95f);
+                    if (urr >= q)
+                        alive = 0;
+                    else
+                        nt = scl(nt, 1.0f / q);
+                }
+                if (!alive) break;
+                o = add(s.p, scl(s.ng, s.eps));
+                d = wi;
+                thr = nt;
+                last_pdf = pdf;
+            }
+        }
+        float inv = 1.0f / (float)J->spp;
+        J->out[3 * (size_t)px] = rad[0] * inv;
+        J->out[3 * (size_t)px + 1] = rad[1] * inv;
+        J->out[3 * (size_t)px + 2] = rad[2] * inv;
+    }
+    return NULL;
+}
+
+int o_pt_render(const float* world, uint32_t ntris, const o_bvh* bvh, const uint32_t* inst5, uint32_t n_inst,
+                const uint32_t* idx, const float* nrm, const float* mat4s, const uint32_t* mats16,
+                const uint32_t* emissive, uint32_t n_emissive, const float* view16, const float* proj16,
+                const float* cam_pos, uint32_t w, uint32_t h, uint32_t spp, uint32_t max_depth, uint32_t rr_start,
+                uint32_t seed, uint32_t frame, uint32_t seed_xor, float* out_rgb, int threads) {
+    oscene S;
+    memset(&S, 0, sizeof(S));
+    S.world = world;
+    S.ntris = ntris;
+    S.bvh = bvh;
+    S.inst5 = inst5;
+    S.n_inst = n_inst;
+    S.idx = idx;
+    S.nrm = nrm;
+    S.mats16 = mats16;
+    S.tri_off = (uint32_t*)malloc(sizeof(uint32_t) * (n_inst + 1));
+    S.tri_off[0] = 0;
+    for (uint32_t i = 0; i < n_inst; ++i) S.tri_off[i + 1] = S.tri_off[i] + inst5[5 * i + 4] / 3;
+    S.nmat = (float*)malloc(sizeof(float) * 9 * (n_inst ? n_inst : 1));
+    for (uint32_t i = 0; i < n_inst; ++i) normal_matrix(mat4s + 16 * (size_t)inst5[5 * i], S.nmat + 9 * (size_t)i);
+    build_emitters_c(&S, emissive, n_emissive);
+    float cam[16];
+    o_camera_matrix(view16, proj16, cam);
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    pthread_t th[256];
+    pt_job jobs[256];
+    uint32_t n = w * h, per = (n + (uint32_t)threads - 1) / (uint32_t)threads;
+    for (int k = 0; k < threads; ++k) {
+        pt_job j = {&S, cam, V(cam_pos[0], cam_pos[1], cam_pos[2]), w, h, spp ? spp : 1, max_depth ? max_depth : 1,
+                    rr_start, seed, frame, seed_xor, out_rgb, 0, 0};
+        j.begin = (uint32_t)k * per < n ? (uint32_t)k * per : n;
+        j.end = j.begin + per < n ? j.begin + per : n;
+        jobs[k] = j;
+        if (pthread_create(&th[k], NULL, pt_worker, &jobs[k]) != 0) {
+            pt_worker(&jobs[k]);
+            th[k] = 0;
+        }
+    }
+    for (int k = 0; k < threads; ++k)
+        if (th[k]) pthread_join(th[k], NULL);
+    free(S.tri_off);
+    free(S.nmat);
+    free(S.e_tri);
+    free(S.e_info);
+    free(S.e_prob);
+    free(S.e_alias);
+    free(S.e_pdf);
+    free(S.gid2e);
+    return 0;
+}

@@ -1,20 +1,494 @@
-// Path-trace pass (placeholder until the wavefront kernels land).
-#include "snkpt_host.h"
+// The path-trace pass: RT::RecordRenderCmdBuf (Core/src/rendering/Raytracing.cpp:227-267)
+// rebuilt as a wavefront path tracer.
+//
+// The reference runs two traceRaysKHR(W, H) dispatches whose raygen shader does
+// primary shading from a G-buffer, ReSTIR and BounceDiffuse with recursive
+// traceRayEXT calls (RayGen.rgen:530-570).  Here one frame is a sequence of
+// kernels over structure-of-arrays queues in HBM:
+//   generate  camera rays through pixel centres (Util.glsl:9-13), one path per pixel
+//   extend    BVH8 closest hit of the ray queue (trace.hip)
+//   shade     hit interpolation + material (RayClosestHit.rchit:88-123), emission with
+//             MIS, next-event estimation to a power-sampled emissive triangle, GGX /
+//             diffuse BRDF sampling, Russian roulette; survivors are appended to the
+//             next ray queue and NEE samples to the shadow queue with one atomic per wave
+//   shadow    any-hit traversal of the shadow queue (ShadowTest, RayGen.rgen:101-121),
+//             unoccluded contributions added to the frame radiance
+//   resolve   running mean over accumulated frames, RGBA32F or RGBA16F out
+// Queue counts stay on the device, so a frame has no host round trip.
+#include <hip/hip_fp16.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "render_common.h"
+#include "render_modes.h"
 
 namespace snkpt {
 
-struct RenderState {};
-struct CacheState {};
+namespace {
 
-snkpt_status render(snkpt_ctx* ctx, const snkpt_camera*, const snkpt_settings*, void*, uint32_t, uint32_t, uint32_t,
-                    hipStream_t) {
-    return fail(ctx, SNKPT_ERR_NOT_BUILT, "render: not available in this build");
+constexpr int kBlock = 256;
+constexpr int kShadowBlock = 128;
+
+__global__ void k_fill_f4(float4* p, uint32_t n, float4 v) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
-snkpt_status oracle_accumulate(snkpt_ctx* ctx, const snkpt_camera*, const snkpt_settings*, double*, uint32_t,
-                               uint32_t, uint32_t, hipStream_t) {
-    return fail(ctx, SNKPT_ERR_NOT_BUILT, "oracle_accumulate: not available in this build");
+
+__global__ void k_set_counts(uint32_t* c, uint32_t i0, uint32_t v0, uint32_t i1, uint32_t v1) {
+    if (threadIdx.x == 0) {
+        c[i0] = v0;
+        c[i1] = v1;
+    }
 }
+
+__global__ void k_generate(CamDev cam, uint32_t w, uint32_t h, uint64_t initstate, float4* __restrict__ q_org,
+                           float4* __restrict__ q_dir, uint32_t* __restrict__ q_path, PathState ps,
+                           uint32_t* __restrict__ counts, uint32_t cur) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n = w * h;
+    if (i == 0) counts[cur] = n;
+    if (i >= n) return;
+    V3 d = camera_dir(cam, i % w, i / w, w, h);
+    q_org[i] = make_float4(cam.pos.x, cam.pos.y, cam.pos.z, 0.0f);
+    q_dir[i] = make_float4(d.x, d.y, d.z, 1e30f);
+    q_path[i] = i;
+    Rng r = rng_seed(initstate, i);
+    ps.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    ps.rng[i] = make_uint2((uint32_t)r.state, (uint32_t)(r.state >> 32));
+    ps.bounce[i] = 0;
+}
+
+// One path vertex of the unbiased path tracer.
+__global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PtParams P, Queue in, HitArrays hit, Queue out,
+                                                  ShadowQueue sq, PathState ps, float4* __restrict__ rad,
+                                                  uint32_t* __restrict__ counts, uint32_t cur, uint32_t nxt,
+                                                  unsigned long long* __restrict__ stats) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < counts[cur];
+    bool want_ext = false, want_sh = false;
+    V3 ext_o, ext_d, sh_o, sh_d, sh_c;
+    float sh_t = 0.0f;
+    uint32_t path = 0;
+    float4 thr4 = make_float4(0, 0, 0, 0);
+    Rng rng{0, 0};
+    uint32_t bounce = 0;
+    float new_pdf = 0.0f;
+    if (valid) {
+        path = in.path[i];
+        thr4 = ps.thr[path];
+        uint2 rs = ps.rng[path];
+        rng.state = (uint64_t)rs.x | ((uint64_t)rs.y << 32);
+        rng.inc = ((uint64_t)path << 1u) | 1u;
+        bounce = ps.bounce[path];
+        float t = hit.t[i];
+        if (t >= 0.0f) {
+            float4 dq = in.dir[i];
+            V3 dir{dq.x, dq.y, dq.z};
+            V3 thr{thr4.x, thr4.y, thr4.z};
+            Surf s = surface_at(S, dir, hit.inst[i], hit.prim[i], hit.u[i], hit.v[i]);
+            if (s.emissive) {
+                float w = 1.0f;
+                if (bounce > 0) w = power_heuristic(thr4.w, light_pdf(S, s.gid, s.ng, dir, t));
+                V3 c = mul(thr, s.le) * w;
+                float4 r = rad[path];
+                rad[path] = make_float4(r.x + c.x, r.y + c.y, r.z + c.z, r.w);
+            } else {
+                V3 wo = dir * -1.0f;
+                float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform(), u3 = rng.uniform();
+                float ul = rng.uniform(), us1 = rng.uniform(), us2 = rng.uniform(), urr = rng.uniform();
+                const bool last = bounce + 1 >= P.max_depth;
+                // next-event estimation
+                LightSample ls = sample_light(S, s.p, u0, u1, u2, u3);
+                if (ls.ok) {
+                    float cs = dot(s.ns, ls.wi);
+                    if (cs > 0.0f && dot(s.ng, ls.wi) > 0.0f) {
+                        V3 f = brdf_eval(s.m, s.ns, wo, ls.wi);
+                        float w = last ? 1.0f : power_heuristic(ls.pdf, brdf_pdf(s.m, s.ns, wo, ls.wi));
+                        sh_c = mul(mul(thr, f), ls.le) * (cs * w / ls.pdf);
+                        if (sh_c.x > 0.0f || sh_c.y > 0.0f || sh_c.z > 0.0f) {
+                            want_sh = true;
+                            sh_o = s.p + s.ng * s.eps;
+                            sh_d = ls.wi;
+                            sh_t = ls.dist * (1.0f - 1e-4f) - s.eps;
+                        }
+                    }
+                }
+                // continuation
+                V3 wi;
+                if (!last && brdf_sample(s.m, s.ns, wo, ul, us1, us2, wi) && dot(s.ng, wi) > 0.0f) {
+                    float pdf = brdf_pdf(s.m, s.ns, wo, wi);
+                    if (pdf > 0.0f) {
+                        V3 f = brdf_eval(s.m, s.ns, wo, wi);
+                        V3 nt = mul(thr, f) * (dot(s.ns, wi) / pdf);
+                        bool alive = nt.x > 0.0f || nt.y > 0.0f || nt.z > 0.0f;
+                        if (alive && P.rr_start > 0 && bounce + 1 >= P.rr_start) {
+                            float q = fminf(fmaxf(fmaxf(nt.x, nt.y), nt.z), 0.95f);
+                            if (urr >= q)
+                                alive = false;
+                            else
+                                nt = nt * (1.0f / q);
+                        }
+                        if (alive) {
+                            want_ext = true;
+                            ext_o = s.p + s.ng * s.eps;
+                            ext_d = wi;
+                            thr4 = make_float4(nt.x, nt.y, nt.z, pdf);
+                            new_pdf = pdf;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    uint32_t slot = wave_append(want_ext, &counts[nxt], stats ? &stats[1] : nullptr);
+    if (want_ext) {
+        out.org[slot] = make_float4(ext_o.x, ext_o.y, ext_o.z, 0.0f);
+        out.dir[slot] = make_float4(ext_d.x, ext_d.y, ext_d.z, 1e30f);
+        out.path[slot] = path;
+        ps.thr[path] = thr4;
+        ps.rng[path] = make_uint2((uint32_t)rng.state, (uint32_t)(rng.state >> 32));
+        ps.bounce[path] = bounce + 1;
+    }
+    (void)new_pdf;
+    uint32_t sslot = wave_append(want_sh, &counts[2], stats ? &stats[2] : nullptr);
+    if (want_sh) {
+        sq.org[sslot] = make_float4(sh_o.x, sh_o.y, sh_o.z, 0.0f);
+        sq.dir[sslot] = make_float4(sh_d.x, sh_d.y, sh_d.z, sh_t);
+        sq.contrib[sslot] = make_float4(sh_c.x, sh_c.y, sh_c.z, __uint_as_float(path));
+    }
+}
+
+template <int STACK>
+__global__ __launch_bounds__(kShadowBlock) void k_shadow(const uint4* __restrict__ nodes,
+                                                         const float4* __restrict__ tris, ShadowQueue sq,
+                                                         const uint32_t* __restrict__ counts,
+                                                         float4* __restrict__ rad) {
+    __shared__ uint2 stk[STACK * kShadowBlock];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= counts[2]) return;
+    float4 a = sq.org[i], d = sq.dir[i];
+    Ray r{a.x, a.y, a.z, a.w, d.x, d.y, d.z, d.w};
+    uint32_t nv = 0, tv = 0;
+    TraceResult res = traverse<true, false, STACK>(nodes, tris, r, &stk[threadIdx.x], kShadowBlock, nv, tv);
+    if (res.rec == 0xFFFFFFFFu) {
+        float4 c = sq.contrib[i];
+        uint32_t px = __float_as_uint(c.w);
+        float4 o = rad[px];
+        rad[px] = make_float4(o.x + c.x, o.y + c.y, o.z + c.z, o.w);
+    }
+}
+
+__global__ void k_resolve(const float4* __restrict__ rad, float4* __restrict__ accum, void* __restrict__ out,
+                          uint32_t n, float spp, uint32_t accumulate, uint32_t fmt) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 r = rad[i];
+    float4 o;
+    if (accumulate) {
+        float4 a = accum[i];
+        a = make_float4(a.x + r.x, a.y + r.y, a.z + r.z, a.w + spp);
+        accum[i] = a;
+        float inv = a.w > 0.0f ? 1.0f / a.w : 0.0f;
+        o = make_float4(a.x * inv, a.y * inv, a.z * inv, 1.0f);
+    } else {
+        float inv = 1.0f / spp;
+        o = make_float4(r.x * inv, r.y * inv, r.z * inv, 1.0f);
+    }
+    if (fmt == SNKPT_FORMAT_RGBA16F) {
+        __half* h = (__half*)out + 4 * (size_t)i;
+        h[0] = __float2half(o.x);
+        h[1] = __float2half(o.y);
+        h[2] = __float2half(o.z);
+        h[3] = __float2half(o.w);
+    } else {
+        ((float4*)out)[i] = o;
+    }
+}
+
+__global__ void k_add_f64(const float4* __restrict__ rad, double* __restrict__ acc, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 r = rad[i];
+    acc[3 * (size_t)i] += (double)r.x;
+    acc[3 * (size_t)i + 1] += (double)r.y;
+    acc[3 * (size_t)i + 2] += (double)r.z;
+}
+
+inline uint32_t nblk(uint32_t n, uint32_t b) { return n == 0 ? 1 : (n + b - 1) / b; }
+
+// inverse of a 4x4 (row-major) by cofactors, in double
+bool inverse4(const double m[16], double inv[16]) {
+    double a[16];
+    a[0] = m[5] * m[10] * m[15] - m[5] * m[11] * m[14] - m[9] * m[6] * m[15] + m[9] * m[7] * m[14] +
+           m[13] * m[6] * m[11] - m[13] * m[7] * m[10];
+    a[4] = -m[4] * m[10] * m[15] + m[4] * m[11] * m[14] + m[8] * m[6] * m[15] - m[8] * m[7] * m[14] -
+           m[12] * m[6] * m[11] + m[12] * m[7] * m[10];
+    a[8] = m[4] * m[9] * m[15] - m[4] * m[11] * m[13] - m[8] * m[5] * m[15] + m[8] * m[7] * m[13] +
+           m[12] * m[5] * m[11] - m[12] * m[7] * m[9];
+    a[12] = -m[4] * m[9] * m[14] + m[4] * m[10] * m[13] + m[8] * m[5] * m[14] - m[8] * m[6] * m[13] -
+            m[12] * m[5] * m[10] + m[12] * m[6] * m[9];
+    a[1] = -m[1] * m[10] * m[15] + m[1] * m[11] * m[14] + m[9] * m[2] * m[15] - m[9] * m[3] * m[14] -
+           m[13] * m[2] * m[11] + m[13] * m[3] * m[10];
+    a[5] = m[0] * m[10] * m[15] - m[0] * m[11] * m[14] - m[8] * m[2] * m[15] + m[8] * m[3] * m[14] +
+           m[12] * m[2] * m[11] - m[12] * m[3] * m[10];
+    a[9] = -m[0] * m[9] * m[15] + m[0] * m[11] * m[13] + m[8] * m[1] * m[15] - m[8] * m[3] * m[13] -
+           m[12] * m[1] * m[11] + m[12] * m[3] * m[9];
+    a[13] = m[0] * m[9] * m[14] - m[0] * m[10] * m[13] - m[8] * m[1] * m[14] + m[8] * m[2] * m[13] +
+            m[12] * m[1] * m[10] - m[12] * m[2] * m[9];
+    a[2] = m[1] * m[6] * m[15] - m[1] * m[7] * m[14] - m[5] * m[2] * m[15] + m[5] * m[3] * m[14] +
+           m[13] * m[2] * m[7] - m[13] * m[3] * m[6];
+    a[6] = -m[0] * m[6] * m[15] + m[0] * m[7] * m[14] + m[4] * m[2] * m[15] - m[4] * m[3] * m[14] -
+           m[12] * m[2] * m[7] + m[12] * m[3] * m[6];
+    a[10] = m[0] * m[5] * m[15] - m[0] * m[7] * m[13] - m[4] * m[1] * m[15] + m[4] * m[3] * m[13] +
+            m[12] * m[1] * m[7] - m[12] * m[3] * m[5];
+    a[14] = -m[0] * m[5] * m[14] + m[0] * m[6] * m[13] + m[4] * m[1] * m[14] - m[4] * m[2] * m[13] -
+            m[12] * m[1] * m[6] + m[12] * m[2] * m[5];
+    a[3] = -m[1] * m[6] * m[11] + m[1] * m[7] * m[10] + m[5] * m[2] * m[11] - m[5] * m[3] * m[10] -
+           m[9] * m[2] * m[7] + m[9] * m[3] * m[6];
+    a[7] = m[0] * m[6] * m[11] - m[0] * m[7] * m[10] - m[4] * m[2] * m[11] + m[4] * m[3] * m[10] +
+           m[8] * m[2] * m[7] - m[8] * m[3] * m[6];
+    a[11] = -m[0] * m[5] * m[11] + m[0] * m[7] * m[9] + m[4] * m[1] * m[11] - m[4] * m[3] * m[9] -
+            m[8] * m[1] * m[7] + m[8] * m[3] * m[5];
+    a[15] = m[0] * m[5] * m[10] - m[0] * m[6] * m[9] - m[4] * m[1] * m[10] + m[4] * m[2] * m[9] +
+            m[8] * m[1] * m[6] - m[8] * m[2] * m[5];
+    double det = m[0] * a[0] + m[1] * a[4] + m[2] * a[8] + m[3] * a[12];
+    if (det == 0.0) return false;
+    for (int i = 0; i < 16; ++i) inv[i] = a[i] / det;
+    return true;
+}
+
+}  // namespace
+
+// WorldPosFromDepth (Util.glsl:1-7): world = inverse(view) * inverse(proj) * clip.
+// The matrix is formed in double on the host and rounded to float once.
+CamDev make_cam(const snkpt_camera* c) {
+    double v[16], p[16], vi[16], pi[16];
+    for (int r = 0; r < 4; ++r)
+        for (int k = 0; k < 4; ++k) {
+            v[4 * r + k] = c->view[4 * k + r];  // column-major -> row-major
+            p[4 * r + k] = c->proj[4 * k + r];
+        }
+    inverse4(v, vi);
+    inverse4(p, pi);
+    CamDev d;
+    for (int r = 0; r < 4; ++r)
+        for (int k = 0; k < 4; ++k) {
+            double acc = 0;
+            for (int j = 0; j < 4; ++j) acc += vi[4 * r + j] * pi[4 * j + k];
+            d.m[4 * r + k] = (float)acc;
+        }
+    d.pos = V3{c->cam_pos[0], c->cam_pos[1], c->cam_pos[2]};
+    return d;
+}
+
+SceneDev scene_dev(snkpt_ctx* ctx) {
+    SceneDev S;
+    S.inst = ctx->d_inst.as<InstD>();
+    S.xf = ctx->d_inst_xf.as<float4>();
+    S.tri_off = ctx->accel.inst_tri_offset.as<uint32_t>();
+    S.mats = ctx->d_mats.as<MatD>();
+    S.idx = ctx->d_idx.as<uint32_t>();
+    S.nrm = ctx->d_nrm.as<float>();
+    S.world = ctx->accel.world_tris.as<float4>();
+    S.emit_tri = ctx->emit.tri.as<float4>();
+    S.emit_info = ctx->emit.info.as<float4>();
+    S.emit_alias = ctx->emit.alias.as<uint2>();
+    S.emit_pdf = ctx->emit.pdf.as<float>();
+    S.gid2e = ctx->emit.gid_to_emitter.as<uint32_t>();
+    S.n_emit = ctx->emit.count;
+    S.nodes = ctx->accel.nodes.as<uint4>();
+    S.leaf = ctx->accel.leaf_tris.as<float4>();
+    return S;
+}
+
+#define HIPR(expr)                                                \
+    do {                                                          \
+        hipError_t e_ = (expr);                                   \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr);    \
+    } while (0)
+
+snkpt_status ensure_state(snkpt_ctx* ctx, uint32_t w, uint32_t h, hipStream_t st) {
+    if (!ctx->render) ctx->render = new RenderState();
+    RenderState& R = *ctx->render;
+    const uint32_t n = w * h;
+    if (n > R.cap) {
+        for (int k = 0; k < 2; ++k) {
+            HIPR(R.q[k].org.reserve((size_t)n * 16));
+            HIPR(R.q[k].dir.reserve((size_t)n * 16));
+            HIPR(R.q[k].path.reserve((size_t)n * 4));
+        }
+        HIPR(R.h_t.reserve((size_t)n * 4));
+        HIPR(R.h_inst.reserve((size_t)n * 4));
+        HIPR(R.h_prim.reserve((size_t)n * 4));
+        HIPR(R.h_u.reserve((size_t)n * 4));
+        HIPR(R.h_v.reserve((size_t)n * 4));
+        HIPR(R.s_org.reserve((size_t)n * 16));
+        HIPR(R.s_dir.reserve((size_t)n * 16));
+        HIPR(R.s_contrib.reserve((size_t)n * 16));
+        HIPR(R.p_thr.reserve((size_t)n * 16));
+        HIPR(R.p_rng.reserve((size_t)n * 8));
+        HIPR(R.p_bounce.reserve((size_t)n * 4));
+        HIPR(R.rad.reserve((size_t)n * 16));
+        HIPR(R.accum.reserve((size_t)n * 16));
+        R.cap = n;
+    }
+    HIPR(R.counts.reserve(64));
+    HIPR(R.stats.reserve(64));
+    if (R.w != w || R.h != h) {
+        R.w = w;
+        R.h = h;
+        k_fill_f4<<<nblk(n, kBlock), kBlock, 0, st>>>(R.accum.as<float4>(), n, make_float4(0, 0, 0, 0));
+        R.accum_frames = 0;
+        R.history_valid = false;
+    }
+    return SNKPT_OK;
+}
+
+Queue queue_of(RenderState& R, int k) {
+    return Queue{R.q[k].org.as<float4>(), R.q[k].dir.as<float4>(), R.q[k].path.as<uint32_t>()};
+}
+HitArrays hits_of(RenderState& R) {
+    return HitArrays{R.h_t.as<float>(), R.h_inst.as<uint32_t>(), R.h_prim.as<uint32_t>(), R.h_u.as<float>(),
+                     R.h_v.as<float>()};
+}
+ShadowQueue shadow_of(RenderState& R) {
+    return ShadowQueue{R.s_org.as<float4>(), R.s_dir.as<float4>(), R.s_contrib.as<float4>()};
+}
+PathState paths_of(RenderState& R) {
+    return PathState{R.p_thr.as<float4>(), R.p_rng.as<uint2>(), R.p_bounce.as<uint32_t>()};
+}
+
+snkpt_status launch_shadow(snkpt_ctx* ctx, RenderState& R, hipStream_t st) {
+    const Accel& A = ctx->accel;
+    if (A.num_tris == 0) return SNKPT_OK;  // no hits, so no shadow rays were queued
+    uint32_t need = A.depth > 0 ? A.depth - 1 : 0;
+    uint32_t grid = nblk(R.w * R.h, kShadowBlock);
+    if (need <= 16)
+        k_shadow<16><<<grid, kShadowBlock, 0, st>>>(A.nodes.as<uint4>(), A.leaf_tris.as<float4>(), shadow_of(R),
+                                                     R.counts.as<uint32_t>(), R.rad.as<float4>());
+    else if (need <= 32)
+        k_shadow<32><<<grid, kShadowBlock, 0, st>>>(A.nodes.as<uint4>(), A.leaf_tris.as<float4>(), shadow_of(R),
+                                                     R.counts.as<uint32_t>(), R.rad.as<float4>());
+    else if (need <= 64)
+        k_shadow<64><<<grid, kShadowBlock, 0, st>>>(A.nodes.as<uint4>(), A.leaf_tris.as<float4>(), shadow_of(R),
+                                                     R.counts.as<uint32_t>(), R.rad.as<float4>());
+    else
+        return fail(ctx, SNKPT_ERR_CAPACITY, "shadow: BVH8 deeper than the 64-entry traversal stack");
+    return SNKPT_OK;
+}
+
+// One pass of 1 sample per pixel of the plain path tracer, added into R.rad.
+snkpt_status pt_pass(snkpt_ctx* ctx, const CamDev& cam, const snkpt_settings* s, uint64_t initstate,
+                     hipStream_t st, uint32_t first_bounce_skip) {
+    RenderState& R = *ctx->render;
+    const uint32_t n = R.w * R.h;
+    SceneDev S = scene_dev(ctx);
+    PtParams P{s->max_depth ? s->max_depth : 1u, s->rr_start};
+    uint32_t* counts = R.counts.as<uint32_t>();
+    unsigned long long* stats = R.stats.as<unsigned long long>();
+    int cur = 0;
+    k_generate<<<nblk(n, kBlock), kBlock, 0, st>>>(cam, R.w, R.h, initstate, R.q[0].org.as<float4>(),
+                                                   R.q[0].dir.as<float4>(), R.q[0].path.as<uint32_t>(),
+                                                   paths_of(R), counts, 0);
+    (void)first_bounce_skip;
+    for (uint32_t b = 0; b < P.max_depth; ++b) {
+        int nxt = cur ^ 1;
+        k_set_counts<<<1, 64, 0, st>>>(counts, (uint32_t)nxt, 0u, 2u, 0u);
+        snkpt_hits h{R.h_t.as<float>(), R.h_inst.as<uint32_t>(), R.h_prim.as<uint32_t>(), R.h_u.as<float>(),
+                     R.h_v.as<float>()};
+        snkpt_status e = launch_trace(ctx, R.q[cur].org.as<float4>(), R.q[cur].dir.as<float4>(), n, h, false,
+                                      nullptr, nullptr, counts + cur, st);
+        if (e != SNKPT_OK) return e;
+        k_shade<<<nblk(n, kBlock), kBlock, 0, st>>>(S, P, queue_of(R, cur), hits_of(R), queue_of(R, nxt),
+                                                    shadow_of(R), paths_of(R), R.rad.as<float4>(), counts,
+                                                    (uint32_t)cur, (uint32_t)nxt, stats);
+        if ((e = launch_shadow(ctx, R, st)) != SNKPT_OK) return e;
+        cur = nxt;
+    }
+    return SNKPT_OK;
+}
+
+snkpt_status begin_frame(snkpt_ctx* ctx, uint32_t w, uint32_t h, hipStream_t st) {
+    snkpt_status e = ensure_state(ctx, w, h, st);
+    if (e != SNKPT_OK) return e;
+    RenderState& R = *ctx->render;
+    const uint32_t n = w * h;
+    k_fill_f4<<<nblk(n, kBlock), kBlock, 0, st>>>(R.rad.as<float4>(), n, make_float4(0, 0, 0, 0));
+    return SNKPT_OK;
+}
+
+static snkpt_status read_stats(snkpt_ctx* ctx, hipStream_t st, uint64_t camera_rays) {
+    RenderState& R = *ctx->render;
+    unsigned long long h[8];
+    HIPR(hipMemcpyAsync(h, R.stats.ptr, sizeof(h), hipMemcpyDeviceToHost, st));
+    HIPR(hipStreamSynchronize(st));
+    ctx->stats.camera_rays = camera_rays;
+    ctx->stats.extension_rays = h[1];
+    ctx->stats.shadow_rays = h[2];
+    ctx->stats.cache_queries = h[3];
+    ctx->stats.cache_train_samples = h[4];
+    return SNKPT_OK;
+}
+
+__global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
+    if (threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
+snkpt_status render(snkpt_ctx* ctx, const snkpt_camera* cam, const snkpt_settings* s, void* out, uint32_t w,
+                    uint32_t h, uint32_t fmt, hipStream_t st) {
+    if (!ctx->accel.built) return fail(ctx, SNKPT_ERR_NOT_BUILT, "snkpt_render: call snkpt_build_accel first");
+    if ((uint64_t)w * h > (1ull << 26)) return fail(ctx, SNKPT_ERR_CAPACITY, "snkpt_render: image above 2^26 pixels");
+    if (s->mode > SNKPT_MODE_CACHE) return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_render: unknown mode");
+    if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, SNKPT_ERR_HIP, "hipSetDevice");
+    snkpt_status e = begin_frame(ctx, w, h, st);
+    if (e != SNKPT_OK) return e;
+    RenderState& R = *ctx->render;
+    k_zero_u64<<<1, 64, 0, st>>>(R.stats.as<unsigned long long>(), 8);
+    if (s->flags & SNKPT_FLAG_RESET) {
+        k_fill_f4<<<nblk(w * h, kBlock), kBlock, 0, st>>>(R.accum.as<float4>(), w * h, make_float4(0, 0, 0, 0));
+        R.accum_frames = 0;
+    }
+    CamDev cd = make_cam(cam);
+    const uint32_t spp = s->spp ? s->spp : 1u;
+    for (uint32_t k = 0; k < spp; ++k) {
+        uint64_t is = pass_state(s->seed, s->frame, k);
+        if (s->mode == SNKPT_MODE_PT)
+            e = pt_pass(ctx, cd, s, is, st, 0);
+        else if (s->mode == SNKPT_MODE_RESTIR)
+            e = restir_pass(ctx, cam, cd, s, is, k, st);
+        else
+            e = cache_pass(ctx, cd, s, is, k, st);
+        if (e != SNKPT_OK) return e;
+    }
+    k_resolve<<<nblk(w * h, kBlock), kBlock, 0, st>>>(R.rad.as<float4>(), R.accum.as<float4>(), out, w * h,
+                                                      (float)spp, (s->flags & SNKPT_FLAG_ACCUMULATE) ? 1u : 0u, fmt);
+    if (s->flags & SNKPT_FLAG_ACCUMULATE) R.accum_frames += spp;
+    hipError_t le = hipGetLastError();
+    if (le != hipSuccess) return hip_fail(ctx, le, "render launch");
+    return read_stats(ctx, st, (uint64_t)w * h * spp);
+}
+
+snkpt_status oracle_accumulate(snkpt_ctx* ctx, const snkpt_camera* cam, const snkpt_settings* s, double* accum,
+                               uint32_t w, uint32_t h, uint32_t spp, hipStream_t st) {
+    if (!ctx->accel.built) return fail(ctx, SNKPT_ERR_NOT_BUILT, "oracle_accumulate: build the accel first");
+    if ((uint64_t)w * h > (1ull << 26)) return fail(ctx, SNKPT_ERR_CAPACITY, "oracle_accumulate: image too large");
+    if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, SNKPT_ERR_HIP, "hipSetDevice");
+    CamDev cd = make_cam(cam);
+    snkpt_status e = ensure_state(ctx, w, h, st);
+    if (e != SNKPT_OK) return e;
+    RenderState& R = *ctx->render;
+    for (uint32_t k = 0; k < spp; ++k) {
+        k_fill_f4<<<nblk(w * h, kBlock), kBlock, 0, st>>>(R.rad.as<float4>(), w * h, make_float4(0, 0, 0, 0));
+        // oracle streams are disjoint from render streams: seed ^ "orac"
+        if ((e = pt_pass(ctx, cd, s, pass_state(s->seed ^ 0x6F726163u, s->frame, k), st, 0)) != SNKPT_OK) return e;
+        k_add_f64<<<nblk(w * h, kBlock), kBlock, 0, st>>>(R.rad.as<float4>(), accum, w * h);
+    }
+    hipError_t le = hipGetLastError();
+    if (le != hipSuccess) return hip_fail(ctx, le, "oracle_accumulate launch");
+    return SNKPT_OK;
+}
+
 void render_state_free(RenderState* r) { delete r; }
-void cache_state_free(CacheState* c) { delete c; }
 
 }  // namespace snkpt

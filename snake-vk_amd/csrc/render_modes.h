@@ -1,0 +1,88 @@
+// Wavefront state shared by the path-trace, ReSTIR and cache passes.
+#pragma once
+
+#include "render_common.h"
+
+namespace snkpt {
+
+struct CamDev {
+    float m[16];  // inverse(view) * inverse(proj), row-major
+    V3 pos;
+};
+
+// Camera ray through the centre of pixel (x, y), row 0 at the top (Util.glsl:9-13).
+__device__ __forceinline__ V3 camera_dir(const CamDev& c, uint32_t x, uint32_t y, uint32_t w, uint32_t h) {
+    float u = ((float)x + 0.5f) / (float)w * 2.0f - 1.0f;
+    float v = ((float)y + 0.5f) / (float)h * 2.0f - 1.0f;
+    float wp[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) wp[r] = ((c.m[4 * r] * u + c.m[4 * r + 1] * v) + c.m[4 * r + 2]) + c.m[4 * r + 3];
+    float iw = 1.0f / fmaxf(wp[3], 1e-6f);
+    V3 p{wp[0] * iw, wp[1] * iw, wp[2] * iw};
+    return normalize(p - c.pos);
+}
+
+struct PtParams {
+    uint32_t max_depth, rr_start;
+};
+struct Queue {
+    float4* org;
+    float4* dir;
+    uint32_t* path;
+};
+struct HitArrays {
+    float* t;
+    uint32_t* inst;
+    uint32_t* prim;
+    float* u;
+    float* v;
+};
+struct ShadowQueue {
+    float4* org;
+    float4* dir;
+    float4* contrib;  // rgb, pixel (bits)
+};
+struct PathState {
+    float4* thr;       // throughput rgb, pdf of the last BRDF sample
+    uint2* rng;        // PCG32 state (the stream is the pixel index)
+    uint32_t* bounce;  // path vertex index of the next hit
+};
+
+struct QueueBufs {
+    DeviceBuffer org, dir, path;
+};
+
+struct RestirState;
+void restir_state_free(RestirState* r);
+
+struct RenderState {
+    uint32_t cap = 0, w = 0, h = 0;
+    QueueBufs q[2];
+    DeviceBuffer h_t, h_inst, h_prim, h_u, h_v;
+    DeviceBuffer s_org, s_dir, s_contrib;
+    DeviceBuffer p_thr, p_rng, p_bounce;
+    DeviceBuffer rad, accum, counts, stats;
+    uint32_t accum_frames = 0;
+    bool history_valid = false;
+    RestirState* restir = nullptr;
+    ~RenderState() { restir_state_free(restir); }
+};
+
+CamDev make_cam(const snkpt_camera* c);
+SceneDev scene_dev(snkpt_ctx* ctx);
+snkpt_status ensure_state(snkpt_ctx* ctx, uint32_t w, uint32_t h, hipStream_t st);
+Queue queue_of(RenderState& R, int k);
+HitArrays hits_of(RenderState& R);
+ShadowQueue shadow_of(RenderState& R);
+PathState paths_of(RenderState& R);
+snkpt_status launch_shadow(snkpt_ctx* ctx, RenderState& R, hipStream_t st);
+snkpt_status pt_pass(snkpt_ctx* ctx, const CamDev& cam, const snkpt_settings* s, uint64_t initstate,
+                     hipStream_t st, uint32_t first_bounce_skip);
+
+// restir.hip / cache.hip
+snkpt_status restir_pass(snkpt_ctx* ctx, const snkpt_camera* cam, const CamDev& cd, const snkpt_settings* s,
+                         uint64_t initstate, uint32_t pass, hipStream_t st);
+snkpt_status cache_pass(snkpt_ctx* ctx, const CamDev& cd, const snkpt_settings* s, uint64_t initstate,
+                        uint32_t pass, hipStream_t st);
+
+}  // namespace snkpt
