@@ -88,8 +88,11 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PtParams P, Queue 
             V3 thr{thr4.x, thr4.y, thr4.z};
             Surf s = surface_at(S, dir, hit.inst[i], hit.prim[i], hit.u[i], hit.v[i]);
             if (s.emissive) {
+                // MIS against the NEE of the previous vertex; a negative stored pdf
+                // marks a vertex whose direct light came from ReSTIR (weight 0 here)
                 float w = 1.0f;
-                if (bounce > 0) w = power_heuristic(thr4.w, light_pdf(S, s.gid, s.ng, dir, t));
+                if (bounce > 0)
+                    w = thr4.w < 0.0f ? 0.0f : power_heuristic(thr4.w, light_pdf(S, s.gid, s.ng, dir, t));
                 V3 c = mul(thr, s.le) * w;
                 float4 r = rad[path];
                 rad[path] = make_float4(r.x + c.x, r.y + c.y, r.z + c.z, r.w);
@@ -378,21 +381,16 @@ snkpt_status launch_shadow(snkpt_ctx* ctx, RenderState& R, hipStream_t st) {
     return SNKPT_OK;
 }
 
-// One pass of 1 sample per pixel of the plain path tracer, added into R.rad.
-snkpt_status pt_pass(snkpt_ctx* ctx, const CamDev& cam, const snkpt_settings* s, uint64_t initstate,
-                     hipStream_t st, uint32_t first_bounce_skip) {
+// Bounces first..max_depth-1 of the queued paths (queue `cur` already holds
+// the rays of bounce `first`).
+snkpt_status pt_bounces(snkpt_ctx* ctx, const snkpt_settings* s, int cur, uint32_t first, hipStream_t st) {
     RenderState& R = *ctx->render;
     const uint32_t n = R.w * R.h;
     SceneDev S = scene_dev(ctx);
     PtParams P{s->max_depth ? s->max_depth : 1u, s->rr_start};
     uint32_t* counts = R.counts.as<uint32_t>();
     unsigned long long* stats = R.stats.as<unsigned long long>();
-    int cur = 0;
-    k_generate<<<nblk(n, kBlock), kBlock, 0, st>>>(cam, R.w, R.h, initstate, R.q[0].org.as<float4>(),
-                                                   R.q[0].dir.as<float4>(), R.q[0].path.as<uint32_t>(),
-                                                   paths_of(R), counts, 0);
-    (void)first_bounce_skip;
-    for (uint32_t b = 0; b < P.max_depth; ++b) {
+    for (uint32_t b = first; b < P.max_depth; ++b) {
         int nxt = cur ^ 1;
         k_set_counts<<<1, 64, 0, st>>>(counts, (uint32_t)nxt, 0u, 2u, 0u);
         snkpt_hits h{R.h_t.as<float>(), R.h_inst.as<uint32_t>(), R.h_prim.as<uint32_t>(), R.h_u.as<float>(),
@@ -407,6 +405,23 @@ snkpt_status pt_pass(snkpt_ctx* ctx, const CamDev& cam, const snkpt_settings* s,
         cur = nxt;
     }
     return SNKPT_OK;
+}
+
+snkpt_status generate_camera(snkpt_ctx* ctx, const CamDev& cam, uint64_t initstate, hipStream_t st) {
+    RenderState& R = *ctx->render;
+    const uint32_t n = R.w * R.h;
+    k_generate<<<nblk(n, kBlock), kBlock, 0, st>>>(cam, R.w, R.h, initstate, R.q[0].org.as<float4>(),
+                                                   R.q[0].dir.as<float4>(), R.q[0].path.as<uint32_t>(),
+                                                   paths_of(R), R.counts.as<uint32_t>(), 0);
+    return SNKPT_OK;
+}
+
+// One pass of 1 sample per pixel of the plain path tracer, added into R.rad.
+snkpt_status pt_pass(snkpt_ctx* ctx, const CamDev& cam, const snkpt_settings* s, uint64_t initstate,
+                     hipStream_t st, uint32_t first_bounce_skip) {
+    (void)first_bounce_skip;
+    generate_camera(ctx, cam, initstate, st);
+    return pt_bounces(ctx, s, 0, 0, st);
 }
 
 snkpt_status begin_frame(snkpt_ctx* ctx, uint32_t w, uint32_t h, hipStream_t st) {

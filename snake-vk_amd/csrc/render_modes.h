@@ -78,6 +78,9 @@ PathState paths_of(RenderState& R);
 snkpt_status launch_shadow(snkpt_ctx* ctx, RenderState& R, hipStream_t st);
 snkpt_status pt_pass(snkpt_ctx* ctx, const CamDev& cam, const snkpt_settings* s, uint64_t initstate,
                      hipStream_t st, uint32_t first_bounce_skip);
+snkpt_status pt_bounces(snkpt_ctx* ctx, const snkpt_settings* s, int cur, uint32_t first, hipStream_t st);
+snkpt_status generate_camera(snkpt_ctx* ctx, const CamDev& cam, uint64_t initstate, hipStream_t st);
+inline uint32_t grid_for(uint32_t n, uint32_t b) { return n == 0 ? 1 : (n + b - 1) / b; }
 
 // restir.hip / cache.hip
 snkpt_status restir_pass(snkpt_ctx* ctx, const snkpt_camera* cam, const CamDev& cd, const snkpt_settings* s,

@@ -1,14 +1,482 @@
-// ReSTIR DI (placeholder until the resampling kernels land).
+// ReSTIR DI for the primary vertex: candidate, temporal and spatial passes.
+//
+// Reference: RayGen.rgen:215-528 — DoInitialWRS (32 point-light + 32 emissive
+// candidates, :296-357), WriteInitialReservoirs (temporal reuse with
+// reprojection, normal test and M_CAP 20, :411-468), ComputeAndWriteSpatial-
+// ReservoirReuse (num_spatial_resamples neighbours in a 30 px box, :470-506),
+// CalculateRadianceFromReservoirs (:508-528).  The reference combine is biased
+// (no MIS / 1/Z weights, Q10) and discards a fresh reservoir without history;
+// this build keeps its structure (candidate -> temporal -> spatial -> shade,
+// reservoirs ping-ponged between passes as in Q9) but uses the unbiased
+// combination of generalized resampled importance sampling (Bitterli et al.
+// 2020; Lin et al. 2022) with balance-heuristic resampling weights:
+//   candidates  RESTIR_CANDIDATES samples from the power-proportional emitter
+//               distribution, RIS weight p_hat / p_src, weighted reservoir sampling
+//   temporal    reproject the primary hit into the previous camera, reject on
+//               normal / distance mismatch, cap the history M at m_cap * M_cur
+//   spatial     k neighbours uniformly in a disk of `restir_radius` pixels
+//   weights     input l contributes m_l(y_l) p_hat(y_l) W_l with
+//               m_l(y) = M_l p_hat_l(y) / sum_j M_j p_hat_j(y), each p_hat_j
+//               evaluated on input j's own surface; W = w_sum / p_hat(y)
+// Target p_hat = luminance(f Le cos_s cos_l / d^2), unshadowed; visibility is
+// traced once for the final sample.  The rest of the path (bounces >= 1)
+// continues with the plain path tracer; BRDF-sampled emission seen from the
+// primary vertex gets MIS weight 0 because direct light there is ReSTIR's.
+#include <cstring>
+
 #include "render_modes.h"
 
 namespace snkpt {
 
-struct RestirState {};
+struct RestirState {
+    uint32_t cap = 0;
+    DeviceBuffer g[2][5];  // per frame parity: p|valid, ns|rough, ng|metal, albedo|flags, wo|t
+    DeviceBuffer r_init, r_tmp, r_fin;  // float4 (emitter bits, b1, b2, W)
+    DeviceBuffer m_init, m_tmp, m_fin;  // M
+    int cur = 0;
+    bool have_prev = false;
+    float prev_pv[16] = {0};
+    float prev_pos[3] = {0, 0, 0};
+};
 void restir_state_free(RestirState* r) { delete r; }
 
-snkpt_status restir_pass(snkpt_ctx* ctx, const snkpt_camera*, const CamDev&, const snkpt_settings*, uint64_t,
-                         uint32_t, hipStream_t) {
-    return fail(ctx, SNKPT_ERR_NOT_BUILT, "ReSTIR mode: not available in this build");
+namespace {
+
+constexpr int kBlock = 256;
+constexpr uint32_t kMaxSpatial = 8;
+
+struct GView {
+    float4 *p, *n, *g, *a, *w;
+};
+
+struct GSurf {
+    V3 p, ns, ng, wo;
+    Mat m;
+    float t;
+    bool valid;
+};
+
+__device__ __forceinline__ GSurf load_g(const GView& G, uint32_t i) {
+    GSurf s;
+    float4 p = G.p[i], n = G.n[i], g = G.g[i], a = G.a[i], w = G.w[i];
+    s.p = V3{p.x, p.y, p.z};
+    s.valid = p.w > 0.0f;
+    s.ns = V3{n.x, n.y, n.z};
+    s.m.roughness = n.w;
+    s.ng = V3{g.x, g.y, g.z};
+    s.m.metallic = g.w;
+    s.m.albedo = V3{a.x, a.y, a.z};
+    s.m.flags = __float_as_uint(a.w);
+    s.wo = V3{w.x, w.y, w.z};
+    s.t = w.w;
+    return s;
+}
+
+struct Res {
+    uint32_t e;
+    float b1, b2, W, M, wsum;
+};
+
+// Unshadowed contribution of emitter sample (e, b1, b2) at surface g.
+__device__ __forceinline__ float target(const SceneDev& S, const GSurf& g, uint32_t e, float b1, float b2, V3* c_out,
+                                        V3* wi_out, float* dist_out) {
+    const float4* t = S.emit_tri + 3 * (size_t)e;
+    float4 a = t[0], b = t[1], c = t[2];
+    V3 A{a.x, a.y, a.z}, B{b.x, b.y, b.z}, C{c.x, c.y, c.z};
+    float b0 = 1.0f - b1 - b2;
+    V3 q = A * b0 + B * b1 + C * b2;
+    V3 d = q - g.p;
+    float d2 = dot(d, d);
+    if (!(d2 > 0.0f)) return 0.0f;
+    float dist = sqrtf(d2);
+    V3 wi = d * (1.0f / dist);
+    float cs = dot(g.ns, wi);
+    if (cs <= 0.0f || dot(g.ng, wi) <= 0.0f) return 0.0f;
+    V3 nl = cross(B - A, C - A);
+    float nll = sqrtf(dot(nl, nl));
+    if (!(nll > 0.0f)) return 0.0f;
+    float cosl = fabsf(dot(nl, wi)) / nll;
+    float4 info = S.emit_info[e];
+    V3 f = brdf_eval(g.m, g.ns, g.wo, wi);
+    V3 cc = mul(f, V3{info.x, info.y, info.z}) * (cs * cosl / d2);
+    if (c_out) *c_out = cc;
+    if (wi_out) *wi_out = wi;
+    if (dist_out) *dist_out = dist;
+    return luminance(cc);
+}
+
+__device__ __forceinline__ bool res_update(Res& r, uint32_t e, float b1, float b2, float w, float u) {
+    if (!(w > 0.0f)) return false;
+    r.wsum += w;
+    if (u * r.wsum < w) {
+        r.e = e;
+        r.b1 = b1;
+        r.b2 = b2;
+        return true;
+    }
+    return false;
+}
+
+__device__ __forceinline__ void store_res(float4* r, float* m, uint32_t i, const Res& x) {
+    r[i] = make_float4(__uint_as_float(x.e), x.b1, x.b2, x.W);
+    m[i] = x.M;
+}
+__device__ __forceinline__ Res load_res(const float4* r, const float* m, uint32_t i) {
+    float4 a = r[i];
+    Res x;
+    x.e = __float_as_uint(a.x);
+    x.b1 = a.y;
+    x.b2 = a.z;
+    x.W = a.w;
+    x.M = m[i];
+    x.wsum = 0.0f;
+    return x;
+}
+
+constexpr uint64_t kPhaseCand = 0x52455354ull, kPhaseTemp = 0x54454d50ull, kPhaseSpat = 0x53504154ull,
+                   kPhaseCont = 0x434f4e54ull;
+
+// Primary vertex: G-buffer, directly seen emission, candidate RIS, continuation ray.
+__global__ __launch_bounds__(kBlock) void k_primary(SceneDev S, GView G, Queue in, HitArrays hit, Queue out,
+                                                    PathState ps, float4* __restrict__ rad, float4* __restrict__ r_out,
+                                                    float* __restrict__ m_out, uint32_t* __restrict__ counts,
+                                                    uint32_t n, uint32_t n_cand, uint32_t max_depth,
+                                                    uint32_t rr_start, uint64_t initstate,
+                                                    unsigned long long* __restrict__ stats) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    bool want_ext = false;
+    V3 ext_o, ext_d;
+    float4 thr4 = make_float4(0, 0, 0, 0);
+    Rng rc{0, 0};
+    if (valid) {
+        Res r{0xFFFFFFFFu, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        float4 dq = in.dir[i];
+        V3 dir{dq.x, dq.y, dq.z};
+        float t = hit.t[i];
+        float4 gp = make_float4(0, 0, 0, 0);
+        if (t >= 0.0f) {
+            Surf s = surface_at(S, dir, hit.inst[i], hit.prim[i], hit.u[i], hit.v[i]);
+            if (s.emissive) {
+                float4 o = rad[i];
+                rad[i] = make_float4(o.x + s.le.x, o.y + s.le.y, o.z + s.le.z, o.w);
+            } else {
+                V3 wo = dir * -1.0f;
+                gp = make_float4(s.p.x, s.p.y, s.p.z, 1.0f);
+                G.n[i] = make_float4(s.ns.x, s.ns.y, s.ns.z, s.m.roughness);
+                G.g[i] = make_float4(s.ng.x, s.ng.y, s.ng.z, s.m.metallic);
+                G.a[i] = make_float4(s.m.albedo.x, s.m.albedo.y, s.m.albedo.z, __uint_as_float(s.m.flags));
+                G.w[i] = make_float4(wo.x, wo.y, wo.z, t);
+                GSurf g;
+                g.p = s.p;
+                g.ns = s.ns;
+                g.ng = s.ng;
+                g.wo = wo;
+                g.m = s.m;
+                Rng rr = rng_seed(initstate ^ kPhaseCand, i);
+                float ph = 0.0f;
+                if (S.n_emit > 0) {
+                    for (uint32_t c = 0; c < n_cand; ++c) {
+                        float u0 = rr.uniform(), u1 = rr.uniform(), u2 = rr.uniform(), u3 = rr.uniform();
+                        float uw = rr.uniform();
+                        uint32_t e = (uint32_t)(u0 * (float)S.n_emit);
+                        if (e >= S.n_emit) e = S.n_emit - 1;
+                        uint2 al = S.emit_alias[e];
+                        if (u1 >= __uint_as_float(al.x)) e = al.y;
+                        float su = sqrtf(u2);
+                        float b1 = su * (1.0f - u3), b2 = su * u3;
+                        float p = target(S, g, e, b1, b2, nullptr, nullptr, nullptr);
+                        float src = S.emit_pdf[e] / S.emit_info[e].w;  // area measure
+                        if (res_update(r, e, b1, b2, p / src, uw)) ph = p;
+                    }
+                }
+                r.M = (float)n_cand;
+                r.W = ph > 0.0f ? r.wsum / (r.M * ph) : 0.0f;
+                // continuation into bounce 1 (direct light at this vertex is ReSTIR's)
+                if (max_depth > 1) {
+                    rc = rng_seed(initstate ^ kPhaseCont, i);
+                    float ul = rc.uniform(), us1 = rc.uniform(), us2 = rc.uniform(), urr = rc.uniform();
+                    V3 wi;
+                    if (brdf_sample(s.m, s.ns, wo, ul, us1, us2, wi) && dot(s.ng, wi) > 0.0f) {
+                        float pdf = brdf_pdf(s.m, s.ns, wo, wi);
+                        if (pdf > 0.0f) {
+                            V3 nt = brdf_eval(s.m, s.ns, wo, wi) * (dot(s.ns, wi) / pdf);
+                            bool alive = nt.x > 0.0f || nt.y > 0.0f || nt.z > 0.0f;
+                            if (alive && rr_start > 0 && 1u >= rr_start) {
+                                float q = fminf(fmaxf(fmaxf(nt.x, nt.y), nt.z), 0.95f);
+                                if (urr >= q)
+                                    alive = false;
+                                else
+                                    nt = nt * (1.0f / q);
+                            }
+                            if (alive) {
+                                want_ext = true;
+                                ext_o = s.p + s.ng * s.eps;
+                                ext_d = wi;
+                                thr4 = make_float4(nt.x, nt.y, nt.z, -1.0f);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        G.p[i] = gp;
+        store_res(r_out, m_out, i, r);
+    }
+    uint32_t slot = wave_append(want_ext, &counts[1], stats ? &stats[1] : nullptr);
+    if (want_ext) {
+        out.org[slot] = make_float4(ext_o.x, ext_o.y, ext_o.z, 0.0f);
+        out.dir[slot] = make_float4(ext_d.x, ext_d.y, ext_d.z, 1e30f);
+        out.path[slot] = i;
+        ps.thr[i] = thr4;
+        ps.rng[i] = make_uint2((uint32_t)rc.state, (uint32_t)(rc.state >> 32));
+        ps.bounce[i] = 1;
+    }
+}
+
+struct PrevCam {
+    float pv[16];  // column-major proj_view of the previous frame
+    V3 pos;
+};
+
+// Temporal reuse (WriteInitialReservoirs, RayGen.rgen:411-468) with 1/Z weights.
+__global__ __launch_bounds__(kBlock) void k_temporal(SceneDev S, GView G, GView Gp, const float4* __restrict__ r_in,
+                                                     const float* __restrict__ m_in, const float4* __restrict__ r_hist,
+                                                     const float* __restrict__ m_hist, float4* __restrict__ r_out,
+                                                     float* __restrict__ m_out, uint32_t w, uint32_t h, PrevCam pc,
+                                                     float m_cap, uint64_t initstate) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w * h) return;
+    Res rc = load_res(r_in, m_in, i);
+    GSurf g = load_g(G, i);
+    if (!g.valid) {
+        store_res(r_out, m_out, i, rc);
+        return;
+    }
+    // reproject into the previous frame
+    float cl[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) cl[r] = ((pc.pv[r] * g.p.x + pc.pv[4 + r] * g.p.y) + pc.pv[8 + r] * g.p.z) + pc.pv[12 + r];
+    int q = -1;
+    if (cl[3] > 1e-6f) {
+        float fx = (cl[0] / cl[3] * 0.5f + 0.5f) * (float)w, fy = (cl[1] / cl[3] * 0.5f + 0.5f) * (float)h;
+        if (fx >= 0.0f && fy >= 0.0f && fx < (float)w && fy < (float)h) q = (int)fy * (int)w + (int)fx;
+    }
+    GSurf gp;
+    gp.valid = false;
+    if (q >= 0) {
+        gp = load_g(Gp, (uint32_t)q);
+        float dprev = sqrtf(dot(g.p - pc.pos, g.p - pc.pos));
+        if (!gp.valid || dot(gp.ns, g.ns) < 0.9f || fabsf(dprev - gp.t) > 0.05f * gp.t) gp.valid = false;
+    }
+    if (!gp.valid) {
+        store_res(r_out, m_out, i, rc);
+        return;
+    }
+    Res rh = load_res(r_hist, m_hist, (uint32_t)q);
+    if (rh.M > m_cap * rc.M) rh.M = m_cap * rc.M;
+    Rng rr = rng_seed(initstate ^ kPhaseTemp, i);
+    Res s{0xFFFFFFFFu, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    float ph = 0.0f;
+    if (rc.e != 0xFFFFFFFFu && rc.W > 0.0f) {
+        float pc = target(S, g, rc.e, rc.b1, rc.b2, nullptr, nullptr, nullptr);
+        float pp = target(S, gp, rc.e, rc.b1, rc.b2, nullptr, nullptr, nullptr);
+        float den = rc.M * pc + rh.M * pp;
+        float m = den > 0.0f ? rc.M * pc / den : 0.0f;
+        if (res_update(s, rc.e, rc.b1, rc.b2, m * pc * rc.W, rr.uniform())) ph = pc;
+    }
+    if (rh.e != 0xFFFFFFFFu && rh.W > 0.0f && rh.e < S.n_emit) {
+        float pc = target(S, g, rh.e, rh.b1, rh.b2, nullptr, nullptr, nullptr);
+        float pp = target(S, gp, rh.e, rh.b1, rh.b2, nullptr, nullptr, nullptr);
+        float den = rc.M * pc + rh.M * pp;
+        float m = den > 0.0f ? rh.M * pp / den : 0.0f;
+        if (res_update(s, rh.e, rh.b1, rh.b2, m * pc * rh.W, rr.uniform())) ph = pc;
+    }
+    s.M = rc.M + rh.M;
+    s.W = ph > 0.0f ? s.wsum / ph : 0.0f;
+    store_res(r_out, m_out, i, s);
+}
+
+// Spatial reuse (ComputeAndWriteSpatialReservoirReuse, RayGen.rgen:470-506) with 1/Z weights.
+__global__ __launch_bounds__(kBlock) void k_spatial(SceneDev S, GView G, const float4* __restrict__ r_in,
+                                                    const float* __restrict__ m_in, float4* __restrict__ r_out,
+                                                    float* __restrict__ m_out, uint32_t w, uint32_t h, uint32_t k,
+                                                    float radius, uint64_t initstate) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w * h) return;
+    Res r0 = load_res(r_in, m_in, i);
+    GSurf g = load_g(G, i);
+    if (!g.valid || k == 0) {
+        store_res(r_out, m_out, i, r0);
+        return;
+    }
+    Rng rr = rng_seed(initstate ^ kPhaseSpat, i);
+    const int x = (int)(i % w), y = (int)(i / w);
+    // inputs: this pixel's reservoir and up to k accepted neighbours
+    uint32_t px[kMaxSpatial + 1];
+    px[0] = i;
+    int ni = 1;
+    if (k > kMaxSpatial) k = kMaxSpatial;
+    for (uint32_t j = 0; j < k; ++j) {
+        float ur = rr.uniform(), ua = rr.uniform();
+        float rad = radius * sqrtf(ur), ang = 2.0f * kPi * ua;
+        int nx = x + (int)lrintf(rad * cosf(ang)), ny = y + (int)lrintf(rad * sinf(ang));
+        if (nx < 0 || ny < 0 || nx >= (int)w || ny >= (int)h || (nx == x && ny == y)) continue;
+        uint32_t q = (uint32_t)ny * w + (uint32_t)nx;
+        GSurf gq = load_g(G, q);
+        if (!gq.valid || dot(gq.ns, g.ns) < 0.9f || fabsf(gq.t - g.t) > 0.1f * g.t) continue;
+        px[ni++] = q;
+    }
+    float Ms[kMaxSpatial + 1];
+    for (int l = 0; l < ni; ++l) Ms[l] = m_in[px[l]];
+    Res s{0xFFFFFFFFu, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    float ph = 0.0f;
+    s.M = 0.0f;
+    for (int l = 0; l < ni; ++l) {
+        s.M += Ms[l];
+        Res rl = load_res(r_in, m_in, px[l]);
+        float uw = rr.uniform();
+        if (rl.e == 0xFFFFFFFFu || !(rl.W > 0.0f)) continue;
+        // balance heuristic over the inputs' targets, each on its own surface
+        float p_self = 0.0f, p_l = 0.0f, den = 0.0f;
+        for (int j = 0; j < ni; ++j) {
+            GSurf gj = j == 0 ? g : load_g(G, px[j]);
+            float pj = target(S, gj, rl.e, rl.b1, rl.b2, nullptr, nullptr, nullptr);
+            if (j == 0) p_self = pj;
+            if (j == l) p_l = pj;
+            den += Ms[j] * pj;
+        }
+        float m = den > 0.0f ? Ms[l] * p_l / den : 0.0f;
+        if (res_update(s, rl.e, rl.b1, rl.b2, m * p_self * rl.W, uw)) ph = p_self;
+    }
+    s.W = ph > 0.0f ? s.wsum / ph : 0.0f;
+    store_res(r_out, m_out, i, s);
+}
+
+// CalculateRadianceFromReservoirs (RayGen.rgen:508-528): f Le G W, visibility by shadow ray.
+__global__ __launch_bounds__(kBlock) void k_rshade(SceneDev S, GView G, const float4* __restrict__ r_in,
+                                                   const float* __restrict__ m_in, ShadowQueue sq,
+                                                   uint32_t* __restrict__ counts, uint32_t n,
+                                                   unsigned long long* __restrict__ stats) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool want = false;
+    V3 c, wi, o;
+    float dist = 0.0f, tmax = 0.0f;
+    if (i < n) {
+        GSurf g = load_g(G, i);
+        if (g.valid) {
+            Res r = load_res(r_in, m_in, i);
+            if (r.e != 0xFFFFFFFFu && r.W > 0.0f) {
+                float p = target(S, g, r.e, r.b1, r.b2, &c, &wi, &dist);
+                if (p > 0.0f) {
+                    c = c * r.W;
+                    float am = fmaxf(fmaxf(fabsf(g.p.x), fabsf(g.p.y)), fabsf(g.p.z));
+                    float eps = 1e-4f * (1.0f + am);
+                    o = g.p + g.ng * eps;
+                    tmax = dist * (1.0f - 1e-4f) - eps;
+                    want = true;
+                }
+            }
+        }
+    }
+    uint32_t slot = wave_append(want, &counts[2], stats ? &stats[2] : nullptr);
+    if (want) {
+        sq.org[slot] = make_float4(o.x, o.y, o.z, 0.0f);
+        sq.dir[slot] = make_float4(wi.x, wi.y, wi.z, tmax);
+        sq.contrib[slot] = make_float4(c.x, c.y, c.z, __uint_as_float(i));
+    }
+}
+
+__global__ void k_counts3(uint32_t* c, uint32_t a, uint32_t b, uint32_t d) {
+    if (threadIdx.x == 0) {
+        c[0] = a;
+        c[1] = b;
+        c[2] = d;
+    }
+}
+
+}  // namespace
+
+#define HIPR(expr)                                                \
+    do {                                                          \
+        hipError_t e_ = (expr);                                   \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr);    \
+    } while (0)
+
+snkpt_status restir_pass(snkpt_ctx* ctx, const snkpt_camera* cam, const CamDev& cd, const snkpt_settings* s,
+                         uint64_t initstate, uint32_t pass, hipStream_t st) {
+    RenderState& R = *ctx->render;
+    if (!R.restir) R.restir = new RestirState();
+    RestirState& X = *R.restir;
+    const uint32_t w = R.w, h = R.h, n = w * h;
+    if (n > X.cap) {
+        for (int f = 0; f < 2; ++f)
+            for (int k = 0; k < 5; ++k) HIPR(X.g[f][k].reserve((size_t)n * 16));
+        HIPR(X.r_init.reserve((size_t)n * 16));
+        HIPR(X.r_tmp.reserve((size_t)n * 16));
+        HIPR(X.r_fin.reserve((size_t)n * 16));
+        HIPR(X.m_init.reserve((size_t)n * 4));
+        HIPR(X.m_tmp.reserve((size_t)n * 4));
+        HIPR(X.m_fin.reserve((size_t)n * 4));
+        X.cap = n;
+        X.have_prev = false;
+    }
+    if (!R.history_valid) X.have_prev = false;
+    SceneDev S = scene_dev(ctx);
+    auto gview = [&](int f) {
+        return GView{X.g[f][0].as<float4>(), X.g[f][1].as<float4>(), X.g[f][2].as<float4>(),
+                     X.g[f][3].as<float4>(), X.g[f][4].as<float4>()};
+    };
+    GView G = gview(X.cur), Gp = gview(X.cur ^ 1);
+    uint32_t* counts = R.counts.as<uint32_t>();
+    unsigned long long* stats = R.stats.as<unsigned long long>();
+    const uint32_t max_depth = s->max_depth ? s->max_depth : 1u;
+    // camera rays + primary hits
+    generate_camera(ctx, cd, initstate, st);
+    snkpt_hits hh{R.h_t.as<float>(), R.h_inst.as<uint32_t>(), R.h_prim.as<uint32_t>(), R.h_u.as<float>(),
+                  R.h_v.as<float>()};
+    snkpt_status e = launch_trace(ctx, R.q[0].org.as<float4>(), R.q[0].dir.as<float4>(), n, hh, false, nullptr,
+                                  nullptr, nullptr, st);
+    if (e != SNKPT_OK) return e;
+    k_counts3<<<1, 64, 0, st>>>(counts, n, 0u, 0u);
+    const uint32_t n_cand = s->restir_candidates ? s->restir_candidates : 32u;
+    k_primary<<<grid_for(n, kBlock), kBlock, 0, st>>>(S, G, queue_of(R, 0), hits_of(R), queue_of(R, 1), paths_of(R),
+                                                      R.rad.as<float4>(), X.r_init.as<float4>(), X.m_init.as<float>(),
+                                                      counts, n, n_cand, max_depth, s->rr_start, initstate, stats);
+    const bool temporal = X.have_prev && s->num_temporal_resamples != 0 && !cam->prev_frame_invalid;
+    const float4* r_sp_in = X.r_init.as<float4>();
+    const float* m_sp_in = X.m_init.as<float>();
+    if (temporal) {
+        PrevCam pc;
+        std::memcpy(pc.pv, X.prev_pv, sizeof(pc.pv));
+        pc.pos = V3{X.prev_pos[0], X.prev_pos[1], X.prev_pos[2]};
+        float m_cap = (float)(s->restir_m_cap ? s->restir_m_cap : 20u);
+        k_temporal<<<grid_for(n, kBlock), kBlock, 0, st>>>(S, G, Gp, X.r_init.as<float4>(), X.m_init.as<float>(),
+                                                           X.r_fin.as<float4>(), X.m_fin.as<float>(),
+                                                           X.r_tmp.as<float4>(), X.m_tmp.as<float>(), w, h, pc, m_cap,
+                                                           initstate);
+        r_sp_in = X.r_tmp.as<float4>();
+        m_sp_in = X.m_tmp.as<float>();
+    }
+    const uint32_t k = s->num_spatial_resamples > 0 ? (uint32_t)s->num_spatial_resamples : 0u;
+    k_spatial<<<grid_for(n, kBlock), kBlock, 0, st>>>(S, G, r_sp_in, m_sp_in, X.r_fin.as<float4>(),
+                                                      X.m_fin.as<float>(), w, h, k,
+                                                      s->restir_radius > 0.0f ? s->restir_radius : 30.0f, initstate);
+    k_rshade<<<grid_for(n, kBlock), kBlock, 0, st>>>(S, G, X.r_fin.as<float4>(), X.m_fin.as<float>(), shadow_of(R),
+                                                     counts, n, stats);
+    if ((e = launch_shadow(ctx, R, st)) != SNKPT_OK) return e;
+    // history for the next frame
+    std::memcpy(X.prev_pv, cam->proj_view, sizeof(X.prev_pv));
+    X.prev_pos[0] = cam->cam_pos[0];
+    X.prev_pos[1] = cam->cam_pos[1];
+    X.prev_pos[2] = cam->cam_pos[2];
+    X.cur ^= 1;
+    X.have_prev = true;
+    R.history_valid = true;
+    (void)pass;
+    // remaining bounces with the plain path tracer
+    if (max_depth > 1) return pt_bounces(ctx, s, 1, 1, st);
+    return SNKPT_OK;
 }
 
 }  // namespace snkpt
