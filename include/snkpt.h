@@ -244,6 +244,33 @@ snkpt_status snkpt_oracle_accumulate(snkpt_ctx* ctx, const snkpt_camera* cam,
                                      const snkpt_settings* settings, double* accum_rgb, uint32_t w,
                                      uint32_t h, uint32_t spp, snkpt_stream stream);
 
+/* ---- MLP radiance cache (no reference counterpart) -------------------------
+ * A training / query sample is 16 floats: position[3], direction[3] (towards the
+ * previous path vertex), normal[3], roughness, albedo[3], target radiance[3]
+ * (ignored for queries).  Sample and output pointers are device memory.  The
+ * cache is a 16-level hash grid (2 FP16 features, 2^19 entries per level) feeding
+ * a 64-wide MLP with 4 hidden ReLU layers and 3 outputs, run on MFMA in FP16
+ * with FP32 accumulation; Adam (beta 0.9 / 0.99, eps 1e-15). */
+#define SNKPT_CACHE_SAMPLE_FLOATS 16
+/* (Re)initialises weights, grid and Adam state.  lo3/hi3: box the positions are
+ * normalised by; NULL = the built scene's bounds. */
+snkpt_status snkpt_cache_reset(snkpt_ctx* ctx, uint32_t seed, const float* lo3, const float* hi3,
+                               snkpt_stream stream);
+/* out_rgba: n x 4 floats (radiance rgb, 0). */
+snkpt_status snkpt_cache_infer(snkpt_ctx* ctx, const float* samples, uint32_t n, float* out_rgba,
+                               snkpt_stream stream);
+/* One optimisation step on n samples (relative L2 loss).  apply_adam = 0 only
+ * computes the gradients (readable through snkpt_cache_params 2 and 3). */
+snkpt_status snkpt_cache_train(snkpt_ctx* ctx, const float* samples, uint32_t n, float lr, int apply_adam,
+                               float* out_loss, snkpt_stream stream);
+/* which: 0 MLP fp32 weights (17,408: W0..W3 64x64, W4 16x64 row-major [out][in], rows
+ * 3..15 of W4 zero), 1 grid fp32 features (16 x 2^19 x 2), 2 MLP gradient of the last
+ * train call, 3 grid gradient of the last train call made without Adam,
+ * 4 encoded inputs of the last call (fp16, n x 64). */
+snkpt_status snkpt_cache_params(snkpt_ctx* ctx, uint32_t which, void* host_dst, size_t bytes,
+                                size_t* out_bytes);
+snkpt_status snkpt_cache_set_params(snkpt_ctx* ctx, uint32_t which, const void* host_src, size_t bytes);
+
 /* ---- synthetic scenes (no reference counterpart: workload generators) ------
  * config 1..5 as described in the scene generator; variant picks one of the
  * scenes of a multi-scene config (config 5: 0..15) and is 0 otherwise.

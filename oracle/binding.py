@@ -43,6 +43,14 @@ def load():
     lib.o_pt_render.argtypes = [VP, U32, C.POINTER(OBvh), VP, U32, VP, VP, VP, VP, VP, U32, VP, VP, VP, U32, U32,
                                 U32, U32, U32, U32, U32, U32, VP, I]
     lib.o_pt_render.restype = I
+    lib.o_cache_geom.argtypes = [VP, VP, VP, VP, VP]
+    lib.o_grid_index.argtypes = [U32, U32, U32, U32, U32]
+    lib.o_grid_index.restype = U32
+    lib.o_cache_encode.argtypes = [VP, VP, VP, VP, VP, U32, VP]
+    lib.o_cache_infer.argtypes = [VP, VP, VP, VP, VP, VP, U32, VP]
+    lib.o_cache_grads.argtypes = [VP, VP, VP, VP, VP, VP, U32, VP, VP, VP, VP]
+    lib.o_cache_grads.restype = C.c_double
+    lib.o_adam.argtypes = [VP, VP, VP, VP, C.c_size_t, U32, C.c_double, C.c_double, C.c_double, C.c_double]
     _lib = lib
     return lib
 
@@ -199,3 +207,67 @@ def scene_arrays(scene):
     nrm = np.ctypeslib.as_array(s.nrm, (s.n_verts * 3,)).copy()
     em = np.ctypeslib.as_array(s.emissive_instances, (s.n_emissive,)).copy().astype(np.uint32)
     return pos, idx, inst, m4, mats, nrm, em
+
+
+GRID_PARAMS = 16 * (1 << 19) * 2
+MLP_PARAMS = 4 * 4096 + 16 * 64
+
+
+class CacheGeom:
+    def __init__(self, lo, hi):
+        self.lo_inv = np.zeros(4, np.float32)
+        self.res = np.zeros(16, np.uint32)
+        self.dense = np.zeros(16, np.uint32)
+        lo = np.ascontiguousarray(lo, np.float32)
+        hi = np.ascontiguousarray(hi, np.float32)
+        load().o_cache_geom(_p(lo), _p(hi), _p(self.lo_inv), _p(self.res), _p(self.dense))
+
+    def args(self):
+        return _p(self.lo_inv), _p(self.res), _p(self.dense)
+
+
+def grid_index(x, y, z, res, dense):
+    return load().o_grid_index(x, y, z, res, dense)
+
+
+def cache_encode(geom, grid, samples):
+    samples = np.ascontiguousarray(samples, np.float32)
+    n = samples.size // 16
+    out = np.zeros((n, 64), np.float64)
+    load().o_cache_encode(*geom.args(), _p(np.ascontiguousarray(grid, np.float32)), _p(samples), n, _p(out))
+    return out
+
+
+def cache_infer(geom, grid, W, samples):
+    samples = np.ascontiguousarray(samples, np.float32)
+    n = samples.size // 16
+    out = np.zeros((n, 3), np.float64)
+    load().o_cache_infer(*geom.args(), _p(np.ascontiguousarray(grid, np.float32)), _p(np.ascontiguousarray(W, np.float32)),
+                         _p(samples), n, _p(out))
+    return out
+
+
+def cache_grads(geom, grid, W, samples, want_grid=True, den=None, want_den=False):
+    """Mean relative-L2 loss and its gradients (stop-gradient through the
+    denominators).  den: frozen denominators (n, 3) for finite differences."""
+    samples = np.ascontiguousarray(samples, np.float32)
+    n = samples.size // 16
+    dW = np.zeros(MLP_PARAMS, np.float64)
+    dg = np.zeros(GRID_PARAMS, np.float64) if want_grid else None
+    dout = np.zeros((n, 3), np.float64) if want_den else None
+    din = None if den is None else np.ascontiguousarray(den, np.float64)
+    loss = load().o_cache_grads(*geom.args(), _p(np.ascontiguousarray(grid, np.float32)),
+                                _p(np.ascontiguousarray(W, np.float32)), _p(samples), n, _p(dW), _p(dg), _p(din),
+                                _p(dout))
+    if want_den:
+        return loss, dW, dg, dout
+    return loss, dW, dg
+
+
+def adam(p, m, v, g, t, lr, b1=0.9, b2=0.99, eps=1e-15):
+    p = np.ascontiguousarray(p, np.float32).copy()
+    m = np.ascontiguousarray(m, np.float32).copy()
+    v = np.ascontiguousarray(v, np.float32).copy()
+    g = np.ascontiguousarray(g, np.float64)
+    load().o_adam(_p(p), _p(m), _p(v), _p(g), p.size, t, lr, b1, b2, eps)
+    return p, m, v

@@ -93,6 +93,19 @@ int o_pt_render(const float* world, uint32_t ntris, const o_bvh* bvh, const uint
                 const float* cam_pos, uint32_t w, uint32_t h, uint32_t spp, uint32_t max_depth, uint32_t rr_start,
                 uint32_t seed, uint32_t frame, uint32_t seed_xor, float* out_rgb, int threads);
 
+/* ---- neural radiance cache (snkpt_oracle_mlp.c), FP64 ----------------------- */
+void o_cache_geom(const float lo[3], const float hi[3], float out_lo_inv[4], uint32_t res[16], uint32_t dense[16]);
+uint32_t o_grid_index(uint32_t x, uint32_t y, uint32_t z, uint32_t res, uint32_t dense);
+void o_cache_encode(const float* lo_inv, const uint32_t* res, const uint32_t* dense, const float* grid,
+                    const float* samples, uint32_t n, double* out);
+void o_cache_infer(const float* lo_inv, const uint32_t* res, const uint32_t* dense, const float* grid, const float* W,
+                   const float* samples, uint32_t n, double* out_y);
+double o_cache_grads(const float* lo_inv, const uint32_t* res, const uint32_t* dense, const float* grid,
+                     const float* W, const float* samples, uint32_t n, double* dW, double* dgrid,
+                     const double* den_in, double* den_out);
+void o_adam(float* p, float* m, float* v, const double* g, size_t n, uint32_t t, double lr, double b1, double b2,
+            double eps);
+
 #ifdef __cplusplus
 }
 #endif

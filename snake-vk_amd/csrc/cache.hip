@@ -1,13 +1,716 @@
-// MLP radiance cache (placeholder until the cache kernels land).
+// Neural radiance cache: hash-grid encoding, fused MLP forward / backward on
+// MFMA (v_mfma_f32_16x16x32_f16, FP16 in, FP32 accumulate), Adam, and the
+// cache render mode (paths terminate into the cache at a fixed depth while a
+// subset of paths trains it online).  No reference counterpart.
+//
+// Fragment layout of v_mfma_f32_16x16x32_f16 (checked on the device): lane l
+// holds A[l%16][8(l/16) .. 8(l/16)+7], B[8(l/16) .. +7][l%16] and
+// D[4(l/16) .. 4(l/16)+3][l%16].
+//
+// Determinism: MLP weight gradients are written as per-wave partials and summed
+// in a fixed order; hash-grid gradients are accumulated with 64-bit fixed-point
+// integer atomics (order independent).  Same seed -> same bits.
+#include <hip/hip_fp16.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "cache_nn.h"
 #include "render_modes.h"
 
 namespace snkpt {
 
-struct CacheState {};
+struct CacheState {
+    bool ready = false;
+    GridGeom geom{};
+    uint32_t step = 0;
+    DeviceBuffer grid32, grid16, grid_m, grid_v, grid_grad;  // grid_grad: int64 fixed point
+    DeviceBuffer mlp32, mlp16, mlp16t, mlp_m, mlp_v, mlp_gradf, grid_gradf;
+    DeviceBuffer partial, loss_part, x0, dx0, y;
+    uint32_t cap = 0;         // samples the work buffers hold
+    uint32_t last_n = 0;
+    // render integration
+    DeviceBuffer q_samples, q_meta, train_vtx, train_meta, t_samples, counts;
+    uint32_t rcap = 0;
+};
 void cache_state_free(CacheState* c) { delete c; }
 
+namespace {
+
+constexpr int kRow = 72;  // LDS row stride in halves (64 + 8: 144 B rows, 16 B aligned)
+constexpr int kWTRow4 = 32;  // W4^T row length (16 outputs padded to the MFMA K of 32)
+constexpr int kMlpTParams = kHidden * kLayerParams + kWidth * kWTRow4;  // 18432
+constexpr double kFixScale = 4294967296.0;  // 2^32 fixed point for grid gradients
+
+__device__ __forceinline__ f32x4_t mfma16(half8_t a, half8_t b, f32x4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+// ------------------------------------------------------------- encoding ----
+__device__ __forceinline__ void encode_one(const float* s, const GridGeom& G, const __half2* __restrict__ grid,
+                                           _Float16* out) {
+    float p[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) p[a] = fminf(fmaxf((s[a] - G.lo[a]) * G.inv_ext, 0.0f), 1.0f);
+    for (int l = 0; l < kGridLevels; ++l) {
+        const float r = (float)G.res[l];
+        float x = p[0] * r, y = p[1] * r, z = p[2] * r;
+        // cell of the point, clamped so that corner + 1 <= res (p may be exactly 1)
+        float fx = fminf(floorf(x), r - 1.0f), fy = fminf(floorf(y), r - 1.0f), fz = fminf(floorf(z), r - 1.0f);
+        uint32_t ix = (uint32_t)fx, iy = (uint32_t)fy, iz = (uint32_t)fz;
+        float wx = x - fx, wy = y - fy, wz = z - fz;
+        float f0 = 0.0f, f1 = 0.0f;
+        const __half2* g = grid + (size_t)l * kGridT;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            uint32_t cx = ix + (c & 1), cy = iy + ((c >> 1) & 1), cz = iz + (c >> 2);
+            float w = ((c & 1) ? wx : 1.0f - wx) * (((c >> 1) & 1) ? wy : 1.0f - wy) * ((c >> 2) ? wz : 1.0f - wz);
+            float2 v = __half22float2(g[grid_index(cx, cy, cz, G.res[l], G.dense[l])]);
+            f0 += w * v.x;
+            f1 += w * v.y;
+        }
+        out[2 * l] = (_Float16)f0;
+        out[2 * l + 1] = (_Float16)f1;
+    }
+    const float* d = s + 3;
+    const float* n = s + 6;
+    int k = 32;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) out[k++] = (_Float16)(0.5f * (d[a] + 1.0f));
+#pragma unroll
+    for (int a = 0; a < 3; ++a) out[k++] = (_Float16)(0.5f * (n[a] + 1.0f));
+    out[k++] = (_Float16)s[9];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) out[k++] = (_Float16)s[10 + a];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        out[k++] = (_Float16)sinf(kPi * d[a]);
+        out[k++] = (_Float16)cosf(kPi * d[a]);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        out[k++] = (_Float16)sinf(kPi * n[a]);
+        out[k++] = (_Float16)cosf(kPi * n[a]);
+    }
+    while (k < 64) out[k++] = (_Float16)0.0f;
+}
+
+__global__ void k_encode(const float* __restrict__ samples, uint32_t n, GridGeom G, const __half2* __restrict__ grid,
+                         _Float16* __restrict__ x0) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    _Float16 row[64];
+    encode_one(samples + (size_t)kSampleFloats * i, G, grid, row);
+    half8_t* o = (half8_t*)(x0 + (size_t)64 * i);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        half8_t v;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = row[8 * j + e];
+        o[j] = v;
+    }
+}
+
+// ------------------------------------------------------------ MLP core ----
+// act: [64][kRow] halves in LDS.  W: [out][64] row-major fp16 in global memory.
+__device__ __forceinline__ half8_t a_frag(const _Float16* act, int m, int kb, int lane) {
+    return *(const half8_t*)(act + (m * 16 + (lane & 15)) * kRow + kb * 32 + 8 * (lane >> 4));
+}
+__device__ __forceinline__ half8_t w_frag(const _Float16* __restrict__ W, int n, int kb, int lane, int row_len) {
+    return *(const half8_t*)(W + (n * 16 + (lane & 15)) * row_len + kb * 32 + 8 * (lane >> 4));
+}
+
+// Hidden layer: act_out = relu(act_in * W^T), 64 samples x 64 outputs.
+__device__ __forceinline__ void layer_hidden(const _Float16* act_in, const _Float16* __restrict__ W, _Float16* act_out,
+                                             int lane) {
+    f32x4_t acc[4][4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int n = 0; n < 4; ++n) acc[m][n] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+        half8_t b[4];
+#pragma unroll
+        for (int n = 0; n < 4; ++n) b[n] = w_frag(W, n, kb, lane, kWidth);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            half8_t a = a_frag(act_in, m, kb, lane);
+#pragma unroll
+            for (int n = 0; n < 4; ++n) acc[m][n] = mfma16(a, b[n], acc[m][n]);
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int n = 0; n < 4; ++n)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                int row = m * 16 + 4 * (lane >> 4) + e, col = n * 16 + (lane & 15);
+                act_out[row * kRow + col] = (_Float16)fmaxf(acc[m][n][e], 0.0f);
+            }
+}
+
+// Output layer: 64 samples x 16 (3 used) outputs, no activation.  acc[m][e] is
+// sample m*16 + 4(lane/16) + e, channel lane%16.
+__device__ __forceinline__ void layer_out(const _Float16* act_in, const _Float16* __restrict__ W4, f32x4_t acc[4],
+                                          int lane) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m) acc[m] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+        half8_t b = w_frag(W4, 0, kb, lane, kWidth);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) acc[m] = mfma16(a_frag(act_in, m, kb, lane), b, acc[m]);
+    }
+}
+
+__device__ __forceinline__ void load_x0_tile(const _Float16* __restrict__ x0, uint32_t base, uint32_t n,
+                                             _Float16* act, int lane) {
+    // 64 rows x 64 halves; lane copies row `lane`
+    uint32_t q = base + lane;
+    const half8_t* src = (const half8_t*)(x0 + (size_t)64 * q);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        half8_t v = q < n ? src[j] : half8_t{0, 0, 0, 0, 0, 0, 0, 0};
+        *(half8_t*)(act + lane * kRow + 8 * j) = v;
+    }
+}
+
+constexpr int kInferWaves = 4;
+
+// Inference: y[q] = MLP(x0[q]) for 64 samples per wave.
+__global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const _Float16* __restrict__ x0, uint32_t n,
+                                                                const _Float16* __restrict__ W,
+                                                                float4* __restrict__ y) {
+    __shared__ _Float16 buf[kInferWaves][2][64 * kRow];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t base = (blockIdx.x * kInferWaves + wv) * 64;  // waves past n run on zero rows
+    _Float16* a0 = buf[wv][0];
+    _Float16* a1 = buf[wv][1];
+    load_x0_tile(x0, base, n, a0, lane);
+    __syncthreads();
+    layer_hidden(a0, W + 0 * kLayerParams, a1, lane);
+    __syncthreads();
+    layer_hidden(a1, W + 1 * kLayerParams, a0, lane);
+    __syncthreads();
+    layer_hidden(a0, W + 2 * kLayerParams, a1, lane);
+    __syncthreads();
+    layer_hidden(a1, W + 3 * kLayerParams, a0, lane);
+    __syncthreads();
+    f32x4_t acc[4];
+    layer_out(a0, W + 4 * kLayerParams, acc, lane);
+    const int c = lane & 15;
+    // gather the three channels per sample through LDS (reuse a1 as fp32 scratch)
+    float* sc = (float*)a1;  // 64 x 4 floats = 1 KB
+    if (c < 4)
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) sc[(m * 16 + 4 * (lane >> 4) + e) * 4 + c] = c < kOut ? acc[m][e] : 0.0f;
+    __syncthreads();
+    uint32_t q = base + lane;
+    if (q < n) y[q] = make_float4(sc[lane * 4], sc[lane * 4 + 1], sc[lane * 4 + 2], 0.0f);
+}
+
+// dX tile (64 samples x 64 inputs) = dZ (64 x K) * W (K x 64), via W^T rows.
+// Optionally multiplied by the ReLU mask of `act_mask` (> 0) and stored as fp16.
+template <int KB, int NT>
+__device__ __forceinline__ void back_dx(const _Float16* g_in, const _Float16* __restrict__ WT, int wt_row,
+                                        f32x4_t acc[4][NT], int lane) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int n = 0; n < NT; ++n) acc[m][n] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+        half8_t b[NT];
+#pragma unroll
+        for (int n = 0; n < NT; ++n) b[n] = w_frag(WT, n, kb, lane, wt_row);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            half8_t a = a_frag(g_in, m, kb, lane);
+#pragma unroll
+            for (int n = 0; n < NT; ++n) acc[m][n] = mfma16(a, b[n], acc[m][n]);
+        }
+    }
+}
+
+// dW (MT*16 outputs x 64 inputs) = dZ^T (outputs x 64 samples) * X (64 samples x 64 inputs).
+template <int MT>
+__device__ __forceinline__ void back_dw(const _Float16* g, const _Float16* x, float* __restrict__ out, int lane) {
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+        f32x4_t acc[4];
+#pragma unroll
+        for (int n = 0; n < 4; ++n) acc[n] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb) {
+            half8_t a;
+            const int o = m * 16 + (lane & 15), q0 = kb * 32 + 8 * (lane >> 4);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) a[e] = g[(q0 + e) * kRow + o];
+#pragma unroll
+            for (int n = 0; n < 4; ++n) {
+                half8_t b;
+                const int i = n * 16 + (lane & 15);
+#pragma unroll
+                for (int e = 0; e < 8; ++e) b[e] = x[(q0 + e) * kRow + i];
+                acc[n] = mfma16(a, b, acc[n]);
+            }
+        }
+#pragma unroll
+        for (int n = 0; n < 4; ++n)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                int o = m * 16 + 4 * (lane >> 4) + e, i = n * 16 + (lane & 15);
+                out[o * kWidth + i] = acc[n][e];
+            }
+    }
+}
+
+// Training step, one wave per 64 samples: forward with stored activations,
+// relative-L2 loss, backward to the weights (partials per wave) and to the 32
+// grid features of each sample.
+__global__ __launch_bounds__(64) void k_mlp_train(const _Float16* __restrict__ x0, const float* __restrict__ samples,
+                                                  uint32_t n, const _Float16* __restrict__ W,
+                                                  const _Float16* __restrict__ WT, float grad_scale,
+                                                  float* __restrict__ partial, float* __restrict__ dx0,
+                                                  float* __restrict__ loss_part) {
+    __shared__ _Float16 act[5][64 * kRow];
+    __shared__ _Float16 gb[2][64 * kRow];
+    const int lane = threadIdx.x;
+    const uint32_t base = blockIdx.x * 64;
+    load_x0_tile(x0, base, n, act[0], lane);
+    __syncthreads();
+    for (int l = 0; l < kHidden; ++l) {
+        layer_hidden(act[l], W + l * kLayerParams, act[l + 1], lane);
+        __syncthreads();
+    }
+    f32x4_t yo[4];
+    layer_out(act[4], W + 4 * kLayerParams, yo, lane);
+    // loss and dZ4 (scaled) into gb[0]: columns 0..31 (3 used)
+    const int c = lane & 15;
+    float lsum = 0.0f;
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int row = m * 16 + 4 * (lane >> 4) + e;
+            uint32_t q = base + row;
+            float d = 0.0f;
+            if (c < kOut && q < n) {
+                float yv = yo[m][e];
+                float t = samples[(size_t)kSampleFloats * q + 13 + c];
+                float den = yv * yv + 0.01f;
+                float diff = yv - t;
+                lsum += diff * diff / den;
+                d = 2.0f * diff / den * grad_scale;
+            }
+            gb[0][row * kRow + c] = (_Float16)d;
+            gb[0][row * kRow + 16 + c] = (_Float16)0.0f;
+        }
+    for (int off = 32; off >= 1; off >>= 1) lsum += __shfl_xor(lsum, off, 64);
+    if (lane == 0) loss_part[blockIdx.x] = lsum;
+    __syncthreads();
+    float* P = partial + (size_t)blockIdx.x * kMlpParams;
+    // layer 4: dW4 = dZ4^T X4 (16 x 64); dX4 = dZ4 * W4 (K = 32 padded)
+    back_dw<1>(gb[0], act[4], P + 4 * kLayerParams, lane);
+    int gi = 0;
+    {
+        f32x4_t acc[4][4];
+        back_dx<1, 4>(gb[0], WT + 4 * kLayerParams, kWTRow4, acc, lane);
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int nn = 0; nn < 4; ++nn)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    int row = m * 16 + 4 * (lane >> 4) + e, col = nn * 16 + (lane & 15);
+                    float v = act[4][row * kRow + col] > (_Float16)0.0f ? acc[m][nn][e] : 0.0f;
+                    gb[1][row * kRow + col] = (_Float16)v;
+                }
+        gi = 1;
+    }
+    __syncthreads();
+    for (int l = kHidden - 1; l >= 0; --l) {
+        // gb[gi] holds dZ_l (64 x 64); dW_l = dZ_l^T X_l
+        back_dw<4>(gb[gi], act[l], P + l * kLayerParams, lane);
+        if (l > 0) {
+            f32x4_t acc[4][4];
+            back_dx<2, 4>(gb[gi], WT + l * kLayerParams, kWidth, acc, lane);
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int nn = 0; nn < 4; ++nn)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        int row = m * 16 + 4 * (lane >> 4) + e, col = nn * 16 + (lane & 15);
+                        float v = act[l][row * kRow + col] > (_Float16)0.0f ? acc[m][nn][e] : 0.0f;
+                        gb[gi ^ 1][row * kRow + col] = (_Float16)v;
+                    }
+            gi ^= 1;
+            __syncthreads();
+        } else {
+            // dX0 for the 32 grid features
+            f32x4_t acc[4][2];
+            back_dx<2, 2>(gb[gi], WT + 0 * kLayerParams, kWidth, acc, lane);
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int nn = 0; nn < 2; ++nn)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        int row = m * 16 + 4 * (lane >> 4) + e, col = nn * 16 + (lane & 15);
+                        uint32_t q = base + row;
+                        if (q < n) dx0[(size_t)32 * q + col] = acc[m][nn][e];
+                    }
+        }
+    }
+}
+
+// Scatter dL/dfeature to the 8 corners of each level (64-bit fixed point).
+__global__ void k_grid_bwd(const float* __restrict__ samples, const float* __restrict__ dx0, uint32_t n, GridGeom G,
+                           unsigned long long* __restrict__ grad) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* s = samples + (size_t)kSampleFloats * i;
+    float p[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) p[a] = fminf(fmaxf((s[a] - G.lo[a]) * G.inv_ext, 0.0f), 1.0f);
+    for (int l = 0; l < kGridLevels; ++l) {
+        float g0 = dx0[(size_t)32 * i + 2 * l], g1 = dx0[(size_t)32 * i + 2 * l + 1];
+        if (g0 == 0.0f && g1 == 0.0f) continue;
+        const float r = (float)G.res[l];
+        float x = p[0] * r, y = p[1] * r, z = p[2] * r;
+        // cell of the point, clamped so that corner + 1 <= res (p may be exactly 1)
+        float fx = fminf(floorf(x), r - 1.0f), fy = fminf(floorf(y), r - 1.0f), fz = fminf(floorf(z), r - 1.0f);
+        uint32_t ix = (uint32_t)fx, iy = (uint32_t)fy, iz = (uint32_t)fz;
+        float wx = x - fx, wy = y - fy, wz = z - fz;
+        for (int c = 0; c < 8; ++c) {
+            uint32_t cx = ix + (c & 1), cy = iy + ((c >> 1) & 1), cz = iz + (c >> 2);
+            float w = ((c & 1) ? wx : 1.0f - wx) * (((c >> 1) & 1) ? wy : 1.0f - wy) * ((c >> 2) ? wz : 1.0f - wz);
+            size_t e = ((size_t)l * kGridT + grid_index(cx, cy, cz, G.res[l], G.dense[l])) * 2;
+            long long v0 = (long long)llrint((double)(w * g0) * kFixScale);
+            long long v1 = (long long)llrint((double)(w * g1) * kFixScale);
+            if (v0) atomicAdd(&grad[e], (unsigned long long)v0);
+            if (v1) atomicAdd(&grad[e + 1], (unsigned long long)v1);
+        }
+    }
+}
+
+struct AdamP {
+    float lr_t;  // lr * sqrt(1 - b2^t) / (1 - b1^t)
+    float b1, b2, eps, inv_scale;
+    int apply;
+};
+
+// Sum the per-wave partials in a fixed order, then Adam.  Keeps fp32 masters,
+// the fp16 weights and the fp16 transposed weights in step.
+__global__ void k_mlp_adam(const float* __restrict__ partial, uint32_t nchunks, float* __restrict__ w32,
+                           float* __restrict__ m, float* __restrict__ v, _Float16* __restrict__ w16,
+                           _Float16* __restrict__ wt16, float* __restrict__ gout, AdamP A) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kMlpParams) return;
+    float g = 0.0f;
+    for (uint32_t c = 0; c < nchunks; ++c) g += partial[(size_t)c * kMlpParams + p];
+    g *= A.inv_scale;
+    gout[p] = g;
+    if (!A.apply) return;
+    float mm = A.b1 * m[p] + (1.0f - A.b1) * g;
+    float vv = A.b2 * v[p] + (1.0f - A.b2) * g * g;
+    m[p] = mm;
+    v[p] = vv;
+    float w = w32[p] - A.lr_t * mm / (sqrtf(vv) + A.eps);
+    w32[p] = w;
+    w16[p] = (_Float16)w;
+    int layer = p / kLayerParams, r = p % kLayerParams;
+    int o = r / kWidth, i = r % kWidth;
+    if (layer < kHidden)
+        wt16[layer * kLayerParams + i * kWidth + o] = (_Float16)w;
+    else if (o < kOutPad)
+        wt16[kHidden * kLayerParams + i * kWTRow4 + o] = (_Float16)w;
+}
+
+__global__ void k_grid_adam(unsigned long long* __restrict__ grad, float* __restrict__ w32, float* __restrict__ m,
+                            float* __restrict__ v, __half* __restrict__ w16, float* __restrict__ gout, size_t n,
+                            AdamP A) {
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    long long gi = (long long)grad[p];
+    grad[p] = 0ull;
+    float g = (float)((double)gi / kFixScale) * A.inv_scale;
+    if (gout) gout[p] = g;
+    if (!A.apply) return;
+    float mm = A.b1 * m[p] + (1.0f - A.b1) * g;
+    float vv = A.b2 * v[p] + (1.0f - A.b2) * g * g;
+    m[p] = mm;
+    v[p] = vv;
+    float w = w32[p] - A.lr_t * mm / (sqrtf(vv) + A.eps);
+    w32[p] = w;
+    w16[p] = __float2half(w);
+}
+
+__global__ void k_grid_init(float* __restrict__ w32, __half* __restrict__ w16, float* __restrict__ m,
+                            float* __restrict__ v, unsigned long long* __restrict__ grad, size_t n, uint64_t seed) {
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    Rng r = rng_seed(seed, p);
+    float w = (r.uniform() * 2.0f - 1.0f) * 1e-4f;
+    w32[p] = w;
+    w16[p] = __float2half(w);
+    m[p] = 0.0f;
+    v[p] = 0.0f;
+    grad[p] = 0ull;
+}
+
+__global__ void k_sync_grid16(const float* __restrict__ w32, __half* __restrict__ w16, size_t n) {
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) w16[p] = __float2half(w32[p]);
+}
+
+// fp16 and transposed fp16 copies of the fp32 MLP masters (the padding columns
+// of W4^T are cleared by the host once and never written).
+__global__ void k_sync_mlp16(const float* __restrict__ w32, _Float16* __restrict__ w16, _Float16* __restrict__ wt16) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kMlpParams) return;
+    float w = w32[p];
+    w16[p] = (_Float16)w;
+    int layer = p / kLayerParams, r = p % kLayerParams;
+    int o = r / kWidth, i = r % kWidth;
+    if (layer < kHidden)
+        wt16[layer * kLayerParams + i * kWidth + o] = (_Float16)w;
+    else
+        wt16[kHidden * kLayerParams + i * kWTRow4 + o] = (_Float16)w;
+}
+
+}  // namespace
+
+#define HIPC(expr)                                                \
+    do {                                                          \
+        hipError_t e_ = (expr);                                   \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr);    \
+    } while (0)
+
+static void make_geom(GridGeom& G, const float lo[3], const float hi[3]) {
+    float ext = 0.0f;
+    for (int a = 0; a < 3; ++a) {
+        G.lo[a] = lo[a];
+        ext = std::max(ext, hi[a] - lo[a]);
+    }
+    G.inv_ext = ext > 0.0f ? 1.0f / ext : 1.0f;
+    const double b = std::exp((std::log(2048.0) - std::log(16.0)) / (kGridLevels - 1));
+    for (int l = 0; l < kGridLevels; ++l) {
+        uint32_t r = (uint32_t)std::floor(16.0 * std::pow(b, (double)l) + 1e-6);
+        G.res[l] = r;
+        uint64_t cells = (uint64_t)(r + 1) * (r + 1) * (r + 1);
+        G.dense[l] = cells <= kGridT ? 1u : 0u;
+    }
+}
+
+// Xavier-uniform MLP initialisation from a seeded PCG32 stream (host side).
+static void init_mlp(std::vector<float>& w, uint32_t seed) {
+    w.assign(kMlpParams, 0.0f);
+    uint64_t state = 0, inc = ((uint64_t)seed << 1) | 1u;
+    auto next = [&]() {
+        uint64_t old = state;
+        state = old * 6364136223846793005ULL + inc;
+        uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u), rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((-rot) & 31u));
+    };
+    next();
+    state += 0x5EEDull;
+    next();
+    for (int l = 0; l <= kHidden; ++l) {
+        int outs = l < kHidden ? kWidth : kOut;
+        float lim = std::sqrt(6.0f / (float)(kWidth + (l < kHidden ? kWidth : kOut)));
+        for (int o = 0; o < outs; ++o)
+            for (int i = 0; i < kWidth; ++i)
+                w[l * kLayerParams + o * kWidth + i] = ((float)(next() >> 8) * (1.0f / 16777216.0f) * 2.0f - 1.0f) * lim;
+    }
+}
+
+snkpt_status cache_reset(snkpt_ctx* ctx, uint32_t seed, const float* lo, const float* hi, hipStream_t st) {
+    if (!ctx->cache) ctx->cache = new CacheState();
+    CacheState& C = *ctx->cache;
+    float l3[3], h3[3];
+    if (lo && hi) {
+        std::memcpy(l3, lo, 12);
+        std::memcpy(h3, hi, 12);
+    } else {
+        if (!ctx->accel.built) return fail(ctx, SNKPT_ERR_NOT_BUILT, "cache_reset: no bounds and no scene");
+        std::memcpy(l3, ctx->accel.lo, 12);
+        std::memcpy(h3, ctx->accel.hi, 12);
+    }
+    make_geom(C.geom, l3, h3);
+    HIPC(C.grid32.reserve(kGridParams * 4));
+    HIPC(C.grid16.reserve(kGridParams * 2));
+    HIPC(C.grid_m.reserve(kGridParams * 4));
+    HIPC(C.grid_v.reserve(kGridParams * 4));
+    HIPC(C.grid_grad.reserve(kGridParams * 8));
+    HIPC(C.mlp32.reserve(kMlpParams * 4));
+    HIPC(C.mlp16.reserve(kMlpParams * 2));
+    HIPC(C.mlp16t.reserve(kMlpTParams * 2));
+    HIPC(C.mlp_m.reserve(kMlpParams * 4));
+    HIPC(C.mlp_v.reserve(kMlpParams * 4));
+    HIPC(C.mlp_gradf.reserve(kMlpParams * 4));
+    k_grid_init<<<(uint32_t)((kGridParams + 255) / 256), 256, 0, st>>>(
+        C.grid32.as<float>(), C.grid16.as<__half>(), C.grid_m.as<float>(), C.grid_v.as<float>(),
+        C.grid_grad.as<unsigned long long>(), kGridParams, 0x6772696400000000ull ^ seed);
+    std::vector<float> w;
+    init_mlp(w, seed);
+    std::vector<float> zeros(kMlpParams, 0.0f);
+    HIPC(hipMemcpyAsync(C.mlp32.ptr, w.data(), kMlpParams * 4, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(C.mlp_m.ptr, zeros.data(), kMlpParams * 4, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync(C.mlp_v.ptr, zeros.data(), kMlpParams * 4, hipMemcpyHostToDevice, st));
+    {
+        std::vector<_Float16> zt(kMlpTParams, (_Float16)0.0f);
+        HIPC(hipMemcpyAsync(C.mlp16t.ptr, zt.data(), kMlpTParams * 2, hipMemcpyHostToDevice, st));
+        k_sync_mlp16<<<(kMlpParams + 255) / 256, 256, 0, st>>>(C.mlp32.as<float>(), C.mlp16.as<_Float16>(),
+                                                               C.mlp16t.as<_Float16>());
+    }
+    HIPC(hipStreamSynchronize(st));
+    C.step = 0;
+    C.ready = true;
+    return SNKPT_OK;
+}
+
+static snkpt_status ensure_work(snkpt_ctx* ctx, CacheState& C, uint32_t n) {
+    if (n <= C.cap) return SNKPT_OK;
+    uint32_t cap = ((n + 63) / 64) * 64;
+    HIPC(C.x0.reserve((size_t)cap * 128));
+    HIPC(C.dx0.reserve((size_t)cap * 128));
+    HIPC(C.y.reserve((size_t)cap * 16));
+    HIPC(C.partial.reserve((size_t)(cap / 64) * kMlpParams * 4));
+    HIPC(C.loss_part.reserve((size_t)(cap / 64) * 4));
+    C.cap = cap;
+    return SNKPT_OK;
+}
+
+snkpt_status cache_infer(snkpt_ctx* ctx, const float* samples, uint32_t n, float* out, hipStream_t st) {
+    CacheState* Cp = ctx->cache;
+    if (!Cp || !Cp->ready) return fail(ctx, SNKPT_ERR_NOT_BUILT, "cache_infer: call snkpt_cache_reset first");
+    CacheState& C = *Cp;
+    if (n == 0) return SNKPT_OK;
+    snkpt_status e = ensure_work(ctx, C, n);
+    if (e != SNKPT_OK) return e;
+    k_encode<<<grid_for(n, 128), 128, 0, st>>>(samples, n, C.geom, C.grid16.as<__half2>(), C.x0.as<_Float16>());
+    const uint32_t waves = (n + 63) / 64;
+    k_mlp_infer<<<(waves + kInferWaves - 1) / kInferWaves, 64 * kInferWaves, 0, st>>>(
+        C.x0.as<_Float16>(), n, C.mlp16.as<_Float16>(), (float4*)out);
+    C.last_n = n;
+    hipError_t le = hipGetLastError();
+    if (le != hipSuccess) return hip_fail(ctx, le, "cache_infer launch");
+    return SNKPT_OK;
+}
+
+snkpt_status cache_train(snkpt_ctx* ctx, const float* samples, uint32_t n, float lr, int apply, float* loss_out,
+                         hipStream_t st) {
+    CacheState* Cp = ctx->cache;
+    if (!Cp || !Cp->ready) return fail(ctx, SNKPT_ERR_NOT_BUILT, "cache_train: call snkpt_cache_reset first");
+    CacheState& C = *Cp;
+    if (n == 0) return SNKPT_OK;
+    snkpt_status e = ensure_work(ctx, C, n);
+    if (e != SNKPT_OK) return e;
+    const uint32_t chunks = (n + 63) / 64;
+    k_encode<<<grid_for(n, 128), 128, 0, st>>>(samples, n, C.geom, C.grid16.as<__half2>(), C.x0.as<_Float16>());
+    const float grad_scale = kLossScale / (3.0f * (float)n);
+    k_mlp_train<<<chunks, 64, 0, st>>>(C.x0.as<_Float16>(), samples, n, C.mlp16.as<_Float16>(),
+                                       C.mlp16t.as<_Float16>(), grad_scale, C.partial.as<float>(), C.dx0.as<float>(),
+                                       C.loss_part.as<float>());
+    k_grid_bwd<<<grid_for(n, 128), 128, 0, st>>>(samples, C.dx0.as<float>(), n, C.geom,
+                                                 C.grid_grad.as<unsigned long long>());
+    if (apply) C.step += 1;
+    const float b1 = 0.9f, b2 = 0.99f;
+    const uint32_t t = C.step ? C.step : 1u;
+    AdamP A;
+    A.b1 = b1;
+    A.b2 = b2;
+    A.eps = 1e-15f;
+    A.lr_t = (float)((double)lr * std::sqrt(1.0 - std::pow((double)b2, (double)t)) / (1.0 - std::pow((double)b1, (double)t)));
+    A.inv_scale = 1.0f / kLossScale;
+    A.apply = apply;
+    k_mlp_adam<<<(kMlpParams + 255) / 256, 256, 0, st>>>(C.partial.as<float>(), chunks, C.mlp32.as<float>(),
+                                                        C.mlp_m.as<float>(), C.mlp_v.as<float>(),
+                                                        C.mlp16.as<_Float16>(), C.mlp16t.as<_Float16>(),
+                                                        C.mlp_gradf.as<float>(), A);
+    // grid: the fixed-point gradient is consumed (and cleared) every call; without
+    // Adam it is kept as floats so it can be read back
+    float* gout = nullptr;
+    if (!apply) {
+        HIPC(C.grid_gradf.reserve(kGridParams * 4));
+        gout = C.grid_gradf.as<float>();
+    }
+    k_grid_adam<<<(uint32_t)((kGridParams + 255) / 256), 256, 0, st>>>(
+        C.grid_grad.as<unsigned long long>(), C.grid32.as<float>(), C.grid_m.as<float>(), C.grid_v.as<float>(),
+        C.grid16.as<__half>(), gout, kGridParams, A);
+    C.last_n = n;
+    if (loss_out) {
+        std::vector<float> lp(chunks);
+        HIPC(hipMemcpyAsync(lp.data(), C.loss_part.ptr, chunks * 4, hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
+        double s = 0;
+        for (float v : lp) s += v;
+        *loss_out = (float)(s / (3.0 * n));
+    }
+    hipError_t le = hipGetLastError();
+    if (le != hipSuccess) return hip_fail(ctx, le, "cache_train launch");
+    return SNKPT_OK;
+}
+
+snkpt_status cache_params(snkpt_ctx* ctx, uint32_t which, void* dst, size_t bytes, size_t* out_bytes, hipStream_t st) {
+    CacheState* Cp = ctx->cache;
+    if (!Cp || !Cp->ready) return fail(ctx, SNKPT_ERR_NOT_BUILT, "cache_params: call snkpt_cache_reset first");
+    CacheState& C = *Cp;
+    size_t sz = 0;
+    switch (which) {
+        case 0: sz = kMlpParams * 4; break;
+        case 1: sz = kGridParams * 4; break;
+        case 2: sz = kMlpParams * 4; break;
+        case 3: sz = kGridParams * 4; break;
+        case 4: sz = (size_t)C.last_n * 128; break;
+        default: return fail(ctx, SNKPT_ERR_INVALID_ARG, "cache_params: unknown array");
+    }
+    if (out_bytes) *out_bytes = sz;
+    if (!dst) return SNKPT_OK;
+    if (bytes < sz) return fail(ctx, SNKPT_ERR_INVALID_ARG, "cache_params: destination too small");
+    if (which == 3 && !C.grid_gradf.ptr) return fail(ctx, SNKPT_ERR_NOT_BUILT, "cache_params: no gradient-only step yet");
+    const void* src = which == 0   ? C.mlp32.ptr
+                      : which == 1 ? C.grid32.ptr
+                      : which == 2 ? C.mlp_gradf.ptr
+                      : which == 3 ? C.grid_gradf.ptr
+                                   : C.x0.ptr;
+    HIPC(hipMemcpyAsync(dst, src, sz, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    return SNKPT_OK;
+}
+
+snkpt_status cache_set_params(snkpt_ctx* ctx, uint32_t which, const void* src, size_t bytes, hipStream_t st) {
+    CacheState* Cp = ctx->cache;
+    if (!Cp || !Cp->ready) return fail(ctx, SNKPT_ERR_NOT_BUILT, "cache_set_params: call snkpt_cache_reset first");
+    CacheState& C = *Cp;
+    if (which == 0) {
+        if (bytes != kMlpParams * 4) return fail(ctx, SNKPT_ERR_INVALID_ARG, "cache_set_params: MLP size");
+        HIPC(hipMemcpyAsync(C.mlp32.ptr, src, bytes, hipMemcpyHostToDevice, st));
+        k_sync_mlp16<<<(kMlpParams + 255) / 256, 256, 0, st>>>(C.mlp32.as<float>(), C.mlp16.as<_Float16>(),
+                                                               C.mlp16t.as<_Float16>());
+    } else if (which == 1) {
+        if (bytes != kGridParams * 4) return fail(ctx, SNKPT_ERR_INVALID_ARG, "cache_set_params: grid size");
+        HIPC(hipMemcpyAsync(C.grid32.ptr, src, bytes, hipMemcpyHostToDevice, st));
+        k_sync_grid16<<<(uint32_t)((kGridParams + 255) / 256), 256, 0, st>>>(C.grid32.as<float>(), C.grid16.as<__half>(),
+                                                                              kGridParams);
+    } else {
+        return fail(ctx, SNKPT_ERR_INVALID_ARG, "cache_set_params: only 0 (MLP) and 1 (grid) can be set");
+    }
+    HIPC(hipStreamSynchronize(st));
+    return SNKPT_OK;
+}
+
 snkpt_status cache_pass(snkpt_ctx* ctx, const CamDev&, const snkpt_settings*, uint64_t, uint32_t, hipStream_t) {
-    return fail(ctx, SNKPT_ERR_NOT_BUILT, "cache mode: not available in this build");
+    return fail(ctx, SNKPT_ERR_NOT_BUILT, "cache mode: render integration not available in this build");
 }
 
 }  // namespace snkpt

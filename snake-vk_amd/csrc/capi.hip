@@ -280,6 +280,40 @@ snkpt_status snkpt_oracle_accumulate(snkpt_ctx* ctx, const snkpt_camera* cam, co
     return oracle_accumulate(ctx, cam, settings, accum_rgb, w, h, spp, (hipStream_t)stream);
 }
 
+snkpt_status snkpt_cache_reset(snkpt_ctx* ctx, uint32_t seed, const float* lo3, const float* hi3,
+                               snkpt_stream stream) {
+    if (!ctx) return SNKPT_ERR_INVALID_ARG;
+    if ((lo3 == nullptr) != (hi3 == nullptr)) return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_cache_reset: lo3/hi3");
+    if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, SNKPT_ERR_HIP, "hipSetDevice");
+    return cache_reset(ctx, seed, lo3, hi3, (hipStream_t)stream);
+}
+
+snkpt_status snkpt_cache_infer(snkpt_ctx* ctx, const float* samples, uint32_t n, float* out_rgba,
+                               snkpt_stream stream) {
+    if (!ctx) return SNKPT_ERR_INVALID_ARG;
+    if (n && (!samples || !out_rgba)) return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_cache_infer: null pointer");
+    if (n > (1u << 26)) return fail(ctx, SNKPT_ERR_CAPACITY, "snkpt_cache_infer: more than 2^26 samples");
+    return cache_infer(ctx, samples, n, out_rgba, (hipStream_t)stream);
+}
+
+snkpt_status snkpt_cache_train(snkpt_ctx* ctx, const float* samples, uint32_t n, float lr, int apply_adam,
+                               float* out_loss, snkpt_stream stream) {
+    if (!ctx) return SNKPT_ERR_INVALID_ARG;
+    if (n && !samples) return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_cache_train: null samples");
+    if (n > (1u << 22)) return fail(ctx, SNKPT_ERR_CAPACITY, "snkpt_cache_train: more than 2^22 samples per step");
+    return cache_train(ctx, samples, n, lr, apply_adam, out_loss, (hipStream_t)stream);
+}
+
+snkpt_status snkpt_cache_params(snkpt_ctx* ctx, uint32_t which, void* host_dst, size_t bytes, size_t* out_bytes) {
+    if (!ctx) return SNKPT_ERR_INVALID_ARG;
+    return cache_params(ctx, which, host_dst, bytes, out_bytes, nullptr);
+}
+
+snkpt_status snkpt_cache_set_params(snkpt_ctx* ctx, uint32_t which, const void* host_src, size_t bytes) {
+    if (!ctx || !host_src) return SNKPT_ERR_INVALID_ARG;
+    return cache_set_params(ctx, which, host_src, bytes, nullptr);
+}
+
 snkpt_status snkpt_scene_generate(uint32_t config, uint32_t variant, snkpt_scene** out) {
     return scene_generate(config, variant, out);
 }

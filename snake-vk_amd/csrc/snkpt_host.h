@@ -127,6 +127,14 @@ snkpt_status oracle_accumulate(snkpt_ctx* ctx, const snkpt_camera* cam, const sn
 void render_state_free(RenderState* r);
 void cache_state_free(CacheState* c);
 
+// cache.hip
+snkpt_status cache_reset(snkpt_ctx* ctx, uint32_t seed, const float* lo, const float* hi, hipStream_t st);
+snkpt_status cache_infer(snkpt_ctx* ctx, const float* samples, uint32_t n, float* out, hipStream_t st);
+snkpt_status cache_train(snkpt_ctx* ctx, const float* samples, uint32_t n, float lr, int apply, float* loss_out,
+                         hipStream_t st);
+snkpt_status cache_params(snkpt_ctx* ctx, uint32_t which, void* dst, size_t bytes, size_t* out_bytes, hipStream_t st);
+snkpt_status cache_set_params(snkpt_ctx* ctx, uint32_t which, const void* src, size_t bytes, hipStream_t st);
+
 // scenegen.cpp
 snkpt_status scene_generate(uint32_t config, uint32_t variant, snkpt_scene** out);
 void scene_free(snkpt_scene* s);

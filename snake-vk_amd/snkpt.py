@@ -43,8 +43,12 @@ EXPORTS = [
     "snkpt_set_materials", "snkpt_set_lights", "snkpt_build_accel", "snkpt_get_accel_info",
     "snkpt_accel_export", "snkpt_trace", "snkpt_trace_counted", "snkpt_render", "snkpt_get_frame_stats",
     "snkpt_oracle_accumulate", "snkpt_scene_generate", "snkpt_scene_free", "snkpt_scene_camera",
-    "snkpt_scene_load",
+    "snkpt_scene_load", "snkpt_cache_reset", "snkpt_cache_infer", "snkpt_cache_train", "snkpt_cache_params",
+    "snkpt_cache_set_params",
 ]
+CACHE_SAMPLE_FLOATS = 16
+CACHE_MLP_PARAMS = 4 * 4096 + 16 * 64
+CACHE_GRID_PARAMS = 16 * (1 << 19) * 2
 
 
 class Instance(C.Structure):
@@ -141,6 +145,11 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "snkpt_scene_free": (None, [P(Scene)]),
         "snkpt_scene_camera": (I32, [U32, U32, U32, U32, U32, P(Camera)]),
         "snkpt_scene_load": (I32, [VP, P(Scene)]),
+        "snkpt_cache_reset": (I32, [VP, U32, VP, VP, VP]),
+        "snkpt_cache_infer": (I32, [VP, VP, U32, VP, VP]),
+        "snkpt_cache_train": (I32, [VP, VP, U32, C.c_float, C.c_int, P(C.c_float), VP]),
+        "snkpt_cache_params": (I32, [VP, U32, VP, SZ, P(SZ)]),
+        "snkpt_cache_set_params": (I32, [VP, U32, VP, SZ]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -333,6 +342,48 @@ class Context:
         st = FrameStats()
         self._check(self.lib.snkpt_get_frame_stats(self._h, C.byref(st)), "snkpt_get_frame_stats")
         return st
+
+    # ---- MLP radiance cache
+    def cache_reset(self, seed: int = 1, lo=None, hi=None, stream=None):
+        import numpy as np
+        lo_a = None if lo is None else np.ascontiguousarray(lo, np.float32)
+        hi_a = None if hi is None else np.ascontiguousarray(hi, np.float32)
+        rc = self.lib.snkpt_cache_reset(self._h, seed, None if lo_a is None else lo_a.ctypes.data_as(C.c_void_p),
+                                        None if hi_a is None else hi_a.ctypes.data_as(C.c_void_p), _stream(stream))
+        self._check(rc, "snkpt_cache_reset")
+
+    def cache_infer(self, samples, stream=None):
+        import torch
+        n = samples.shape[0]
+        out = torch.empty(n, 4, dtype=torch.float32, device=samples.device)
+        self._check(self.lib.snkpt_cache_infer(self._h, _ptr(samples), n, _ptr(out), _stream(stream)),
+                    "snkpt_cache_infer")
+        return out
+
+    def cache_train(self, samples, lr: float = 1e-2, apply: bool = True, stream=None) -> float:
+        loss = C.c_float()
+        rc = self.lib.snkpt_cache_train(self._h, _ptr(samples), samples.shape[0], lr, int(apply), C.byref(loss),
+                                        _stream(stream))
+        self._check(rc, "snkpt_cache_train")
+        return loss.value
+
+    def cache_params(self, which: int):
+        import numpy as np
+        nb = C.c_size_t()
+        self._check(self.lib.snkpt_cache_params(self._h, which, None, 0, C.byref(nb)), "snkpt_cache_params")
+        if which == 4:
+            buf = np.zeros(max(nb.value, 2) // 2, np.float16)
+        else:
+            buf = np.zeros(max(nb.value, 4) // 4, np.float32)
+        self._check(self.lib.snkpt_cache_params(self._h, which, buf.ctypes.data_as(C.c_void_p), buf.nbytes,
+                                                C.byref(nb)), "snkpt_cache_params")
+        return buf
+
+    def cache_set_params(self, which: int, arr):
+        import numpy as np
+        a = np.ascontiguousarray(arr, np.float32)
+        self._check(self.lib.snkpt_cache_set_params(self._h, which, a.ctypes.data_as(C.c_void_p), a.nbytes),
+                    "snkpt_cache_set_params")
 
     def oracle_accumulate(self, cam: Camera, settings: Settings, accum, w: int, h: int, spp: int, stream=None):
         rc = self.lib.snkpt_oracle_accumulate(self._h, C.byref(cam), C.byref(settings), _ptr(accum), w, h, spp,
