@@ -123,7 +123,7 @@ typedef struct snkpt_settings {
     uint32_t restir_candidates;     /* RIS candidates per pixel (LIGHT_RIS_ITERS, RayGen.rgen:95) */
     uint32_t restir_m_cap;          /* temporal M cap (M_CAP, RayGen.rgen:97) */
     float restir_radius;            /* spatial radius in pixels (RayGen.rgen:488) */
-    uint32_t cache_query_depth;     /* cache mode: bounce at which the cache terminates paths */
+    uint32_t cache_query_depth;     /* cache mode: path vertex answered by the cache (1 = primary hit; 0 = 2) */
     uint32_t cache_train_paths;     /* cache mode: training paths per frame */
     float cache_lr;                 /* cache mode: Adam learning rate */
     uint32_t flags;                 /* SNKPT_FLAG_* */
@@ -266,7 +266,10 @@ snkpt_status snkpt_cache_train(snkpt_ctx* ctx, const float* samples, uint32_t n,
 /* which: 0 MLP fp32 weights (17,408: W0..W3 64x64, W4 16x64 row-major [out][in], rows
  * 3..15 of W4 zero), 1 grid fp32 features (16 x 2^19 x 2), 2 MLP gradient of the last
  * train call, 3 grid gradient of the last train call made without Adam,
- * 4 encoded inputs of the last call (fp16, n x 64). */
+ * 4 encoded inputs of the last call (fp16, n x 64); of the last cache-mode render
+ * pass: 5 the queries (16 floats each, target fields unused), 6 their records
+ * (float4: throughput rgb at the query vertex, path index as uint32 bits),
+ * 7 the training samples built from that pass's training paths. */
 snkpt_status snkpt_cache_params(snkpt_ctx* ctx, uint32_t which, void* host_dst, size_t bytes,
                                 size_t* out_bytes);
 snkpt_status snkpt_cache_set_params(snkpt_ctx* ctx, uint32_t which, const void* host_src, size_t bytes);

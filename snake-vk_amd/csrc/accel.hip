@@ -113,16 +113,6 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_down(const uint32_t* __rest
     }
 }
 
-// Exclusive scan of n u32 (n <= 4096 * 4096); `sums` needs ceil(n/4096) entries.
-// The grand total is written to *total (device) when total != nullptr.
-static void scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* sums, uint32_t* total,
-                           hipStream_t st) {
-    uint32_t nb = (n + kScanTile - 1) / kScanTile;
-    if (nb == 0) nb = 1;
-    k_scan_reduce<<<nb, kScanBlock, 0, st>>>(in, n, sums);
-    k_scan_top<<<1, kScanBlock, 0, st>>>(sums, nb, total);
-    k_scan_down<<<nb, kScanBlock, 0, st>>>(in, out, n, sums);
-}
 
 // ---------------------------------------------------------- radix sort -----
 constexpr int kSortBlock = 256;
@@ -666,6 +656,17 @@ __global__ void k_collapse_emit(BinView bv, uint32_t* __restrict__ work, uint32_
 inline uint32_t blocks(uint32_t n, uint32_t b) { return n == 0 ? 1 : (n + b - 1) / b; }
 
 }  // namespace
+
+// Exclusive scan of n u32 (n <= 4096 * 4096); `sums` needs ceil(n/4096) entries.
+// The grand total is written to *total (device) when total != nullptr.
+void scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* sums, uint32_t* total,
+                    hipStream_t st) {
+    uint32_t nb = (n + kScanTile - 1) / kScanTile;
+    if (nb == 0) nb = 1;
+    k_scan_reduce<<<nb, kScanBlock, 0, st>>>(in, n, sums);
+    k_scan_top<<<1, kScanBlock, 0, st>>>(sums, nb, total);
+    k_scan_down<<<nb, kScanBlock, 0, st>>>(in, out, n, sums);
+}
 
 #define HIP_CHECK(expr)                                          \
     do {                                                         \

@@ -29,7 +29,8 @@ struct CacheState {
     DeviceBuffer grid32, grid16, grid_m, grid_v, grid_grad;  // grid_grad: int64 fixed point
     DeviceBuffer mlp32, mlp16, mlp16t, mlp_m, mlp_v, mlp_gradf, grid_gradf;
     DeviceBuffer partial, loss_part, x0, dx0, y;
-    uint32_t cap = 0;         // samples the work buffers hold
+    uint32_t cap = 0;         // samples the inference work buffers hold
+    uint32_t tcap = 0;        // samples the training work buffers hold
     uint32_t last_n = 0;
     // render integration
     DeviceBuffer q_samples, q_meta, train_vtx, train_meta, t_samples, counts;
@@ -97,9 +98,10 @@ __device__ __forceinline__ void encode_one(const float* s, const GridGeom& G, co
     while (k < 64) out[k++] = (_Float16)0.0f;
 }
 
-__global__ void k_encode(const float* __restrict__ samples, uint32_t n, GridGeom G, const __half2* __restrict__ grid,
-                         _Float16* __restrict__ x0) {
+__global__ void k_encode(const float* __restrict__ samples, uint32_t n, const uint32_t* __restrict__ n_dev, GridGeom G,
+                         const __half2* __restrict__ grid, _Float16* __restrict__ x0) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n_dev) n = *n_dev;
     if (i >= n) return;
     _Float16 row[64];
     encode_one(samples + (size_t)kSampleFloats * i, G, grid, row);
@@ -183,10 +185,13 @@ constexpr int kInferWaves = 4;
 
 // Inference: y[q] = MLP(x0[q]) for 64 samples per wave.
 __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const _Float16* __restrict__ x0, uint32_t n,
+                                                                const uint32_t* __restrict__ n_dev,
                                                                 const _Float16* __restrict__ W,
                                                                 float4* __restrict__ y) {
     __shared__ _Float16 buf[kInferWaves][2][64 * kRow];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (n_dev) n = *n_dev;
+    if (blockIdx.x * kInferWaves * 64 >= n) return;  // the whole block is past the end
     const uint32_t base = (blockIdx.x * kInferWaves + wv) * 64;  // waves past n run on zero rows
     _Float16* a0 = buf[wv][0];
     _Float16* a1 = buf[wv][1];
@@ -578,15 +583,22 @@ snkpt_status cache_reset(snkpt_ctx* ctx, uint32_t seed, const float* lo, const f
     return SNKPT_OK;
 }
 
-static snkpt_status ensure_work(snkpt_ctx* ctx, CacheState& C, uint32_t n) {
-    if (n <= C.cap) return SNKPT_OK;
-    uint32_t cap = ((n + 63) / 64) * 64;
-    HIPC(C.x0.reserve((size_t)cap * 128));
-    HIPC(C.dx0.reserve((size_t)cap * 128));
-    HIPC(C.y.reserve((size_t)cap * 16));
-    HIPC(C.partial.reserve((size_t)(cap / 64) * kMlpParams * 4));
-    HIPC(C.loss_part.reserve((size_t)(cap / 64) * 4));
-    C.cap = cap;
+// Work buffers for n samples: encodings and outputs for inference; with `train`
+// also the backward buffers (per-wave weight-gradient partials: 68 KiB per 64
+// samples, so they are sized by the training batch only).
+static snkpt_status ensure_work(snkpt_ctx* ctx, CacheState& C, uint32_t n, bool train) {
+    const uint32_t cap = ((n + 63) / 64) * 64;
+    if (cap > C.cap) {
+        HIPC(C.x0.reserve((size_t)cap * 128));
+        HIPC(C.y.reserve((size_t)cap * 16));
+        C.cap = cap;
+    }
+    if (train && cap > C.tcap) {
+        HIPC(C.dx0.reserve((size_t)cap * 128));
+        HIPC(C.partial.reserve((size_t)(cap / 64) * kMlpParams * 4));
+        HIPC(C.loss_part.reserve((size_t)(cap / 64) * 4));
+        C.tcap = cap;
+    }
     return SNKPT_OK;
 }
 
@@ -595,12 +607,13 @@ snkpt_status cache_infer(snkpt_ctx* ctx, const float* samples, uint32_t n, float
     if (!Cp || !Cp->ready) return fail(ctx, SNKPT_ERR_NOT_BUILT, "cache_infer: call snkpt_cache_reset first");
     CacheState& C = *Cp;
     if (n == 0) return SNKPT_OK;
-    snkpt_status e = ensure_work(ctx, C, n);
+    snkpt_status e = ensure_work(ctx, C, n, false);
     if (e != SNKPT_OK) return e;
-    k_encode<<<grid_for(n, 128), 128, 0, st>>>(samples, n, C.geom, C.grid16.as<__half2>(), C.x0.as<_Float16>());
+    k_encode<<<grid_for(n, 128), 128, 0, st>>>(samples, n, nullptr, C.geom, C.grid16.as<__half2>(),
+                                               C.x0.as<_Float16>());
     const uint32_t waves = (n + 63) / 64;
     k_mlp_infer<<<(waves + kInferWaves - 1) / kInferWaves, 64 * kInferWaves, 0, st>>>(
-        C.x0.as<_Float16>(), n, C.mlp16.as<_Float16>(), (float4*)out);
+        C.x0.as<_Float16>(), n, nullptr, C.mlp16.as<_Float16>(), (float4*)out);
     C.last_n = n;
     hipError_t le = hipGetLastError();
     if (le != hipSuccess) return hip_fail(ctx, le, "cache_infer launch");
@@ -613,10 +626,11 @@ snkpt_status cache_train(snkpt_ctx* ctx, const float* samples, uint32_t n, float
     if (!Cp || !Cp->ready) return fail(ctx, SNKPT_ERR_NOT_BUILT, "cache_train: call snkpt_cache_reset first");
     CacheState& C = *Cp;
     if (n == 0) return SNKPT_OK;
-    snkpt_status e = ensure_work(ctx, C, n);
+    snkpt_status e = ensure_work(ctx, C, n, true);
     if (e != SNKPT_OK) return e;
     const uint32_t chunks = (n + 63) / 64;
-    k_encode<<<grid_for(n, 128), 128, 0, st>>>(samples, n, C.geom, C.grid16.as<__half2>(), C.x0.as<_Float16>());
+    k_encode<<<grid_for(n, 128), 128, 0, st>>>(samples, n, nullptr, C.geom, C.grid16.as<__half2>(),
+                                               C.x0.as<_Float16>());
     const float grad_scale = kLossScale / (3.0f * (float)n);
     k_mlp_train<<<chunks, 64, 0, st>>>(C.x0.as<_Float16>(), samples, n, C.mlp16.as<_Float16>(),
                                        C.mlp16t.as<_Float16>(), grad_scale, C.partial.as<float>(), C.dx0.as<float>(),
@@ -672,6 +686,18 @@ snkpt_status cache_params(snkpt_ctx* ctx, uint32_t which, void* dst, size_t byte
         case 2: sz = kMlpParams * 4; break;
         case 3: sz = kGridParams * 4; break;
         case 4: sz = (size_t)C.last_n * 128; break;
+        case 5:
+        case 6:
+        case 7: {
+            // query / training records of the last cache-mode pass; counts live on the device
+            uint32_t cnt[2] = {0, 0};
+            if (C.counts.ptr) {
+                HIPC(hipMemcpyAsync(cnt, C.counts.ptr, 8, hipMemcpyDeviceToHost, st));
+                HIPC(hipStreamSynchronize(st));
+            }
+            sz = which == 7 ? (size_t)cnt[1] * 64 : (size_t)cnt[0] * (which == 5 ? 64 : 16);
+            break;
+        }
         default: return fail(ctx, SNKPT_ERR_INVALID_ARG, "cache_params: unknown array");
     }
     if (out_bytes) *out_bytes = sz;
@@ -682,7 +708,11 @@ snkpt_status cache_params(snkpt_ctx* ctx, uint32_t which, void* dst, size_t byte
                       : which == 1 ? C.grid32.ptr
                       : which == 2 ? C.mlp_gradf.ptr
                       : which == 3 ? C.grid_gradf.ptr
-                                   : C.x0.ptr;
+                      : which == 4 ? C.x0.ptr
+                      : which == 5 ? C.q_samples.ptr
+                      : which == 6 ? C.q_meta.ptr
+                                   : C.t_samples.ptr;
+    if (sz == 0) return SNKPT_OK;
     HIPC(hipMemcpyAsync(dst, src, sz, hipMemcpyDeviceToHost, st));
     HIPC(hipStreamSynchronize(st));
     return SNKPT_OK;
@@ -709,8 +739,167 @@ snkpt_status cache_set_params(snkpt_ctx* ctx, uint32_t which, const void* src, s
     return SNKPT_OK;
 }
 
-snkpt_status cache_pass(snkpt_ctx* ctx, const CamDev&, const snkpt_settings*, uint64_t, uint32_t, hipStream_t) {
-    return fail(ctx, SNKPT_ERR_NOT_BUILT, "cache mode: render integration not available in this build");
+namespace {
+
+// rad[path] += throughput * max(cache output, 0) for every query of this pass.
+__global__ void k_cache_apply(const float4* __restrict__ meta, const float4* __restrict__ y,
+                              const uint32_t* __restrict__ count, float4* __restrict__ rad) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *count) return;
+    float4 m = meta[i], v = y[i];
+    uint32_t path = __float_as_uint(m.w);
+    float4 r = rad[path];
+    rad[path] = make_float4(r.x + m.x * fmaxf(v.x, 0.0f), r.y + m.y * fmaxf(v.y, 0.0f), r.z + m.z * fmaxf(v.z, 0.0f),
+                            r.w);
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+    if (threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
+__global__ void k_fill4(float4* p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// One training sample per training path: a uniformly chosen recorded vertex,
+// target = (radiance of the whole path - radiance before the vertex) / throughput
+// at the vertex = an unbiased estimate of L_o(x, wo) there.  Written at the
+// path's own slot with a keep flag; k_train_compact packs them in path order
+// so that the batch (and the gradient sums over it) do not depend on timing.
+__global__ void k_train_targets(const float* __restrict__ rec, const uint32_t* __restrict__ nrec, uint32_t max_rec,
+                                const float4* __restrict__ rad, uint32_t n, uint64_t seed, float* __restrict__ stage,
+                                uint32_t* __restrict__ keep) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    bool want = false;
+    float smp[16];
+    {
+        uint32_t k = nrec[j];
+        if (k > 0) {
+            Rng r = rng_seed(seed, j);
+            uint32_t v = (uint32_t)(r.uniform() * (float)k);
+            if (v >= k) v = k - 1;
+            const float* e = rec + ((size_t)j * max_rec + v) * kRecFloats;
+            float4 L = rad[j];
+            float b[3] = {e[16], e[17], e[18]};
+            float lt[3] = {L.x - e[13], L.y - e[14], L.z - e[15]};
+            want = b[0] > 1e-6f && b[1] > 1e-6f && b[2] > 1e-6f;
+#pragma unroll
+            for (int c = 0; c < 13; ++c) smp[c] = e[c];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float t = fmaxf(lt[c], 0.0f) / fmaxf(b[c], 1e-6f);
+                smp[13 + c] = fminf(t, 1000.0f);
+                want = want && __builtin_isfinite(t);
+            }
+        }
+    }
+    keep[j] = want ? 1u : 0u;
+    if (want) {
+        float4* o = (float4*)(stage + (size_t)j * 16);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) o[c] = make_float4(smp[4 * c], smp[4 * c + 1], smp[4 * c + 2], smp[4 * c + 3]);
+    }
+}
+
+__global__ void k_train_compact(const float* __restrict__ stage, const uint32_t* __restrict__ keep,
+                                const uint32_t* __restrict__ off, uint32_t n, float* __restrict__ out,
+                                const uint32_t* __restrict__ total, unsigned long long* __restrict__ stat) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j == 0) atomicAdd(stat, (unsigned long long)*total);
+    if (j >= n || !keep[j]) return;
+    const float4* src = (const float4*)(stage + (size_t)j * 16);
+    float4* dst = (float4*)(out + (size_t)off[j] * 16);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) dst[c] = src[c];
+}
+
+constexpr uint64_t kTrainPaths = 0x545241494eull;
+
+}  // namespace
+
+snkpt_status cache_pass(snkpt_ctx* ctx, const CamDev& cd, const snkpt_settings* s, uint64_t initstate, uint32_t pass,
+                        hipStream_t st) {
+    RenderState& R = *ctx->render;
+    const uint32_t n = R.w * R.h;
+    if (!ctx->cache || !ctx->cache->ready) {
+        snkpt_status e = cache_reset(ctx, s->seed, nullptr, nullptr, st);
+        if (e != SNKPT_OK) return e;
+    }
+    CacheState& C = *ctx->cache;
+    const uint32_t depth = s->max_depth ? s->max_depth : 1u;
+    // the Q-th path vertex (1 = primary hit) is answered by the cache
+    const uint32_t Q = s->cache_query_depth ? s->cache_query_depth : 2u;
+    const uint32_t NT = std::min<uint32_t>(s->cache_train_paths ? s->cache_train_paths : 65536u, n);
+    if (n > C.rcap) {
+        HIPC(C.q_samples.reserve((size_t)n * 64));
+        HIPC(C.q_meta.reserve((size_t)n * 16));
+        C.rcap = n;
+    }
+    HIPC(C.counts.reserve(64));
+    HIPC(C.train_vtx.reserve((size_t)NT * depth * kRecFloats * 4));
+    // train_meta: nrec | keep | offsets (u32 each) | scan sums | radiance (float4) | staged samples (16 floats)
+    const size_t u32_bytes = ((size_t)NT * 4 + 255) & ~(size_t)255;
+    const size_t sums_bytes = (((size_t)(NT + 4095) / 4096) * 4 + 255) & ~(size_t)255;
+    const size_t trad_off = 3 * u32_bytes + sums_bytes;
+    const size_t stage_off = trad_off + (size_t)NT * 16;
+    HIPC(C.train_meta.reserve(stage_off + (size_t)NT * 64));
+    HIPC(C.t_samples.reserve((size_t)NT * 64));
+    snkpt_status e = ensure_work(ctx, C, n, false);
+    if (e != SNKPT_OK) return e;
+    uint32_t* qc = C.counts.as<uint32_t>();
+    unsigned long long* stats = R.stats.as<unsigned long long>();
+    // 1. render paths, terminated into the cache at vertex Q
+    generate_camera(ctx, cd, initstate, st);
+    k_zero_u32<<<1, 64, 0, st>>>(qc, 4);
+    ShadeHooks H;
+    H.kind = 1;
+    H.query_at = Q - 1;  // bounce index of that vertex
+    H.q_samples = C.q_samples.as<float>();
+    H.q_meta = C.q_meta.as<float4>();
+    H.q_count = qc;
+    // with depth >= Q the vertex before the query keeps its continuation (MIS
+    // against the cache vertex's emission); with depth < Q this is plain PT
+    if ((e = pt_bounces(ctx, s, 0, 0, st, &H, std::min(depth, Q))) != SNKPT_OK) return e;
+    // 2. query the cache
+    k_encode<<<grid_for(n, 128), 128, 0, st>>>(C.q_samples.as<float>(), n, qc, C.geom, C.grid16.as<__half2>(),
+                                               C.x0.as<_Float16>());
+    const uint32_t waves = (n + 63) / 64;
+    k_mlp_infer<<<(waves + kInferWaves - 1) / kInferWaves, 64 * kInferWaves, 0, st>>>(
+        C.x0.as<_Float16>(), n, qc, C.mlp16.as<_Float16>(), C.y.as<float4>());
+    k_cache_apply<<<grid_for(n, 256), 256, 0, st>>>(C.q_meta.as<float4>(), C.y.as<float4>(), qc, R.rad.as<float4>());
+    if (s->flags & SNKPT_FLAG_NO_TRAIN) return SNKPT_OK;
+    // 3. training paths: full-length unbiased paths through a strided pixel subset
+    uint32_t* nrec = C.train_meta.as<uint32_t>();
+    float4* trad = (float4*)(C.train_meta.as<uint8_t>() + trad_off);
+    const uint32_t stride = std::max<uint32_t>(n / NT, 1u);
+    const uint32_t start = (uint32_t)((initstate >> 17) % stride);
+    k_fill4<<<grid_for(NT, 256), 256, 0, st>>>(trad, NT);
+    generate_camera(ctx, cd, initstate ^ kTrainPaths, st, NT, start, stride, nrec);
+    ShadeHooks T;
+    T.kind = 2;
+    T.rec = C.train_vtx.as<float>();
+    T.nrec = nrec;
+    T.max_rec = depth;
+    T.rad = trad;
+    if ((e = pt_bounces(ctx, s, 0, 0, st, &T)) != SNKPT_OK) return e;
+    uint8_t* tm = C.train_meta.as<uint8_t>();
+    uint32_t* keep = (uint32_t*)(tm + u32_bytes);
+    uint32_t* off = (uint32_t*)(tm + 2 * u32_bytes);
+    uint32_t* sums = (uint32_t*)(tm + 3 * u32_bytes);
+    float* stage = (float*)(tm + stage_off);
+    k_train_targets<<<grid_for(NT, 256), 256, 0, st>>>(C.train_vtx.as<float>(), nrec, depth, trad, NT,
+                                                        initstate ^ 0x7461726765ull, stage, keep);
+    scan_exclusive(keep, off, NT, sums, qc + 1, st);
+    k_train_compact<<<grid_for(NT, 256), 256, 0, st>>>(stage, keep, off, NT, C.t_samples.as<float>(), qc + 1,
+                                                        stats + 4);
+    uint32_t nt = 0;
+    HIPC(hipMemcpyAsync(&nt, qc + 1, 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    (void)pass;
+    if (nt == 0) return SNKPT_OK;
+    return cache_train(ctx, C.t_samples.as<float>(), nt, s->cache_lr > 0.0f ? s->cache_lr : 1e-2f, 1, nullptr, st);
 }
 
 }  // namespace snkpt

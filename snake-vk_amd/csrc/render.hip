@@ -42,14 +42,16 @@ __global__ void k_set_counts(uint32_t* c, uint32_t i0, uint32_t v0, uint32_t i1,
     }
 }
 
-__global__ void k_generate(CamDev cam, uint32_t w, uint32_t h, uint64_t initstate, float4* __restrict__ q_org,
-                           float4* __restrict__ q_dir, uint32_t* __restrict__ q_path, PathState ps,
-                           uint32_t* __restrict__ counts, uint32_t cur) {
+__global__ void k_generate(CamDev cam, uint32_t w, uint32_t h, uint32_t n, uint32_t start, uint32_t stride,
+                           uint64_t initstate, float4* __restrict__ q_org, float4* __restrict__ q_dir,
+                           uint32_t* __restrict__ q_path, PathState ps, uint32_t* __restrict__ counts, uint32_t cur,
+                           uint32_t* __restrict__ nrec) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t n = w * h;
     if (i == 0) counts[cur] = n;
     if (i >= n) return;
-    V3 d = camera_dir(cam, i % w, i / w, w, h);
+    uint32_t px = (uint32_t)(((uint64_t)start + (uint64_t)i * stride) % ((uint64_t)w * h));
+    if (nrec) nrec[i] = 0;
+    V3 d = camera_dir(cam, px % w, px / w, w, h);
     q_org[i] = make_float4(cam.pos.x, cam.pos.y, cam.pos.z, 0.0f);
     q_dir[i] = make_float4(d.x, d.y, d.z, 1e30f);
     q_path[i] = i;
@@ -59,14 +61,16 @@ __global__ void k_generate(CamDev cam, uint32_t w, uint32_t h, uint64_t initstat
     ps.bounce[i] = 0;
 }
 
-// One path vertex of the unbiased path tracer.
+// One path vertex of the unbiased path tracer (KIND: see ShadeHooks).
+template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PtParams P, Queue in, HitArrays hit, Queue out,
                                                   ShadowQueue sq, PathState ps, float4* __restrict__ rad,
                                                   uint32_t* __restrict__ counts, uint32_t cur, uint32_t nxt,
-                                                  unsigned long long* __restrict__ stats) {
+                                                  unsigned long long* __restrict__ stats, ShadeHooks H) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < counts[cur];
-    bool want_ext = false, want_sh = false;
+    bool want_ext = false, want_sh = false, want_q = false;
+    float qs[13];
     V3 ext_o, ext_d, sh_o, sh_d, sh_c;
     float sh_t = 0.0f;
     uint32_t path = 0;
@@ -96,8 +100,27 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PtParams P, Queue 
                 V3 c = mul(thr, s.le) * w;
                 float4 r = rad[path];
                 rad[path] = make_float4(r.x + c.x, r.y + c.y, r.z + c.z, r.w);
+            } else if (KIND == 1 && bounce == H.query_at) {
+                // terminate into the radiance cache: L_o(x, wo) from the network
+                V3 wo = dir * -1.0f;
+                want_q = true;
+                float q[13] = {s.p.x, s.p.y, s.p.z, wo.x, wo.y, wo.z, s.ns.x, s.ns.y, s.ns.z, s.m.roughness,
+                               s.m.albedo.x, s.m.albedo.y, s.m.albedo.z};
+#pragma unroll
+                for (int k = 0; k < 13; ++k) qs[k] = q[k];
             } else {
                 V3 wo = dir * -1.0f;
+                if (KIND == 2 && bounce < H.max_rec) {
+                    // training record: sample fields, radiance so far, throughput at arrival
+                    float4 r = rad[path];
+                    float* o = H.rec + ((size_t)path * H.max_rec + bounce) * kRecFloats;
+                    float q[kRecFloats] = {s.p.x, s.p.y, s.p.z, wo.x, wo.y, wo.z, s.ns.x, s.ns.y, s.ns.z,
+                                           s.m.roughness, s.m.albedo.x, s.m.albedo.y, s.m.albedo.z, r.x, r.y, r.z,
+                                           thr.x, thr.y, thr.z, 0.0f};
+#pragma unroll
+                    for (int k = 0; k < kRecFloats; ++k) o[k] = q[k];
+                    H.nrec[path] = bounce + 1;
+                }
                 float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform(), u3 = rng.uniform();
                 float ul = rng.uniform(), us1 = rng.uniform(), us2 = rng.uniform(), urr = rng.uniform();
                 const bool last = bounce + 1 >= P.max_depth;
@@ -159,6 +182,16 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PtParams P, Queue 
         sq.org[sslot] = make_float4(sh_o.x, sh_o.y, sh_o.z, 0.0f);
         sq.dir[sslot] = make_float4(sh_d.x, sh_d.y, sh_d.z, sh_t);
         sq.contrib[sslot] = make_float4(sh_c.x, sh_c.y, sh_c.z, __uint_as_float(path));
+    }
+    if (KIND == 1) {
+        uint32_t qslot = wave_append(want_q, H.q_count, stats ? &stats[3] : nullptr);
+        if (want_q) {
+            float* o = H.q_samples + (size_t)qslot * 16;
+#pragma unroll
+            for (int k = 0; k < 13; ++k) o[k] = qs[k];
+            o[13] = o[14] = o[15] = 0.0f;
+            H.q_meta[qslot] = make_float4(thr4.x, thr4.y, thr4.z, __uint_as_float(path));
+        }
     }
 }
 
@@ -362,20 +395,20 @@ PathState paths_of(RenderState& R) {
     return PathState{R.p_thr.as<float4>(), R.p_rng.as<uint2>(), R.p_bounce.as<uint32_t>()};
 }
 
-snkpt_status launch_shadow(snkpt_ctx* ctx, RenderState& R, hipStream_t st) {
+snkpt_status launch_shadow(snkpt_ctx* ctx, RenderState& R, float4* rad, hipStream_t st) {
     const Accel& A = ctx->accel;
     if (A.num_tris == 0) return SNKPT_OK;  // no hits, so no shadow rays were queued
     uint32_t need = A.depth > 0 ? A.depth - 1 : 0;
     uint32_t grid = nblk(R.w * R.h, kShadowBlock);
     if (need <= 16)
         k_shadow<16><<<grid, kShadowBlock, 0, st>>>(A.nodes.as<uint4>(), A.leaf_tris.as<float4>(), shadow_of(R),
-                                                     R.counts.as<uint32_t>(), R.rad.as<float4>());
+                                                     R.counts.as<uint32_t>(), rad);
     else if (need <= 32)
         k_shadow<32><<<grid, kShadowBlock, 0, st>>>(A.nodes.as<uint4>(), A.leaf_tris.as<float4>(), shadow_of(R),
-                                                     R.counts.as<uint32_t>(), R.rad.as<float4>());
+                                                     R.counts.as<uint32_t>(), rad);
     else if (need <= 64)
         k_shadow<64><<<grid, kShadowBlock, 0, st>>>(A.nodes.as<uint4>(), A.leaf_tris.as<float4>(), shadow_of(R),
-                                                     R.counts.as<uint32_t>(), R.rad.as<float4>());
+                                                     R.counts.as<uint32_t>(), rad);
     else
         return fail(ctx, SNKPT_ERR_CAPACITY, "shadow: BVH8 deeper than the 64-entry traversal stack");
     return SNKPT_OK;
@@ -383,11 +416,14 @@ snkpt_status launch_shadow(snkpt_ctx* ctx, RenderState& R, hipStream_t st) {
 
 // Bounces first..max_depth-1 of the queued paths (queue `cur` already holds
 // the rays of bounce `first`).
-snkpt_status pt_bounces(snkpt_ctx* ctx, const snkpt_settings* s, int cur, uint32_t first, hipStream_t st) {
+snkpt_status pt_bounces(snkpt_ctx* ctx, const snkpt_settings* s, int cur, uint32_t first, hipStream_t st,
+                        const ShadeHooks* hooks, uint32_t max_depth_override) {
     RenderState& R = *ctx->render;
     const uint32_t n = R.w * R.h;
     SceneDev S = scene_dev(ctx);
-    PtParams P{s->max_depth ? s->max_depth : 1u, s->rr_start};
+    PtParams P{max_depth_override ? max_depth_override : (s->max_depth ? s->max_depth : 1u), s->rr_start};
+    ShadeHooks H = hooks ? *hooks : ShadeHooks{};
+    float4* rad = H.rad ? H.rad : R.rad.as<float4>();
     uint32_t* counts = R.counts.as<uint32_t>();
     unsigned long long* stats = R.stats.as<unsigned long long>();
     for (uint32_t b = first; b < P.max_depth; ++b) {
@@ -398,21 +434,32 @@ snkpt_status pt_bounces(snkpt_ctx* ctx, const snkpt_settings* s, int cur, uint32
         snkpt_status e = launch_trace(ctx, R.q[cur].org.as<float4>(), R.q[cur].dir.as<float4>(), n, h, false,
                                       nullptr, nullptr, counts + cur, st);
         if (e != SNKPT_OK) return e;
-        k_shade<<<nblk(n, kBlock), kBlock, 0, st>>>(S, P, queue_of(R, cur), hits_of(R), queue_of(R, nxt),
-                                                    shadow_of(R), paths_of(R), R.rad.as<float4>(), counts,
-                                                    (uint32_t)cur, (uint32_t)nxt, stats);
-        if ((e = launch_shadow(ctx, R, st)) != SNKPT_OK) return e;
+        if (H.kind == 1)
+            k_shade<1><<<nblk(n, kBlock), kBlock, 0, st>>>(S, P, queue_of(R, cur), hits_of(R), queue_of(R, nxt),
+                                                           shadow_of(R), paths_of(R), rad, counts, (uint32_t)cur,
+                                                           (uint32_t)nxt, stats, H);
+        else if (H.kind == 2)
+            k_shade<2><<<nblk(n, kBlock), kBlock, 0, st>>>(S, P, queue_of(R, cur), hits_of(R), queue_of(R, nxt),
+                                                           shadow_of(R), paths_of(R), rad, counts, (uint32_t)cur,
+                                                           (uint32_t)nxt, stats, H);
+        else
+            k_shade<0><<<nblk(n, kBlock), kBlock, 0, st>>>(S, P, queue_of(R, cur), hits_of(R), queue_of(R, nxt),
+                                                           shadow_of(R), paths_of(R), rad, counts, (uint32_t)cur,
+                                                           (uint32_t)nxt, stats, H);
+        if ((e = launch_shadow(ctx, R, rad, st)) != SNKPT_OK) return e;
         cur = nxt;
     }
     return SNKPT_OK;
 }
 
-snkpt_status generate_camera(snkpt_ctx* ctx, const CamDev& cam, uint64_t initstate, hipStream_t st) {
+snkpt_status generate_camera(snkpt_ctx* ctx, const CamDev& cam, uint64_t initstate, hipStream_t st, uint32_t n_paths,
+                             uint32_t start, uint32_t stride, uint32_t* nrec_clear) {
     RenderState& R = *ctx->render;
-    const uint32_t n = R.w * R.h;
-    k_generate<<<nblk(n, kBlock), kBlock, 0, st>>>(cam, R.w, R.h, initstate, R.q[0].org.as<float4>(),
-                                                   R.q[0].dir.as<float4>(), R.q[0].path.as<uint32_t>(),
-                                                   paths_of(R), R.counts.as<uint32_t>(), 0);
+    const uint32_t n = n_paths ? n_paths : R.w * R.h;
+    k_generate<<<nblk(n, kBlock), kBlock, 0, st>>>(cam, R.w, R.h, n, start, stride, initstate,
+                                                   R.q[0].org.as<float4>(), R.q[0].dir.as<float4>(),
+                                                   R.q[0].path.as<uint32_t>(), paths_of(R), R.counts.as<uint32_t>(), 0,
+                                                   nrec_clear);
     return SNKPT_OK;
 }
 

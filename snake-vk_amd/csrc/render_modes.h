@@ -48,6 +48,25 @@ struct PathState {
     uint32_t* bounce;  // path vertex index of the next hit
 };
 
+// Optional behaviour of the shade kernel for the cache mode.
+//   kind 0: plain path tracing
+//   kind 1: at vertex `query_at`, a non-emissive hit writes a cache query
+//           (position, direction, normal, roughness, albedo) and the path ends
+//   kind 2: every non-emissive vertex below `max_rec` is recorded for training
+//           (sample fields, the radiance accumulated so far, the throughput)
+struct ShadeHooks {
+    uint32_t kind = 0;
+    uint32_t query_at = 0xFFFFFFFFu;
+    float* q_samples = nullptr;  // 16 floats per query
+    float4* q_meta = nullptr;    // throughput rgb, path (bits)
+    uint32_t* q_count = nullptr;
+    float* rec = nullptr;        // kRecFloats per (path, vertex)
+    uint32_t* nrec = nullptr;    // recorded vertices per path
+    uint32_t max_rec = 0;
+    float4* rad = nullptr;       // radiance target (defaults to the frame radiance)
+};
+constexpr int kRecFloats = 20;  // 16 sample floats (target = radiance so far) + throughput rgb + pad
+
 struct QueueBufs {
     DeviceBuffer org, dir, path;
 };
@@ -75,11 +94,15 @@ Queue queue_of(RenderState& R, int k);
 HitArrays hits_of(RenderState& R);
 ShadowQueue shadow_of(RenderState& R);
 PathState paths_of(RenderState& R);
-snkpt_status launch_shadow(snkpt_ctx* ctx, RenderState& R, hipStream_t st);
+snkpt_status launch_shadow(snkpt_ctx* ctx, RenderState& R, float4* rad, hipStream_t st);
 snkpt_status pt_pass(snkpt_ctx* ctx, const CamDev& cam, const snkpt_settings* s, uint64_t initstate,
                      hipStream_t st, uint32_t first_bounce_skip);
-snkpt_status pt_bounces(snkpt_ctx* ctx, const snkpt_settings* s, int cur, uint32_t first, hipStream_t st);
-snkpt_status generate_camera(snkpt_ctx* ctx, const CamDev& cam, uint64_t initstate, hipStream_t st);
+snkpt_status pt_bounces(snkpt_ctx* ctx, const snkpt_settings* s, int cur, uint32_t first, hipStream_t st,
+                        const ShadeHooks* hooks = nullptr, uint32_t max_depth_override = 0);
+// Camera rays for paths j < n_paths through pixels (start + j * stride) mod (w h).
+snkpt_status generate_camera(snkpt_ctx* ctx, const CamDev& cam, uint64_t initstate, hipStream_t st,
+                             uint32_t n_paths = 0, uint32_t start = 0, uint32_t stride = 1,
+                             uint32_t* nrec_clear = nullptr);
 inline uint32_t grid_for(uint32_t n, uint32_t b) { return n == 0 ? 1 : (n + b - 1) / b; }
 
 // restir.hip / cache.hip

@@ -464,7 +464,7 @@ snkpt_status restir_pass(snkpt_ctx* ctx, const snkpt_camera* cam, const CamDev& 
                                                       s->restir_radius > 0.0f ? s->restir_radius : 30.0f, initstate);
     k_rshade<<<grid_for(n, kBlock), kBlock, 0, st>>>(S, G, X.r_fin.as<float4>(), X.m_fin.as<float>(), shadow_of(R),
                                                      counts, n, stats);
-    if ((e = launch_shadow(ctx, R, st)) != SNKPT_OK) return e;
+    if ((e = launch_shadow(ctx, R, R.rad.as<float4>(), st)) != SNKPT_OK) return e;
     // history for the next frame
     std::memcpy(X.prev_pv, cam->proj_view, sizeof(X.prev_pv));
     X.prev_pos[0] = cam->cam_pos[0];

@@ -111,6 +111,10 @@ snkpt_status fail(snkpt_ctx* ctx, snkpt_status code, const std::string& msg);
 snkpt_status hip_fail(snkpt_ctx* ctx, hipError_t e, const char* what);
 
 snkpt_status build_accel(snkpt_ctx* ctx, hipStream_t stream);
+// Exclusive scan of n u32 (n <= 2^24); `sums` needs ceil(n / 4096) entries; the
+// grand total goes to *total (device memory) when total != nullptr.
+void scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* sums, uint32_t* total,
+                    hipStream_t st);
 snkpt_status accel_export(snkpt_ctx* ctx, uint32_t which, void* dst, size_t bytes, size_t* out_bytes);
 snkpt_status build_emitters(snkpt_ctx* ctx, hipStream_t stream);
 
