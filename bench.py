@@ -58,6 +58,142 @@ def cpu_baseline(world, rays_np, threads):
                       f"triangles took {build_s:.2f} s"}
 
 
+def frame_rates(stream, frames=10, warmup=3):
+    """Frames/s per configuration (1 spp frames of the render pass, event-timed
+    on the launch stream).  Cache frames include the online training step."""
+    import torch
+
+    import snkpt
+    runs = [(1, "pt", snkpt.MODE_PT), (2, "pt", snkpt.MODE_PT), (3, "restir", snkpt.MODE_RESTIR),
+            (4, "pt", snkpt.MODE_PT), (4, "cache", snkpt.MODE_CACHE)]
+    res = {}
+    for cfg, label, mode in runs:
+        scene = snkpt.SceneData(cfg)
+        ctx = snkpt.Context(0)
+        ctx.load_scene(scene)
+        with torch.cuda.stream(stream):
+            ctx.build_accel(stream)
+            w, h = scene.s.width, scene.s.height
+            cam = scene.camera(0, w, h)
+            out = torch.zeros(h, w, 4, device="cuda")
+            if mode == snkpt.MODE_CACHE:
+                ctx.cache_reset(1, stream=stream)
+
+            def frame(f):
+                s = snkpt.default_settings(mode=mode, max_depth=scene.s.max_depth, rr_start=scene.s.rr_start,
+                                           seed=scene.s.seed, frame=f, cache_query_depth=3)
+                if cfg == 3:
+                    s.num_temporal_resamples, s.num_spatial_resamples = 1, 5
+                ctx.render(cam, s, out, w, h, stream=stream)
+            for f in range(warmup):
+                frame(f)
+            evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                   for _ in range(frames)]
+            for k, (a, b) in enumerate(evs):
+                a.record(stream)
+                frame(warmup + k)
+                b.record(stream)
+            stream.synchronize()
+        ms = sum(a.elapsed_time(b) for a, b in evs) / frames
+        st = ctx.frame_stats()
+        key = f"c{cfg}_{label}"
+        res[key] = {"fps": round(1e3 / ms, 2), "ms_per_frame": round(ms, 3), "res": f"{w}x{h}",
+                    "depth": scene.s.max_depth, "rays_per_frame": int(st.camera_rays + st.extension_rays +
+                                                                      st.shadow_rays)}
+        res[key]["mrays_s"] = round(res[key]["rays_per_frame"] / (ms * 1e-3) / 1e6, 1)
+        log(f"bench: {key}: {ms:.3f} ms/frame ({res[key]['mrays_s']} Mrays/s incl. shadow)")
+        ctx.close()
+    return res
+
+
+def time_to_rmse(stream, ref_spp=4096, max_frames=768, cache_restart=32, query=3):
+    """1080p room (config 4): wall time for the running mean of 1-spp frames to
+    reach an RMSE against the FP64 oracle, plain PT vs the MLP cache (trained
+    from a freshly initialised network; its warm-up frames are timed and the
+    running mean restarts after `cache_restart` frames).  Frame times are GPU
+    event times of the render calls; the error checks between frames are not
+    timed.  Thresholds: absolute RMSE 0.02 and RMSE relative to the oracle's
+    mean radiance."""
+    import torch
+
+    import snkpt
+    scene = snkpt.SceneData(4)
+    ctx = snkpt.Context(0)
+    ctx.load_scene(scene)
+    w, h = scene.s.width, scene.s.height
+    depth, rr = scene.s.max_depth, scene.s.rr_start
+    with torch.cuda.stream(stream):
+        ctx.build_accel(stream)
+        cam = scene.camera(0, w, h)
+        acc = torch.zeros(h * w * 3, dtype=torch.float64, device="cuda")
+        s = snkpt.default_settings(max_depth=depth, rr_start=rr, seed=scene.s.seed ^ 0x0AC1E)
+        t0 = time.perf_counter()
+        for k in range(ref_spp // 64):
+            s.frame = k
+            ctx.oracle_accumulate(cam, s, acc, w, h, 64, stream=stream)
+        stream.synchronize()
+        ref = (acc / ref_spp).view(h, w, 3)
+        ref_mean = float(ref.mean())
+        log(f"bench: FP64 oracle {ref_spp} spp in {time.perf_counter() - t0:.1f} s, mean {ref_mean:.5f}")
+        out = torch.zeros(h, w, 4, device="cuda")
+        rel_thr = [0.2, 0.1, 0.05, 0.03]
+
+        def race(mode, **kw):
+            if mode == snkpt.MODE_CACHE:
+                ctx.cache_reset(1, stream=stream)
+            t_ms, first_rmse, hit_abs, hit_rel, curve = 0.0, None, None, {}, []
+            start = 0
+            for f in range(max_frames):
+                restart = f == 0 or (mode == snkpt.MODE_CACHE and f == cache_restart)
+                if restart:
+                    start = f
+                fl = snkpt.FLAG_ACCUMULATE | (snkpt.FLAG_RESET if restart else 0)
+                st = snkpt.default_settings(mode=mode, max_depth=depth, rr_start=rr, seed=scene.s.seed, frame=f,
+                                            flags=fl, **kw)
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record(stream)
+                ctx.render(cam, st, out, w, h, stream=stream)
+                b.record(stream)
+                stream.synchronize()
+                t_ms += a.elapsed_time(b)
+                e = float(torch.sqrt(((out[..., :3].double() - ref) ** 2).mean()))
+                if f == 0:
+                    first_rmse = e
+                if mode == snkpt.MODE_CACHE and f < cache_restart:
+                    continue
+                n_acc = f - start + 1
+                if (n_acc & (n_acc - 1)) == 0:
+                    curve.append([n_acc, round(t_ms, 2), round(e / ref_mean, 5)])
+                if hit_abs is None and e <= 0.02:
+                    hit_abs = round(t_ms, 2)
+                for thr in rel_thr:
+                    if str(thr) not in hit_rel and e <= thr * ref_mean:
+                        hit_rel[str(thr)] = round(t_ms, 2)
+                if len(hit_rel) == len(rel_thr):
+                    break
+            return {"ms_to_rmse_0.02": hit_abs, "ms_to_rel_rmse": hit_rel, "frames": f + 1,
+                    "ms_per_frame": round(t_ms / (f + 1), 3), "first_frame_rel_rmse": round(first_rmse / ref_mean, 4),
+                    "final_rel_rmse": round(e / ref_mean, 5), "curve_frames_ms_relrmse": curve}
+
+        pt = race(snkpt.MODE_PT)
+        log(f"bench: PT time-to-RMSE {pt['ms_to_rmse_0.02']} ms (abs 0.02), rel {pt['ms_to_rel_rmse']}")
+        cache = race(snkpt.MODE_CACHE, cache_query_depth=query)
+        log(f"bench: cache time-to-RMSE {cache['ms_to_rmse_0.02']} ms (abs 0.02), rel {cache['ms_to_rel_rmse']}")
+    ratio = {}
+    for k, v in pt["ms_to_rel_rmse"].items():
+        c = cache["ms_to_rel_rmse"].get(k)
+        ratio[k] = round(v / c, 3) if c else None
+    if pt["ms_to_rmse_0.02"] and cache["ms_to_rmse_0.02"]:
+        ratio["abs_0.02"] = round(pt["ms_to_rmse_0.02"] / cache["ms_to_rmse_0.02"], 3)
+    ctx.close()
+    return {"workload": "config 4: room at 1920x1080, 1 spp frames, depth 8, RR from bounce 3; cache queried at "
+                        f"path vertex {query} (after 2 bounces), 65,536 training paths/frame, Adam lr 1e-2",
+            "oracle": {"spp": ref_spp, "mean_radiance": round(ref_mean, 6),
+                       "rel_noise_est": round(pt["first_frame_rel_rmse"] / ref_spp ** 0.5, 5)},
+            "cache_restart_frame": cache_restart, "pt": pt, "cache": cache,
+            "speedup_pt_time_over_cache_time": ratio}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,6 +201,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only-trace", action="store_true", help="time only the incoherent trace (profiling)")
+    ap.add_argument("--no-frames", action="store_true", help="skip the per-config frame rates")
+    ap.add_argument("--no-rmse", action="store_true", help="skip the time-to-RMSE race (PT vs cache)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -185,6 +323,11 @@ def main():
                   "secondary_hit_fraction": round(secondary_hit, 4), "mean_nodes_visited": round(n_node, 3),
                   "mean_tris_visited": round(n_tri, 3), "bvh8_nodes": info.num_nodes, "bvh8_depth": info.depth,
                   "gpu_build_ms": round(min(build_ms), 3), "launch_ms": round(launch_ms, 4)})
+
+    if not args.only_trace and not args.no_frames:
+        extra["frames"] = frame_rates(stream)
+    if not args.only_trace and not args.no_rmse:
+        extra["time_to_rmse"] = time_to_rmse(stream)
 
     cpu = None
     if not args.no_cpu and not args.only_trace:

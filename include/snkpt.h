@@ -106,6 +106,8 @@ typedef struct snkpt_pointlight {
 #define SNKPT_FLAG_ACCUMULATE 1u /* add this frame to the running mean and output the mean */
 #define SNKPT_FLAG_RESET 2u      /* clear the running mean first */
 #define SNKPT_FLAG_NO_TRAIN 4u   /* cache mode: query only, skip the training step */
+#define SNKPT_FLAG_CACHE_INDIRECT 8u /* cache mode: NEE at the query vertex, the cache learns and answers
+                                        only the indirect radiance there */
 
 #define SNKPT_FORMAT_RGBA32F 0u
 #define SNKPT_FORMAT_RGBA16F 1u

@@ -762,57 +762,101 @@ __global__ void k_fill4(float4* p, uint32_t n) {
     if (i < n) p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-// One training sample per training path: a uniformly chosen recorded vertex,
-// target = (radiance of the whole path - radiance before the vertex) / throughput
-// at the vertex = an unbiased estimate of L_o(x, wo) there.  Written at the
-// path's own slot with a keep flag; k_train_compact packs them in path order
-// so that the batch (and the gradient sums over it) do not depend on timing.
-__global__ void k_train_targets(const float* __restrict__ rec, const uint32_t* __restrict__ nrec, uint32_t max_rec,
-                                const float4* __restrict__ rad, uint32_t n, uint64_t seed, float* __restrict__ stage,
-                                uint32_t* __restrict__ keep) {
-    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    bool want = false;
-    float smp[16];
-    {
-        uint32_t k = nrec[j];
-        if (k > 0) {
-            Rng r = rng_seed(seed, j);
-            uint32_t v = (uint32_t)(r.uniform() * (float)k);
-            if (v >= k) v = k - 1;
-            const float* e = rec + ((size_t)j * max_rec + v) * kRecFloats;
-            float4 L = rad[j];
-            float b[3] = {e[16], e[17], e[18]};
-            float lt[3] = {L.x - e[13], L.y - e[14], L.z - e[15]};
-            want = b[0] > 1e-6f && b[1] > 1e-6f && b[2] > 1e-6f;
-#pragma unroll
-            for (int c = 0; c < 13; ++c) smp[c] = e[c];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                float t = fmaxf(lt[c], 0.0f) / fmaxf(b[c], 1e-6f);
-                smp[13 + c] = fminf(t, 1000.0f);
-                want = want && __builtin_isfinite(t);
-            }
+// Training samples from the recorded vertices of the training paths.  Vertex
+// v of a path with throughput beta_v and radiance snapshot S_v (radiance of
+// the path before v's own contributions) gets the target
+//     (L - S_v) / beta_v                 = an unbiased estimate of L_o(x_v, wo_v),
+// or, with `indirect`, (L - S_{v+1}) / beta_v (0 when the path has no
+// recorded vertex v + 1): v's NEE and the MIS-weighted emission its
+// continuation hits are direct light, everything later is indirect.
+// Every vertex is a candidate and is kept with one probability p for the
+// whole frame (p = min(1, budget / all recorded vertices)), decided by a hash
+// of (seed, path, vertex): selection never depends on how a path continues,
+// which would bias the targets (picking one vertex per path favours short
+// paths).  Samples are emitted in (path, vertex) order through a scan, so the
+// batch, and the gradient sums over it, do not depend on timing.
+__device__ __forceinline__ bool train_sample(const float* __restrict__ rec, uint32_t max_rec, uint32_t j, uint32_t k,
+                                             uint32_t v, float4 L, uint32_t indirect, float smp[16]) {
+    const float* e = rec + ((size_t)j * max_rec + v) * kRecFloats;
+    const float b[3] = {e[16], e[17], e[18]};
+    float lt[3] = {L.x - e[13], L.y - e[14], L.z - e[15]};
+    if (indirect) {
+        if (v + 1 < k) {
+            const float* e1 = e + kRecFloats;
+            lt[0] = L.x - e1[13], lt[1] = L.y - e1[14], lt[2] = L.z - e1[15];
+        } else {
+            lt[0] = lt[1] = lt[2] = 0.0f;
         }
     }
-    keep[j] = want ? 1u : 0u;
-    if (want) {
-        float4* o = (float4*)(stage + (size_t)j * 16);
+    bool ok = b[0] > 1e-6f && b[1] > 1e-6f && b[2] > 1e-6f;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) o[c] = make_float4(smp[4 * c], smp[4 * c + 1], smp[4 * c + 2], smp[4 * c + 3]);
+    for (int c = 0; c < 13; ++c) smp[c] = e[c];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float t = fmaxf(lt[c], 0.0f) / fmaxf(b[c], 1e-6f);
+        smp[13 + c] = fminf(t, 1000.0f);
+        ok = ok && __builtin_isfinite(t);
     }
+    return ok;
 }
 
-__global__ void k_train_compact(const float* __restrict__ stage, const uint32_t* __restrict__ keep,
-                                const uint32_t* __restrict__ off, uint32_t n, float* __restrict__ out,
-                                const uint32_t* __restrict__ total, unsigned long long* __restrict__ stat) {
+__device__ __forceinline__ bool keep_vertex(uint64_t seed, uint32_t j, uint32_t v, uint64_t thr) {
+    uint64_t x = seed ^ (((uint64_t)j << 8) | v) * 0x9E3779B97F4A7C15ull;
+    x ^= x >> 31;
+    x *= 0x94D049BB133111EBull;
+    x ^= x >> 29;
+    return (x >> 32) < thr;
+}
+
+// keep threshold in 2^-32 units from the frame's recorded-vertex total
+__device__ __forceinline__ uint64_t keep_threshold(const uint32_t* __restrict__ cnt, uint32_t budget) {
+    uint32_t total = cnt[2];
+    if (total <= budget) return 1ull << 32;
+    return (uint64_t)(((double)budget / (double)total) * 4294967296.0);
+}
+
+__global__ void k_train_total(const uint32_t* __restrict__ nrec, uint32_t n, uint32_t* __restrict__ cnt) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j == 0) atomicAdd(stat, (unsigned long long)*total);
-    if (j >= n || !keep[j]) return;
-    const float4* src = (const float4*)(stage + (size_t)j * 16);
-    float4* dst = (float4*)(out + (size_t)off[j] * 16);
+    uint32_t k = j < n ? nrec[j] : 0u;
+    // wave sum, one atomic per wave
+    for (int o = 32; o > 0; o >>= 1) k += __shfl_xor(k, o, 64);
+    if ((threadIdx.x & 63) == 0 && k) atomicAdd(&cnt[2], k);
+}
+
+__global__ void k_train_count(const float* __restrict__ rec, const uint32_t* __restrict__ nrec, uint32_t max_rec,
+                              const float4* __restrict__ rad, uint32_t n, uint64_t seed, uint32_t indirect,
+                              const uint32_t* __restrict__ cnt, uint32_t budget, uint32_t* __restrict__ count) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint64_t thr = keep_threshold(cnt, budget);
+    const uint32_t k = nrec[j];
+    const float4 L = rad[j];
+    uint32_t c = 0;
+    float smp[16];
+    for (uint32_t v = 0; v < k; ++v)
+        if (keep_vertex(seed, j, v, thr) && train_sample(rec, max_rec, j, k, v, L, indirect, smp)) ++c;
+    count[j] = c;
+}
+
+__global__ void k_train_emit(const float* __restrict__ rec, const uint32_t* __restrict__ nrec, uint32_t max_rec,
+                             const float4* __restrict__ rad, uint32_t n, uint64_t seed, uint32_t indirect,
+                             const uint32_t* __restrict__ cnt, uint32_t budget, const uint32_t* __restrict__ off,
+                             float* __restrict__ out, unsigned long long* __restrict__ stat) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j == 0) atomicAdd(stat, (unsigned long long)cnt[1]);
+    if (j >= n) return;
+    const uint64_t thr = keep_threshold(cnt, budget);
+    const uint32_t k = nrec[j];
+    const float4 L = rad[j];
+    uint32_t slot = off[j];
+    float smp[16];
+    for (uint32_t v = 0; v < k; ++v) {
+        if (!keep_vertex(seed, j, v, thr) || !train_sample(rec, max_rec, j, k, v, L, indirect, smp)) continue;
+        float4* o = (float4*)(out + (size_t)slot * 16);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) dst[c] = src[c];
+        for (int c = 0; c < 4; ++c) o[c] = make_float4(smp[4 * c], smp[4 * c + 1], smp[4 * c + 2], smp[4 * c + 3]);
+        ++slot;
+    }
 }
 
 constexpr uint64_t kTrainPaths = 0x545241494eull;
@@ -839,13 +883,12 @@ snkpt_status cache_pass(snkpt_ctx* ctx, const CamDev& cd, const snkpt_settings* 
     }
     HIPC(C.counts.reserve(64));
     HIPC(C.train_vtx.reserve((size_t)NT * depth * kRecFloats * 4));
-    // train_meta: nrec | keep | offsets (u32 each) | scan sums | radiance (float4) | staged samples (16 floats)
+    // train_meta: nrec | per-path sample counts | offsets (u32 each) | scan sums | radiance (float4)
     const size_t u32_bytes = ((size_t)NT * 4 + 255) & ~(size_t)255;
     const size_t sums_bytes = (((size_t)(NT + 4095) / 4096) * 4 + 255) & ~(size_t)255;
     const size_t trad_off = 3 * u32_bytes + sums_bytes;
-    const size_t stage_off = trad_off + (size_t)NT * 16;
-    HIPC(C.train_meta.reserve(stage_off + (size_t)NT * 64));
-    HIPC(C.t_samples.reserve((size_t)NT * 64));
+    HIPC(C.train_meta.reserve(trad_off + (size_t)NT * 16));
+    HIPC(C.t_samples.reserve((size_t)NT * depth * 64));
     snkpt_status e = ensure_work(ctx, C, n, false);
     if (e != SNKPT_OK) return e;
     uint32_t* qc = C.counts.as<uint32_t>();
@@ -856,6 +899,7 @@ snkpt_status cache_pass(snkpt_ctx* ctx, const CamDev& cd, const snkpt_settings* 
     ShadeHooks H;
     H.kind = 1;
     H.query_at = Q - 1;  // bounce index of that vertex
+    H.indirect = (s->flags & SNKPT_FLAG_CACHE_INDIRECT) ? 1u : 0u;
     H.q_samples = C.q_samples.as<float>();
     H.q_meta = C.q_meta.as<float4>();
     H.q_count = qc;
@@ -883,17 +927,19 @@ snkpt_status cache_pass(snkpt_ctx* ctx, const CamDev& cd, const snkpt_settings* 
     T.nrec = nrec;
     T.max_rec = depth;
     T.rad = trad;
-    if ((e = pt_bounces(ctx, s, 0, 0, st, &T)) != SNKPT_OK) return e;
+    if ((e = pt_bounces(ctx, s, 0, 0, st, &T, 0, NT)) != SNKPT_OK) return e;
     uint8_t* tm = C.train_meta.as<uint8_t>();
-    uint32_t* keep = (uint32_t*)(tm + u32_bytes);
+    uint32_t* pcount = (uint32_t*)(tm + u32_bytes);
     uint32_t* off = (uint32_t*)(tm + 2 * u32_bytes);
     uint32_t* sums = (uint32_t*)(tm + 3 * u32_bytes);
-    float* stage = (float*)(tm + stage_off);
-    k_train_targets<<<grid_for(NT, 256), 256, 0, st>>>(C.train_vtx.as<float>(), nrec, depth, trad, NT,
-                                                        initstate ^ 0x7461726765ull, stage, keep);
-    scan_exclusive(keep, off, NT, sums, qc + 1, st);
-    k_train_compact<<<grid_for(NT, 256), 256, 0, st>>>(stage, keep, off, NT, C.t_samples.as<float>(), qc + 1,
-                                                        stats + 4);
+    const uint64_t kseed = initstate ^ 0x7461726765ull;
+    const uint32_t budget = NT;  // training samples per frame
+    k_train_total<<<grid_for(NT, 256), 256, 0, st>>>(nrec, NT, qc);
+    k_train_count<<<grid_for(NT, 256), 256, 0, st>>>(C.train_vtx.as<float>(), nrec, depth, trad, NT, kseed,
+                                                      H.indirect, qc, budget, pcount);
+    scan_exclusive(pcount, off, NT, sums, qc + 1, st);
+    k_train_emit<<<grid_for(NT, 256), 256, 0, st>>>(C.train_vtx.as<float>(), nrec, depth, trad, NT, kseed, H.indirect,
+                                                     qc, budget, off, C.t_samples.as<float>(), stats + 4);
     uint32_t nt = 0;
     HIPC(hipMemcpyAsync(&nt, qc + 1, 4, hipMemcpyDeviceToHost, st));
     HIPC(hipStreamSynchronize(st));

@@ -100,7 +100,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PtParams P, Queue 
                 V3 c = mul(thr, s.le) * w;
                 float4 r = rad[path];
                 rad[path] = make_float4(r.x + c.x, r.y + c.y, r.z + c.z, r.w);
-            } else if (KIND == 1 && bounce == H.query_at) {
+            } else if (KIND == 1 && bounce == H.query_at && !H.indirect) {
                 // terminate into the radiance cache: L_o(x, wo) from the network
                 V3 wo = dir * -1.0f;
                 want_q = true;
@@ -110,6 +110,15 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PtParams P, Queue 
                 for (int k = 0; k < 13; ++k) qs[k] = q[k];
             } else {
                 V3 wo = dir * -1.0f;
+                // indirect split: direct light here by NEE alone, the cache answers the rest
+                const bool query = KIND == 1 && bounce == H.query_at;
+                if (query) {
+                    want_q = true;
+                    float q[13] = {s.p.x, s.p.y, s.p.z, wo.x, wo.y, wo.z, s.ns.x, s.ns.y, s.ns.z, s.m.roughness,
+                                   s.m.albedo.x, s.m.albedo.y, s.m.albedo.z};
+#pragma unroll
+                    for (int k = 0; k < 13; ++k) qs[k] = q[k];
+                }
                 if (KIND == 2 && bounce < H.max_rec) {
                     // training record: sample fields, radiance so far, throughput at arrival
                     float4 r = rad[path];
@@ -123,7 +132,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PtParams P, Queue 
                 }
                 float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform(), u3 = rng.uniform();
                 float ul = rng.uniform(), us1 = rng.uniform(), us2 = rng.uniform(), urr = rng.uniform();
-                const bool last = bounce + 1 >= P.max_depth;
+                const bool last = bounce + 1 >= P.max_depth || query;
                 // next-event estimation
                 LightSample ls = sample_light(S, s.p, u0, u1, u2, u3);
                 if (ls.ok) {
@@ -395,11 +404,11 @@ PathState paths_of(RenderState& R) {
     return PathState{R.p_thr.as<float4>(), R.p_rng.as<uint2>(), R.p_bounce.as<uint32_t>()};
 }
 
-snkpt_status launch_shadow(snkpt_ctx* ctx, RenderState& R, float4* rad, hipStream_t st) {
+snkpt_status launch_shadow(snkpt_ctx* ctx, RenderState& R, float4* rad, hipStream_t st, uint32_t n_max) {
     const Accel& A = ctx->accel;
     if (A.num_tris == 0) return SNKPT_OK;  // no hits, so no shadow rays were queued
     uint32_t need = A.depth > 0 ? A.depth - 1 : 0;
-    uint32_t grid = nblk(R.w * R.h, kShadowBlock);
+    uint32_t grid = nblk(n_max ? n_max : R.w * R.h, kShadowBlock);
     if (need <= 16)
         k_shadow<16><<<grid, kShadowBlock, 0, st>>>(A.nodes.as<uint4>(), A.leaf_tris.as<float4>(), shadow_of(R),
                                                      R.counts.as<uint32_t>(), rad);
@@ -417,9 +426,9 @@ snkpt_status launch_shadow(snkpt_ctx* ctx, RenderState& R, float4* rad, hipStrea
 // Bounces first..max_depth-1 of the queued paths (queue `cur` already holds
 // the rays of bounce `first`).
 snkpt_status pt_bounces(snkpt_ctx* ctx, const snkpt_settings* s, int cur, uint32_t first, hipStream_t st,
-                        const ShadeHooks* hooks, uint32_t max_depth_override) {
+                        const ShadeHooks* hooks, uint32_t max_depth_override, uint32_t n_paths) {
     RenderState& R = *ctx->render;
-    const uint32_t n = R.w * R.h;
+    const uint32_t n = n_paths ? std::min(n_paths, R.w * R.h) : R.w * R.h;
     SceneDev S = scene_dev(ctx);
     PtParams P{max_depth_override ? max_depth_override : (s->max_depth ? s->max_depth : 1u), s->rr_start};
     ShadeHooks H = hooks ? *hooks : ShadeHooks{};
@@ -446,7 +455,7 @@ snkpt_status pt_bounces(snkpt_ctx* ctx, const snkpt_settings* s, int cur, uint32
             k_shade<0><<<nblk(n, kBlock), kBlock, 0, st>>>(S, P, queue_of(R, cur), hits_of(R), queue_of(R, nxt),
                                                            shadow_of(R), paths_of(R), rad, counts, (uint32_t)cur,
                                                            (uint32_t)nxt, stats, H);
-        if ((e = launch_shadow(ctx, R, rad, st)) != SNKPT_OK) return e;
+        if ((e = launch_shadow(ctx, R, rad, st, n)) != SNKPT_OK) return e;
         cur = nxt;
     }
     return SNKPT_OK;

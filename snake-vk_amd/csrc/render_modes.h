@@ -51,12 +51,15 @@ struct PathState {
 // Optional behaviour of the shade kernel for the cache mode.
 //   kind 0: plain path tracing
 //   kind 1: at vertex `query_at`, a non-emissive hit writes a cache query
-//           (position, direction, normal, roughness, albedo) and the path ends
+//           (position, direction, normal, roughness, albedo) and the path ends;
+//           with `indirect` the vertex first takes its direct light by NEE
+//           (weight 1) and the cache stands for the indirect part only
 //   kind 2: every non-emissive vertex below `max_rec` is recorded for training
 //           (sample fields, the radiance accumulated so far, the throughput)
 struct ShadeHooks {
     uint32_t kind = 0;
     uint32_t query_at = 0xFFFFFFFFu;
+    uint32_t indirect = 0;
     float* q_samples = nullptr;  // 16 floats per query
     float4* q_meta = nullptr;    // throughput rgb, path (bits)
     uint32_t* q_count = nullptr;
@@ -94,11 +97,13 @@ Queue queue_of(RenderState& R, int k);
 HitArrays hits_of(RenderState& R);
 ShadowQueue shadow_of(RenderState& R);
 PathState paths_of(RenderState& R);
-snkpt_status launch_shadow(snkpt_ctx* ctx, RenderState& R, float4* rad, hipStream_t st);
+// n_max: upper bound of the queued shadow rays (0 = one per pixel)
+snkpt_status launch_shadow(snkpt_ctx* ctx, RenderState& R, float4* rad, hipStream_t st, uint32_t n_max = 0);
 snkpt_status pt_pass(snkpt_ctx* ctx, const CamDev& cam, const snkpt_settings* s, uint64_t initstate,
                      hipStream_t st, uint32_t first_bounce_skip);
+// n_paths: number of paths generated (0 = one per pixel); sizes the launches.
 snkpt_status pt_bounces(snkpt_ctx* ctx, const snkpt_settings* s, int cur, uint32_t first, hipStream_t st,
-                        const ShadeHooks* hooks = nullptr, uint32_t max_depth_override = 0);
+                        const ShadeHooks* hooks = nullptr, uint32_t max_depth_override = 0, uint32_t n_paths = 0);
 // Camera rays for paths j < n_paths through pixels (start + j * stride) mod (w h).
 snkpt_status generate_camera(snkpt_ctx* ctx, const CamDev& cam, uint64_t initstate, hipStream_t st,
                              uint32_t n_paths = 0, uint32_t start = 0, uint32_t stride = 1,

@@ -34,6 +34,7 @@ MODE_CACHE = 2
 FLAG_ACCUMULATE = 1
 FLAG_RESET = 2
 FLAG_NO_TRAIN = 4
+FLAG_CACHE_INDIRECT = 8
 FORMAT_RGBA32F = 0
 FORMAT_RGBA16F = 1
 

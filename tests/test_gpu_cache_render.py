@@ -86,6 +86,28 @@ def test_zero_cache_equals_pt_truncated_before_query(gpu):
     assert rel < 0.05
 
 
+def test_zero_indirect_cache_equals_pt_to_query_depth(gpu):
+    """Indirect split, Q = 2, zero network: the query vertex keeps its NEE
+    (weight 1) and nothing else, so E[image] = PT with max_depth 2."""
+    import snkpt
+    w = h = 32
+    scene, ctx, cam = _room(w, h)
+    _zero_net(ctx)
+    ref = _oracle(ctx, cam, w, h, 2048, 2)
+    acc = np.zeros((h, w, 3))
+    frames = 128
+    for f in range(frames):
+        acc += _render(ctx, cam, w, h, mode=snkpt.MODE_CACHE, max_depth=6, cache_query_depth=2, frame=f, seed=4,
+                       flags=snkpt.FLAG_NO_TRAIN | snkpt.FLAG_CACHE_INDIRECT)
+    mean = acc / frames
+    tr = mean.reshape(4, 8, 4, 8, 3).mean((1, 3))
+    to = ref.reshape(4, 8, 4, 8, 3).mean((1, 3))
+    rel = np.abs(tr - to).sum() / to.sum()
+    _say(f"zero indirect cache: mean {mean.mean():.5f} vs PT depth 2 {ref.mean():.5f}, tile L1 rel {rel:.4f}")
+    assert abs(mean.mean() - ref.mean()) < 0.02 * ref.mean()
+    assert rel < 0.05
+
+
 def test_cache_term_is_throughput_times_network_output(gpu):
     """render(W) - render(0) with the same seed = scatter of thr * max(y, 0)
     over the exported queries, y from snkpt_cache_infer on those queries."""
@@ -228,3 +250,52 @@ def test_training_is_deterministic(gpu):
         outs.append((img, ctx.cache_params(0).copy()))
     assert np.array_equal(outs[0][0], outs[1][0])
     assert np.array_equal(outs[0][1], outs[1][1])
+
+
+def _wall_grid(verts, nz=120, ny=36):
+    """Midpoint quadrature of the wall x = 1.2, z in [-3, 3], y in [0, 1.8]:
+    points, cell area and the direct irradiance at each point."""
+    zs = -3.0 + (np.arange(nz) + 0.5) * 6.0 / nz
+    ys = (np.arange(ny) + 0.5) * 1.8 / ny
+    Z, Y = np.meshgrid(zs, ys)
+    P = np.stack([np.full_like(Z, 1.2), Y, Z], -1).reshape(-1, 3)
+    E = np.array([polygon_irradiance(p, [-1, 0, 0], verts, 10.0) for p in P])
+    return P, (6.0 / nz) * (1.8 / ny), E
+
+
+def test_indirect_training_targets_match_one_bounce_quadrature(gpu):
+    """Indirect split, max_depth 2: a floor vertex's target is the radiance
+    its continuation brings back from the wall (the wall's NEE, the last
+    vertex), excluding the floor's own direct light.  Its expectation is
+    rho_f/pi * integral over the wall of (rho_w/pi) E_d(y) cos cos / r^2 dA,
+    with E_d from Lambert's polygon formula.  Wall vertices (last bounce)
+    have no indirect light: their targets are exactly 0."""
+    import snkpt
+    sc, verts = _wall_scene()
+    ctx = snkpt.Context(0)
+    sc.load_into(ctx)
+    w = h = 64
+    cam = look_at_camera([0.0, 1.5, 0.0], [0.0, 0.0, -0.3], fov=60.0, w=w, h=h)
+    ctx.cache_reset(3)
+    P, dA, Ew = _wall_grid(verts)
+    Lw = 0.7 / np.pi * Ew
+    got, want, n_wall = 0.0, 0.0, 0
+    for f in range(24):
+        _render(ctx, cam, w, h, mode=snkpt.MODE_CACHE, max_depth=2, cache_query_depth=2, frame=f, seed=17,
+                cache_lr=1e-9, flags=snkpt.FLAG_CACHE_INDIRECT)
+        t = ctx.cache_params(7).reshape(-1, 16).astype(np.float64)
+        on_wall = np.abs(t[:, 0] - 1.2) < 1e-4
+        assert np.all(t[on_wall, 13:16] == 0.0)
+        n_wall += int(on_wall.sum())
+        fl = t[np.abs(t[:, 1]) < 1e-4]
+        d = P[None, :, :] - fl[:, None, 0:3]
+        r2 = (d ** 2).sum(-1)
+        cx = d[..., 1] / np.sqrt(r2)
+        cy = d[..., 0] / np.sqrt(r2)
+        e = 0.5 / np.pi * (Lw[None, :] * cx * cy / r2).sum(1) * dA
+        got += fl[:, 13].sum()
+        want += e.sum()
+    _say(f"indirect targets at the floor: sum {got:.3f} vs quadrature {want:.3f} (ratio {got / want:.4f}); "
+         f"{n_wall} wall samples, all 0")
+    assert n_wall > 1000
+    assert abs(got / want - 1.0) < 0.03
