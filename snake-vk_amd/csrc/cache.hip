@@ -28,7 +28,7 @@ struct CacheState {
     uint32_t step = 0;
     DeviceBuffer grid32, grid16, grid_m, grid_v, grid_grad;  // grid_grad: int64 fixed point
     DeviceBuffer mlp32, mlp16, mlp16t, mlp_m, mlp_v, mlp_gradf, grid_gradf;
-    DeviceBuffer partial, loss_part, x0, dx0, y;
+    DeviceBuffer partial, loss_part, x0, dx0, y, mlp_red;
     uint32_t cap = 0;         // samples the inference work buffers hold
     uint32_t tcap = 0;        // samples the training work buffers hold
     uint32_t last_n = 0;
@@ -50,70 +50,95 @@ __device__ __forceinline__ f32x4_t mfma16(half8_t a, half8_t b, f32x4_t c) {
 }
 
 // ------------------------------------------------------------- encoding ----
-__device__ __forceinline__ void encode_one(const float* s, const GridGeom& G, const __half2* __restrict__ grid,
-                                           _Float16* out) {
-    float p[3];
+typedef _Float16 half2v_t __attribute__((ext_vector_type(2)));
+
+// Position normalised to the cache box, clamped to [0, 1].
+__device__ __forceinline__ void norm_pos(const float* s, const GridGeom& G, float p[3]) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) p[a] = fminf(fmaxf((s[a] - G.lo[a]) * G.inv_ext, 0.0f), 1.0f);
-    for (int l = 0; l < kGridLevels; ++l) {
-        const float r = (float)G.res[l];
-        float x = p[0] * r, y = p[1] * r, z = p[2] * r;
-        // cell of the point, clamped so that corner + 1 <= res (p may be exactly 1)
-        float fx = fminf(floorf(x), r - 1.0f), fy = fminf(floorf(y), r - 1.0f), fz = fminf(floorf(z), r - 1.0f);
-        uint32_t ix = (uint32_t)fx, iy = (uint32_t)fy, iz = (uint32_t)fz;
-        float wx = x - fx, wy = y - fy, wz = z - fz;
-        float f0 = 0.0f, f1 = 0.0f;
-        const __half2* g = grid + (size_t)l * kGridT;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            uint32_t cx = ix + (c & 1), cy = iy + ((c >> 1) & 1), cz = iz + (c >> 2);
-            float w = ((c & 1) ? wx : 1.0f - wx) * (((c >> 1) & 1) ? wy : 1.0f - wy) * ((c >> 2) ? wz : 1.0f - wz);
-            float2 v = __half22float2(g[grid_index(cx, cy, cz, G.res[l], G.dense[l])]);
-            f0 += w * v.x;
-            f1 += w * v.y;
-        }
-        out[2 * l] = (_Float16)f0;
-        out[2 * l + 1] = (_Float16)f1;
-    }
-    const float* d = s + 3;
-    const float* n = s + 6;
-    int k = 32;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) out[k++] = (_Float16)(0.5f * (d[a] + 1.0f));
-#pragma unroll
-    for (int a = 0; a < 3; ++a) out[k++] = (_Float16)(0.5f * (n[a] + 1.0f));
-    out[k++] = (_Float16)s[9];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) out[k++] = (_Float16)s[10 + a];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        out[k++] = (_Float16)sinf(kPi * d[a]);
-        out[k++] = (_Float16)cosf(kPi * d[a]);
-    }
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        out[k++] = (_Float16)sinf(kPi * n[a]);
-        out[k++] = (_Float16)cosf(kPi * n[a]);
-    }
-    while (k < 64) out[k++] = (_Float16)0.0f;
 }
 
-__global__ void k_encode(const float* __restrict__ samples, uint32_t n, const uint32_t* __restrict__ n_dev, GridGeom G,
-                         const __half2* __restrict__ grid, _Float16* __restrict__ x0) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n_dev) n = *n_dev;
-    if (i >= n) return;
-    _Float16 row[64];
-    encode_one(samples + (size_t)kSampleFloats * i, G, grid, row);
-    half8_t* o = (half8_t*)(x0 + (size_t)64 * i);
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        half8_t v;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) v[e] = row[8 * j + e];
-        o[j] = v;
-    }
+// Cell of p at resolution r and the trilinear fractions; the cell is clamped
+// so that corner + 1 <= r (p may be exactly 1).
+struct Cell {
+    uint32_t ix, iy, iz;
+    float wx, wy, wz;
+};
+__device__ __forceinline__ Cell cell_of(const float p[3], float r) {
+    float x = p[0] * r, y = p[1] * r, z = p[2] * r;
+    float fx = fminf(floorf(x), r - 1.0f), fy = fminf(floorf(y), r - 1.0f), fz = fminf(floorf(z), r - 1.0f);
+    return Cell{(uint32_t)fx, (uint32_t)fy, (uint32_t)fz, x - fx, y - fy, z - fz};
 }
+__device__ __forceinline__ float corner_weight(const Cell& c, int k) {
+    return ((k & 1) ? c.wx : 1.0f - c.wx) * (((k >> 1) & 1) ? c.wy : 1.0f - c.wy) * ((k >> 2) ? c.wz : 1.0f - c.wz);
+}
+__device__ __forceinline__ uint32_t corner_index(const Cell& c, int k, const GridGeom& G, int l) {
+    return grid_index(c.ix + (k & 1), c.iy + ((k >> 1) & 1), c.iz + (k >> 2), G.res[l], G.dense[l]);
+}
+
+// The two features of level l: trilinear blend of the 8 corner entries.
+__device__ __forceinline__ float2 encode_level(const float p[3], const GridGeom& G, int l,
+                                               const __half2* __restrict__ grid) {
+    const Cell c = cell_of(p, (float)G.res[l]);
+    const __half2* g = grid + (size_t)l * kGridT;
+    float2 v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = __half22float2(g[corner_index(c, k, G, l)]);
+    float f0 = 0.0f, f1 = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        float w = corner_weight(c, k);
+        f0 += w * v[k].x;
+        f1 += w * v[k].y;
+    }
+    return make_float2(f0, f1);
+}
+
+// Input feature 32 + k (k < 32) of a sample: direction and normal mapped to
+// [0, 1], roughness, albedo, sin/cos(pi x) of each direction and normal
+// component, zero padding.
+__device__ __forceinline__ float extra_feature(const float* s, int k) {
+    const float* d = s + 3;
+    const float* n = s + 6;
+    if (k < 3) return 0.5f * (d[k] + 1.0f);
+    if (k < 6) return 0.5f * (n[k - 3] + 1.0f);
+    if (k == 6) return s[9];
+    if (k < 10) return s[10 + (k - 7)];
+    if (k < 16) {
+        float a = kPi * d[(k - 10) >> 1];
+        return ((k - 10) & 1) ? cosf(a) : sinf(a);
+    }
+    if (k < 22) {
+        float a = kPi * n[(k - 16) >> 1];
+        return ((k - 16) & 1) ? cosf(a) : sinf(a);
+    }
+    return 0.0f;
+}
+
+// One thread per (sample, level): wave l of a block encodes level l of 64
+// consecutive samples and writes input features 32 + 2l, 33 + 2l as well, so
+// the 16 waves of a block fill the 128 B input rows of their 64 samples.
+constexpr int kEncBlock = 64 * kGridLevels;
+__global__ __launch_bounds__(kEncBlock) void k_encode(const float* __restrict__ samples, uint32_t n,
+                                                     const uint32_t* __restrict__ n_dev, GridGeom G,
+                                                     const __half2* __restrict__ grid, _Float16* __restrict__ x0) {
+    if (n_dev) n = *n_dev;
+    const int l = threadIdx.x >> 6;
+    const uint32_t i = blockIdx.x * 64 + (threadIdx.x & 63);
+    if (i >= n) return;
+    const float* s = samples + (size_t)kSampleFloats * i;
+    float p[3];
+    norm_pos(s, G, p);
+    float2 f = encode_level(p, G, l, grid);
+    half2v_t v, e;
+    v[0] = (_Float16)f.x;
+    v[1] = (_Float16)f.y;
+    e[0] = (_Float16)extra_feature(s, 2 * l);
+    e[1] = (_Float16)extra_feature(s, 2 * l + 1);
+    *(half2v_t*)(x0 + (size_t)64 * i + 2 * l) = v;
+    *(half2v_t*)(x0 + (size_t)64 * i + 32 + 2 * l) = e;
+}
+inline uint32_t enc_blocks(uint32_t n) { return n == 0 ? 1u : (n + 63) / 64; }
 
 // ------------------------------------------------------------ MLP core ----
 // act: [64][kRow] halves in LDS.  W: [out][64] row-major fp16 in global memory.
@@ -377,28 +402,24 @@ __global__ __launch_bounds__(64) void k_mlp_train(const _Float16* __restrict__ x
     }
 }
 
-// Scatter dL/dfeature to the 8 corners of each level (64-bit fixed point).
+// Scatter dL/dfeature to the 8 corners of each level (64-bit fixed point, so
+// the sums do not depend on the order of the atomics).  One thread per sample
+// walks the levels in turn: at any moment the threads of the grid spread
+// their atomics over several levels, which keeps the coarse levels' few
+// entries less contended than one thread per (sample, level) does.
 __global__ void k_grid_bwd(const float* __restrict__ samples, const float* __restrict__ dx0, uint32_t n, GridGeom G,
                            unsigned long long* __restrict__ grad) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float* s = samples + (size_t)kSampleFloats * i;
     float p[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) p[a] = fminf(fmaxf((s[a] - G.lo[a]) * G.inv_ext, 0.0f), 1.0f);
+    norm_pos(samples + (size_t)kSampleFloats * i, G, p);
     for (int l = 0; l < kGridLevels; ++l) {
-        float g0 = dx0[(size_t)32 * i + 2 * l], g1 = dx0[(size_t)32 * i + 2 * l + 1];
+        const float g0 = dx0[(size_t)32 * i + 2 * l], g1 = dx0[(size_t)32 * i + 2 * l + 1];
         if (g0 == 0.0f && g1 == 0.0f) continue;
-        const float r = (float)G.res[l];
-        float x = p[0] * r, y = p[1] * r, z = p[2] * r;
-        // cell of the point, clamped so that corner + 1 <= res (p may be exactly 1)
-        float fx = fminf(floorf(x), r - 1.0f), fy = fminf(floorf(y), r - 1.0f), fz = fminf(floorf(z), r - 1.0f);
-        uint32_t ix = (uint32_t)fx, iy = (uint32_t)fy, iz = (uint32_t)fz;
-        float wx = x - fx, wy = y - fy, wz = z - fz;
-        for (int c = 0; c < 8; ++c) {
-            uint32_t cx = ix + (c & 1), cy = iy + ((c >> 1) & 1), cz = iz + (c >> 2);
-            float w = ((c & 1) ? wx : 1.0f - wx) * (((c >> 1) & 1) ? wy : 1.0f - wy) * ((c >> 2) ? wz : 1.0f - wz);
-            size_t e = ((size_t)l * kGridT + grid_index(cx, cy, cz, G.res[l], G.dense[l])) * 2;
+        const Cell c = cell_of(p, (float)G.res[l]);
+        for (int k = 0; k < 8; ++k) {
+            float w = corner_weight(c, k);
+            size_t e = ((size_t)l * kGridT + corner_index(c, k, G, l)) * 2;
             long long v0 = (long long)llrint((double)(w * g0) * kFixScale);
             long long v1 = (long long)llrint((double)(w * g1) * kFixScale);
             if (v0) atomicAdd(&grad[e], (unsigned long long)v0);
@@ -413,15 +434,29 @@ struct AdamP {
     int apply;
 };
 
-// Sum the per-wave partials in a fixed order, then Adam.  Keeps fp32 masters,
-// the fp16 weights and the fp16 transposed weights in step.
-__global__ void k_mlp_adam(const float* __restrict__ partial, uint32_t nchunks, float* __restrict__ w32,
+// Deterministic two-stage sum of the per-wave gradient partials: group y of
+// the grid sums chunks [y * per, (y + 1) * per) in order, then k_mlp_adam sums
+// the groups in order.
+constexpr uint32_t kRedGroups = 64;
+__global__ void k_mlp_reduce(const float* __restrict__ partial, uint32_t nchunks, uint32_t per,
+                             float* __restrict__ red) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kMlpParams) return;
+    uint32_t c0 = blockIdx.y * per, c1 = min(nchunks, c0 + per);
+    float g = 0.0f;
+    for (uint32_t c = c0; c < c1; ++c) g += partial[(size_t)c * kMlpParams + p];
+    red[(size_t)blockIdx.y * kMlpParams + p] = g;
+}
+
+// Adam on the reduced gradient.  Keeps fp32 masters, the fp16 weights and the
+// fp16 transposed weights in step.
+__global__ void k_mlp_adam(const float* __restrict__ red, uint32_t ngroups, float* __restrict__ w32,
                            float* __restrict__ m, float* __restrict__ v, _Float16* __restrict__ w16,
                            _Float16* __restrict__ wt16, float* __restrict__ gout, AdamP A) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= kMlpParams) return;
     float g = 0.0f;
-    for (uint32_t c = 0; c < nchunks; ++c) g += partial[(size_t)c * kMlpParams + p];
+    for (uint32_t c = 0; c < ngroups; ++c) g += red[(size_t)c * kMlpParams + p];
     g *= A.inv_scale;
     gout[p] = g;
     if (!A.apply) return;
@@ -562,6 +597,7 @@ snkpt_status cache_reset(snkpt_ctx* ctx, uint32_t seed, const float* lo, const f
     HIPC(C.mlp_m.reserve(kMlpParams * 4));
     HIPC(C.mlp_v.reserve(kMlpParams * 4));
     HIPC(C.mlp_gradf.reserve(kMlpParams * 4));
+    HIPC(C.mlp_red.reserve((size_t)kRedGroups * kMlpParams * 4));
     k_grid_init<<<(uint32_t)((kGridParams + 255) / 256), 256, 0, st>>>(
         C.grid32.as<float>(), C.grid16.as<__half>(), C.grid_m.as<float>(), C.grid_v.as<float>(),
         C.grid_grad.as<unsigned long long>(), kGridParams, 0x6772696400000000ull ^ seed);
@@ -609,7 +645,7 @@ snkpt_status cache_infer(snkpt_ctx* ctx, const float* samples, uint32_t n, float
     if (n == 0) return SNKPT_OK;
     snkpt_status e = ensure_work(ctx, C, n, false);
     if (e != SNKPT_OK) return e;
-    k_encode<<<grid_for(n, 128), 128, 0, st>>>(samples, n, nullptr, C.geom, C.grid16.as<__half2>(),
+    k_encode<<<enc_blocks(n), kEncBlock, 0, st>>>(samples, n, nullptr, C.geom, C.grid16.as<__half2>(),
                                                C.x0.as<_Float16>());
     const uint32_t waves = (n + 63) / 64;
     k_mlp_infer<<<(waves + kInferWaves - 1) / kInferWaves, 64 * kInferWaves, 0, st>>>(
@@ -629,7 +665,7 @@ snkpt_status cache_train(snkpt_ctx* ctx, const float* samples, uint32_t n, float
     snkpt_status e = ensure_work(ctx, C, n, true);
     if (e != SNKPT_OK) return e;
     const uint32_t chunks = (n + 63) / 64;
-    k_encode<<<grid_for(n, 128), 128, 0, st>>>(samples, n, nullptr, C.geom, C.grid16.as<__half2>(),
+    k_encode<<<enc_blocks(n), kEncBlock, 0, st>>>(samples, n, nullptr, C.geom, C.grid16.as<__half2>(),
                                                C.x0.as<_Float16>());
     const float grad_scale = kLossScale / (3.0f * (float)n);
     k_mlp_train<<<chunks, 64, 0, st>>>(C.x0.as<_Float16>(), samples, n, C.mlp16.as<_Float16>(),
@@ -647,7 +683,10 @@ snkpt_status cache_train(snkpt_ctx* ctx, const float* samples, uint32_t n, float
     A.lr_t = (float)((double)lr * std::sqrt(1.0 - std::pow((double)b2, (double)t)) / (1.0 - std::pow((double)b1, (double)t)));
     A.inv_scale = 1.0f / kLossScale;
     A.apply = apply;
-    k_mlp_adam<<<(kMlpParams + 255) / 256, 256, 0, st>>>(C.partial.as<float>(), chunks, C.mlp32.as<float>(),
+    const uint32_t groups = std::min(kRedGroups, chunks), per = (chunks + groups - 1) / groups;
+    k_mlp_reduce<<<dim3((kMlpParams + 255) / 256, groups), 256, 0, st>>>(C.partial.as<float>(), chunks, per,
+                                                                         C.mlp_red.as<float>());
+    k_mlp_adam<<<(kMlpParams + 255) / 256, 256, 0, st>>>(C.mlp_red.as<float>(), groups, C.mlp32.as<float>(),
                                                         C.mlp_m.as<float>(), C.mlp_v.as<float>(),
                                                         C.mlp16.as<_Float16>(), C.mlp16t.as<_Float16>(),
                                                         C.mlp_gradf.as<float>(), A);
@@ -743,8 +782,10 @@ namespace {
 
 // rad[path] += throughput * max(cache output, 0) for every query of this pass.
 __global__ void k_cache_apply(const float4* __restrict__ meta, const float4* __restrict__ y,
-                              const uint32_t* __restrict__ count, float4* __restrict__ rad) {
+                              const uint32_t* __restrict__ count, float4* __restrict__ rad,
+                              unsigned long long* __restrict__ stat) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) *stat += *count;
     if (i >= *count) return;
     float4 m = meta[i], v = y[i];
     uint32_t path = __float_as_uint(m.w);
@@ -907,12 +948,13 @@ snkpt_status cache_pass(snkpt_ctx* ctx, const CamDev& cd, const snkpt_settings* 
     // against the cache vertex's emission); with depth < Q this is plain PT
     if ((e = pt_bounces(ctx, s, 0, 0, st, &H, std::min(depth, Q))) != SNKPT_OK) return e;
     // 2. query the cache
-    k_encode<<<grid_for(n, 128), 128, 0, st>>>(C.q_samples.as<float>(), n, qc, C.geom, C.grid16.as<__half2>(),
+    k_encode<<<enc_blocks(n), kEncBlock, 0, st>>>(C.q_samples.as<float>(), n, qc, C.geom, C.grid16.as<__half2>(),
                                                C.x0.as<_Float16>());
     const uint32_t waves = (n + 63) / 64;
     k_mlp_infer<<<(waves + kInferWaves - 1) / kInferWaves, 64 * kInferWaves, 0, st>>>(
         C.x0.as<_Float16>(), n, qc, C.mlp16.as<_Float16>(), C.y.as<float4>());
-    k_cache_apply<<<grid_for(n, 256), 256, 0, st>>>(C.q_meta.as<float4>(), C.y.as<float4>(), qc, R.rad.as<float4>());
+    k_cache_apply<<<grid_for(n, 256), 256, 0, st>>>(C.q_meta.as<float4>(), C.y.as<float4>(), qc, R.rad.as<float4>(),
+                                                      stats + 3);
     if (s->flags & SNKPT_FLAG_NO_TRAIN) return SNKPT_OK;
     // 3. training paths: full-length unbiased paths through a strided pixel subset
     uint32_t* nrec = C.train_meta.as<uint32_t>();

@@ -42,6 +42,26 @@ __global__ void k_set_counts(uint32_t* c, uint32_t i0, uint32_t v0, uint32_t i1,
     }
 }
 
+// Start of a bounce: count the extension rays queued for it (count_ext) and
+// the shadow rays traced in the previous bounce (count_shadow), then clear the
+// queues this bounce fills.  Counting from the queue sizes keeps per-wave
+// statistics atomics out of the shade kernel.
+__global__ void k_bounce_begin(uint32_t* c, uint32_t cur, uint32_t nxt, unsigned long long* stats,
+                               uint32_t count_ext, uint32_t count_shadow) {
+    if (threadIdx.x == 0) {
+        if (count_ext) stats[1] += c[cur];
+        if (count_shadow) stats[2] += c[2];
+        c[nxt] = 0;
+        c[2] = 0;
+    }
+}
+__global__ void k_bounce_end(uint32_t* c, unsigned long long* stats) {
+    if (threadIdx.x == 0) {
+        stats[2] += c[2];
+        c[2] = 0;
+    }
+}
+
 __global__ void k_generate(CamDev cam, uint32_t w, uint32_t h, uint32_t n, uint32_t start, uint32_t stride,
                            uint64_t initstate, float4* __restrict__ q_org, float4* __restrict__ q_dir,
                            uint32_t* __restrict__ q_path, PathState ps, uint32_t* __restrict__ counts, uint32_t cur,
@@ -176,7 +196,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PtParams P, Queue 
             }
         }
     }
-    uint32_t slot = wave_append(want_ext, &counts[nxt], stats ? &stats[1] : nullptr);
+    __shared__ uint32_t lds_app[kBlock / 64 + 1];
+    uint32_t slot = block_append<kBlock>(want_ext, &counts[nxt], lds_app);
     if (want_ext) {
         out.org[slot] = make_float4(ext_o.x, ext_o.y, ext_o.z, 0.0f);
         out.dir[slot] = make_float4(ext_d.x, ext_d.y, ext_d.z, 1e30f);
@@ -186,14 +207,14 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PtParams P, Queue 
         ps.bounce[path] = bounce + 1;
     }
     (void)new_pdf;
-    uint32_t sslot = wave_append(want_sh, &counts[2], stats ? &stats[2] : nullptr);
+    uint32_t sslot = block_append<kBlock>(want_sh, &counts[2], lds_app);
     if (want_sh) {
         sq.org[sslot] = make_float4(sh_o.x, sh_o.y, sh_o.z, 0.0f);
         sq.dir[sslot] = make_float4(sh_d.x, sh_d.y, sh_d.z, sh_t);
         sq.contrib[sslot] = make_float4(sh_c.x, sh_c.y, sh_c.z, __uint_as_float(path));
     }
     if (KIND == 1) {
-        uint32_t qslot = wave_append(want_q, H.q_count, stats ? &stats[3] : nullptr);
+        uint32_t qslot = block_append<kBlock>(want_q, H.q_count, lds_app);
         if (want_q) {
             float* o = H.q_samples + (size_t)qslot * 16;
 #pragma unroll
@@ -437,7 +458,8 @@ snkpt_status pt_bounces(snkpt_ctx* ctx, const snkpt_settings* s, int cur, uint32
     unsigned long long* stats = R.stats.as<unsigned long long>();
     for (uint32_t b = first; b < P.max_depth; ++b) {
         int nxt = cur ^ 1;
-        k_set_counts<<<1, 64, 0, st>>>(counts, (uint32_t)nxt, 0u, 2u, 0u);
+        k_bounce_begin<<<1, 64, 0, st>>>(counts, (uint32_t)cur, (uint32_t)nxt, stats, b > first ? 1u : 0u,
+                                         b > first ? 1u : 0u);
         snkpt_hits h{R.h_t.as<float>(), R.h_inst.as<uint32_t>(), R.h_prim.as<uint32_t>(), R.h_u.as<float>(),
                      R.h_v.as<float>()};
         snkpt_status e = launch_trace(ctx, R.q[cur].org.as<float4>(), R.q[cur].dir.as<float4>(), n, h, false,
@@ -458,6 +480,7 @@ snkpt_status pt_bounces(snkpt_ctx* ctx, const snkpt_settings* s, int cur, uint32
         if ((e = launch_shadow(ctx, R, rad, st, n)) != SNKPT_OK) return e;
         cur = nxt;
     }
+    if (first < P.max_depth) k_bounce_end<<<1, 64, 0, st>>>(counts, stats);
     return SNKPT_OK;
 }
 
