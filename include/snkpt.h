@@ -129,7 +129,8 @@ typedef struct snkpt_settings {
     uint32_t cache_train_paths;     /* cache mode: training paths per frame */
     float cache_lr;                 /* cache mode: Adam learning rate */
     uint32_t flags;                 /* SNKPT_FLAG_* */
-    uint32_t reserved[5];
+    float cache_ema;                /* cache mode: decay of the weight average used for queries (0 = raw weights) */
+    uint32_t reserved[4];
 } snkpt_settings;
 
 /* Ray and hit streams for snkpt_trace: structure-of-arrays, device pointers.

@@ -28,6 +28,8 @@ struct CacheState {
     uint32_t step = 0;
     DeviceBuffer grid32, grid16, grid_m, grid_v, grid_grad;  // grid_grad: int64 fixed point
     DeviceBuffer mlp32, mlp16, mlp16t, mlp_m, mlp_v, mlp_gradf, grid_gradf;
+    // exponential moving averages of the weights (fp32 and the fp16 copies queries read)
+    DeviceBuffer grid_e32, grid_e16, mlp_e32, mlp_e16;
     DeviceBuffer partial, loss_part, x0, dx0, y, mlp_red;
     uint32_t cap = 0;         // samples the inference work buffers hold
     uint32_t tcap = 0;        // samples the training work buffers hold
@@ -431,6 +433,7 @@ __global__ void k_grid_bwd(const float* __restrict__ samples, const float* __res
 struct AdamP {
     float lr_t;  // lr * sqrt(1 - b2^t) / (1 - b1^t)
     float b1, b2, eps, inv_scale;
+    float ema;   // weight-average decay of this step (0: the average is set to the weights)
     int apply;
 };
 
@@ -452,7 +455,8 @@ __global__ void k_mlp_reduce(const float* __restrict__ partial, uint32_t nchunks
 // fp16 transposed weights in step.
 __global__ void k_mlp_adam(const float* __restrict__ red, uint32_t ngroups, float* __restrict__ w32,
                            float* __restrict__ m, float* __restrict__ v, _Float16* __restrict__ w16,
-                           _Float16* __restrict__ wt16, float* __restrict__ gout, AdamP A) {
+                           _Float16* __restrict__ wt16, float* __restrict__ gout, float* __restrict__ e32,
+                           _Float16* __restrict__ e16, AdamP A) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= kMlpParams) return;
     float g = 0.0f;
@@ -467,6 +471,9 @@ __global__ void k_mlp_adam(const float* __restrict__ red, uint32_t ngroups, floa
     float w = w32[p] - A.lr_t * mm / (sqrtf(vv) + A.eps);
     w32[p] = w;
     w16[p] = (_Float16)w;
+    const float e = A.ema * e32[p] + (1.0f - A.ema) * w;
+    e32[p] = e;
+    e16[p] = (_Float16)e;
     int layer = p / kLayerParams, r = p % kLayerParams;
     int o = r / kWidth, i = r % kWidth;
     if (layer < kHidden)
@@ -476,8 +483,8 @@ __global__ void k_mlp_adam(const float* __restrict__ red, uint32_t ngroups, floa
 }
 
 __global__ void k_grid_adam(unsigned long long* __restrict__ grad, float* __restrict__ w32, float* __restrict__ m,
-                            float* __restrict__ v, __half* __restrict__ w16, float* __restrict__ gout, size_t n,
-                            AdamP A) {
+                            float* __restrict__ v, __half* __restrict__ w16, float* __restrict__ gout,
+                            float* __restrict__ e32, __half* __restrict__ e16, size_t n, AdamP A) {
     size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     long long gi = (long long)grad[p];
@@ -492,6 +499,9 @@ __global__ void k_grid_adam(unsigned long long* __restrict__ grad, float* __rest
     float w = w32[p] - A.lr_t * mm / (sqrtf(vv) + A.eps);
     w32[p] = w;
     w16[p] = __float2half(w);
+    const float e = A.ema * e32[p] + (1.0f - A.ema) * w;
+    e32[p] = e;
+    e16[p] = __float2half(e);
 }
 
 __global__ void k_grid_init(float* __restrict__ w32, __half* __restrict__ w16, float* __restrict__ m,
@@ -573,6 +583,15 @@ static void init_mlp(std::vector<float>& w, uint32_t seed) {
     }
 }
 
+// weight averages := weights
+static hipError_t sync_ema(CacheState& C, hipStream_t st) {
+    hipError_t e;
+    if ((e = hipMemcpyAsync(C.grid_e32.ptr, C.grid32.ptr, kGridParams * 4, hipMemcpyDeviceToDevice, st))) return e;
+    if ((e = hipMemcpyAsync(C.grid_e16.ptr, C.grid16.ptr, kGridParams * 2, hipMemcpyDeviceToDevice, st))) return e;
+    if ((e = hipMemcpyAsync(C.mlp_e32.ptr, C.mlp32.ptr, kMlpParams * 4, hipMemcpyDeviceToDevice, st))) return e;
+    return hipMemcpyAsync(C.mlp_e16.ptr, C.mlp16.ptr, kMlpParams * 2, hipMemcpyDeviceToDevice, st);
+}
+
 snkpt_status cache_reset(snkpt_ctx* ctx, uint32_t seed, const float* lo, const float* hi, hipStream_t st) {
     if (!ctx->cache) ctx->cache = new CacheState();
     CacheState& C = *ctx->cache;
@@ -613,6 +632,11 @@ snkpt_status cache_reset(snkpt_ctx* ctx, uint32_t seed, const float* lo, const f
         k_sync_mlp16<<<(kMlpParams + 255) / 256, 256, 0, st>>>(C.mlp32.as<float>(), C.mlp16.as<_Float16>(),
                                                                C.mlp16t.as<_Float16>());
     }
+    HIPC(C.grid_e32.reserve(kGridParams * 4));
+    HIPC(C.grid_e16.reserve(kGridParams * 2));
+    HIPC(C.mlp_e32.reserve(kMlpParams * 4));
+    HIPC(C.mlp_e16.reserve(kMlpParams * 2));
+    HIPC(sync_ema(C, st));
     HIPC(hipStreamSynchronize(st));
     C.step = 0;
     C.ready = true;
@@ -658,6 +682,13 @@ snkpt_status cache_infer(snkpt_ctx* ctx, const float* samples, uint32_t n, float
 
 snkpt_status cache_train(snkpt_ctx* ctx, const float* samples, uint32_t n, float lr, int apply, float* loss_out,
                          hipStream_t st) {
+    return cache_train_ema(ctx, samples, n, lr, apply, 0.0f, loss_out, st);
+}
+
+// One optimisation step; `ema` is the decay of the weight averages the render
+// queries read (0 sets them to the new weights).
+snkpt_status cache_train_ema(snkpt_ctx* ctx, const float* samples, uint32_t n, float lr, int apply, float ema,
+                             float* loss_out, hipStream_t st) {
     CacheState* Cp = ctx->cache;
     if (!Cp || !Cp->ready) return fail(ctx, SNKPT_ERR_NOT_BUILT, "cache_train: call snkpt_cache_reset first");
     CacheState& C = *Cp;
@@ -683,13 +714,16 @@ snkpt_status cache_train(snkpt_ctx* ctx, const float* samples, uint32_t n, float
     A.lr_t = (float)((double)lr * std::sqrt(1.0 - std::pow((double)b2, (double)t)) / (1.0 - std::pow((double)b1, (double)t)));
     A.inv_scale = 1.0f / kLossScale;
     A.apply = apply;
+    // decay ramp (1 + t) / (10 + t) so early averages are not dominated by the initial weights
+    A.ema = std::min(ema, (1.0f + (float)t) / (10.0f + (float)t));
     const uint32_t groups = std::min(kRedGroups, chunks), per = (chunks + groups - 1) / groups;
     k_mlp_reduce<<<dim3((kMlpParams + 255) / 256, groups), 256, 0, st>>>(C.partial.as<float>(), chunks, per,
                                                                          C.mlp_red.as<float>());
     k_mlp_adam<<<(kMlpParams + 255) / 256, 256, 0, st>>>(C.mlp_red.as<float>(), groups, C.mlp32.as<float>(),
                                                         C.mlp_m.as<float>(), C.mlp_v.as<float>(),
                                                         C.mlp16.as<_Float16>(), C.mlp16t.as<_Float16>(),
-                                                        C.mlp_gradf.as<float>(), A);
+                                                        C.mlp_gradf.as<float>(), C.mlp_e32.as<float>(),
+                                                        C.mlp_e16.as<_Float16>(), A);
     // grid: the fixed-point gradient is consumed (and cleared) every call; without
     // Adam it is kept as floats so it can be read back
     float* gout = nullptr;
@@ -699,7 +733,7 @@ snkpt_status cache_train(snkpt_ctx* ctx, const float* samples, uint32_t n, float
     }
     k_grid_adam<<<(uint32_t)((kGridParams + 255) / 256), 256, 0, st>>>(
         C.grid_grad.as<unsigned long long>(), C.grid32.as<float>(), C.grid_m.as<float>(), C.grid_v.as<float>(),
-        C.grid16.as<__half>(), gout, kGridParams, A);
+        C.grid16.as<__half>(), gout, C.grid_e32.as<float>(), C.grid_e16.as<__half>(), kGridParams, A);
     C.last_n = n;
     if (loss_out) {
         std::vector<float> lp(chunks);
@@ -774,6 +808,7 @@ snkpt_status cache_set_params(snkpt_ctx* ctx, uint32_t which, const void* src, s
     } else {
         return fail(ctx, SNKPT_ERR_INVALID_ARG, "cache_set_params: only 0 (MLP) and 1 (grid) can be set");
     }
+    HIPC(sync_ema(C, st));
     HIPC(hipStreamSynchronize(st));
     return SNKPT_OK;
 }
@@ -948,11 +983,12 @@ snkpt_status cache_pass(snkpt_ctx* ctx, const CamDev& cd, const snkpt_settings* 
     // against the cache vertex's emission); with depth < Q this is plain PT
     if ((e = pt_bounces(ctx, s, 0, 0, st, &H, std::min(depth, Q))) != SNKPT_OK) return e;
     // 2. query the cache
-    k_encode<<<enc_blocks(n), kEncBlock, 0, st>>>(C.q_samples.as<float>(), n, qc, C.geom, C.grid16.as<__half2>(),
-                                               C.x0.as<_Float16>());
+    const bool avg = s->cache_ema > 0.0f;  // query the weight averages
+    k_encode<<<enc_blocks(n), kEncBlock, 0, st>>>(C.q_samples.as<float>(), n, qc, C.geom,
+                                                  (avg ? C.grid_e16 : C.grid16).as<__half2>(), C.x0.as<_Float16>());
     const uint32_t waves = (n + 63) / 64;
     k_mlp_infer<<<(waves + kInferWaves - 1) / kInferWaves, 64 * kInferWaves, 0, st>>>(
-        C.x0.as<_Float16>(), n, qc, C.mlp16.as<_Float16>(), C.y.as<float4>());
+        C.x0.as<_Float16>(), n, qc, (avg ? C.mlp_e16 : C.mlp16).as<_Float16>(), C.y.as<float4>());
     k_cache_apply<<<grid_for(n, 256), 256, 0, st>>>(C.q_meta.as<float4>(), C.y.as<float4>(), qc, R.rad.as<float4>(),
                                                       stats + 3);
     if (s->flags & SNKPT_FLAG_NO_TRAIN) return SNKPT_OK;
@@ -987,7 +1023,8 @@ snkpt_status cache_pass(snkpt_ctx* ctx, const CamDev& cd, const snkpt_settings* 
     HIPC(hipStreamSynchronize(st));
     (void)pass;
     if (nt == 0) return SNKPT_OK;
-    return cache_train(ctx, C.t_samples.as<float>(), nt, s->cache_lr > 0.0f ? s->cache_lr : 1e-2f, 1, nullptr, st);
+    return cache_train_ema(ctx, C.t_samples.as<float>(), nt, s->cache_lr > 0.0f ? s->cache_lr : 1e-2f, 1,
+                           std::min(std::max(s->cache_ema, 0.0f), 0.9999f), nullptr, st);
 }
 
 }  // namespace snkpt

@@ -223,7 +223,8 @@ __global__ __launch_bounds__(kBlock) void k_primary(SceneDev S, GView G, Queue i
         G.p[i] = gp;
         store_res(r_out, m_out, i, r);
     }
-    uint32_t slot = wave_append(want_ext, &counts[1], stats ? &stats[1] : nullptr);
+    __shared__ uint32_t lds_app[kBlock / 64 + 1];
+    uint32_t slot = block_append<kBlock>(want_ext, &counts[1], lds_app);
     if (want_ext) {
         out.org[slot] = make_float4(ext_o.x, ext_o.y, ext_o.z, 0.0f);
         out.dir[slot] = make_float4(ext_d.x, ext_d.y, ext_d.z, 1e30f);
@@ -379,11 +380,20 @@ __global__ __launch_bounds__(kBlock) void k_rshade(SceneDev S, GView G, const fl
             }
         }
     }
-    uint32_t slot = wave_append(want, &counts[2], stats ? &stats[2] : nullptr);
+    __shared__ uint32_t lds_app[kBlock / 64 + 1];
+    uint32_t slot = block_append<kBlock>(want, &counts[2], lds_app);
     if (want) {
         sq.org[slot] = make_float4(o.x, o.y, o.z, 0.0f);
         sq.dir[slot] = make_float4(wi.x, wi.y, wi.z, tmax);
         sq.contrib[slot] = make_float4(c.x, c.y, c.z, __uint_as_float(i));
+    }
+}
+
+// extension rays queued by k_primary and shadow rays of k_rshade
+__global__ void k_restir_stats(const uint32_t* c, unsigned long long* stats) {
+    if (threadIdx.x == 0) {
+        stats[1] += c[1];
+        stats[2] += c[2];
     }
 }
 
@@ -465,6 +475,7 @@ snkpt_status restir_pass(snkpt_ctx* ctx, const snkpt_camera* cam, const CamDev& 
     k_rshade<<<grid_for(n, kBlock), kBlock, 0, st>>>(S, G, X.r_fin.as<float4>(), X.m_fin.as<float>(), shadow_of(R),
                                                      counts, n, stats);
     if ((e = launch_shadow(ctx, R, R.rad.as<float4>(), st)) != SNKPT_OK) return e;
+    k_restir_stats<<<1, 64, 0, st>>>(counts, stats);
     // history for the next frame
     std::memcpy(X.prev_pv, cam->proj_view, sizeof(X.prev_pv));
     X.prev_pos[0] = cam->cam_pos[0];

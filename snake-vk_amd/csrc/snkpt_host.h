@@ -136,6 +136,8 @@ snkpt_status cache_reset(snkpt_ctx* ctx, uint32_t seed, const float* lo, const f
 snkpt_status cache_infer(snkpt_ctx* ctx, const float* samples, uint32_t n, float* out, hipStream_t st);
 snkpt_status cache_train(snkpt_ctx* ctx, const float* samples, uint32_t n, float lr, int apply, float* loss_out,
                          hipStream_t st);
+snkpt_status cache_train_ema(snkpt_ctx* ctx, const float* samples, uint32_t n, float lr, int apply, float ema,
+                             float* loss_out, hipStream_t st);
 snkpt_status cache_params(snkpt_ctx* ctx, uint32_t which, void* dst, size_t bytes, size_t* out_bytes, hipStream_t st);
 snkpt_status cache_set_params(snkpt_ctx* ctx, uint32_t which, const void* src, size_t bytes, hipStream_t st);
 

@@ -79,7 +79,7 @@ class Settings(C.Structure):
                 ("restir_candidates", C.c_uint32), ("restir_m_cap", C.c_uint32),
                 ("restir_radius", C.c_float), ("cache_query_depth", C.c_uint32),
                 ("cache_train_paths", C.c_uint32), ("cache_lr", C.c_float), ("flags", C.c_uint32),
-                ("reserved", C.c_uint32 * 5)]
+                ("cache_ema", C.c_float), ("reserved", C.c_uint32 * 4)]
 
 
 class Rays(C.Structure):
@@ -408,6 +408,7 @@ def default_settings(**kw) -> Settings:
     s.cache_query_depth = 2
     s.cache_train_paths = 65536
     s.cache_lr = 1e-2
+    s.cache_ema = 0.99
     s.flags = 0
     for k, v in kw.items():
         setattr(s, k, v)
