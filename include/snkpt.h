@@ -309,6 +309,50 @@ snkpt_status snkpt_scene_camera(uint32_t config, uint32_t variant, uint32_t fram
                                 uint32_t h, snkpt_camera* out);
 /* upload_mesh + set_instances + set_materials + set_lights for a generated scene. */
 snkpt_status snkpt_scene_load(snkpt_ctx* ctx, const snkpt_scene* scene);
+/* Batched workload (config 5): view v (0..63) shows scene variant v / 4 at a
+ * w x h resolution (side log-uniform in [128, 1024]). */
+snkpt_status snkpt_scene_view(uint32_t config, uint32_t view, uint32_t* out_variant, uint32_t* out_w,
+                              uint32_t* out_h);
+
+/* ---- batched rendering (no reference counterpart: RT::RecordRenderCmdBuf,
+ * Raytracing.cpp:227-267, renders one view of one scene per command buffer) ----
+ * Many views of many scenes share one set of wavefront queues; each stage is
+ * one persistent launch per bounce over every view, and a frame can be
+ * captured once as a hipGraph and replayed.  Every view's image is
+ * bit-identical to snkpt_render of that view alone with the view's seed
+ * (plain path tracing, 1 spp per frame).  Scenes are borrowed contexts with a
+ * built acceleration structure; they and the output buffers must outlive the
+ * batch.  A batch is externally synchronised like a context. */
+typedef struct snkpt_batch snkpt_batch;
+typedef struct snkpt_batch_tuning {
+    uint32_t trace_block;         /* lanes per traversal block: 64 or 128 (extension and shadow rays) */
+    uint32_t trace_blocks_per_cu; /* persistent traversal grid = CUs x this */
+    uint32_t shade_blocks_per_cu; /* persistent shade grid (256-lane blocks) = CUs x this */
+    uint32_t queue_capacity;      /* paths per group of views processed together; 0 = all views at once */
+    float ms_per_frame;           /* set by snkpt_batch_autotune: best measured frame time */
+    uint32_t reserved[3];
+} snkpt_batch_tuning;
+snkpt_status snkpt_batch_create(int hip_device, snkpt_batch** out);
+void snkpt_batch_destroy(snkpt_batch* batch);
+const char* snkpt_batch_last_error(const snkpt_batch* batch);
+snkpt_status snkpt_batch_add_scene(snkpt_batch* batch, snkpt_ctx* scene, uint32_t* out_scene);
+/* out_rgba_device: w*h RGBA pixels in out_format, row 0 at the top. */
+snkpt_status snkpt_batch_add_view(snkpt_batch* batch, uint32_t scene, const snkpt_camera* cam, uint32_t w,
+                                  uint32_t h, uint32_t seed, void* out_rgba_device, uint32_t out_format,
+                                  uint32_t* out_view);
+/* One frame of every view.  Uses mode (PT only), max_depth, rr_start, frame
+ * and the ACCUMULATE / RESET flags of `s`.  use_graph: capture the frame on the
+ * first call (or after the views, tuning, depth or stream change) and replay
+ * it afterwards; needs a created (non-null) stream. */
+snkpt_status snkpt_batch_render(snkpt_batch* batch, const snkpt_settings* s, int use_graph, snkpt_stream stream);
+/* Times whole frames of this batch over traversal block size, persistent grid
+ * sizes and queue capacity, keeps the fastest (images do not depend on it). */
+snkpt_status snkpt_batch_autotune(snkpt_batch* batch, const snkpt_settings* s, snkpt_stream stream,
+                                  snkpt_batch_tuning* out);
+snkpt_status snkpt_batch_set_tuning(snkpt_batch* batch, const snkpt_batch_tuning* tuning);
+snkpt_status snkpt_batch_get_tuning(const snkpt_batch* batch, snkpt_batch_tuning* out);
+/* Ray counts of the last frame (waits for the stream). */
+snkpt_status snkpt_batch_frame_stats(snkpt_batch* batch, snkpt_frame_stats* out, snkpt_stream stream);
 
 #ifdef __cplusplus
 }

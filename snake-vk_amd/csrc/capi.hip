@@ -325,6 +325,14 @@ snkpt_status snkpt_scene_camera(uint32_t config, uint32_t variant, uint32_t fram
     return scene_camera(config, variant, frame, w, h, out);
 }
 
+snkpt_status snkpt_scene_view(uint32_t config, uint32_t view, uint32_t* out_variant, uint32_t* out_w,
+                              uint32_t* out_h) {
+    if (config != 5 || view >= 64 || !out_variant || !out_w || !out_h) return SNKPT_ERR_INVALID_ARG;
+    *out_variant = view / 4;
+    *out_w = *out_h = c5_view_side(view);
+    return SNKPT_OK;
+}
+
 snkpt_status snkpt_scene_load(snkpt_ctx* ctx, const snkpt_scene* s) {
     if (!ctx || !s) return SNKPT_ERR_INVALID_ARG;
     ctx->pos.clear();

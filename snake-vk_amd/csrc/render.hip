@@ -21,6 +21,7 @@
 #include <cstring>
 
 #include "render_common.h"
+#include "pt_step.h"
 #include "render_modes.h"
 
 namespace snkpt {
@@ -81,148 +82,17 @@ __global__ void k_generate(CamDev cam, uint32_t w, uint32_t h, uint32_t n, uint3
     ps.bounce[i] = 0;
 }
 
-// One path vertex of the unbiased path tracer (KIND: see ShadeHooks).
+// One path vertex of the unbiased path tracer over the queue of one scene.
 template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PtParams P, Queue in, HitArrays hit, Queue out,
                                                   ShadowQueue sq, PathState ps, float4* __restrict__ rad,
                                                   uint32_t* __restrict__ counts, uint32_t cur, uint32_t nxt,
                                                   unsigned long long* __restrict__ stats, ShadeHooks H) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = i < counts[cur];
-    bool want_ext = false, want_sh = false, want_q = false;
-    float qs[13];
-    V3 ext_o, ext_d, sh_o, sh_d, sh_c;
-    float sh_t = 0.0f;
-    uint32_t path = 0;
-    float4 thr4 = make_float4(0, 0, 0, 0);
-    Rng rng{0, 0};
-    uint32_t bounce = 0;
-    float new_pdf = 0.0f;
-    if (valid) {
-        path = in.path[i];
-        thr4 = ps.thr[path];
-        uint2 rs = ps.rng[path];
-        rng.state = (uint64_t)rs.x | ((uint64_t)rs.y << 32);
-        rng.inc = ((uint64_t)path << 1u) | 1u;
-        bounce = ps.bounce[path];
-        float t = hit.t[i];
-        if (t >= 0.0f) {
-            float4 dq = in.dir[i];
-            V3 dir{dq.x, dq.y, dq.z};
-            V3 thr{thr4.x, thr4.y, thr4.z};
-            Surf s = surface_at(S, dir, hit.inst[i], hit.prim[i], hit.u[i], hit.v[i]);
-            if (s.emissive) {
-                // MIS against the NEE of the previous vertex; a negative stored pdf
-                // marks a vertex whose direct light came from ReSTIR (weight 0 here)
-                float w = 1.0f;
-                if (bounce > 0)
-                    w = thr4.w < 0.0f ? 0.0f : power_heuristic(thr4.w, light_pdf(S, s.gid, s.ng, dir, t));
-                V3 c = mul(thr, s.le) * w;
-                float4 r = rad[path];
-                rad[path] = make_float4(r.x + c.x, r.y + c.y, r.z + c.z, r.w);
-            } else if (KIND == 1 && bounce == H.query_at && !H.indirect) {
-                // terminate into the radiance cache: L_o(x, wo) from the network
-                V3 wo = dir * -1.0f;
-                want_q = true;
-                float q[13] = {s.p.x, s.p.y, s.p.z, wo.x, wo.y, wo.z, s.ns.x, s.ns.y, s.ns.z, s.m.roughness,
-                               s.m.albedo.x, s.m.albedo.y, s.m.albedo.z};
-#pragma unroll
-                for (int k = 0; k < 13; ++k) qs[k] = q[k];
-            } else {
-                V3 wo = dir * -1.0f;
-                // indirect split: direct light here by NEE alone, the cache answers the rest
-                const bool query = KIND == 1 && bounce == H.query_at;
-                if (query) {
-                    want_q = true;
-                    float q[13] = {s.p.x, s.p.y, s.p.z, wo.x, wo.y, wo.z, s.ns.x, s.ns.y, s.ns.z, s.m.roughness,
-                                   s.m.albedo.x, s.m.albedo.y, s.m.albedo.z};
-#pragma unroll
-                    for (int k = 0; k < 13; ++k) qs[k] = q[k];
-                }
-                if (KIND == 2 && bounce < H.max_rec) {
-                    // training record: sample fields, radiance so far, throughput at arrival
-                    float4 r = rad[path];
-                    float* o = H.rec + ((size_t)path * H.max_rec + bounce) * kRecFloats;
-                    float q[kRecFloats] = {s.p.x, s.p.y, s.p.z, wo.x, wo.y, wo.z, s.ns.x, s.ns.y, s.ns.z,
-                                           s.m.roughness, s.m.albedo.x, s.m.albedo.y, s.m.albedo.z, r.x, r.y, r.z,
-                                           thr.x, thr.y, thr.z, 0.0f};
-#pragma unroll
-                    for (int k = 0; k < kRecFloats; ++k) o[k] = q[k];
-                    H.nrec[path] = bounce + 1;
-                }
-                float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform(), u3 = rng.uniform();
-                float ul = rng.uniform(), us1 = rng.uniform(), us2 = rng.uniform(), urr = rng.uniform();
-                const bool last = bounce + 1 >= P.max_depth || query;
-                // next-event estimation
-                LightSample ls = sample_light(S, s.p, u0, u1, u2, u3);
-                if (ls.ok) {
-                    float cs = dot(s.ns, ls.wi);
-                    if (cs > 0.0f && dot(s.ng, ls.wi) > 0.0f) {
-                        V3 f = brdf_eval(s.m, s.ns, wo, ls.wi);
-                        float w = last ? 1.0f : power_heuristic(ls.pdf, brdf_pdf(s.m, s.ns, wo, ls.wi));
-                        sh_c = mul(mul(thr, f), ls.le) * (cs * w / ls.pdf);
-                        if (sh_c.x > 0.0f || sh_c.y > 0.0f || sh_c.z > 0.0f) {
-                            want_sh = true;
-                            sh_o = s.p + s.ng * s.eps;
-                            sh_d = ls.wi;
-                            sh_t = ls.dist * (1.0f - 1e-4f) - s.eps;
-                        }
-                    }
-                }
-                // continuation
-                V3 wi;
-                if (!last && brdf_sample(s.m, s.ns, wo, ul, us1, us2, wi) && dot(s.ng, wi) > 0.0f) {
-                    float pdf = brdf_pdf(s.m, s.ns, wo, wi);
-                    if (pdf > 0.0f) {
-                        V3 f = brdf_eval(s.m, s.ns, wo, wi);
-                        V3 nt = mul(thr, f) * (dot(s.ns, wi) / pdf);
-                        bool alive = nt.x > 0.0f || nt.y > 0.0f || nt.z > 0.0f;
-                        if (alive && P.rr_start > 0 && bounce + 1 >= P.rr_start) {
-                            float q = fminf(fmaxf(fmaxf(nt.x, nt.y), nt.z), 0.95f);
-                            if (urr >= q)
-                                alive = false;
-                            else
-                                nt = nt * (1.0f / q);
-                        }
-                        if (alive) {
-                            want_ext = true;
-                            ext_o = s.p + s.ng * s.eps;
-                            ext_d = wi;
-                            thr4 = make_float4(nt.x, nt.y, nt.z, pdf);
-                            new_pdf = pdf;
-                        }
-                    }
-                }
-            }
-        }
-    }
     __shared__ uint32_t lds_app[kBlock / 64 + 1];
-    uint32_t slot = block_append<kBlock>(want_ext, &counts[nxt], lds_app);
-    if (want_ext) {
-        out.org[slot] = make_float4(ext_o.x, ext_o.y, ext_o.z, 0.0f);
-        out.dir[slot] = make_float4(ext_d.x, ext_d.y, ext_d.z, 1e30f);
-        out.path[slot] = path;
-        ps.thr[path] = thr4;
-        ps.rng[path] = make_uint2((uint32_t)rng.state, (uint32_t)(rng.state >> 32));
-        ps.bounce[path] = bounce + 1;
-    }
-    (void)new_pdf;
-    uint32_t sslot = block_append<kBlock>(want_sh, &counts[2], lds_app);
-    if (want_sh) {
-        sq.org[sslot] = make_float4(sh_o.x, sh_o.y, sh_o.z, 0.0f);
-        sq.dir[sslot] = make_float4(sh_d.x, sh_d.y, sh_d.z, sh_t);
-        sq.contrib[sslot] = make_float4(sh_c.x, sh_c.y, sh_c.z, __uint_as_float(path));
-    }
-    if (KIND == 1) {
-        uint32_t qslot = block_append<kBlock>(want_q, H.q_count, lds_app);
-        if (want_q) {
-            float* o = H.q_samples + (size_t)qslot * 16;
-#pragma unroll
-            for (int k = 0; k < 13; ++k) o[k] = qs[k];
-            o[13] = o[14] = o[15] = 0.0f;
-            H.q_meta[qslot] = make_float4(thr4.x, thr4.y, thr4.z, __uint_as_float(path));
-        }
-    }
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const SceneOne ss{S};
+    (void)stats;
+    shade_step<KIND, kBlock>(ss, P, in, hit, out, sq, ps, rad, counts, nxt, H, i, i < counts[cur], lds_app);
 }
 
 template <int STACK>
@@ -233,16 +103,7 @@ __global__ __launch_bounds__(kShadowBlock) void k_shadow(const uint4* __restrict
     __shared__ uint2 stk[STACK * kShadowBlock];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= counts[2]) return;
-    float4 a = sq.org[i], d = sq.dir[i];
-    Ray r{a.x, a.y, a.z, a.w, d.x, d.y, d.z, d.w};
-    uint32_t nv = 0, tv = 0;
-    TraceResult res = traverse<true, false, STACK>(nodes, tris, r, &stk[threadIdx.x], kShadowBlock, nv, tv);
-    if (res.rec == 0xFFFFFFFFu) {
-        float4 c = sq.contrib[i];
-        uint32_t px = __float_as_uint(c.w);
-        float4 o = rad[px];
-        rad[px] = make_float4(o.x + c.x, o.y + c.y, o.z + c.z, o.w);
-    }
+    shadow_step<STACK>(nodes, tris, sq, i, rad, &stk[threadIdx.x], kShadowBlock);
 }
 
 __global__ void k_resolve(const float4* __restrict__ rad, float4* __restrict__ accum, void* __restrict__ out,

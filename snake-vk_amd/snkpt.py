@@ -45,7 +45,9 @@ EXPORTS = [
     "snkpt_accel_export", "snkpt_trace", "snkpt_trace_counted", "snkpt_render", "snkpt_get_frame_stats",
     "snkpt_oracle_accumulate", "snkpt_scene_generate", "snkpt_scene_free", "snkpt_scene_camera",
     "snkpt_scene_load", "snkpt_cache_reset", "snkpt_cache_infer", "snkpt_cache_train", "snkpt_cache_params",
-    "snkpt_cache_set_params",
+    "snkpt_cache_set_params", "snkpt_scene_view", "snkpt_batch_create", "snkpt_batch_destroy",
+    "snkpt_batch_last_error", "snkpt_batch_add_scene", "snkpt_batch_add_view", "snkpt_batch_render",
+    "snkpt_batch_autotune", "snkpt_batch_set_tuning", "snkpt_batch_get_tuning", "snkpt_batch_frame_stats",
 ]
 CACHE_SAMPLE_FLOATS = 16
 CACHE_MLP_PARAMS = 4 * 4096 + 16 * 64
@@ -102,6 +104,17 @@ class FrameStats(C.Structure):
                 ("cache_queries", C.c_uint64), ("cache_train_samples", C.c_uint64)]
 
 
+class BatchTuning(C.Structure):
+    _fields_ = [("trace_block", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32),
+                ("shade_blocks_per_cu", C.c_uint32), ("queue_capacity", C.c_uint32), ("ms_per_frame", C.c_float),
+                ("reserved", C.c_uint32 * 3)]
+
+    def as_dict(self):
+        return {"trace_block": self.trace_block, "trace_blocks_per_cu": self.trace_blocks_per_cu,
+                "shade_blocks_per_cu": self.shade_blocks_per_cu, "queue_capacity": self.queue_capacity,
+                "ms_per_frame": round(self.ms_per_frame, 4)}
+
+
 class Scene(C.Structure):
     _fields_ = [("pos", C.POINTER(C.c_float)), ("nrm", C.POINTER(C.c_float)), ("uv", C.POINTER(C.c_float)),
                 ("tan", C.POINTER(C.c_float)), ("n_verts", C.c_uint32), ("idx", C.POINTER(C.c_uint32)),
@@ -151,6 +164,17 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "snkpt_cache_train": (I32, [VP, VP, U32, C.c_float, C.c_int, P(C.c_float), VP]),
         "snkpt_cache_params": (I32, [VP, U32, VP, SZ, P(SZ)]),
         "snkpt_cache_set_params": (I32, [VP, U32, VP, SZ]),
+        "snkpt_scene_view": (I32, [U32, U32, P(U32), P(U32), P(U32)]),
+        "snkpt_batch_create": (I32, [C.c_int, P(VP)]),
+        "snkpt_batch_destroy": (None, [VP]),
+        "snkpt_batch_last_error": (C.c_char_p, [VP]),
+        "snkpt_batch_add_scene": (I32, [VP, VP, P(U32)]),
+        "snkpt_batch_add_view": (I32, [VP, U32, P(Camera), U32, U32, U32, VP, U32, P(U32)]),
+        "snkpt_batch_render": (I32, [VP, P(Settings), C.c_int, VP]),
+        "snkpt_batch_autotune": (I32, [VP, P(Settings), VP, P(BatchTuning)]),
+        "snkpt_batch_set_tuning": (I32, [VP, P(BatchTuning)]),
+        "snkpt_batch_get_tuning": (I32, [VP, P(BatchTuning)]),
+        "snkpt_batch_frame_stats": (I32, [VP, P(FrameStats), VP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -216,6 +240,83 @@ class SceneData:
         if getattr(self, "_p", None) and _lib is not None:
             _lib.snkpt_scene_free(self._p)
             self._p = None
+
+
+def scene_view(config: int, view: int):
+    """(variant, w, h) of view `view` of the batched workload (config 5)."""
+    v, w, h = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    rc = load().snkpt_scene_view(config, view, C.byref(v), C.byref(w), C.byref(h))
+    if rc != OK:
+        raise SnkptError(rc, "scene_view")
+    return v.value, w.value, h.value
+
+
+class Batch:
+    """Many views of many scenes rendered together (snkpt_batch_*).  Scenes
+    are Contexts with a built acceleration structure; outputs are device
+    tensors (h, w, 4) float32 or float16 that must stay alive."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        self._h = C.c_void_p()
+        rc = self.lib.snkpt_batch_create(device, C.byref(self._h))
+        if rc != OK:
+            raise SnkptError(rc, "snkpt_batch_create")
+        self._keep = []
+
+    def _check(self, rc: int, what: str):
+        if rc != OK:
+            msg = self.lib.snkpt_batch_last_error(self._h)
+            raise SnkptError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def add_scene(self, ctx) -> int:
+        out = C.c_uint32()
+        self._check(self.lib.snkpt_batch_add_scene(self._h, ctx._h, C.byref(out)), "snkpt_batch_add_scene")
+        self._keep.append(ctx)
+        return out.value
+
+    def add_view(self, scene: int, cam: Camera, w: int, h: int, seed: int, out, fmt: int = FORMAT_RGBA32F) -> int:
+        v = C.c_uint32()
+        self._check(self.lib.snkpt_batch_add_view(self._h, scene, C.byref(cam), w, h, seed, _ptr(out), fmt,
+                                                  C.byref(v)), "snkpt_batch_add_view")
+        self._keep.append(out)
+        return v.value
+
+    def render(self, settings: Settings, use_graph: bool = False, stream=None):
+        self._check(self.lib.snkpt_batch_render(self._h, C.byref(settings), int(use_graph), _stream(stream)),
+                    "snkpt_batch_render")
+
+    def autotune(self, settings: Settings, stream=None) -> BatchTuning:
+        t = BatchTuning()
+        self._check(self.lib.snkpt_batch_autotune(self._h, C.byref(settings), _stream(stream), C.byref(t)),
+                    "snkpt_batch_autotune")
+        return t
+
+    def set_tuning(self, t: BatchTuning):
+        self._check(self.lib.snkpt_batch_set_tuning(self._h, C.byref(t)), "snkpt_batch_set_tuning")
+
+    def tuning(self) -> BatchTuning:
+        t = BatchTuning()
+        self._check(self.lib.snkpt_batch_get_tuning(self._h, C.byref(t)), "snkpt_batch_get_tuning")
+        return t
+
+    def frame_stats(self, stream=None) -> FrameStats:
+        st = FrameStats()
+        self._check(self.lib.snkpt_batch_frame_stats(self._h, C.byref(st), _stream(stream)),
+                    "snkpt_batch_frame_stats")
+        return st
+
+    def close(self):
+        if self._h:
+            self.lib.snkpt_batch_destroy(self._h)
+            self._h = C.c_void_p()
+        self._keep = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Context:

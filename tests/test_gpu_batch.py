@@ -1,0 +1,171 @@
+"""Batched rendering (snkpt_batch_*): many views of several scenes in one set
+of queues must give, view by view, exactly the image snkpt_render gives for
+that view alone (same per-view seed, same frame); grouping, launch shapes,
+graph capture/replay and the autotuner must not change a single bit."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+# (config, variant) of the scenes; (scene, w, h, seed) of the views
+SCENES = [(5, 1), (5, 2), (1, 0), (5, 11)]
+VIEWS = [(0, 96, 64, 11), (1, 128, 128, 12), (2, 200, 150, 13), (0, 64, 64, 14), (3, 150, 100, 15),
+         (2, 33, 17, 16)]
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _setup():
+    import torch
+
+    import snkpt
+    scenes, ctxs = [], []
+    for cfg, var in SCENES:
+        sd = snkpt.SceneData(cfg, var)
+        c = snkpt.Context(0)
+        c.load_scene(sd)
+        c.build_accel()
+        scenes.append(sd)
+        ctxs.append(c)
+    b = snkpt.Batch(0)
+    for c in ctxs:
+        b.add_scene(c)
+    outs, cams = [], []
+    for k, (si, w, h, seed) in enumerate(VIEWS):
+        cam = scenes[si].camera(k, w, h)
+        out = torch.zeros(h, w, 4, device="cuda")
+        b.add_view(si, cam, w, h, seed, out)
+        outs.append(out)
+        cams.append(cam)
+    return scenes, ctxs, b, outs, cams
+
+
+def _settings(**kw):
+    import snkpt
+    base = dict(mode=snkpt.MODE_PT, max_depth=6, rr_start=3)
+    base.update(kw)
+    return snkpt.default_settings(**base)
+
+
+def _single(ctxs, cams, frame):
+    import torch
+    res = []
+    for k, (si, w, h, seed) in enumerate(VIEWS):
+        o = torch.zeros(h, w, 4, device="cuda")
+        ctxs[si].render(cams[k], _settings(seed=seed, frame=frame), o, w, h)
+        res.append(o.cpu().numpy())
+    return res
+
+
+def _grab(outs):
+    import torch
+    torch.cuda.synchronize()
+    return [o.cpu().numpy().copy() for o in outs]
+
+
+def test_batch_equals_single_view_renders(gpu):
+    scenes, ctxs, b, outs, cams = _setup()
+    for frame in (0, 3):
+        b.render(_settings(frame=frame))
+        got = _grab(outs)
+        want = _single(ctxs, cams, frame)
+        for k in range(len(VIEWS)):
+            assert np.array_equal(got[k], want[k]), (frame, k, np.abs(got[k] - want[k]).max())
+        _say(f"frame {frame}: {len(VIEWS)} views bit-identical, mean {np.mean([g.mean() for g in got]):.5f}")
+    st = b.frame_stats()
+    assert st.camera_rays == sum(w * h for _, w, h, _ in VIEWS)
+    assert st.extension_rays > 0 and st.shadow_rays > 0
+
+
+def test_grouping_and_launch_shapes_do_not_change_images(gpu):
+    import snkpt
+    scenes, ctxs, b, outs, cams = _setup()
+    b.render(_settings(frame=5))
+    ref = _grab(outs)
+    for tb, tpc, spc, cap in ((64, 4, 2, 20000), (128, 32, 16, 1), (64, 8, 4, 0)):
+        t = snkpt.BatchTuning()
+        t.trace_block, t.trace_blocks_per_cu, t.shade_blocks_per_cu, t.queue_capacity = tb, tpc, spc, cap
+        b.set_tuning(t)
+        for o in outs:
+            o.zero_()
+        b.render(_settings(frame=5))
+        got = _grab(outs)
+        for k in range(len(VIEWS)):
+            assert np.array_equal(got[k], ref[k]), (tb, tpc, spc, cap, k)
+
+
+def test_graph_replays_match_eager_frames(gpu):
+    import torch
+    scenes, ctxs, b, outs, cams = _setup()
+    st = torch.cuda.Stream()
+    eager = []
+    for f in range(4):
+        b.render(_settings(frame=f), use_graph=False, stream=st)
+        st.synchronize()
+        eager.append(_grab(outs))
+    for f in range(4):
+        for o in outs:
+            o.zero_()  # the program's own eager work between replays
+        torch.cuda.synchronize()
+        b.render(_settings(frame=f), use_graph=True, stream=st)
+        st.synchronize()
+        got = _grab(outs)
+        for k in range(len(VIEWS)):
+            assert np.array_equal(got[k], eager[f][k]), (f, k)
+    _say("4 graph replays bit-identical to eager frames")
+
+
+def test_accumulation_is_the_running_mean(gpu):
+    import snkpt
+    scenes, ctxs, b, outs, cams = _setup()
+    frames = []
+    for f in range(3):
+        b.render(_settings(frame=f))
+        frames.append(_grab(outs))
+    for f in range(3):
+        fl = snkpt.FLAG_ACCUMULATE | (snkpt.FLAG_RESET if f == 0 else 0)
+        b.render(_settings(frame=f, flags=fl))
+    got = _grab(outs)
+    for k in range(len(VIEWS)):
+        want = (frames[0][k].astype(np.float64) + frames[1][k] + frames[2][k]) / 3.0
+        assert np.allclose(got[k][..., :3], want[..., :3], rtol=1e-5, atol=1e-6)
+
+
+def test_autotune_keeps_images(gpu):
+    import torch
+    scenes, ctxs, b, outs, cams = _setup()
+    st = torch.cuda.Stream()
+    b.render(_settings(frame=2), stream=st)
+    st.synchronize()
+    ref = _grab(outs)
+    t = b.autotune(_settings(frame=0), stream=st)
+    _say(f"autotuned: {t.as_dict()}")
+    assert t.ms_per_frame > 0 and t.trace_block in (64, 128)
+    b.render(_settings(frame=2), stream=st)
+    st.synchronize()
+    got = _grab(outs)
+    for k in range(len(VIEWS)):
+        assert np.array_equal(got[k], ref[k])
+
+
+def test_batch_rejects_bad_input(gpu):
+    import snkpt
+    b = snkpt.Batch(0)
+    with pytest.raises(snkpt.SnkptError):
+        b.render(_settings())  # no views
+    c = snkpt.Context(0)
+    with pytest.raises(snkpt.SnkptError):
+        b.add_view(0, snkpt.Camera(), 8, 8, 1, None)
+    b.add_scene(c)  # not built: rejected at render time
+    import torch
+    out = torch.zeros(8, 8, 4, device="cuda")
+    b.add_view(0, snkpt.Camera(), 8, 8, 1, out)
+    with pytest.raises(snkpt.SnkptError):
+        b.render(_settings())
+    with pytest.raises(snkpt.SnkptError):
+        b.render(_settings(mode=snkpt.MODE_RESTIR))
