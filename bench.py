@@ -106,6 +106,86 @@ def frame_rates(stream, frames=10, warmup=3):
     return res
 
 
+def batched_config5(stream, frames=10, warmup=2):
+    """Config 5: 16 scenes (10k-2M triangles) x 64 views (128^2-1024^2) in one
+    batch, depth 6, RR from bounce 3, 1 spp.  Reports frames/s of the whole
+    batch eagerly, with the frame captured as a hipGraph, after the autotuner,
+    and the same 64 views rendered one snkpt_render call at a time."""
+    import torch
+
+    import snkpt
+    t0 = time.perf_counter()
+    scenes, ctxs = [], []
+    for v in range(16):
+        sd = snkpt.SceneData(5, v)
+        c = snkpt.Context(0)
+        c.load_scene(sd)
+        with torch.cuda.stream(stream):
+            c.build_accel(stream)
+        scenes.append(sd)
+        ctxs.append(c)
+    stream.synchronize()
+    tris = [int(sd.s.n_idx // 3) for sd in scenes]
+    log(f"bench: config 5: 16 scenes built ({min(tris)}..{max(tris)} tris) in {time.perf_counter() - t0:.1f} s")
+    b = snkpt.Batch(0)
+    for c in ctxs:
+        b.add_scene(c)
+    views, outs = [], []
+    for view in range(64):
+        var, w, h = snkpt.scene_view(5, view)
+        cam = scenes[var].camera(view, w, h)
+        out = torch.zeros(h, w, 4, device="cuda")
+        b.add_view(var, cam, w, h, 0x5000 + view, out)
+        views.append((var, w, h, cam))
+        outs.append(out)
+    pixels = sum(w * h for _, w, h, _ in views)
+    depth, rr = 6, 3
+
+    def settings(f):
+        return snkpt.default_settings(mode=snkpt.MODE_PT, max_depth=depth, rr_start=rr, frame=f)
+
+    def timed(fn):
+        for f in range(warmup):
+            fn(f)
+        a, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for f in range(frames):
+            fn(warmup + f)
+        e.record(stream)
+        stream.synchronize()
+        return a.elapsed_time(e) / frames
+
+    eager_ms = timed(lambda f: b.render(settings(f), use_graph=False, stream=stream))
+    st = b.frame_stats(stream)
+    rays = int(st.camera_rays + st.extension_rays + st.shadow_rays)
+    graph_ms = timed(lambda f: b.render(settings(f), use_graph=True, stream=stream))
+    t0 = time.perf_counter()
+    tuning = b.autotune(settings(100), stream=stream)
+    tune_s = time.perf_counter() - t0
+    tuned_ms = timed(lambda f: b.render(settings(f), use_graph=True, stream=stream))
+
+    def one_by_one(f):
+        for k, (var, w, h, cam) in enumerate(views):
+            s = settings(f)
+            s.seed = 0x5000 + k
+            ctxs[var].render(cam, s, outs[k], w, h, stream=stream)
+
+    single_ms = timed(one_by_one)
+    log(f"bench: config 5 ({pixels} px / frame): eager {eager_ms:.2f} ms, graph {graph_ms:.2f} ms, "
+        f"autotuned graph {tuned_ms:.2f} ms, one view per call {single_ms:.2f} ms")
+    res = {"workload": "config 5: 16 seeded room scenes, 10k-2M triangles log-uniform, 64 views of 128^2-1024^2 "
+                       "log-uniform, plain PT depth 6, RR from bounce 3, 1 spp",
+           "triangles_min_max": [min(tris), max(tris)], "pixels_per_frame": pixels, "rays_per_frame": rays,
+           "eager_ms": round(eager_ms, 3), "graph_ms": round(graph_ms, 3), "autotuned_graph_ms": round(tuned_ms, 3),
+           "one_view_per_call_ms": round(single_ms, 3), "fps": round(1e3 / tuned_ms, 2),
+           "mrays_s": round(rays / (tuned_ms * 1e-3) / 1e6, 1), "tuning": tuning.as_dict(),
+           "autotune_s": round(tune_s, 2)}
+    b.close()
+    for c in ctxs:
+        c.close()
+    return res
+
+
 def time_to_rmse(stream, ref_spp=4096, max_frames=768, cache_restart=16):
     """1080p room (config 4): wall time for the running mean of 1-spp frames to
     reach an RMSE against the FP64 oracle, plain PT vs the MLP cache (trained
@@ -335,6 +415,8 @@ def main():
 
     if not args.only_trace and not args.no_frames:
         extra["frames"] = frame_rates(stream)
+    if not args.only_trace and not args.no_frames:
+        extra["frames"]["c5_batched_pt"] = batched_config5(stream)
     if not args.only_trace and not args.no_rmse:
         extra["time_to_rmse"] = time_to_rmse(stream)
 
