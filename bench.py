@@ -58,16 +58,24 @@ def cpu_baseline(world, rays_np, threads):
                       f"triangles took {build_s:.2f} s"}
 
 
-def frame_rates(stream, frames=10, warmup=3):
+FRAME_RUNS = [(1, "pt"), (2, "pt"), (3, "restir"), (4, "pt"), (4, "cache"), (4, "cache_indirect")]
+
+
+def frame_rates(stream, frames=10, warmup=3, only=None):
     """Frames/s per configuration (1 spp frames of the render pass, event-timed
-    on the launch stream).  Cache frames include the online training step."""
+    on the launch stream).  Cache frames include the online training step
+    (cache: L_o queried at path vertex 3; cache_indirect: NEE at the primary
+    hit and the cache's indirect light there)."""
     import torch
 
     import snkpt
-    runs = [(1, "pt", snkpt.MODE_PT), (2, "pt", snkpt.MODE_PT), (3, "restir", snkpt.MODE_RESTIR),
-            (4, "pt", snkpt.MODE_PT), (4, "cache", snkpt.MODE_CACHE)]
+    modes = {"pt": snkpt.MODE_PT, "restir": snkpt.MODE_RESTIR, "cache": snkpt.MODE_CACHE,
+             "cache_indirect": snkpt.MODE_CACHE}
     res = {}
-    for cfg, label, mode in runs:
+    for cfg, label in FRAME_RUNS:
+        if only is not None and f"c{cfg}_{label}" != only:
+            continue
+        mode = modes[label]
         scene = snkpt.SceneData(cfg)
         ctx = snkpt.Context(0)
         ctx.load_scene(scene)
@@ -82,6 +90,8 @@ def frame_rates(stream, frames=10, warmup=3):
             def frame(f):
                 s = snkpt.default_settings(mode=mode, max_depth=scene.s.max_depth, rr_start=scene.s.rr_start,
                                            seed=scene.s.seed, frame=f, cache_query_depth=3)
+                if label == "cache_indirect":
+                    s.cache_query_depth, s.flags = 1, snkpt.FLAG_CACHE_INDIRECT
                 if cfg == 3:
                     s.num_temporal_resamples, s.num_spatial_resamples = 1, 5
                 ctx.render(cam, s, out, w, h, stream=stream)
@@ -292,6 +302,9 @@ def main():
     ap.add_argument("--only-trace", action="store_true", help="time only the incoherent trace (profiling)")
     ap.add_argument("--no-frames", action="store_true", help="skip the per-config frame rates")
     ap.add_argument("--no-rmse", action="store_true", help="skip the time-to-RMSE race (PT vs cache)")
+    ap.add_argument("--leg", default=None,
+                    help="run one workload only and print its record (profiling): "
+                         + ", ".join(f"c{c}_{l}" for c, l in FRAME_RUNS) + ", c5_batched, rmse")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -319,6 +332,18 @@ def main():
     torch.cuda.init()
     dev = torch.device("cuda:0")
     stream = torch.cuda.Stream(device=dev)
+    if args.leg:
+        if args.leg == "c5_batched":
+            rec = batched_config5(stream)
+        elif args.leg == "rmse":
+            rec = time_to_rmse(stream)
+        else:
+            rec = frame_rates(stream, frames=args.steps, warmup=args.warmup, only=args.leg)
+            if not rec:
+                print(f"unknown leg {args.leg}", file=sys.stderr)
+                sys.exit(2)
+        print(json.dumps({"leg": args.leg, "record": rec}), flush=True)
+        return
     W, H = 1920, 1080
     log("bench: generating the 1.02M-triangle scene")
     scene = snkpt.SceneData(2)

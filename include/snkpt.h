@@ -106,6 +106,7 @@ typedef struct snkpt_pointlight {
 #define SNKPT_FLAG_ACCUMULATE 1u /* add this frame to the running mean and output the mean */
 #define SNKPT_FLAG_RESET 2u      /* clear the running mean first */
 #define SNKPT_FLAG_NO_TRAIN 4u   /* cache mode: query only, skip the training step */
+#define SNKPT_FLAG_WHOLE_PATHS 16u   /* plain PT: one thread runs each whole path (no queues); same image */
 #define SNKPT_FLAG_CACHE_INDIRECT 8u /* cache mode: NEE at the query vertex, the cache learns and answers
                                         only the indirect radiance there */
 

@@ -30,7 +30,7 @@ struct CacheState {
     DeviceBuffer mlp32, mlp16, mlp16t, mlp_m, mlp_v, mlp_gradf, grid_gradf;
     // exponential moving averages of the weights (fp32 and the fp16 copies queries read)
     DeviceBuffer grid_e32, grid_e16, mlp_e32, mlp_e16;
-    DeviceBuffer partial, loss_part, x0, dx0, y, mlp_red;
+    DeviceBuffer partial, loss_part, x0, dx0, y, mlp_red, grid_priv;
     uint32_t cap = 0;         // samples the inference work buffers hold
     uint32_t tcap = 0;        // samples the training work buffers hold
     uint32_t last_n = 0;
@@ -404,15 +404,30 @@ __global__ __launch_bounds__(64) void k_mlp_train(const _Float16* __restrict__ x
     }
 }
 
+// Gradient privatisation of the coarsest levels: their few entries (17^3,
+// 23^3, 31^3 dense cells) take most same-address atomics, so blocks spread
+// them over kPrivCopies copies (copy = block index mod kPrivCopies) that a
+// second pass sums back in a fixed order.  Integer sums: the result does not
+// depend on the split.
+constexpr int kPrivLevels = 3;
+constexpr int kPrivCopies = 8;
+__host__ __device__ constexpr uint32_t priv_cells(int l) {
+    return l == 0 ? 17u * 17u * 17u : l == 1 ? 23u * 23u * 23u : 31u * 31u * 31u;
+}
+__host__ __device__ constexpr uint32_t priv_base(int l) {
+    return l == 0 ? 0u : l == 1 ? priv_cells(0) : priv_cells(0) + priv_cells(1);
+}
+constexpr uint32_t kPrivEntries = priv_base(kPrivLevels - 1) + priv_cells(kPrivLevels - 1);  // cells per copy
+
 // Scatter dL/dfeature to the 8 corners of each level (64-bit fixed point, so
 // the sums do not depend on the order of the atomics).  One thread per sample
 // walks the levels in turn: at any moment the threads of the grid spread
-// their atomics over several levels, which keeps the coarse levels' few
-// entries less contended than one thread per (sample, level) does.
+// their atomics over several levels.
 __global__ void k_grid_bwd(const float* __restrict__ samples, const float* __restrict__ dx0, uint32_t n, GridGeom G,
-                           unsigned long long* __restrict__ grad) {
+                           unsigned long long* __restrict__ grad, unsigned long long* __restrict__ priv) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    unsigned long long* mine = priv + (size_t)(blockIdx.x % kPrivCopies) * kPrivEntries * 2;
     float p[3];
     norm_pos(samples + (size_t)kSampleFloats * i, G, p);
     for (int l = 0; l < kGridLevels; ++l) {
@@ -421,13 +436,32 @@ __global__ void k_grid_bwd(const float* __restrict__ samples, const float* __res
         const Cell c = cell_of(p, (float)G.res[l]);
         for (int k = 0; k < 8; ++k) {
             float w = corner_weight(c, k);
-            size_t e = ((size_t)l * kGridT + corner_index(c, k, G, l)) * 2;
+            const uint32_t idx = corner_index(c, k, G, l);
+            unsigned long long* dst =
+                l < kPrivLevels ? mine + (size_t)(priv_base(l) + idx) * 2 : grad + ((size_t)l * kGridT + idx) * 2;
             long long v0 = (long long)llrint((double)(w * g0) * kFixScale);
             long long v1 = (long long)llrint((double)(w * g1) * kFixScale);
-            if (v0) atomicAdd(&grad[e], (unsigned long long)v0);
-            if (v1) atomicAdd(&grad[e + 1], (unsigned long long)v1);
+            if (v0) atomicAdd(&dst[0], (unsigned long long)v0);
+            if (v1) atomicAdd(&dst[1], (unsigned long long)v1);
         }
     }
+}
+
+// Folds the private copies into the gradient (copies in a fixed order) and clears them.
+__global__ void k_grid_priv_fold(unsigned long long* __restrict__ priv, unsigned long long* __restrict__ grad) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;  // entry * 2 + feature
+    if (e >= kPrivEntries * 2) return;
+    const uint32_t cell = e >> 1, f = e & 1u;
+    const int l = cell < priv_base(1) ? 0 : cell < priv_base(2) ? 1 : 2;
+    unsigned long long s = 0;
+#pragma unroll
+    for (int c = 0; c < kPrivCopies; ++c) {
+        unsigned long long* q = priv + (size_t)c * kPrivEntries * 2 + e;
+        s += *q;
+        *q = 0ull;
+    }
+    const size_t g = ((size_t)l * kGridT + (cell - priv_base(l))) * 2 + f;
+    grad[g] += s;
 }
 
 struct AdamP {
@@ -605,6 +639,11 @@ snkpt_status cache_reset(snkpt_ctx* ctx, uint32_t seed, const float* lo, const f
         std::memcpy(h3, ctx->accel.hi, 12);
     }
     make_geom(C.geom, l3, h3);
+    for (int l = 0; l < kPrivLevels; ++l) {
+        const uint64_t cells = (uint64_t)(C.geom.res[l] + 1) * (C.geom.res[l] + 1) * (C.geom.res[l] + 1);
+        if (!C.geom.dense[l] || cells != priv_cells(l))
+            return fail(ctx, SNKPT_ERR_INVALID_ARG, "cache_reset: unexpected coarse-level resolution");
+    }
     HIPC(C.grid32.reserve(kGridParams * 4));
     HIPC(C.grid16.reserve(kGridParams * 2));
     HIPC(C.grid_m.reserve(kGridParams * 4));
@@ -617,6 +656,8 @@ snkpt_status cache_reset(snkpt_ctx* ctx, uint32_t seed, const float* lo, const f
     HIPC(C.mlp_v.reserve(kMlpParams * 4));
     HIPC(C.mlp_gradf.reserve(kMlpParams * 4));
     HIPC(C.mlp_red.reserve((size_t)kRedGroups * kMlpParams * 4));
+    HIPC(C.grid_priv.reserve((size_t)kPrivCopies * kPrivEntries * 2 * 8));
+    HIPC(hipMemsetAsync(C.grid_priv.ptr, 0, (size_t)kPrivCopies * kPrivEntries * 2 * 8, st));
     k_grid_init<<<(uint32_t)((kGridParams + 255) / 256), 256, 0, st>>>(
         C.grid32.as<float>(), C.grid16.as<__half>(), C.grid_m.as<float>(), C.grid_v.as<float>(),
         C.grid_grad.as<unsigned long long>(), kGridParams, 0x6772696400000000ull ^ seed);
@@ -703,7 +744,10 @@ snkpt_status cache_train_ema(snkpt_ctx* ctx, const float* samples, uint32_t n, f
                                        C.mlp16t.as<_Float16>(), grad_scale, C.partial.as<float>(), C.dx0.as<float>(),
                                        C.loss_part.as<float>());
     k_grid_bwd<<<grid_for(n, 128), 128, 0, st>>>(samples, C.dx0.as<float>(), n, C.geom,
-                                                 C.grid_grad.as<unsigned long long>());
+                                                 C.grid_grad.as<unsigned long long>(),
+                                                 C.grid_priv.as<unsigned long long>());
+    k_grid_priv_fold<<<grid_for(kPrivEntries * 2, 256), 256, 0, st>>>(C.grid_priv.as<unsigned long long>(),
+                                                                      C.grid_grad.as<unsigned long long>());
     if (apply) C.step += 1;
     const float b1 = 0.9f, b2 = 0.99f;
     const uint32_t t = C.step ? C.step : 1u;
@@ -998,14 +1042,15 @@ snkpt_status cache_pass(snkpt_ctx* ctx, const CamDev& cd, const snkpt_settings* 
     const uint32_t stride = std::max<uint32_t>(n / NT, 1u);
     const uint32_t start = (uint32_t)((initstate >> 17) % stride);
     k_fill4<<<grid_for(NT, 256), 256, 0, st>>>(trad, NT);
-    generate_camera(ctx, cd, initstate ^ kTrainPaths, st, NT, start, stride, nrec);
     ShadeHooks T;
     T.kind = 2;
     T.rec = C.train_vtx.as<float>();
     T.nrec = nrec;
     T.max_rec = depth;
     T.rad = trad;
-    if ((e = pt_bounces(ctx, s, 0, 0, st, &T, 0, NT)) != SNKPT_OK) return e;
+    // the training paths are few (latency-bound per bounce as a wavefront): one thread runs each whole path
+    if ((e = whole_paths(ctx, cd, s, initstate ^ kTrainPaths, NT, start, stride, &T, trad, st)) != SNKPT_OK)
+        return e;
     uint8_t* tm = C.train_meta.as<uint8_t>();
     uint32_t* pcount = (uint32_t*)(tm + u32_bytes);
     uint32_t* off = (uint32_t*)(tm + 2 * u32_bytes);

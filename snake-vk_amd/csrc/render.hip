@@ -106,6 +106,39 @@ __global__ __launch_bounds__(kShadowBlock) void k_shadow(const uint4* __restrict
     shadow_step<STACK>(nodes, tris, sq, i, rad, &stk[threadIdx.x], kShadowBlock);
 }
 
+// Whole paths, one per thread (KIND 0: radiance into rad[i]; KIND 2: training
+// records as the wavefront's kind-2 shade).  Ray counts: one atomic per block.
+constexpr int kMegaBlock = 128;
+template <int KIND, int STACK>
+__global__ __launch_bounds__(kMegaBlock) void k_pt_mega(SceneDev S, CamDev cam, uint32_t w, uint32_t h, uint32_t n,
+                                                        uint32_t start, uint32_t stride, uint64_t initstate,
+                                                        PtParams P, ShadeHooks H, float4* __restrict__ rad,
+                                                        unsigned long long* __restrict__ stats) {
+    __shared__ uint2 stk[STACK * kMegaBlock];
+    __shared__ uint32_t red[2];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (threadIdx.x == 0) red[0] = red[1] = 0u;
+    __syncthreads();
+    uint32_t n_ext = 0, n_sh = 0;
+    if (i < n) {
+        rad[i] = whole_path<KIND, STACK>(S, cam, w, h, i, start, stride, initstate, P, H, &stk[threadIdx.x],
+                                         kMegaBlock, rad[i], n_ext, n_sh);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        n_ext += __shfl_xor(n_ext, o, 64);
+        n_sh += __shfl_xor(n_sh, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&red[0], n_ext);
+        atomicAdd(&red[1], n_sh);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && stats && (red[0] | red[1])) {
+        atomicAdd(&stats[1], (unsigned long long)red[0]);
+        atomicAdd(&stats[2], (unsigned long long)red[1]);
+    }
+}
+
 __global__ void k_resolve(const float4* __restrict__ rad, float4* __restrict__ accum, void* __restrict__ out,
                           uint32_t n, float spp, uint32_t accumulate, uint32_t fmt) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -360,8 +393,47 @@ snkpt_status generate_camera(snkpt_ctx* ctx, const CamDev& cam, uint64_t initsta
 snkpt_status pt_pass(snkpt_ctx* ctx, const CamDev& cam, const snkpt_settings* s, uint64_t initstate,
                      hipStream_t st, uint32_t first_bounce_skip) {
     (void)first_bounce_skip;
+    if (s->flags & SNKPT_FLAG_WHOLE_PATHS) {
+        RenderState& R = *ctx->render;
+        return whole_paths(ctx, cam, s, initstate, R.w * R.h, 0, 1, nullptr, R.rad.as<float4>(), st);
+    }
     generate_camera(ctx, cam, initstate, st);
     return pt_bounces(ctx, s, 0, 0, st);
+}
+
+snkpt_status whole_paths(snkpt_ctx* ctx, const CamDev& cam, const snkpt_settings* s, uint64_t initstate,
+                         uint32_t n_paths, uint32_t start, uint32_t stride, const ShadeHooks* hooks, float4* rad,
+                         hipStream_t st) {
+    RenderState& R = *ctx->render;
+    const Accel& A = ctx->accel;
+    if (n_paths == 0) return SNKPT_OK;
+    if (A.num_tris == 0) return SNKPT_OK;  // nothing to hit: radiance stays 0
+    SceneDev S = scene_dev(ctx);
+    PtParams P{s->max_depth ? s->max_depth : 1u, s->rr_start};
+    const uint32_t need = A.depth > 0 ? A.depth - 1 : 0;
+    unsigned long long* stats = R.stats.as<unsigned long long>();
+    const uint32_t grid = nblk(n_paths, kMegaBlock);
+    const bool rec = hooks && hooks->kind == 2;
+    ShadeHooks H = hooks ? *hooks : ShadeHooks{};
+#define SNKPT_MEGA(STK)                                                                                        \
+    do {                                                                                                       \
+        if (rec)                                                                                               \
+            k_pt_mega<2, STK><<<grid, kMegaBlock, 0, st>>>(S, cam, R.w, R.h, n_paths, start, stride, initstate, \
+                                                           P, H, rad, stats);                                  \
+        else                                                                                                   \
+            k_pt_mega<0, STK><<<grid, kMegaBlock, 0, st>>>(S, cam, R.w, R.h, n_paths, start, stride, initstate, \
+                                                           P, H, rad, stats);                                  \
+    } while (0)
+    if (need <= 16)
+        SNKPT_MEGA(16);
+    else if (need <= 32)
+        SNKPT_MEGA(32);
+    else if (need <= 64)
+        SNKPT_MEGA(64);
+    else
+        return fail(ctx, SNKPT_ERR_CAPACITY, "whole paths: BVH8 deeper than the 64-entry traversal stack");
+#undef SNKPT_MEGA
+    return SNKPT_OK;
 }
 
 snkpt_status begin_frame(snkpt_ctx* ctx, uint32_t w, uint32_t h, hipStream_t st) {

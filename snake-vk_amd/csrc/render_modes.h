@@ -104,6 +104,11 @@ snkpt_status pt_pass(snkpt_ctx* ctx, const CamDev& cam, const snkpt_settings* s,
 // n_paths: number of paths generated (0 = one per pixel); sizes the launches.
 snkpt_status pt_bounces(snkpt_ctx* ctx, const snkpt_settings* s, int cur, uint32_t first, hipStream_t st,
                         const ShadeHooks* hooks = nullptr, uint32_t max_depth_override = 0, uint32_t n_paths = 0);
+// Whole paths, one per thread: path j < n_paths through pixel (start + j * stride)
+// mod (w h), radiance added into rad[j]; hooks of kind 2 record training vertices.
+snkpt_status whole_paths(snkpt_ctx* ctx, const CamDev& cam, const snkpt_settings* s, uint64_t initstate,
+                         uint32_t n_paths, uint32_t start, uint32_t stride, const ShadeHooks* hooks, float4* rad,
+                         hipStream_t st);
 // Camera rays for paths j < n_paths through pixels (start + j * stride) mod (w h).
 snkpt_status generate_camera(snkpt_ctx* ctx, const CamDev& cam, uint64_t initstate, hipStream_t st,
                              uint32_t n_paths = 0, uint32_t start = 0, uint32_t stride = 1,

@@ -128,3 +128,27 @@ def test_spheres_1m_frame_renders(gpu):
     w, h = 480, 270
     img = _render(ctx, scene.camera(0, w, h), w, h, spp=1, max_depth=8, rr_start=3, seed=1)
     assert np.isfinite(img).all() and img[..., :3].mean() > 0
+
+
+@pytest.mark.parametrize("config,w,h", [(1, 96, 64), (4, 160, 90), (2, 128, 72)])
+def test_whole_path_kernel_matches_wavefront(gpu, config, w, h):
+    """One thread per whole path (no queues) gives the wavefront's image bit
+    for bit: same vertex logic, random stream and order of additions."""
+    import torch
+
+    import snkpt
+    scene = snkpt.SceneData(config)
+    ctx = snkpt.Context(0)
+    ctx.load_scene(scene)
+    ctx.build_accel()
+    cam = scene.camera(0, w, h)
+    outs = []
+    for flags in (0, snkpt.FLAG_WHOLE_PATHS):
+        o = torch.zeros(h, w, 4, device="cuda")
+        s = snkpt.default_settings(max_depth=scene.s.max_depth, rr_start=scene.s.rr_start, seed=5, frame=2,
+                                   flags=flags, spp=2)
+        ctx.render(cam, s, o, w, h)
+        st = ctx.frame_stats()
+        outs.append((o.cpu().numpy(), st.extension_rays, st.shadow_rays))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert outs[0][1:] == outs[1][1:]
