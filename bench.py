@@ -196,14 +196,16 @@ def batched_config5(stream, frames=10, warmup=2):
     return res
 
 
-def time_to_rmse(stream, ref_spp=4096, max_frames=768, cache_restart=16):
+def time_to_rmse(stream, ref_spp=4096, max_frames=768, warm_frames=256):
     """1080p room (config 4): wall time for the running mean of 1-spp frames to
-    reach an RMSE against the FP64 oracle, plain PT vs the MLP cache (trained
-    from a freshly initialised network; its warm-up frames are timed and the
-    running mean restarts after `cache_restart` frames).  Frame times are GPU
-    event times of the render calls; the error checks between frames are not
-    timed.  Thresholds: absolute RMSE 0.02 and RMSE relative to the oracle's
-    mean radiance."""
+    reach an RMSE against the FP64 oracle, plain PT vs the MLP cache.  Cold
+    start: the cache starts from a freshly initialised network, all its frames
+    are timed and its running mean restarts at frames 16, 32, 64, ... (early
+    frames of a still-learning network would otherwise stay in the mean).
+    Warm start: the cache was trained for `warm_frames` frames beforehand (not
+    timed), as in interactive use.  Frame times are GPU event times of the
+    render calls; the error checks between frames are not timed.  Thresholds:
+    absolute RMSE 0.02 and RMSE relative to the oracle's mean radiance."""
     import torch
 
     import snkpt
@@ -228,34 +230,42 @@ def time_to_rmse(stream, ref_spp=4096, max_frames=768, cache_restart=16):
         out = torch.zeros(h, w, 4, device="cuda")
         rel_thr = [0.2, 0.1, 0.05, 0.03]
 
-        def race(mode, **kw):
+        def settings(mode, f, fl, kw):
+            kw2 = dict(kw)
+            fl |= kw2.pop("flags", 0)
+            return snkpt.default_settings(mode=mode, max_depth=depth, rr_start=rr, seed=scene.s.seed, frame=f,
+                                          flags=fl, **kw2)
+
+        def race(mode, warm=0, **kw):
+            restarts = {0}
             if mode == snkpt.MODE_CACHE:
                 ctx.cache_reset(1, stream=stream)
+                for f in range(warm):
+                    ctx.render(cam, settings(mode, 100000 + f, 0, kw), out, w, h, stream=stream)
+                if not warm:
+                    restarts = {0} | {16 << k for k in range(10)}
+            first = min(r for r in restarts if r > 0) if len(restarts) > 1 else 0
             t_ms, first_rmse, hit_abs, hit_rel, curve = 0.0, None, None, {}, []
             start = 0
             for f in range(max_frames):
-                restart = f == 0 or (mode == snkpt.MODE_CACHE and f == cache_restart)
+                restart = f in restarts
                 if restart:
                     start = f
                 fl = snkpt.FLAG_ACCUMULATE | (snkpt.FLAG_RESET if restart else 0)
-                kw2 = dict(kw)
-                fl |= kw2.pop("flags", 0)
-                st = snkpt.default_settings(mode=mode, max_depth=depth, rr_start=rr, seed=scene.s.seed, frame=f,
-                                            flags=fl, **kw2)
                 a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 a.record(stream)
-                ctx.render(cam, st, out, w, h, stream=stream)
+                ctx.render(cam, settings(mode, f, fl, kw), out, w, h, stream=stream)
                 b.record(stream)
                 stream.synchronize()
                 t_ms += a.elapsed_time(b)
                 e = float(torch.sqrt(((out[..., :3].double() - ref) ** 2).mean()))
                 if f == 0:
                     first_rmse = e
-                if mode == snkpt.MODE_CACHE and f < cache_restart:
+                if f < first:
                     continue
                 n_acc = f - start + 1
                 if (n_acc & (n_acc - 1)) == 0:
-                    curve.append([n_acc, round(t_ms, 2), round(e / ref_mean, 5)])
+                    curve.append([f + 1, round(t_ms, 2), round(e / ref_mean, 5)])
                 if hit_abs is None and e <= 0.02:
                     hit_abs = round(t_ms, 2)
                 for thr in rel_thr:
@@ -265,12 +275,16 @@ def time_to_rmse(stream, ref_spp=4096, max_frames=768, cache_restart=16):
                     break
             return {"ms_to_rmse_0.02": hit_abs, "ms_to_rel_rmse": hit_rel, "frames": f + 1,
                     "ms_per_frame": round(t_ms / (f + 1), 3), "first_frame_rel_rmse": round(first_rmse / ref_mean, 4),
-                    "final_rel_rmse": round(e / ref_mean, 5), "curve_frames_ms_relrmse": curve}
+                    "final_rel_rmse": round(e / ref_mean, 5), "warm_frames_untimed": warm,
+                    "curve_frame_ms_relrmse": curve}
 
         pt = race(snkpt.MODE_PT)
         log(f"bench: PT time-to-RMSE {pt['ms_to_rmse_0.02']} ms (abs 0.02), rel {pt['ms_to_rel_rmse']}")
         forms = {"cache_q3_full": dict(cache_query_depth=3),
-                 "cache_q1_indirect": dict(cache_query_depth=1, flags=snkpt.FLAG_CACHE_INDIRECT)}
+                 "cache_q1_indirect": dict(cache_query_depth=1, flags=snkpt.FLAG_CACHE_INDIRECT),
+                 "cache_q3_full_warm": dict(cache_query_depth=3, warm=warm_frames),
+                 "cache_q1_indirect_warm": dict(cache_query_depth=1, flags=snkpt.FLAG_CACHE_INDIRECT,
+                                                warm=warm_frames)}
         caches = {}
         for name, kw in forms.items():
             caches[name] = race(snkpt.MODE_CACHE, **kw)
@@ -290,7 +304,7 @@ def time_to_rmse(stream, ref_spp=4096, max_frames=768, cache_restart=16):
                         "bounces); cache_q1_indirect: NEE at the primary hit, the cache answers its indirect light",
             "oracle": {"spp": ref_spp, "mean_radiance": round(ref_mean, 6),
                        "rel_noise_est": round(pt["first_frame_rel_rmse"] / ref_spp ** 0.5, 5)},
-            "cache_restart_frame": cache_restart, "pt": pt, **caches}
+            "pt": pt, **caches}
 
 
 def main():

@@ -124,21 +124,33 @@ constexpr int kEncBlock = 64 * kGridLevels;
 __global__ __launch_bounds__(kEncBlock) void k_encode(const float* __restrict__ samples, uint32_t n,
                                                      const uint32_t* __restrict__ n_dev, GridGeom G,
                                                      const __half2* __restrict__ grid, _Float16* __restrict__ x0) {
+    // the 64 input rows of the block are assembled in LDS and written out as
+    // whole 128 B rows (4 B stores at a 128 B stride would be 64 partial lines)
+    __shared__ half2v_t tile[64][33];
     if (n_dev) n = *n_dev;
-    const int l = threadIdx.x >> 6;
-    const uint32_t i = blockIdx.x * 64 + (threadIdx.x & 63);
-    if (i >= n) return;
-    const float* s = samples + (size_t)kSampleFloats * i;
-    float p[3];
-    norm_pos(s, G, p);
-    float2 f = encode_level(p, G, l, grid);
-    half2v_t v, e;
-    v[0] = (_Float16)f.x;
-    v[1] = (_Float16)f.y;
-    e[0] = (_Float16)extra_feature(s, 2 * l);
-    e[1] = (_Float16)extra_feature(s, 2 * l + 1);
-    *(half2v_t*)(x0 + (size_t)64 * i + 2 * l) = v;
-    *(half2v_t*)(x0 + (size_t)64 * i + 32 + 2 * l) = e;
+    const int l = threadIdx.x >> 6, r = threadIdx.x & 63;
+    const uint32_t i = blockIdx.x * 64 + r;
+    if (i < n) {
+        const float* s = samples + (size_t)kSampleFloats * i;
+        float p[3];
+        norm_pos(s, G, p);
+        float2 f = encode_level(p, G, l, grid);
+        half2v_t v, e;
+        v[0] = (_Float16)f.x;
+        v[1] = (_Float16)f.y;
+        e[0] = (_Float16)extra_feature(s, 2 * l);
+        e[1] = (_Float16)extra_feature(s, 2 * l + 1);
+        tile[r][l] = v;
+        tile[r][16 + l] = e;
+    }
+    __syncthreads();
+    // 64 rows x 32 half2 = 2048 words: two per thread, consecutive threads on consecutive words
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int wdx = threadIdx.x + k * kEncBlock, row = wdx >> 5, col = wdx & 31;
+        const uint32_t gi = blockIdx.x * 64 + row;
+        if (gi < n) *(half2v_t*)(x0 + (size_t)64 * gi + 2 * col) = tile[row][col];
+    }
 }
 inline uint32_t enc_blocks(uint32_t n) { return n == 0 ? 1u : (n + 63) / 64; }
 
