@@ -111,6 +111,9 @@ def frame_rates(stream, frames=10, warmup=3, only=None):
                     "depth": scene.s.max_depth, "rays_per_frame": int(st.camera_rays + st.extension_rays +
                                                                       st.shadow_rays)}
         res[key]["mrays_s"] = round(res[key]["rays_per_frame"] / (ms * 1e-3) / 1e6, 1)
+        if mode == snkpt.MODE_CACHE:
+            res[key]["cache_queries"] = int(st.cache_queries)
+            res[key]["cache_train_samples"] = int(st.cache_train_samples)
         log(f"bench: {key}: {ms:.3f} ms/frame ({res[key]['mrays_s']} Mrays/s incl. shadow)")
         ctx.close()
     return res
