@@ -115,25 +115,32 @@ __global__ void k_bbounce_end(uint32_t* c, unsigned long long* stats) {
     }
 }
 
-// Closest hit of the queued rays, each against its own scene's BVH8.
+// Closest hit of the queued rays, each against its own scene's BVH8:
+// persistent waves with ray refill over a contiguous slice each.
 template <int STACK, int TB>
 __global__ __launch_bounds__(TB) void k_btrace(SceneMany ss, const float4* __restrict__ org,
                                                const float4* __restrict__ dir, const uint32_t* __restrict__ q_path,
                                                const uint32_t* __restrict__ count, HitArrays hit) {
     __shared__ uint2 stk[STACK * TB];
     const uint32_t n = *count;
-    for (uint32_t i = blockIdx.x * TB + threadIdx.x; i < n; i += gridDim.x * TB) {
-        const SceneDev& S = ss.at(q_path[i]);
-        const uint4* nodes = S.nodes;
-        const float4* tris = S.leaf;
+    const uint32_t nwaves = gridDim.x * (TB / 64);
+    const uint32_t wave = blockIdx.x * (TB / 64) + (threadIdx.x >> 6);
+    auto load = [&](uint32_t i, Ray& ray, const uint4*& nd, const float4*& tr) {
         float4 a = org[i], d = dir[i];
-        Ray r{a.x, a.y, a.z, a.w, d.x, d.y, d.z, d.w};
-        uint32_t nv = 0, tv = 0;
-        TraceResult res = traverse<false, false, STACK>(nodes, tris, r, &stk[threadIdx.x], TB, nv, tv);
+        ray = Ray{a.x, a.y, a.z, a.w, d.x, d.y, d.z, d.w};
+        const SceneDev& S = ss.at(q_path[i]);
+        nd = S.nodes;
+        tr = S.leaf;
+    };
+    auto done = [&](uint32_t i, const TravState& st, uint32_t, uint32_t) {
+        const TraceResult& res = st.res;
         if (res.rec != 0xFFFFFFFFu) {
+            const float4* tris = ss.at(q_path[i]).leaf;
             float4 r2 = tris[(size_t)res.rec * kTriWords + 2];
+            float4 a = org[i], d = dir[i];
+            Ray ray{a.x, a.y, a.z, a.w, d.x, d.y, d.z, d.w};
             float u, v;
-            hit_uv(tris, res.rec, r, u, v);
+            hit_uv(tris, res.rec, ray, u, v);
             hit.t[i] = res.t;
             hit.inst[i] = __float_as_uint(r2.y);
             hit.prim[i] = __float_as_uint(r2.z);
@@ -146,7 +153,8 @@ __global__ __launch_bounds__(TB) void k_btrace(SceneMany ss, const float4* __res
             hit.u[i] = 0.0f;
             hit.v[i] = 0.0f;
         }
-    }
+    };
+    refill_trace<false, false, STACK, kRefillLanes>(n, wave, nwaves, &stk[threadIdx.x], TB, load, done);
 }
 
 __global__ __launch_bounds__(kBBlock) void k_bshade(SceneMany ss, PtParams P, Queue in, HitArrays hit, Queue out,
@@ -167,10 +175,16 @@ __global__ __launch_bounds__(TB) void k_bshadow(SceneMany ss, ShadowQueue sq, co
                                                 float4* __restrict__ rad) {
     __shared__ uint2 stk[STACK * TB];
     const uint32_t n = counts[2];
-    for (uint32_t i = blockIdx.x * TB + threadIdx.x; i < n; i += gridDim.x * TB) {
+    const uint32_t nwaves = gridDim.x * (TB / 64);
+    const uint32_t wave = blockIdx.x * (TB / 64) + (threadIdx.x >> 6);
+    auto load = [&](uint32_t i, Ray& ray, const uint4*& nd, const float4*& tr) {
+        ray = shadow_ray(sq, i);
         const SceneDev& S = ss.at(__float_as_uint(sq.contrib[i].w));
-        shadow_step<STACK>(S.nodes, S.leaf, sq, i, rad, &stk[threadIdx.x], TB);
-    }
+        nd = S.nodes;
+        tr = S.leaf;
+    };
+    auto done = [&](uint32_t i, const TravState& st, uint32_t, uint32_t) { shadow_done(sq, i, st.res, rad); };
+    refill_trace<true, false, STACK, kRefillLanes>(n, wave, nwaves, &stk[threadIdx.x], TB, load, done);
 }
 
 // Radiance of the group's paths -> each view's output (and running mean).

@@ -213,15 +213,16 @@ __device__ __forceinline__ float4 whole_path(const SceneDev& S, const CamDev& ca
     return L;
 }
 
-// Any-hit visibility of shadow-queue entry i; adds its contribution when unoccluded.
-template <int STACK>
-__device__ __forceinline__ void shadow_step(const uint4* __restrict__ nodes, const float4* __restrict__ tris,
-                                            const ShadowQueue& sq, uint32_t i, float4* __restrict__ rad,
-                                            uint2* stk, int stride) {
+// Lanes a persistent traversal wave waits for before refilling idle lanes.
+constexpr int kRefillLanes = 16;
+
+// Shadow-queue entry i as a ray; its contribution, added when unoccluded.
+__device__ __forceinline__ Ray shadow_ray(const ShadowQueue& sq, uint32_t i) {
     float4 a = sq.org[i], d = sq.dir[i];
-    Ray r{a.x, a.y, a.z, a.w, d.x, d.y, d.z, d.w};
-    uint32_t nv = 0, tv = 0;
-    TraceResult res = traverse<true, false, STACK>(nodes, tris, r, stk, stride, nv, tv);
+    return Ray{a.x, a.y, a.z, a.w, d.x, d.y, d.z, d.w};
+}
+__device__ __forceinline__ void shadow_done(const ShadowQueue& sq, uint32_t i, const TraceResult& res,
+                                            float4* __restrict__ rad) {
     if (res.rec == 0xFFFFFFFFu) {
         float4 c = sq.contrib[i];
         uint32_t px = __float_as_uint(c.w);

@@ -17,6 +17,7 @@
 // Queue counts stay on the device, so a frame has no host round trip.
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -95,15 +96,24 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PtParams P, Queue 
     shade_step<KIND, kBlock>(ss, P, in, hit, out, sq, ps, rad, counts, nxt, H, i, i < counts[cur], lds_app);
 }
 
+// Shadow rays: persistent any-hit traversal with ray refill; each wave owns a
+// contiguous slice of the shadow queue.
 template <int STACK>
 __global__ __launch_bounds__(kShadowBlock) void k_shadow(const uint4* __restrict__ nodes,
                                                          const float4* __restrict__ tris, ShadowQueue sq,
                                                          const uint32_t* __restrict__ counts,
                                                          float4* __restrict__ rad) {
     __shared__ uint2 stk[STACK * kShadowBlock];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= counts[2]) return;
-    shadow_step<STACK>(nodes, tris, sq, i, rad, &stk[threadIdx.x], kShadowBlock);
+    const uint32_t nn = counts[2];
+    const uint32_t nwaves = gridDim.x * (kShadowBlock / 64);
+    const uint32_t wave = blockIdx.x * (kShadowBlock / 64) + (threadIdx.x >> 6);
+    auto load = [&](uint32_t i, Ray& ray, const uint4*& nd, const float4*& tr) {
+        ray = shadow_ray(sq, i);
+        nd = nodes;
+        tr = tris;
+    };
+    auto done = [&](uint32_t i, const TravState& st, uint32_t, uint32_t) { shadow_done(sq, i, st.res, rad); };
+    refill_trace<true, false, STACK, kRefillLanes>(nn, wave, nwaves, &stk[threadIdx.x], kShadowBlock, load, done);
 }
 
 // Whole paths, one per thread (KIND 0: radiance into rad[i]; KIND 2: training
@@ -323,7 +333,10 @@ snkpt_status launch_shadow(snkpt_ctx* ctx, RenderState& R, float4* rad, hipStrea
     const Accel& A = ctx->accel;
     if (A.num_tris == 0) return SNKPT_OK;  // no hits, so no shadow rays were queued
     uint32_t need = A.depth > 0 ? A.depth - 1 : 0;
-    uint32_t grid = nblk(n_max ? n_max : R.w * R.h, kShadowBlock);
+    const uint32_t cus = ctx->props.multiProcessorCount > 0 ? (uint32_t)ctx->props.multiProcessorCount : 256u;
+    // persistent grid: as many blocks as the LDS of the stack size lets stay resident
+    const uint32_t resident = need <= 16 ? 10u : need <= 32 ? 5u : 2u;
+    uint32_t grid = std::min(nblk(n_max ? n_max : R.w * R.h, kShadowBlock), cus * resident);
     if (need <= 16)
         k_shadow<16><<<grid, kShadowBlock, 0, st>>>(A.nodes.as<uint4>(), A.leaf_tris.as<float4>(), shadow_of(R),
                                                      R.counts.as<uint32_t>(), rad);

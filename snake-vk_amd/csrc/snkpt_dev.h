@@ -154,115 +154,195 @@ __device__ __forceinline__ uint32_t parallel_miss(const RayPre& p, uint4 w0, uin
     return miss;
 }
 
-// BVH8 traversal for one ray.  `stk` points to this lane's first stack entry in
-// LDS; consecutive entries are `stride` uint2 apart.  Stack entries are node
-// groups: (child_base, hit bits << 24 | imask).  The stack never holds more
-// than depth-1 entries, so STACK >= depth-1 is checked by the launcher.
-template <bool ANY, bool COUNT, int STACK>
-__device__ __forceinline__ TraceResult traverse(const uint4* __restrict__ nodes, const float4* __restrict__ tris,
-                                                const Ray& ray, uint2* stk, int stride, uint32_t& nvis,
-                                                uint32_t& tvis) {
-    TraceResult res{ray.tmax, 0xFFFFFFFFu, 0xFFFFFFFFu};
-    // Non-finite rays, a zero direction or an empty [tmin, tmax] miss at once:
-    // a NaN would otherwise pass every box test and walk the whole tree.
+// BVH8 traversal state of one ray.  Stack entries are node groups:
+// (child_base, hit bits << 24 | imask); the stack never holds more than depth-1
+// entries, so STACK >= depth-1 is checked by the launcher.
+struct TravState {
+    RayPre p;
+    float tmin;
+    uint32_t ng_base, ng_hits;
+    int sp;
+    TraceResult res;
+};
+
+// Sets up the traversal of `ray`; false when the ray is invalid (it misses):
+// non-finite rays, a zero direction or an empty [tmin, tmax] miss at once, as a
+// NaN would otherwise pass every box test and walk the whole tree.
+__device__ __forceinline__ bool trav_init(const Ray& ray, TravState& s) {
+    s.res = TraceResult{ray.tmax, 0xFFFFFFFFu, 0xFFFFFFFFu};
     const bool valid = __builtin_isfinite(ray.ox) && __builtin_isfinite(ray.oy) && __builtin_isfinite(ray.oz) &&
                        __builtin_isfinite(ray.dx) && __builtin_isfinite(ray.dy) && __builtin_isfinite(ray.dz) &&
                        (ray.dx != 0.0f || ray.dy != 0.0f || ray.dz != 0.0f) && ray.tmin <= ray.tmax &&
                        !__builtin_isnan(ray.tmin) && !__builtin_isnan(ray.tmax);
-    if (!valid) return res;
-    RayPre p = ray_setup(ray);
-    const float tmin = ray.tmin;
-    const bool nx = p.ix < 0.0f, ny = p.iy < 0.0f, nz = p.iz < 0.0f;
-    int sp = 0;
+    if (!valid) return false;
+    s.p = ray_setup(ray);
+    s.tmin = ray.tmin;
+    s.sp = 0;
     // start with a node group holding the root in slot 0
-    uint32_t ng_base = 0, ng_hits = (1u << (24 + (0u ^ p.oct))) | 1u;
-    while (true) {
-        uint32_t tg_base = 0, tg_hits = 0;
-        if (ng_hits & 0xFF000000u) {
-            uint32_t k = __builtin_ctz(ng_hits >> 24);
-            ng_hits &= ~(1u << (24 + k));
-            uint32_t s = k ^ p.oct;
-            uint32_t node = ng_base + __builtin_popcount(ng_hits & 0xFFu & ((1u << s) - 1u));
-            if (ng_hits & 0xFF000000u) {
-                stk[sp * stride] = make_uint2(ng_base, ng_hits);
-                ++sp;
-            }
-            const uint4* np = nodes + (size_t)node * kNodeWords;
-            uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
-            if (COUNT) ++nvis;
-            float sx = __uint_as_float((w0.w & 0xFFu) << 23) * p.ix;
-            float sy = __uint_as_float(((w0.w >> 8) & 0xFFu) << 23) * p.iy;
-            float sz = __uint_as_float(((w0.w >> 16) & 0xFFu) << 23) * p.iz;
-            float ax = (__uint_as_float(w0.x) - p.ox) * p.ix;
-            float ay = (__uint_as_float(w0.y) - p.oy) * p.iy;
-            float az = (__uint_as_float(w0.z) - p.oz) * p.iz;
-            uint32_t imask = w0.w >> 24;
-            // near / far plane bytes per axis (qlo first for a positive direction)
-            uint32_t nx0 = nx ? w3.z : w2.x, nx1 = nx ? w3.w : w2.y;
-            uint32_t fx0 = nx ? w2.x : w3.z, fx1 = nx ? w2.y : w3.w;
-            uint32_t ny0 = ny ? w4.x : w2.z, ny1 = ny ? w4.y : w2.w;
-            uint32_t fy0 = ny ? w2.z : w4.x, fy1 = ny ? w2.w : w4.y;
-            uint32_t nz0 = nz ? w4.z : w3.x, nz1 = nz ? w4.w : w3.y;
-            uint32_t fz0 = nz ? w3.x : w4.z, fz1 = nz ? w3.y : w4.w;
-            // parallel axes: exact origin-in-slab test on the decoded planes
-            uint32_t pmiss = 0;
-            if (p.par) pmiss = parallel_miss(p, w0, w2, w3, w4);
-            uint32_t int_hits = 0, tri_hits = 0;
+    s.ng_base = 0;
+    s.ng_hits = (1u << (24 + (0u ^ s.p.oct))) | 1u;
+    return true;
+}
+
+// One traversal step: the next node of the current group (its 8 child boxes),
+// then every triangle of its hit leaves; pops the stack when the group is
+// empty.  Returns true when the ray is finished.  `stk` points to this lane's
+// first stack entry in LDS; consecutive entries are `stride` uint2 apart.
+template <bool ANY, bool COUNT, int STACK>
+__device__ __forceinline__ bool trav_step(const uint4* __restrict__ nodes, const float4* __restrict__ tris,
+                                          TravState& st, uint2* stk, int stride, uint32_t& nvis, uint32_t& tvis) {
+    const RayPre& p = st.p;
+    const float tmin = st.tmin;
+    uint32_t tg_base = 0, tg_hits = 0;
+    if (st.ng_hits & 0xFF000000u) {
+        const bool nx = p.ix < 0.0f, ny = p.iy < 0.0f, nz = p.iz < 0.0f;
+        uint32_t k = __builtin_ctz(st.ng_hits >> 24);
+        st.ng_hits &= ~(1u << (24 + k));
+        uint32_t s = k ^ p.oct;
+        uint32_t node = st.ng_base + __builtin_popcount(st.ng_hits & 0xFFu & ((1u << s) - 1u));
+        if (st.ng_hits & 0xFF000000u) {
+            stk[st.sp * stride] = make_uint2(st.ng_base, st.ng_hits);
+            ++st.sp;
+        }
+        const uint4* np = nodes + (size_t)node * kNodeWords;
+        uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
+        if (COUNT) ++nvis;
+        float sx = __uint_as_float((w0.w & 0xFFu) << 23) * p.ix;
+        float sy = __uint_as_float(((w0.w >> 8) & 0xFFu) << 23) * p.iy;
+        float sz = __uint_as_float(((w0.w >> 16) & 0xFFu) << 23) * p.iz;
+        float ax = (__uint_as_float(w0.x) - p.ox) * p.ix;
+        float ay = (__uint_as_float(w0.y) - p.oy) * p.iy;
+        float az = (__uint_as_float(w0.z) - p.oz) * p.iz;
+        uint32_t imask = w0.w >> 24;
+        // near / far plane bytes per axis (qlo first for a positive direction)
+        uint32_t nx0 = nx ? w3.z : w2.x, nx1 = nx ? w3.w : w2.y;
+        uint32_t fx0 = nx ? w2.x : w3.z, fx1 = nx ? w2.y : w3.w;
+        uint32_t ny0 = ny ? w4.x : w2.z, ny1 = ny ? w4.y : w2.w;
+        uint32_t fy0 = ny ? w2.z : w4.x, fy1 = ny ? w2.w : w4.y;
+        uint32_t nz0 = nz ? w4.z : w3.x, nz1 = nz ? w4.w : w3.y;
+        uint32_t fz0 = nz ? w3.x : w4.z, fz1 = nz ? w3.y : w4.w;
+        // parallel axes: exact origin-in-slab test on the decoded planes
+        uint32_t pmiss = 0;
+        if (p.par) pmiss = parallel_miss(p, w0, w2, w3, w4);
+        uint32_t int_hits = 0, tri_hits = 0;
 #pragma unroll
-            for (int s2 = 0; s2 < 8; ++s2) {
-                const int b = s2 & 3;
-                uint32_t nxw = s2 < 4 ? nx0 : nx1, fxw = s2 < 4 ? fx0 : fx1;
-                uint32_t nyw = s2 < 4 ? ny0 : ny1, fyw = s2 < 4 ? fy0 : fy1;
-                uint32_t nzw = s2 < 4 ? nz0 : nz1, fzw = s2 < 4 ? fz0 : fz1;
-                uint32_t metaw = s2 < 4 ? w1.z : w1.w;
-                float tnx = fmaf(ubyte_f(nxw, b), sx, ax);
-                float tny = fmaf(ubyte_f(nyw, b), sy, ay);
-                float tnz = fmaf(ubyte_f(nzw, b), sz, az);
-                float tfx = fmaf(ubyte_f(fxw, b), sx, ax);
-                float tfy = fmaf(ubyte_f(fyw, b), sy, ay);
-                float tfz = fmaf(ubyte_f(fzw, b), sz, az);
-                float tn = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), tmin);
-                float tf = fminf(fminf(fminf(tfx, tfy), tfz), res.t) * kBoxRobust;
-                uint32_t meta = (metaw >> (8 * b)) & 0xFFu;
-                if (meta != 0u && tn <= tf && !((pmiss >> s2) & 1u)) {
-                    if ((imask >> s2) & 1u)
-                        int_hits |= 1u << (24 + (s2 ^ p.oct));
-                    else
-                        tri_hits |= (meta >> 5) << (meta & 31u);
-                }
-            }
-            ng_base = w1.x;
-            ng_hits = int_hits | imask;
-            tg_base = w1.y;
-            tg_hits = tri_hits;
-        }
-        while (tg_hits) {
-            uint32_t j = __builtin_ctz(tg_hits);
-            tg_hits &= tg_hits - 1u;
-            uint32_t rec = tg_base + j;
-            const float4* tp = tris + (size_t)rec * kTriWords;
-            float4 r0 = tp[0], r1 = tp[1], r2 = tp[2];
-            if (COUNT) ++tvis;
-            float t, U, V, W, det;
-            if (intersect_tri(p, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, t, U, V, W, det)) {
-                uint32_t gid = __float_as_uint(r2.w);
-                if (t >= tmin && (t < res.t || (t == res.t && gid < res.gid))) {
-                    res.t = t;
-                    res.gid = gid;
-                    res.rec = rec;
-                    if (ANY) return res;
-                }
+        for (int s2 = 0; s2 < 8; ++s2) {
+            const int b = s2 & 3;
+            uint32_t nxw = s2 < 4 ? nx0 : nx1, fxw = s2 < 4 ? fx0 : fx1;
+            uint32_t nyw = s2 < 4 ? ny0 : ny1, fyw = s2 < 4 ? fy0 : fy1;
+            uint32_t nzw = s2 < 4 ? nz0 : nz1, fzw = s2 < 4 ? fz0 : fz1;
+            uint32_t metaw = s2 < 4 ? w1.z : w1.w;
+            float tnx = fmaf(ubyte_f(nxw, b), sx, ax);
+            float tny = fmaf(ubyte_f(nyw, b), sy, ay);
+            float tnz = fmaf(ubyte_f(nzw, b), sz, az);
+            float tfx = fmaf(ubyte_f(fxw, b), sx, ax);
+            float tfy = fmaf(ubyte_f(fyw, b), sy, ay);
+            float tfz = fmaf(ubyte_f(fzw, b), sz, az);
+            float tn = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), tmin);
+            float tf = fminf(fminf(fminf(tfx, tfy), tfz), st.res.t) * kBoxRobust;
+            uint32_t meta = (metaw >> (8 * b)) & 0xFFu;
+            if (meta != 0u && tn <= tf && !((pmiss >> s2) & 1u)) {
+                if ((imask >> s2) & 1u)
+                    int_hits |= 1u << (24 + (s2 ^ p.oct));
+                else
+                    tri_hits |= (meta >> 5) << (meta & 31u);
             }
         }
-        if (!(ng_hits & 0xFF000000u)) {
-            if (sp == 0) break;
-            --sp;
-            uint2 e = stk[sp * stride];
-            ng_base = e.x;
-            ng_hits = e.y;
+        st.ng_base = w1.x;
+        st.ng_hits = int_hits | imask;
+        tg_base = w1.y;
+        tg_hits = tri_hits;
+    }
+    while (tg_hits) {
+        uint32_t j = __builtin_ctz(tg_hits);
+        tg_hits &= tg_hits - 1u;
+        uint32_t rec = tg_base + j;
+        const float4* tp = tris + (size_t)rec * kTriWords;
+        float4 r0 = tp[0], r1 = tp[1], r2 = tp[2];
+        if (COUNT) ++tvis;
+        float t, U, V, W, det;
+        if (intersect_tri(p, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, t, U, V, W, det)) {
+            uint32_t gid = __float_as_uint(r2.w);
+            if (t >= tmin && (t < st.res.t || (t == st.res.t && gid < st.res.gid))) {
+                st.res.t = t;
+                st.res.gid = gid;
+                st.res.rec = rec;
+                if (ANY) return true;
+            }
         }
     }
-    return res;
+    if (!(st.ng_hits & 0xFF000000u)) {
+        if (st.sp == 0) return true;
+        --st.sp;
+        uint2 e = stk[st.sp * stride];
+        st.ng_base = e.x;
+        st.ng_hits = e.y;
+    }
+    return false;
+}
+
+// BVH8 traversal of one ray to completion (closest hit, or any hit with ANY).
+template <bool ANY, bool COUNT, int STACK>
+__device__ __forceinline__ TraceResult traverse(const uint4* __restrict__ nodes, const float4* __restrict__ tris,
+                                                const Ray& ray, uint2* stk, int stride, uint32_t& nvis,
+                                                uint32_t& tvis) {
+    TravState st;
+    if (!trav_init(ray, st)) return st.res;
+    while (!trav_step<ANY, COUNT, STACK>(nodes, tris, st, stk, stride, nvis, tvis)) {
+    }
+    return st.res;
+}
+
+// Persistent traversal with ray refill (after Aila & Laine 2009, "Understanding
+// the efficiency of ray traversal on GPUs"), for the calling wave.  The rays
+// [0, n) are split into 64-ray chunks; wave w of nwaves owns chunks w,
+// w + nwaves, w + 2 nwaves, ... (every wave samples the whole stream, so the
+// static split stays balanced; a chunk keeps neighbouring rays together).  Its
+// rays are handed out in order to idle lanes whenever at least REFILL lanes
+// are idle (or all are), so a wave does not wait for its slowest ray with the
+// other lanes masked off.  load(id, ray, nodes, tris) fetches ray `id` and the
+// BVH it is traced against; done(id, state, nvis, tvis) is called once per ray
+// with its finished state, also for invalid rays (which miss).  Every ray's
+// traversal is the one of traverse(), so results are identical.  All 64 lanes
+// of the wave must call it; no atomics (a wave's cursor is uniform and private).
+template <bool ANY, bool COUNT, int STACK, int REFILL, class Load, class Done>
+__device__ __forceinline__ void refill_trace(uint32_t n, uint32_t wave, uint32_t nwaves, uint2* stk, int stride,
+                                             Load load, Done done) {
+    const uint32_t lane = __lane_id();
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t chunks = (n + 63) / 64;
+    const uint32_t mine = wave < chunks ? (chunks - wave + nwaves - 1) / nwaves : 0u;
+    const uint32_t end = mine * 64u;  // this wave's rays, counted in its own order
+    uint32_t cursor = 0;
+    bool active = false;
+    uint32_t id = 0, nv = 0, tv = 0;
+    const uint4* nodes = nullptr;
+    const float4* tris = nullptr;
+    TravState st;
+    while (true) {
+        const uint64_t idle = __ballot(!active);
+        const uint32_t nidle = (uint32_t)__popcll(idle);
+        if (nidle == 64u && cursor >= end) break;
+        if (cursor < end && (nidle >= (uint32_t)REFILL || nidle == 64u)) {
+            if (!active) {
+                const uint32_t k = cursor + (uint32_t)__popcll(idle & below);
+                const uint32_t r = (wave + (k >> 6) * nwaves) * 64u + (k & 63u);
+                if (k < end && r < n) {
+                    id = r;
+                    Ray ray;
+                    load(id, ray, nodes, tris);
+                    nv = tv = 0;
+                    active = trav_init(ray, st);
+                    if (!active) done(id, st, nv, tv);
+                }
+            }
+            cursor += nidle;
+        }
+        if (active && trav_step<ANY, COUNT, STACK>(nodes, tris, st, stk, stride, nv, tv)) {
+            active = false;
+            done(id, st, nv, tv);
+        }
+    }
 }
 
 // Barycentrics (u, v) of a leaf record hit; bary = (1-u-v, u, v).

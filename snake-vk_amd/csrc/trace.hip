@@ -5,6 +5,8 @@
 // (ShadowTest, RayGen.rgen:101-121).  One lane per ray; the traversal stack
 // lives in LDS (STACK entries of 8 B per lane, interleaved across the block so
 // lanes at equal depth hit distinct banks).
+#include <algorithm>
+
 #include "snkpt_dev.h"
 #include "snkpt_host.h"
 
@@ -80,19 +82,77 @@ __global__ void k_trace_empty(uint32_t n, const uint32_t* __restrict__ count_dev
     if (tvis_out) tvis_out[i] = 0;
 }
 
+// Persistent closest / any hit over the ray stream: each wave owns a
+// contiguous slice of the rays and refills its idle lanes from it (see
+// refill_trace).  No atomics: a wave's cursor is uniform and private to it.
+template <bool ANY, bool COUNT, int STACK, int REFILL>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_refill(const float4* __restrict__ org_tmin,
+                                                              const float4* __restrict__ dir_tmax, uint32_t n,
+                                                              const uint32_t* __restrict__ count_dev,
+                                                              const uint4* __restrict__ nodes,
+                                                              const float4* __restrict__ tris,
+                                                              float* __restrict__ t_out, uint32_t* __restrict__ inst_out,
+                                                              uint32_t* __restrict__ prim_out, float* __restrict__ u_out,
+                                                              float* __restrict__ v_out, uint32_t* __restrict__ nvis_out,
+                                                              uint32_t* __restrict__ tvis_out) {
+    __shared__ uint2 stk[STACK * kTraceBlock];
+    const uint32_t nn = count_dev ? *count_dev : n;
+    const uint32_t nwaves = gridDim.x * (kTraceBlock / 64);
+    const uint32_t wave = xcd_block(blockIdx.x, gridDim.x) * (kTraceBlock / 64) + (threadIdx.x >> 6);
+    auto load = [&](uint32_t id, Ray& ray, const uint4*& nd, const float4*& tr) {
+        float4 a = org_tmin[id], d = dir_tmax[id];
+        ray = Ray{a.x, a.y, a.z, a.w, d.x, d.y, d.z, d.w};
+        nd = nodes;
+        tr = tris;
+    };
+    auto done = [&](uint32_t id, const TravState& st, uint32_t nv, uint32_t tv) {
+        const TraceResult& res = st.res;
+        if (res.rec != 0xFFFFFFFFu) {
+            float4 r2 = tris[(size_t)res.rec * kTriWords + 2];
+            t_out[id] = res.t;
+            if (inst_out) inst_out[id] = __float_as_uint(r2.y);
+            if (prim_out) prim_out[id] = __float_as_uint(r2.z);
+            if (u_out || v_out) {
+                float4 a = org_tmin[id], d = dir_tmax[id];
+                Ray ray{a.x, a.y, a.z, a.w, d.x, d.y, d.z, d.w};
+                float u, v;
+                hit_uv(tris, res.rec, ray, u, v);
+                if (u_out) u_out[id] = u;
+                if (v_out) v_out[id] = v;
+            }
+        } else {
+            t_out[id] = -1.0f;
+            if (inst_out) inst_out[id] = 0xFFFFFFFFu;
+            if (prim_out) prim_out[id] = 0xFFFFFFFFu;
+            if (u_out) u_out[id] = 0.0f;
+            if (v_out) v_out[id] = 0.0f;
+        }
+        if (COUNT) {
+            if (nvis_out) nvis_out[id] = nv;
+            if (tvis_out) tvis_out[id] = tv;
+        }
+    };
+    refill_trace<ANY, COUNT, STACK, REFILL>(nn, wave, nwaves, &stk[threadIdx.x], kTraceBlock, load, done);
+}
+
+constexpr int kRefill = 16;
+constexpr uint32_t kRefillBlocksPerCU = 10;  // LDS of STACK 16: 10 blocks of 128 lanes per CU
+
 template <bool ANY, bool COUNT>
-void launch_stack(uint32_t stack, uint32_t grid, hipStream_t st, const float4* o, const float4* d, uint32_t n,
-                  const uint32_t* cnt, const uint4* nodes, const float4* tris, const snkpt_hits& h, uint32_t* nv,
-                  uint32_t* tv) {
+void launch_stack(uint32_t stack, uint32_t grid, uint32_t pgrid, hipStream_t st, const float4* o, const float4* d,
+                  uint32_t n, const uint32_t* cnt, const uint4* nodes, const float4* tris, const snkpt_hits& h,
+                  uint32_t* nv, uint32_t* tv) {
+    // persistent refill launch, its grid capped by the (maximum) ray count
+    const uint32_t g = std::min(pgrid, grid);
     if (stack <= 16)
-        k_trace<ANY, COUNT, 16><<<grid, kTraceBlock, 0, st>>>(o, d, n, cnt, nodes, tris, h.t, h.inst, h.prim, h.u,
-                                                              h.v, nv, tv);
+        k_trace_refill<ANY, COUNT, 16, kRefill><<<g, kTraceBlock, 0, st>>>(o, d, n, cnt, nodes, tris, h.t, h.inst,
+                                                                          h.prim, h.u, h.v, nv, tv);
     else if (stack <= 32)
-        k_trace<ANY, COUNT, 32><<<grid, kTraceBlock, 0, st>>>(o, d, n, cnt, nodes, tris, h.t, h.inst, h.prim, h.u,
-                                                              h.v, nv, tv);
+        k_trace_refill<ANY, COUNT, 32, kRefill><<<g, kTraceBlock, 0, st>>>(o, d, n, cnt, nodes, tris, h.t, h.inst,
+                                                                          h.prim, h.u, h.v, nv, tv);
     else
-        k_trace<ANY, COUNT, 64><<<grid, kTraceBlock, 0, st>>>(o, d, n, cnt, nodes, tris, h.t, h.inst, h.prim, h.u,
-                                                              h.v, nv, tv);
+        k_trace_refill<ANY, COUNT, 64, kRefill><<<g, kTraceBlock, 0, st>>>(o, d, n, cnt, nodes, tris, h.t, h.inst,
+                                                                          h.prim, h.u, h.v, nv, tv);
 }
 
 }  // namespace
@@ -112,21 +172,23 @@ snkpt_status launch_trace(snkpt_ctx* ctx, const float4* org_tmin, const float4* 
         uint32_t need = A.depth > 0 ? A.depth - 1 : 0;
         if (need > 64) return fail(ctx, SNKPT_ERR_CAPACITY, "trace: BVH8 deeper than the 64-entry traversal stack");
         bool count = node_visits || tri_visits;
+        const uint32_t cus = ctx->props.multiProcessorCount > 0 ? (uint32_t)ctx->props.multiProcessorCount : 256u;
+        const uint32_t pgrid = cus * (need <= 16 ? kRefillBlocksPerCU : need <= 32 ? 5u : 2u);
         const uint4* nodes = A.nodes.as<uint4>();
         const float4* tris = A.leaf_tris.as<float4>();
         if (any_hit) {
             if (count)
-                launch_stack<true, true>(need, grid, st, org_tmin, dir_tmax, n, count_dev, nodes, tris, hits,
+                launch_stack<true, true>(need, grid, pgrid, st, org_tmin, dir_tmax, n, count_dev, nodes, tris, hits,
                                          node_visits, tri_visits);
             else
-                launch_stack<true, false>(need, grid, st, org_tmin, dir_tmax, n, count_dev, nodes, tris, hits,
+                launch_stack<true, false>(need, grid, pgrid, st, org_tmin, dir_tmax, n, count_dev, nodes, tris, hits,
                                           nullptr, nullptr);
         } else {
             if (count)
-                launch_stack<false, true>(need, grid, st, org_tmin, dir_tmax, n, count_dev, nodes, tris, hits,
+                launch_stack<false, true>(need, grid, pgrid, st, org_tmin, dir_tmax, n, count_dev, nodes, tris, hits,
                                           node_visits, tri_visits);
             else
-                launch_stack<false, false>(need, grid, st, org_tmin, dir_tmax, n, count_dev, nodes, tris, hits,
+                launch_stack<false, false>(need, grid, pgrid, st, org_tmin, dir_tmax, n, count_dev, nodes, tris, hits,
                                            nullptr, nullptr);
         }
     }
