@@ -172,6 +172,8 @@ snkpt_status snkpt_set_lights(snkpt_ctx* ctx, const void* light_block, size_t by
             return fail(ctx, SNKPT_ERR_CAPACITY, "snkpt_set_lights: light counts above 1024");
         if (bytes < 112 + (size_t)np * 48)
             return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_set_lights: block shorter than its point lights");
+        if (ns && bytes < 112 + 48 * (size_t)SNKPT_MAX_POINTLIGHTS + (size_t)ns * 64)
+            return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_set_lights: block shorter than its spot lights");
         ctx->light_block.assign((const uint8_t*)light_block, (const uint8_t*)light_block + bytes);
     } else {
         ctx->light_block.clear();

@@ -118,7 +118,53 @@ snkpt_status build_emitters(snkpt_ctx* ctx, hipStream_t st) {
         }
         return SNKPT_OK;
     };
+    // analytic lights of the light block (LightBuffers.glsl:1-63 layout): 4 float4 each
+    std::vector<float> al;
+    const std::vector<uint8_t>& lb = ctx->light_block;
+    auto f32 = [&](size_t off) {
+        float v;
+        std::memcpy(&v, lb.data() + off, 4);
+        return v;
+    };
+    auto push = [&](const float* w) { al.insert(al.end(), w, w + 16); };
+    if (lb.size() >= 112) {
+        const float dc[3] = {f32(0), f32(4), f32(8)};
+        if (dc[0] > 0.0f || dc[1] > 0.0f || dc[2] > 0.0f) {
+            double dx = f32(16), dy = f32(20), dz = f32(24), l = std::sqrt(dx * dx + dy * dy + dz * dz);
+            if (l > 0.0) {
+                const float w[16] = {dc[0], dc[1], dc[2], 0.0f, (float)(dx / l), (float)(dy / l), (float)(dz / l), 0.0f,
+                                     0, 0, 0, 0, 0, 0, 0, 0};
+                push(w);
+            }
+        }
+        uint32_t np, ns;
+        std::memcpy(&np, lb.data() + 96, 4);
+        std::memcpy(&ns, lb.data() + 100, 4);
+        for (uint32_t i = 0; i < np; ++i) {
+            const size_t o = 112 + 48 * (size_t)i;
+            const float c[3] = {f32(o), f32(o + 4), f32(o + 8)};
+            if (!(c[0] > 0.0f || c[1] > 0.0f || c[2] > 0.0f)) continue;
+            const float w[16] = {c[0], c[1], c[2], 1.0f, f32(o + 16), f32(o + 20), f32(o + 24), 0.0f, 0, 0, 0, 0,
+                                 f32(o + 40), f32(o + 36), f32(o + 32), 0.0f};
+            push(w);
+        }
+        const size_t so = 112 + 48 * (size_t)SNKPT_MAX_POINTLIGHTS;
+        for (uint32_t i = 0; i < ns && so + 64 * (size_t)(i + 1) <= lb.size(); ++i) {
+            const size_t o = so + 64 * (size_t)i;
+            const float c[3] = {f32(o), f32(o + 4), f32(o + 8)};
+            if (!(c[0] > 0.0f || c[1] > 0.0f || c[2] > 0.0f)) continue;
+            double dx = f32(o + 32), dy = f32(o + 36), dz = f32(o + 40), l = std::sqrt(dx * dx + dy * dy + dz * dz);
+            if (!(l > 0.0)) continue;
+            const float w[16] = {c[0], c[1], c[2], 2.0f, f32(o + 16), f32(o + 20), f32(o + 24), f32(o + 60),
+                                 (float)(dx / l), (float)(dy / l), (float)(dz / l), 0.0f,
+                                 f32(o + 56), f32(o + 52), f32(o + 48), 0.0f};
+            push(w);
+        }
+    }
+    E.n_analytic = (uint32_t)(al.size() / 16);
+    E.q_tri = ne == 0 ? (E.n_analytic ? 0.0f : 1.0f) : (E.n_analytic ? 0.5f : 1.0f);
     snkpt_status s;
+    if ((s = up(E.analytic, al.data(), al.size() * 4)) != SNKPT_OK) return s;
     if ((s = up(E.tri, tri.data(), tri.size() * 4)) != SNKPT_OK) return s;
     if ((s = up(E.info, info.data(), info.size() * 4)) != SNKPT_OK) return s;
     if ((s = up(E.alias, alias_packed.data(), alias_packed.size() * 4)) != SNKPT_OK) return s;

@@ -73,12 +73,12 @@ __device__ __forceinline__ void path_vertex(const SceneDev& S, const PtParams& P
     float ul = rng.uniform(), us1 = rng.uniform(), us2 = rng.uniform(), urr = rng.uniform();
     const bool last = bounce + 1 >= P.max_depth || query;
     // next-event estimation
-    LightSample ls = sample_light(S, s.p, u0, u1, u2, u3);
+    LightSample ls = sample_direct(S, s.p, u0, u1, u2, u3);
     if (ls.ok) {
         float cs = dot(s.ns, ls.wi);
         if (cs > 0.0f && dot(s.ng, ls.wi) > 0.0f) {
             V3 f = brdf_eval(s.m, s.ns, wo, ls.wi);
-            float w = last ? 1.0f : power_heuristic(ls.pdf, brdf_pdf(s.m, s.ns, wo, ls.wi));
+            float w = (last || ls.delta) ? 1.0f : power_heuristic(ls.pdf, brdf_pdf(s.m, s.ns, wo, ls.wi));
             o.sh_c = mul(mul(thr, f), ls.le) * (cs * w / ls.pdf);
             if (o.sh_c.x > 0.0f || o.sh_c.y > 0.0f || o.sh_c.z > 0.0f) {
                 o.sh = true;

@@ -269,6 +269,9 @@ SceneDev scene_dev(snkpt_ctx* ctx) {
     S.n_emit = ctx->emit.count;
     S.nodes = ctx->accel.nodes.as<uint4>();
     S.leaf = ctx->accel.leaf_tris.as<float4>();
+    S.alights = ctx->emit.analytic.as<float4>();
+    S.n_alights = ctx->emit.n_analytic;
+    S.q_tri = ctx->emit.q_tri;
     return S;
 }
 

@@ -33,6 +33,12 @@ struct SceneDev {
     uint32_t n_emit;
     const uint4* nodes;
     const float4* leaf;
+    // analytic lights of the light block (directional, point, spot), 4 float4 each:
+    // colour rgb | type (0 dir, 1 point, 2 spot); position (direction towards a
+    // directional light) | spot aperture; spot direction; attenuation c, l, e
+    const float4* alights;
+    uint32_t n_alights;
+    float q_tri;  // probability that NEE samples the emissive triangles (else an analytic light)
 };
 
 // Surface data at a hit.
@@ -88,8 +94,9 @@ __device__ __forceinline__ Surf surface_at(const SceneDev& S, V3 dir, uint32_t i
 
 struct LightSample {
     V3 wi, le;
-    float dist, pdf;  // solid-angle pdf
+    float dist, pdf;  // solid-angle pdf; for a delta light the selection probability
     bool ok;
+    bool delta;       // analytic light: BRDF sampling cannot hit it (no MIS)
 };
 
 // Power-proportional emitter choice (alias table), uniform point on it.
@@ -97,6 +104,7 @@ __device__ __forceinline__ LightSample sample_light(const SceneDev& S, V3 p, flo
                                                     float u3) {
     LightSample ls;
     ls.ok = false;
+    ls.delta = false;
     if (S.n_emit == 0) return ls;
     uint32_t e = (uint32_t)(u0 * (float)S.n_emit);
     if (e >= S.n_emit) e = S.n_emit - 1;
@@ -126,14 +134,75 @@ __device__ __forceinline__ LightSample sample_light(const SceneDev& S, V3 p, flo
     return ls;
 }
 
-// Solid-angle pdf with which sample_light would pick the point hit on global triangle gid.
+// Analytic light k of the light block, uniformly chosen by u (LightBuffers.glsl:
+// CalcDirectionalLight, CalcPointlight, CalcSpotlight): radiance arriving from
+// a delta light, attenuation c + l d + e d^2, spot falloff
+// 1 - (1 - cos) / (1 - aperture) inside the aperture, nothing beyond it or where
+// the attenuation exceeds 5000.  pdf = selection probability 1 / n.
+__device__ __forceinline__ LightSample sample_analytic(const SceneDev& S, V3 p, float u) {
+    LightSample ls;
+    ls.ok = false;
+    ls.delta = true;
+    if (S.n_alights == 0) return ls;
+    uint32_t k = (uint32_t)(u * (float)S.n_alights);
+    if (k >= S.n_alights) k = S.n_alights - 1;
+    const float4* L = S.alights + 4 * (size_t)k;
+    const float4 w0 = L[0], w1 = L[1];
+    const uint32_t type = (uint32_t)w0.w;
+    V3 col{w0.x, w0.y, w0.z};
+    ls.pdf = 1.0f / (float)S.n_alights;
+    if (type == 0u) {
+        ls.wi = V3{w1.x, w1.y, w1.z};
+        ls.dist = 1e30f;
+        ls.le = col;
+        ls.ok = true;
+        return ls;
+    }
+    V3 d = V3{w1.x, w1.y, w1.z} - p;
+    float d2 = dot(d, d);
+    if (!(d2 > 0.0f)) return ls;
+    float dist = sqrtf(d2);
+    ls.wi = d * (1.0f / dist);
+    const float4 w3 = L[3];
+    float atten = (w3.x + w3.y * dist) + w3.z * d2;
+    if (!(atten > 0.0f)) return ls;
+    float f = 1.0f;
+    if (type == 2u) {
+        const float4 w2 = L[2];
+        float sf = -dot(ls.wi, V3{w2.x, w2.y, w2.z});
+        if (sf < 0.0001f || sf < w1.w || atten > 5000.0f) return ls;
+        f = 1.0f - (1.0f - sf) / fmaxf(1.0f - w1.w, 1e-5f);
+    }
+    ls.le = col * (f / atten);
+    ls.dist = dist;
+    ls.ok = ls.le.x > 0.0f || ls.le.y > 0.0f || ls.le.z > 0.0f;
+    return ls;
+}
+
+// Next-event estimation: the emissive triangles with probability q_tri (u0
+// rescaled, so with q_tri = 1 this is exactly sample_light), else one analytic
+// light; the returned pdf includes the choice of set.
+__device__ __forceinline__ LightSample sample_direct(const SceneDev& S, V3 p, float u0, float u1, float u2,
+                                                     float u3) {
+    if (u0 < S.q_tri) {
+        LightSample ls = sample_light(S, p, u0 / S.q_tri, u1, u2, u3);
+        ls.pdf *= S.q_tri;
+        return ls;
+    }
+    LightSample ls = sample_analytic(S, p, (u0 - S.q_tri) / (1.0f - S.q_tri));
+    ls.pdf *= 1.0f - S.q_tri;
+    return ls;
+}
+
+// Solid-angle pdf with which sample_direct would pick the point hit on global
+// triangle gid.
 __device__ __forceinline__ float light_pdf(const SceneDev& S, uint32_t gid, V3 ng_hit, V3 dir, float t) {
     if (S.n_emit == 0) return 0.0f;
     uint32_t e = S.gid2e[gid];
     if (e == 0xFFFFFFFFu) return 0.0f;
     float cosl = fabsf(dot(ng_hit, dir));
     if (cosl <= 0.0f) return 0.0f;
-    return S.emit_pdf[e] * t * t / (S.emit_info[e].w * cosl);
+    return S.q_tri * (S.emit_pdf[e] * t * t / (S.emit_info[e].w * cosl));
 }
 
 // Wave-level queue append: every lane calls it (convergent); lanes with

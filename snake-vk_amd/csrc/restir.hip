@@ -389,6 +389,54 @@ __global__ __launch_bounds__(kBlock) void k_rshade(SceneDev S, GView G, const fl
     }
 }
 
+// Analytic lights at the primary hit (ReSTIR resamples the emissive triangles
+// only): one light per pixel, uniformly chosen from a stream of its own, as
+// NEE with a shadow ray (delta lights: no MIS partner).
+__global__ __launch_bounds__(kBlock) void k_rshade_analytic(SceneDev S, GView G, ShadowQueue sq,
+                                                            uint32_t* __restrict__ counts, uint32_t n,
+                                                            uint64_t initstate) {
+    __shared__ uint32_t lds_app[kBlock / 64 + 1];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool want = false;
+    V3 c, wi, o;
+    float tmax = 0.0f;
+    if (i < n) {
+        GSurf g = load_g(G, i);
+        if (g.valid) {
+            Rng r = rng_seed(initstate ^ 0x414E414C59544943ull, i);
+            LightSample ls = sample_analytic(S, g.p, r.uniform());
+            if (ls.ok) {
+                float cs = dot(g.ns, ls.wi);
+                if (cs > 0.0f && dot(g.ng, ls.wi) > 0.0f) {
+                    c = mul(brdf_eval(g.m, g.ns, g.wo, ls.wi), ls.le) * (cs / ls.pdf);
+                    if (c.x > 0.0f || c.y > 0.0f || c.z > 0.0f) {
+                        float am = fmaxf(fmaxf(fabsf(g.p.x), fabsf(g.p.y)), fabsf(g.p.z));
+                        float eps = 1e-4f * (1.0f + am);
+                        o = g.p + g.ng * eps;
+                        wi = ls.wi;
+                        tmax = ls.dist * (1.0f - 1e-4f) - eps;
+                        want = true;
+                    }
+                }
+            }
+        }
+    }
+    uint32_t slot = block_append<kBlock>(want, &counts[2], lds_app);
+    if (want) {
+        sq.org[slot] = make_float4(o.x, o.y, o.z, 0.0f);
+        sq.dir[slot] = make_float4(wi.x, wi.y, wi.z, tmax);
+        sq.contrib[slot] = make_float4(c.x, c.y, c.z, __uint_as_float(i));
+    }
+}
+
+// shadow rays of k_rshade_analytic
+__global__ void k_restir_stats2(uint32_t* c, unsigned long long* stats) {
+    if (threadIdx.x == 0) {
+        if (stats) stats[2] += c[2];
+        c[2] = 0;
+    }
+}
+
 // extension rays queued by k_primary and shadow rays of k_rshade
 __global__ void k_restir_stats(const uint32_t* c, unsigned long long* stats) {
     if (threadIdx.x == 0) {
@@ -476,6 +524,13 @@ snkpt_status restir_pass(snkpt_ctx* ctx, const snkpt_camera* cam, const CamDev& 
                                                      counts, n, stats);
     if ((e = launch_shadow(ctx, R, R.rad.as<float4>(), st)) != SNKPT_OK) return e;
     k_restir_stats<<<1, 64, 0, st>>>(counts, stats);
+    if (S.n_alights > 0) {
+        // the shadow queue is reused for the analytic lights of the primary hits
+        k_restir_stats2<<<1, 64, 0, st>>>(counts, nullptr);
+        k_rshade_analytic<<<grid_for(n, kBlock), kBlock, 0, st>>>(S, G, shadow_of(R), counts, n, initstate);
+        if ((e = launch_shadow(ctx, R, R.rad.as<float4>(), st)) != SNKPT_OK) return e;
+        k_restir_stats2<<<1, 64, 0, st>>>(counts, stats);
+    }
     // history for the next frame
     std::memcpy(X.prev_pv, cam->proj_view, sizeof(X.prev_pv));
     X.prev_pos[0] = cam->cam_pos[0];

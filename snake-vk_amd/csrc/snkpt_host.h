@@ -71,6 +71,9 @@ struct Emitters {
     DeviceBuffer alias;  // alias table: float prob, u32 alias
     DeviceBuffer pdf;    // per emitter: selection probability
     DeviceBuffer gid_to_emitter;  // per global triangle: emitter index or ~0u
+    DeviceBuffer analytic;        // directional / point / spot lights, 4 float4 each
+    uint32_t n_analytic = 0;
+    float q_tri = 1.0f;           // NEE: probability of sampling the triangles rather than an analytic light
 };
 
 struct RenderState;  // defined in render.hip

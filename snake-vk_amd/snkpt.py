@@ -243,6 +243,21 @@ class SceneData:
             self._p = None
 
 
+def light_block(dir_colour=(0.0, 0.0, 0.0), dir_to_light=(0.0, 1.0, 0.0), points=(), spots=()) -> bytes:
+    """The reference's LightData SSBO (LightBuffers.glsl:1-63, 114,800 B):
+    points = [(colour3, pos3, (atten_exp, atten_linear, atten_constant))],
+    spots = [(colour3, pos3, dir3, (atten_exp, atten_linear, atten_constant), aperture)]."""
+    import struct
+    b = bytearray(112 + 48 * 1024 + 64 * 1024)
+    struct.pack_into("4f4f", b, 0, *dir_colour, 1.0, *dir_to_light, 0.0)
+    struct.pack_into("2I", b, 96, len(points), len(spots))
+    for i, (c, p, att) in enumerate(points):
+        struct.pack_into("4f4f4f", b, 112 + 48 * i, *c, 1.0, *p, 1.0, *att, 0.0)
+    for i, (c, p, d, att, ap) in enumerate(spots):
+        struct.pack_into("4f4f4f4f", b, 112 + 48 * 1024 + 64 * i, *c, 1.0, *p, 1.0, *d, 0.0, *att, ap)
+    return bytes(b)
+
+
 def scene_view(config: int, view: int):
     """(variant, w, h) of view `view` of the batched workload (config 5)."""
     v, w, h = C.c_uint32(), C.c_uint32(), C.c_uint32()

@@ -94,7 +94,7 @@ class ArrayScene:
         pos, idx, inst, m4, mats, nrm, em = self.arrays()
         return O.flatten(pos, idx, inst, m4)
 
-    def load_into(self, ctx):
+    def load_into(self, ctx, lights=None):
         import snkpt
         pos, idx, inst, m4, mats, nrm, em = self.arrays()
         ctx.upload_mesh(pos, idx, nrm=nrm)
@@ -104,7 +104,7 @@ class ArrayScene:
             mm = snkpt.Material.from_buffer_copy(m.tobytes())
             ms.append(mm)
         ctx.set_materials(ms)
-        ctx.set_lights(None, em)
+        ctx.set_lights(lights, em)
         ctx.build_accel()
 
 
