@@ -157,17 +157,79 @@ __global__ __launch_bounds__(TB) void k_btrace(SceneMany ss, const float4* __res
     refill_trace<false, false, STACK, kRefillLanes>(n, wave, nwaves, &stk[threadIdx.x], TB, load, done);
 }
 
+template <int KIND>
 __global__ __launch_bounds__(kBBlock) void k_bshade(SceneMany ss, PtParams P, Queue in, HitArrays hit, Queue out,
                                                     ShadowQueue sq, PathState ps, float4* __restrict__ rad,
-                                                    uint32_t* __restrict__ counts, uint32_t cur, uint32_t nxt) {
+                                                    uint32_t* __restrict__ counts, uint32_t cur, uint32_t nxt,
+                                                    ShadeHooks H) {
     __shared__ uint32_t lds_app[kBBlock / 64 + 1];
     const uint32_t n = counts[cur];
-    const ShadeHooks H{};
     // block-uniform loop: the appends inside the step are block-wide
     for (uint32_t base = blockIdx.x * kBBlock; base < n; base += gridDim.x * kBBlock) {
         const uint32_t i = base + threadIdx.x;
-        shade_step<0, kBBlock>(ss, P, in, hit, out, sq, ps, rad, counts, nxt, H, i, i < n, lds_app);
+        shade_step<KIND, kBBlock>(ss, P, in, hit, out, sq, ps, rad, counts, nxt, H, i, i < n, lds_app);
     }
+}
+
+// Cache queries of the batch grouped by scene (each scene's network answers its
+// own contiguous segment).  Per block: an LDS histogram over the scenes, one
+// global atomic per (block, scene) to reserve room, ranks from LDS atomics.
+constexpr uint32_t kMaxHistScenes = 1024;
+__global__ __launch_bounds__(kBBlock) void k_bq_hist(const float4* __restrict__ meta, const uint32_t* __restrict__ nq,
+                                                     const JobD* __restrict__ jobs,
+                                                     const uint32_t* __restrict__ path_job, uint32_t nscenes,
+                                                     uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[kMaxHistScenes];
+    for (uint32_t k = threadIdx.x; k < nscenes; k += blockDim.x) h[k] = 0u;
+    __syncthreads();
+    const uint32_t n = *nq;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(&h[jobs[path_job[__float_as_uint(meta[i].w)]].scene], 1u);
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < nscenes; k += blockDim.x)
+        if (h[k]) atomicAdd(&hist[k], h[k]);
+}
+
+__global__ __launch_bounds__(kBBlock) void k_bq_scatter(const float* __restrict__ samples,
+                                                        const float4* __restrict__ meta,
+                                                        const uint32_t* __restrict__ nq, const JobD* __restrict__ jobs,
+                                                        const uint32_t* __restrict__ path_job, uint32_t nscenes,
+                                                        uint32_t* __restrict__ cursor, float* __restrict__ s_out,
+                                                        float4* __restrict__ m_out) {
+    __shared__ uint32_t h[kMaxHistScenes];
+    for (uint32_t k = threadIdx.x; k < nscenes; k += blockDim.x) h[k] = 0u;
+    __syncthreads();
+    const uint32_t n = *nq;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t sc = 0, rank = 0;
+    if (i < n) {
+        sc = jobs[path_job[__float_as_uint(meta[i].w)]].scene;
+        rank = atomicAdd(&h[sc], 1u);
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < nscenes; k += blockDim.x)
+        if (h[k]) h[k] = atomicAdd(&cursor[k], h[k]);
+    __syncthreads();
+    if (i < n) {
+        const uint32_t slot = h[sc] + rank;
+        const float4* src = (const float4*)(samples + (size_t)i * 16);
+        float4* dst = (float4*)(s_out + (size_t)slot * 16);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) dst[c] = src[c];
+        m_out[slot] = meta[i];
+    }
+}
+
+// rad[path] += throughput * max(cache output, 0) for the n sorted queries.
+__global__ void k_bq_apply(const float4* __restrict__ meta, const float4* __restrict__ y, uint32_t n,
+                           float4* __restrict__ rad) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 m = meta[i], v = y[i];
+    const uint32_t path = __float_as_uint(m.w);
+    const float4 r = rad[path];
+    rad[path] = make_float4(r.x + m.x * fmaxf(v.x, 0.0f), r.y + m.y * fmaxf(v.y, 0.0f), r.z + m.z * fmaxf(v.z, 0.0f),
+                            r.w);
 }
 
 template <int STACK, int TB>
@@ -250,6 +312,9 @@ struct snkpt_batch {
     DeviceBuffer h_t, h_inst, h_prim, h_u, h_v;
     DeviceBuffer s_org, s_dir, s_contrib;
     DeviceBuffer p_thr, p_rng, p_bounce, path_job, rad, accum, counts, stats;
+    // cache form: queries, their per-scene sorting, outputs
+    DeviceBuffer q_samples, q_meta, s_samples, s_meta, q_y, hist;
+    uint32_t qcap = 0;
     snkpt::FrameD frame_host{};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -429,8 +494,8 @@ snkpt_status record_frame(snkpt_batch* B, const snkpt_settings* s, hipStream_t s
                 launch_traces_tb<128>(B, ss, cur, tgrid, st, false);
             Queue qin{B->q_org[cur].as<float4>(), B->q_dir[cur].as<float4>(), B->q_path[cur].as<uint32_t>()};
             Queue qout{B->q_org[nxt].as<float4>(), B->q_dir[nxt].as<float4>(), B->q_path[nxt].as<uint32_t>()};
-            k_bshade<<<sgrid, kBBlock, 0, st>>>(ss, P, qin, hit, qout, sq, ps, B->rad.as<float4>(), counts,
-                                                (uint32_t)cur, (uint32_t)nxt);
+            k_bshade<0><<<sgrid, kBBlock, 0, st>>>(ss, P, qin, hit, qout, sq, ps, B->rad.as<float4>(), counts,
+                                                   (uint32_t)cur, (uint32_t)nxt, ShadeHooks{});
             if (tb == 64)
                 launch_traces_tb<64>(B, ss, cur, tgrid, st, true);
             else
@@ -441,6 +506,152 @@ snkpt_status record_frame(snkpt_batch* B, const snkpt_settings* s, hipStream_t s
         k_bresolve<<<ggrid, kBBlock, 0, st>>>(B->d_jobs.as<JobD>(), B->path_job.as<uint32_t>(),
                                               B->rad.as<float4>(), B->accum.as<float4>(), g.total,
                                               B->d_frame.as<FrameD>());
+    }
+    hipError_t le = hipGetLastError();
+    if (le != hipSuccess) return bfail(B, SNKPT_ERR_HIP, std::string("batch launch: ") + hipGetErrorString(le));
+    return SNKPT_OK;
+}
+
+__global__ void k_bset(uint32_t* c, uint32_t i, uint32_t v) {
+    if (threadIdx.x == 0) c[i] = v;
+}
+__global__ void k_bstat_add(const uint32_t* c, uint32_t i, unsigned long long* stat) {
+    if (threadIdx.x == 0) *stat += c[i];
+}
+
+// One frame of every view in the cache form: the batched paths end in a cache
+// query at vertex Q (or take NEE there and query the indirect light); the
+// queries are grouped by scene and each scene's own network answers them; then
+// every scene takes one training step from full paths through its first view
+// (cache_train_paths split over the scenes in proportion to their pixels).
+// Eager only: the grouping and the training read counts back to the host.
+snkpt_status record_cache_frame(snkpt_batch* B, const snkpt_settings* s, hipStream_t st) {
+    const uint32_t nsc = (uint32_t)B->scenes.size();
+    if (nsc > kMaxHistScenes) return bfail(B, SNKPT_ERR_CAPACITY, "batch cache form: more than 1024 scenes");
+    for (snkpt_ctx* c : B->scenes) {
+        snkpt_status e = cache_ensure(c, s, st);
+        if (e != SNKPT_OK) return bfail(B, e, std::string("batch cache form: ") + c->last_error);
+    }
+    const uint32_t depth = s->max_depth ? s->max_depth : 1u;
+    const uint32_t Q = s->cache_query_depth ? s->cache_query_depth : 2u;
+    const uint32_t cap = B->cap;
+    if (cap > B->qcap) {
+        BHIP(B->q_samples.reserve((size_t)cap * 64));
+        BHIP(B->q_meta.reserve((size_t)cap * 16));
+        BHIP(B->s_samples.reserve((size_t)cap * 64));
+        BHIP(B->s_meta.reserve((size_t)cap * 16));
+        BHIP(B->q_y.reserve((size_t)cap * 16));
+        B->qcap = cap;
+    }
+    BHIP(B->hist.reserve((size_t)nsc * 8));
+    const bool avg = s->cache_ema > 0.0f;
+    // with depth >= Q the vertex before the query keeps its continuation; with depth < Q this is plain PT
+    PtParams P{std::min(depth, Q), s->rr_start};
+    SceneMany ss{B->d_scenes.as<SceneDev>(), B->d_jobs.as<JobD>(), B->path_job.as<uint32_t>()};
+    PathState ps{B->p_thr.as<float4>(), B->p_rng.as<uint2>(), B->p_bounce.as<uint32_t>()};
+    ShadowQueue sq{B->s_org.as<float4>(), B->s_dir.as<float4>(), B->s_contrib.as<float4>()};
+    HitArrays hit{B->h_t.as<float>(), B->h_inst.as<uint32_t>(), B->h_prim.as<uint32_t>(), B->h_u.as<float>(),
+                  B->h_v.as<float>()};
+    uint32_t* counts = B->counts.as<uint32_t>();
+    unsigned long long* stats = B->stats.as<unsigned long long>();
+    ShadeHooks H;
+    H.kind = 1;
+    H.query_at = Q - 1;
+    H.indirect = (s->flags & SNKPT_FLAG_CACHE_INDIRECT) ? 1u : 0u;
+    H.q_samples = B->q_samples.as<float>();
+    H.q_meta = B->q_meta.as<float4>();
+    H.q_count = counts + 3;
+    const uint32_t tb = B->tuning.trace_block == 64 ? 64u : 128u;
+    const uint32_t tgrid = (uint32_t)B->cus * std::max(1u, B->tuning.trace_blocks_per_cu);
+    const uint32_t sgrid = (uint32_t)B->cus * std::max(1u, B->tuning.shade_blocks_per_cu);
+    const uint32_t ggrid = (uint32_t)B->cus * 8u;
+    k_bzero<<<1, 64, 0, st>>>(stats, 8);
+    std::vector<uint32_t> hist(nsc), off(nsc);
+    for (const auto& g : B->groups) {
+        k_bgenerate<<<ggrid, kBBlock, 0, st>>>(B->d_jobs.as<JobD>(), g.j0, g.j1, g.total, B->d_frame.as<FrameD>(),
+                                               B->q_org[0].as<float4>(), B->q_dir[0].as<float4>(),
+                                               B->q_path[0].as<uint32_t>(), ps, B->path_job.as<uint32_t>(),
+                                               B->rad.as<float4>(), counts);
+        k_bset<<<1, 64, 0, st>>>(counts, 3, 0u);
+        int cur = 0;
+        for (uint32_t b = 0; b < P.max_depth; ++b) {
+            const int nxt = cur ^ 1;
+            k_bbounce<<<1, 64, 0, st>>>(counts, (uint32_t)cur, (uint32_t)nxt, stats, b > 0 ? 1u : 0u);
+            if (tb == 64)
+                launch_traces_tb<64>(B, ss, cur, tgrid, st, false);
+            else
+                launch_traces_tb<128>(B, ss, cur, tgrid, st, false);
+            Queue qin{B->q_org[cur].as<float4>(), B->q_dir[cur].as<float4>(), B->q_path[cur].as<uint32_t>()};
+            Queue qout{B->q_org[nxt].as<float4>(), B->q_dir[nxt].as<float4>(), B->q_path[nxt].as<uint32_t>()};
+            k_bshade<1><<<sgrid, kBBlock, 0, st>>>(ss, P, qin, hit, qout, sq, ps, B->rad.as<float4>(), counts,
+                                                   (uint32_t)cur, (uint32_t)nxt, H);
+            if (tb == 64)
+                launch_traces_tb<64>(B, ss, cur, tgrid, st, true);
+            else
+                launch_traces_tb<128>(B, ss, cur, tgrid, st, true);
+            cur = nxt;
+        }
+        k_bbounce_end<<<1, 64, 0, st>>>(counts, stats);
+        k_bstat_add<<<1, 64, 0, st>>>(counts, 3, stats + 3);
+        // group the queries by scene and query each scene's network
+        uint32_t nq = 0;
+        BHIP(hipMemcpyAsync(&nq, counts + 3, 4, hipMemcpyDeviceToHost, st));
+        BHIP(hipStreamSynchronize(st));
+        if (nq > 0) {
+            uint32_t* d_hist = B->hist.as<uint32_t>();
+            uint32_t* d_cursor = d_hist + nsc;
+            std::fill(hist.begin(), hist.end(), 0u);
+            BHIP(hipMemcpyAsync(d_hist, hist.data(), (size_t)nsc * 4, hipMemcpyHostToDevice, st));
+            k_bq_hist<<<grid_for(nq, kBBlock), kBBlock, 0, st>>>(B->q_meta.as<float4>(), counts + 3,
+                                                                 B->d_jobs.as<JobD>(), B->path_job.as<uint32_t>(),
+                                                                 nsc, d_hist);
+            BHIP(hipMemcpyAsync(hist.data(), d_hist, (size_t)nsc * 4, hipMemcpyDeviceToHost, st));
+            BHIP(hipStreamSynchronize(st));
+            uint32_t run = 0;
+            for (uint32_t k = 0; k < nsc; ++k) {
+                off[k] = run;
+                run += hist[k];
+            }
+            BHIP(hipMemcpyAsync(d_cursor, off.data(), (size_t)nsc * 4, hipMemcpyHostToDevice, st));
+            k_bq_scatter<<<grid_for(nq, kBBlock), kBBlock, 0, st>>>(B->q_samples.as<float>(), B->q_meta.as<float4>(),
+                                                                    counts + 3, B->d_jobs.as<JobD>(),
+                                                                    B->path_job.as<uint32_t>(), nsc, d_cursor,
+                                                                    B->s_samples.as<float>(), B->s_meta.as<float4>());
+            for (uint32_t k = 0; k < nsc; ++k) {
+                if (!hist[k]) continue;
+                snkpt_status e = cache_query(B->scenes[k], B->s_samples.as<float>() + (size_t)off[k] * 16, hist[k],
+                                             B->q_y.as<float4>() + off[k], avg, st);
+                if (e != SNKPT_OK) return bfail(B, e, std::string("batch cache query: ") + B->scenes[k]->last_error);
+            }
+            k_bq_apply<<<grid_for(nq, 256), 256, 0, st>>>(B->s_meta.as<float4>(), B->q_y.as<float4>(), nq,
+                                                          B->rad.as<float4>());
+        }
+        k_bresolve<<<ggrid, kBBlock, 0, st>>>(B->d_jobs.as<JobD>(), B->path_job.as<uint32_t>(),
+                                              B->rad.as<float4>(), B->accum.as<float4>(), g.total,
+                                              B->d_frame.as<FrameD>());
+    }
+    if (!(s->flags & SNKPT_FLAG_NO_TRAIN)) {
+        // training: full paths through each scene's first view
+        std::vector<uint64_t> px(nsc, 0);
+        std::vector<int64_t> first(nsc, -1);
+        uint64_t total = 0;
+        for (size_t v = 0; v < B->views.size(); ++v) {
+            const auto& V = B->views[v];
+            px[V.scene] += (uint64_t)V.w * V.h;
+            total += (uint64_t)V.w * V.h;
+            if (first[V.scene] < 0) first[V.scene] = (int64_t)v;
+        }
+        const uint64_t budget = s->cache_train_paths ? s->cache_train_paths : 65536u;
+        for (uint32_t k = 0; k < nsc; ++k) {
+            if (first[k] < 0) continue;
+            const auto& V = B->views[(size_t)first[k]];
+            uint64_t nt = std::max<uint64_t>(1024u, budget * px[k] / std::max<uint64_t>(total, 1));
+            nt = std::min<uint64_t>(nt, (uint64_t)V.w * V.h);
+            const CamDev cd = make_cam(&V.cam);
+            snkpt_status e = cache_train_frame(B->scenes[k], cd, V.w, V.h, s, pass_state(V.seed, s->frame, 0),
+                                               (uint32_t)nt, stats, st);
+            if (e != SNKPT_OK) return bfail(B, e, std::string("batch cache training: ") + B->scenes[k]->last_error);
+        }
     }
     hipError_t le = hipGetLastError();
     if (le != hipSuccess) return bfail(B, SNKPT_ERR_HIP, std::string("batch launch: ") + hipGetErrorString(le));
@@ -464,7 +675,10 @@ uint64_t graph_key(const snkpt_batch* B, const snkpt_settings* s, hipStream_t st
 
 snkpt_status batch_render(snkpt_batch* B, const snkpt_settings* s, int use_graph, hipStream_t st) {
     if (!s) return bfail(B, SNKPT_ERR_INVALID_ARG, "batch_render: settings is NULL");
-    if (s->mode != SNKPT_MODE_PT) return bfail(B, SNKPT_ERR_INVALID_ARG, "batch_render: only SNKPT_MODE_PT is batched");
+    if (s->mode != SNKPT_MODE_PT && s->mode != SNKPT_MODE_CACHE)
+        return bfail(B, SNKPT_ERR_INVALID_ARG, "batch_render: SNKPT_MODE_PT or SNKPT_MODE_CACHE");
+    if (s->mode == SNKPT_MODE_CACHE && use_graph)
+        return bfail(B, SNKPT_ERR_INVALID_ARG, "batch_render: the cache form runs eagerly (use_graph = 0)");
     if (s->spp > 1) return bfail(B, SNKPT_ERR_INVALID_ARG, "batch_render: one sample per pixel per frame");
     if (use_graph && st == nullptr)
         return bfail(B, SNKPT_ERR_INVALID_ARG, "batch_render: graph capture needs a created (non-null) stream");
@@ -477,6 +691,7 @@ snkpt_status batch_render(snkpt_batch* B, const snkpt_settings* s, int use_graph
     B->frame_host.accumulate = (s->flags & SNKPT_FLAG_ACCUMULATE) ? 1u : 0u;
     B->frame_host.reset = (s->flags & SNKPT_FLAG_RESET) ? 1u : 0u;
     BHIP(hipMemcpyAsync(B->d_frame.ptr, &B->frame_host, sizeof(FrameD), hipMemcpyHostToDevice, st));
+    if (s->mode == SNKPT_MODE_CACHE) return record_cache_frame(B, s, st);
     if (!use_graph) return record_frame(B, s, st);
     const uint64_t key = graph_key(B, s, st);
     if (!B->exec || key != B->graph_key) {
@@ -675,6 +890,8 @@ snkpt_status snkpt_batch_frame_stats(snkpt_batch* B, snkpt_frame_stats* out, snk
     out->camera_rays = B->camera_rays;
     out->extension_rays = h[1];
     out->shadow_rays = h[2];
+    out->cache_queries = h[3];
+    out->cache_train_samples = h[4];
     return SNKPT_OK;
 }
 

@@ -885,6 +885,14 @@ __global__ void k_cache_apply(const float4* __restrict__ meta, const float4* __r
                             r.w);
 }
 
+// training counters: samples emitted, recorded vertices
+__global__ void k_train_clear(uint32_t* c) {
+    if (threadIdx.x == 0) {
+        c[1] = 0u;
+        c[2] = 0u;
+    }
+}
+
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
     if (threadIdx.x < n) p[threadIdx.x] = 0u;
 }
@@ -995,24 +1003,37 @@ constexpr uint64_t kTrainPaths = 0x545241494eull;
 
 }  // namespace
 
-snkpt_status cache_pass(snkpt_ctx* ctx, const CamDev& cd, const snkpt_settings* s, uint64_t initstate, uint32_t pass,
-                        hipStream_t st) {
-    RenderState& R = *ctx->render;
-    const uint32_t n = R.w * R.h;
-    if (!ctx->cache || !ctx->cache->ready) {
-        snkpt_status e = cache_reset(ctx, s->seed, nullptr, nullptr, st);
-        if (e != SNKPT_OK) return e;
-    }
+snkpt_status cache_ensure(snkpt_ctx* ctx, const snkpt_settings* s, hipStream_t st) {
+    if (!ctx->cache || !ctx->cache->ready) return cache_reset(ctx, s->seed, nullptr, nullptr, st);
+    return SNKPT_OK;
+}
+
+// Encoding + MLP on n queries (device memory) into y; `averaged`: the weight averages.
+snkpt_status cache_query(snkpt_ctx* ctx, const float* samples, uint32_t n, float4* y, bool averaged,
+                         hipStream_t st) {
     CacheState& C = *ctx->cache;
+    if (n == 0) return SNKPT_OK;
+    snkpt_status e = ensure_work(ctx, C, n, false);
+    if (e != SNKPT_OK) return e;
+    k_encode<<<enc_blocks(n), kEncBlock, 0, st>>>(samples, n, nullptr, C.geom,
+                                                  (averaged ? C.grid_e16 : C.grid16).as<__half2>(),
+                                                  C.x0.as<_Float16>());
+    const uint32_t waves = (n + 63) / 64;
+    k_mlp_infer<<<(waves + kInferWaves - 1) / kInferWaves, 64 * kInferWaves, 0, st>>>(
+        C.x0.as<_Float16>(), n, nullptr, (averaged ? C.mlp_e16 : C.mlp16).as<_Float16>(), y);
+    return SNKPT_OK;
+}
+
+// Training half of a cache frame: n_paths full paths through a strided pixel
+// subset of a w x h view of camera cd record their vertices; the thinned
+// vertices become one optimisation step (Adam, weight averages of decay cache_ema).
+snkpt_status cache_train_frame(snkpt_ctx* ctx, const CamDev& cd, uint32_t w, uint32_t h, const snkpt_settings* s,
+                               uint64_t initstate, uint32_t n_paths, unsigned long long* stats, hipStream_t st) {
+    CacheState& C = *ctx->cache;
+    const uint32_t n = w * h;
     const uint32_t depth = s->max_depth ? s->max_depth : 1u;
-    // the Q-th path vertex (1 = primary hit) is answered by the cache
-    const uint32_t Q = s->cache_query_depth ? s->cache_query_depth : 2u;
-    const uint32_t NT = std::min<uint32_t>(s->cache_train_paths ? s->cache_train_paths : 65536u, n);
-    if (n > C.rcap) {
-        HIPC(C.q_samples.reserve((size_t)n * 64));
-        HIPC(C.q_meta.reserve((size_t)n * 16));
-        C.rcap = n;
-    }
+    const uint32_t NT = std::min<uint32_t>(n_paths, n);
+    if (NT == 0) return SNKPT_OK;
     HIPC(C.counts.reserve(64));
     HIPC(C.train_vtx.reserve((size_t)NT * depth * kRecFloats * 4));
     // train_meta: nrec | per-path sample counts | offsets (u32 each) | scan sums | radiance (float4)
@@ -1021,8 +1042,60 @@ snkpt_status cache_pass(snkpt_ctx* ctx, const CamDev& cd, const snkpt_settings* 
     const size_t trad_off = 3 * u32_bytes + sums_bytes;
     HIPC(C.train_meta.reserve(trad_off + (size_t)NT * 16));
     HIPC(C.t_samples.reserve((size_t)NT * depth * 64));
-    snkpt_status e = ensure_work(ctx, C, n, false);
+    uint32_t* qc = C.counts.as<uint32_t>();
+    const uint32_t indirect = (s->flags & SNKPT_FLAG_CACHE_INDIRECT) ? 1u : 0u;
+    uint32_t* nrec = C.train_meta.as<uint32_t>();
+    float4* trad = (float4*)(C.train_meta.as<uint8_t>() + trad_off);
+    const uint32_t stride = std::max<uint32_t>(n / NT, 1u);
+    const uint32_t start = (uint32_t)((initstate >> 17) % stride);
+    k_train_clear<<<1, 64, 0, st>>>(qc);
+    k_fill4<<<grid_for(NT, 256), 256, 0, st>>>(trad, NT);
+    ShadeHooks T;
+    T.kind = 2;
+    T.rec = C.train_vtx.as<float>();
+    T.nrec = nrec;
+    T.max_rec = depth;
+    T.rad = trad;
+    // the training paths are few (latency-bound per bounce as a wavefront): one thread runs each whole path
+    snkpt_status e = whole_paths(ctx, cd, w, h, s, initstate ^ kTrainPaths, NT, start, stride, &T, trad, stats, st);
     if (e != SNKPT_OK) return e;
+    uint8_t* tm = C.train_meta.as<uint8_t>();
+    uint32_t* pcount = (uint32_t*)(tm + u32_bytes);
+    uint32_t* off = (uint32_t*)(tm + 2 * u32_bytes);
+    uint32_t* sums = (uint32_t*)(tm + 3 * u32_bytes);
+    const uint64_t kseed = initstate ^ 0x7461726765ull;
+    const uint32_t budget = NT;  // training samples per frame
+    k_train_total<<<grid_for(NT, 256), 256, 0, st>>>(nrec, NT, qc);
+    k_train_count<<<grid_for(NT, 256), 256, 0, st>>>(C.train_vtx.as<float>(), nrec, depth, trad, NT, kseed,
+                                                      indirect, qc, budget, pcount);
+    scan_exclusive(pcount, off, NT, sums, qc + 1, st);
+    k_train_emit<<<grid_for(NT, 256), 256, 0, st>>>(C.train_vtx.as<float>(), nrec, depth, trad, NT, kseed, indirect,
+                                                     qc, budget, off, C.t_samples.as<float>(), stats + 4);
+    uint32_t nt = 0;
+    HIPC(hipMemcpyAsync(&nt, qc + 1, 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    if (nt == 0) return SNKPT_OK;
+    return cache_train_ema(ctx, C.t_samples.as<float>(), nt, s->cache_lr > 0.0f ? s->cache_lr : 1e-2f, 1,
+                           std::min(std::max(s->cache_ema, 0.0f), 0.9999f), nullptr, st);
+}
+
+snkpt_status cache_pass(snkpt_ctx* ctx, const CamDev& cd, const snkpt_settings* s, uint64_t initstate, uint32_t pass,
+                        hipStream_t st) {
+    RenderState& R = *ctx->render;
+    const uint32_t n = R.w * R.h;
+    snkpt_status e = cache_ensure(ctx, s, st);
+    if (e != SNKPT_OK) return e;
+    CacheState& C = *ctx->cache;
+    const uint32_t depth = s->max_depth ? s->max_depth : 1u;
+    // the Q-th path vertex (1 = primary hit) is answered by the cache
+    const uint32_t Q = s->cache_query_depth ? s->cache_query_depth : 2u;
+    if (n > C.rcap) {
+        HIPC(C.q_samples.reserve((size_t)n * 64));
+        HIPC(C.q_meta.reserve((size_t)n * 16));
+        C.rcap = n;
+    }
+    HIPC(C.counts.reserve(64));
+    if ((e = ensure_work(ctx, C, n, false)) != SNKPT_OK) return e;
     uint32_t* qc = C.counts.as<uint32_t>();
     unsigned long long* stats = R.stats.as<unsigned long long>();
     // 1. render paths, terminated into the cache at vertex Q
@@ -1047,41 +1120,11 @@ snkpt_status cache_pass(snkpt_ctx* ctx, const CamDev& cd, const snkpt_settings* 
         C.x0.as<_Float16>(), n, qc, (avg ? C.mlp_e16 : C.mlp16).as<_Float16>(), C.y.as<float4>());
     k_cache_apply<<<grid_for(n, 256), 256, 0, st>>>(C.q_meta.as<float4>(), C.y.as<float4>(), qc, R.rad.as<float4>(),
                                                       stats + 3);
+    (void)pass;
     if (s->flags & SNKPT_FLAG_NO_TRAIN) return SNKPT_OK;
     // 3. training paths: full-length unbiased paths through a strided pixel subset
-    uint32_t* nrec = C.train_meta.as<uint32_t>();
-    float4* trad = (float4*)(C.train_meta.as<uint8_t>() + trad_off);
-    const uint32_t stride = std::max<uint32_t>(n / NT, 1u);
-    const uint32_t start = (uint32_t)((initstate >> 17) % stride);
-    k_fill4<<<grid_for(NT, 256), 256, 0, st>>>(trad, NT);
-    ShadeHooks T;
-    T.kind = 2;
-    T.rec = C.train_vtx.as<float>();
-    T.nrec = nrec;
-    T.max_rec = depth;
-    T.rad = trad;
-    // the training paths are few (latency-bound per bounce as a wavefront): one thread runs each whole path
-    if ((e = whole_paths(ctx, cd, s, initstate ^ kTrainPaths, NT, start, stride, &T, trad, st)) != SNKPT_OK)
-        return e;
-    uint8_t* tm = C.train_meta.as<uint8_t>();
-    uint32_t* pcount = (uint32_t*)(tm + u32_bytes);
-    uint32_t* off = (uint32_t*)(tm + 2 * u32_bytes);
-    uint32_t* sums = (uint32_t*)(tm + 3 * u32_bytes);
-    const uint64_t kseed = initstate ^ 0x7461726765ull;
-    const uint32_t budget = NT;  // training samples per frame
-    k_train_total<<<grid_for(NT, 256), 256, 0, st>>>(nrec, NT, qc);
-    k_train_count<<<grid_for(NT, 256), 256, 0, st>>>(C.train_vtx.as<float>(), nrec, depth, trad, NT, kseed,
-                                                      H.indirect, qc, budget, pcount);
-    scan_exclusive(pcount, off, NT, sums, qc + 1, st);
-    k_train_emit<<<grid_for(NT, 256), 256, 0, st>>>(C.train_vtx.as<float>(), nrec, depth, trad, NT, kseed, H.indirect,
-                                                     qc, budget, off, C.t_samples.as<float>(), stats + 4);
-    uint32_t nt = 0;
-    HIPC(hipMemcpyAsync(&nt, qc + 1, 4, hipMemcpyDeviceToHost, st));
-    HIPC(hipStreamSynchronize(st));
-    (void)pass;
-    if (nt == 0) return SNKPT_OK;
-    return cache_train_ema(ctx, C.t_samples.as<float>(), nt, s->cache_lr > 0.0f ? s->cache_lr : 1e-2f, 1,
-                           std::min(std::max(s->cache_ema, 0.0f), 0.9999f), nullptr, st);
+    return cache_train_frame(ctx, cd, R.w, R.h, s, initstate, s->cache_train_paths ? s->cache_train_paths : 65536u,
+                             stats, st);
 }
 
 }  // namespace snkpt

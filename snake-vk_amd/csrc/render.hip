@@ -411,33 +411,32 @@ snkpt_status pt_pass(snkpt_ctx* ctx, const CamDev& cam, const snkpt_settings* s,
     (void)first_bounce_skip;
     if (s->flags & SNKPT_FLAG_WHOLE_PATHS) {
         RenderState& R = *ctx->render;
-        return whole_paths(ctx, cam, s, initstate, R.w * R.h, 0, 1, nullptr, R.rad.as<float4>(), st);
+        return whole_paths(ctx, cam, R.w, R.h, s, initstate, R.w * R.h, 0, 1, nullptr, R.rad.as<float4>(),
+                           R.stats.as<unsigned long long>(), st);
     }
     generate_camera(ctx, cam, initstate, st);
     return pt_bounces(ctx, s, 0, 0, st);
 }
 
-snkpt_status whole_paths(snkpt_ctx* ctx, const CamDev& cam, const snkpt_settings* s, uint64_t initstate,
-                         uint32_t n_paths, uint32_t start, uint32_t stride, const ShadeHooks* hooks, float4* rad,
-                         hipStream_t st) {
-    RenderState& R = *ctx->render;
+snkpt_status whole_paths(snkpt_ctx* ctx, const CamDev& cam, uint32_t w, uint32_t h, const snkpt_settings* s,
+                         uint64_t initstate, uint32_t n_paths, uint32_t start, uint32_t stride,
+                         const ShadeHooks* hooks, float4* rad, unsigned long long* stats, hipStream_t st) {
     const Accel& A = ctx->accel;
     if (n_paths == 0) return SNKPT_OK;
     if (A.num_tris == 0) return SNKPT_OK;  // nothing to hit: radiance stays 0
     SceneDev S = scene_dev(ctx);
     PtParams P{s->max_depth ? s->max_depth : 1u, s->rr_start};
     const uint32_t need = A.depth > 0 ? A.depth - 1 : 0;
-    unsigned long long* stats = R.stats.as<unsigned long long>();
     const uint32_t grid = nblk(n_paths, kMegaBlock);
     const bool rec = hooks && hooks->kind == 2;
     ShadeHooks H = hooks ? *hooks : ShadeHooks{};
 #define SNKPT_MEGA(STK)                                                                                        \
     do {                                                                                                       \
         if (rec)                                                                                               \
-            k_pt_mega<2, STK><<<grid, kMegaBlock, 0, st>>>(S, cam, R.w, R.h, n_paths, start, stride, initstate, \
+            k_pt_mega<2, STK><<<grid, kMegaBlock, 0, st>>>(S, cam, w, h, n_paths, start, stride, initstate,     \
                                                            P, H, rad, stats);                                  \
         else                                                                                                   \
-            k_pt_mega<0, STK><<<grid, kMegaBlock, 0, st>>>(S, cam, R.w, R.h, n_paths, start, stride, initstate, \
+            k_pt_mega<0, STK><<<grid, kMegaBlock, 0, st>>>(S, cam, w, h, n_paths, start, stride, initstate,     \
                                                            P, H, rad, stats);                                  \
     } while (0)
     if (need <= 16)

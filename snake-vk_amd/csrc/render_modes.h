@@ -106,9 +106,15 @@ snkpt_status pt_bounces(snkpt_ctx* ctx, const snkpt_settings* s, int cur, uint32
                         const ShadeHooks* hooks = nullptr, uint32_t max_depth_override = 0, uint32_t n_paths = 0);
 // Whole paths, one per thread: path j < n_paths through pixel (start + j * stride)
 // mod (w h), radiance added into rad[j]; hooks of kind 2 record training vertices.
-snkpt_status whole_paths(snkpt_ctx* ctx, const CamDev& cam, const snkpt_settings* s, uint64_t initstate,
-                         uint32_t n_paths, uint32_t start, uint32_t stride, const ShadeHooks* hooks, float4* rad,
+snkpt_status whole_paths(snkpt_ctx* ctx, const CamDev& cam, uint32_t w, uint32_t h, const snkpt_settings* s,
+                         uint64_t initstate, uint32_t n_paths, uint32_t start, uint32_t stride,
+                         const ShadeHooks* hooks, float4* rad, unsigned long long* stats, hipStream_t st);
+// cache.hip: cache query and training for callers that trace the render paths themselves (the batch)
+snkpt_status cache_ensure(snkpt_ctx* ctx, const snkpt_settings* s, hipStream_t st);
+snkpt_status cache_query(snkpt_ctx* ctx, const float* samples, uint32_t n, float4* y, bool averaged,
                          hipStream_t st);
+snkpt_status cache_train_frame(snkpt_ctx* ctx, const CamDev& cd, uint32_t w, uint32_t h, const snkpt_settings* s,
+                               uint64_t initstate, uint32_t n_paths, unsigned long long* stats, hipStream_t st);
 // Camera rays for paths j < n_paths through pixels (start + j * stride) mod (w h).
 snkpt_status generate_camera(snkpt_ctx* ctx, const CamDev& cam, uint64_t initstate, hipStream_t st,
                              uint32_t n_paths = 0, uint32_t start = 0, uint32_t stride = 1,

@@ -169,3 +169,77 @@ def test_batch_rejects_bad_input(gpu):
         b.render(_settings())
     with pytest.raises(snkpt.SnkptError):
         b.render(_settings(mode=snkpt.MODE_RESTIR))
+
+
+def _contexts(scene_ids):
+    import snkpt
+    out = []
+    for cfg, var in scene_ids:
+        sd = snkpt.SceneData(cfg, var)
+        c = snkpt.Context(0)
+        c.load_scene(sd)
+        c.build_accel()
+        out.append((sd, c))
+    return out
+
+
+@pytest.mark.parametrize("indirect", [False, True])
+def test_batch_cache_form_equals_single_view_cache_renders(gpu, indirect):
+    """Cache form, one view per scene, training on: each batched view equals
+    the single-context cache render of that view (own networks, same seeds),
+    frame after frame, bit for bit."""
+    import torch
+
+    import snkpt
+    ids = [(5, 1), (1, 0), (5, 2)]
+    views = [(0, 96, 64, 21), (1, 64, 64, 22), (2, 80, 48, 23)]
+    A = _contexts(ids)
+    Bs = _contexts(ids)
+    for (_, c) in A + Bs:
+        c.cache_reset(7)
+    b = snkpt.Batch(0)
+    for _, c in A:
+        b.add_scene(c)
+    outs, cams = [], []
+    for k, (si, w, h, seed) in enumerate(views):
+        cam = A[si][0].camera(k, w, h)
+        o = torch.zeros(h, w, 4, device="cuda")
+        b.add_view(si, cam, w, h, seed, o)
+        outs.append(o)
+        cams.append(cam)
+    kw = dict(mode=snkpt.MODE_CACHE, max_depth=4, rr_start=3, cache_train_paths=1 << 20,
+              cache_query_depth=1 if indirect else 2, flags=snkpt.FLAG_CACHE_INDIRECT if indirect else 0)
+    for frame in range(3):
+        b.render(_settings(frame=frame, **kw))
+        got = _grab(outs)
+        for k, (si, w, h, seed) in enumerate(views):
+            o = torch.zeros(h, w, 4, device="cuda")
+            Bs[si][1].render(cams[k], _settings(frame=frame, seed=seed, **kw), o, w, h)
+            torch.cuda.synchronize()
+            assert np.array_equal(got[k], o.cpu().numpy()), (frame, k)
+    st = b.frame_stats()
+    _say(f"cache form: {st.cache_queries} queries, {st.cache_train_samples} training samples in the last frame")
+    assert st.cache_queries > 0 and st.cache_train_samples > 0
+    for (_, ca), (_, cb) in zip(A, Bs):
+        assert np.array_equal(ca.cache_params(0), cb.cache_params(0))
+
+
+def test_batch_cache_form_frozen_many_views(gpu):
+    """Several views per scene, networks frozen: every batched view equals the
+    single-context cache render of that view."""
+    import torch
+
+    import snkpt
+    scenes, ctxs, b, outs, cams = _setup()
+    single = _contexts(SCENES)
+    for (_, c), c2 in zip(single, ctxs):
+        c.cache_reset(3)
+        c2.cache_reset(3)
+    kw = dict(mode=snkpt.MODE_CACHE, max_depth=5, rr_start=3, cache_query_depth=2, flags=snkpt.FLAG_NO_TRAIN)
+    b.render(_settings(frame=1, **kw))
+    got = _grab(outs)
+    for k, (si, w, h, seed) in enumerate(VIEWS):
+        o = torch.zeros(h, w, 4, device="cuda")
+        single[si][1].render(cams[k], _settings(frame=1, seed=seed, **kw), o, w, h)
+        torch.cuda.synchronize()
+        assert np.array_equal(got[k], o.cpu().numpy()), k
