@@ -111,6 +111,13 @@ def frame_rates(stream, frames=10, warmup=3, only=None):
                     "depth": scene.s.max_depth, "rays_per_frame": int(st.camera_rays + st.extension_rays +
                                                                       st.shadow_rays)}
         res[key]["mrays_s"] = round(res[key]["rays_per_frame"] / (ms * 1e-3) / 1e6, 1)
+        # compulsory bytes of a frame (BVH node/triangle reads excluded): 56 B per closest-hit ray
+        # (ray 32 + path id 4 + hit 20), 40 B per shadow ray, 370 B per shaded path vertex
+        # (hit, ray, path state, indices + normals, instance, material, inverse transform read;
+        # next ray, path state, shadow ray written)
+        hits = int(st.camera_rays + st.extension_rays)
+        comp = (56.0 + 370.0) * hits + 40.0 * int(st.shadow_rays)
+        res[key]["hbm_frac_compulsory"] = round(comp / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)
         if mode == snkpt.MODE_CACHE:
             res[key]["cache_queries"] = int(st.cache_queries)
             res[key]["cache_train_samples"] = int(st.cache_train_samples)

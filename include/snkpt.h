@@ -231,6 +231,32 @@ snkpt_status snkpt_render(snkpt_ctx* ctx, const snkpt_camera* cam, const snkpt_s
                           void* out_rgba, uint32_t w, uint32_t h, uint32_t out_format,
                           snkpt_stream stream);
 
+/* Rasterized G-buffer of the primary visibility, in the reference's image
+ * formats (EditorLayer.cpp:227-302, written by GBuffer.frag:46-72).  Device
+ * pointers, w*h texels each, row-major, row 0 at the top, tightly packed,
+ * each aligned to its texel size. */
+typedef struct snkpt_gbuffer {
+    const float* depth;        /* D32_SFLOAT: clip-space depth in [0, 1]; >= 1 (cleared) = nothing rasterized */
+    const uint16_t* normal;    /* R16G16B16A16_SFLOAT: world-space shading normal in rgb */
+    const int8_t* albedo;      /* R8G8B8A8_SNORM: albedo in rgb (the emission of an emissive material) */
+    const uint16_t* rma;       /* R16G16B16A16_SFLOAT: roughness, metallic, ao */
+    const uint16_t* mat_flags; /* R16_UINT: material flags (SNKPT_MAT_FLAG_*) */
+    const uint16_t* motion;    /* R16G16_SFLOAT: history texcoord - texcoord (GBuffer.frag:40-42);
+                                  NULL: reproject with the previous frame's camera */
+} snkpt_gbuffer;
+
+/* RayGen.rgen:531-555: the frame of snkpt_render with the primary vertex read
+ * from a G-buffer instead of traced camera rays.  World position from depth
+ * through inverse(view) * inverse(proj) (Util.glsl:1-7); rays leave it with the
+ * reference's bias 0.01 + max(0.001 |cam - p|, 0.01) along the normal
+ * (RayGen.rgen:552); an emissive texel's radiance is its albedo
+ * (RayGen.rgen:539-542).  Every mode; in the cache mode the training paths
+ * still start with camera rays.  Same output, accumulation and statistics
+ * (camera_rays = 0). */
+snkpt_status snkpt_render_gbuffer(snkpt_ctx* ctx, const snkpt_camera* cam, const snkpt_settings* settings,
+                                  const snkpt_gbuffer* gbuffer, void* out_rgba, uint32_t w, uint32_t h,
+                                  uint32_t out_format, snkpt_stream stream);
+
 /* Rays traced by the last snkpt_render, by kind. */
 typedef struct snkpt_frame_stats {
     uint64_t camera_rays;

@@ -43,6 +43,13 @@ def load():
     lib.o_pt_render.argtypes = [VP, U32, C.POINTER(OBvh), VP, U32, VP, VP, VP, VP, VP, U32, VP, VP, VP, U32, U32,
                                 U32, U32, U32, U32, U32, U32, VP, I]
     lib.o_pt_render.restype = I
+    lib.o_pt_render_gbuffer.argtypes = [VP, U32, C.POINTER(OBvh), VP, U32, VP, VP, VP, VP, VP, U32, VP, VP, VP, U32,
+                                        U32, U32, U32, U32, U32, U32, C.POINTER(OGbuffer), VP, I]
+    lib.o_pt_render_gbuffer.restype = I
+    lib.o_half_to_float.argtypes = [C.c_uint16]
+    lib.o_half_to_float.restype = F
+    lib.o_primary_surfaces.argtypes = [VP, U32, C.POINTER(OBvh), VP, U32, VP, VP, VP, VP, VP, VP, VP, VP, U32, U32,
+                                       VP, VP, VP, VP, VP]
     lib.o_cache_geom.argtypes = [VP, VP, VP, VP, VP]
     lib.o_grid_index.argtypes = [U32, U32, U32, U32, U32]
     lib.o_grid_index.restype = U32
@@ -53,6 +60,11 @@ def load():
     lib.o_adam.argtypes = [VP, VP, VP, VP, C.c_size_t, U32, C.c_double, C.c_double, C.c_double, C.c_double]
     _lib = lib
     return lib
+
+
+class OGbuffer(C.Structure):
+    _fields_ = [("depth", C.c_void_p), ("normal", C.c_void_p), ("albedo", C.c_void_p), ("rma", C.c_void_p),
+                ("flags", C.c_void_p)]
 
 
 class OBvh(C.Structure):
@@ -197,6 +209,73 @@ def pt_render(arrs, bvh: "Bvh", cam, w, h, spp, max_depth, rr_start, seed, frame
     load().o_pt_render(_p(bvh.world), bvh.n, C.byref(bvh.b), _p(inst), inst.shape[0], _p(idx), _p(nrm), _p(m4),
                        _p(mats), _p(em), em.size, _p(view), _p(proj), _p(cpos), w, h, spp, max_depth, rr_start,
                        seed, frame, seed_xor, _p(out), threads)
+    return out
+
+
+def half_to_float(h: int) -> float:
+    return load().o_half_to_float(h)
+
+
+def primary_surfaces(arrs, bvh: "Bvh", cam, w, h):
+    """Surfaces of the camera rays: depth (h, w) clip depth (1 = miss), normal
+    (h, w, 3), albedo (h, w, 3), roughness/metallic (h, w, 2), flags (h, w)."""
+    pos, idx, inst, m4, mats, nrm, em = arrs
+    inst = np.ascontiguousarray(inst, np.uint32)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    m4 = np.ascontiguousarray(m4, np.float32)
+    mats = np.ascontiguousarray(mats, np.uint32)
+    nrm = np.ascontiguousarray(nrm, np.float32)
+    view = np.array(cam.view, np.float32)
+    proj = np.array(cam.proj, np.float32)
+    pv = np.array(cam.proj_view, np.float32)
+    cpos = np.array(cam.cam_pos[:3], np.float32)
+    depth = np.zeros((h, w), np.float32)
+    n3 = np.zeros((h, w, 3), np.float32)
+    a3 = np.zeros((h, w, 3), np.float32)
+    rm = np.zeros((h, w, 2), np.float32)
+    fl = np.zeros((h, w), np.uint32)
+    load().o_primary_surfaces(_p(bvh.world), bvh.n, C.byref(bvh.b), _p(inst), inst.shape[0], _p(idx), _p(nrm),
+                              _p(m4), _p(mats), _p(view), _p(proj), _p(pv), _p(cpos), w, h, _p(depth), _p(n3),
+                              _p(a3), _p(rm), _p(fl))
+    return depth, n3, a3, rm, fl
+
+
+def encode_gbuffer(depth, normal3, albedo3, rm2, flags, ao=1.0):
+    """Host G-buffer in the reference's formats from float surfaces: depth f32,
+    normal RGBA16F, albedo RGBA8 SNORM (round(c * 127)), rma RGBA16F, flags u16."""
+    h, w = depth.shape
+    nrm = np.ones((h, w, 4), np.float16)
+    nrm[..., :3] = normal3.astype(np.float16)
+    alb = np.full((h, w, 4), 127, np.int8)
+    alb[..., :3] = np.clip(np.rint(albedo3 * 127.0), -127, 127).astype(np.int8)
+    rma = np.ones((h, w, 4), np.float16)
+    rma[..., :2] = rm2.astype(np.float16)
+    rma[..., 2] = ao
+    return {"depth": np.ascontiguousarray(depth, np.float32), "normal": nrm, "albedo": alb, "rma": rma,
+            "mat_flags": flags.astype(np.uint16)}
+
+
+def pt_render_gbuffer(arrs, bvh: "Bvh", cam, gb, w, h, spp, max_depth, rr_start, seed, frame=0, threads=8):
+    """CPU path tracer with each pixel's primary vertex read from the G-buffer
+    `gb` (host arrays as encode_gbuffer returns them)."""
+    pos, idx, inst, m4, mats, nrm, em = arrs
+    inst = np.ascontiguousarray(inst, np.uint32)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    m4 = np.ascontiguousarray(m4, np.float32)
+    mats = np.ascontiguousarray(mats, np.uint32)
+    nrm = np.ascontiguousarray(nrm, np.float32)
+    em = np.ascontiguousarray(em, np.uint32)
+    view = np.array(cam.view, np.float32)
+    proj = np.array(cam.proj, np.float32)
+    cpos = np.array(cam.cam_pos[:3], np.float32)
+    keep = {k: np.ascontiguousarray(gb[k]) for k in ("depth", "normal", "albedo", "rma", "mat_flags")}
+    g = OGbuffer(_p(keep["depth"]), _p(keep["normal"]), _p(keep["albedo"]), _p(keep["rma"]), _p(keep["mat_flags"]))
+    out = np.zeros((h, w, 3), np.float32)
+    rc = load().o_pt_render_gbuffer(_p(bvh.world), bvh.n, C.byref(bvh.b), _p(inst), inst.shape[0], _p(idx), _p(nrm),
+                                    _p(m4), _p(mats), _p(em), em.size, _p(view), _p(proj), _p(cpos), w, h, spp,
+                                    max_depth, rr_start, seed, frame, C.byref(g), _p(out), threads)
+    if rc != 0:
+        raise ValueError("o_pt_render_gbuffer: incomplete G-buffer")
     return out
 
 

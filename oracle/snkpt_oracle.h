@@ -93,6 +93,28 @@ int o_pt_render(const float* world, uint32_t ntris, const o_bvh* bvh, const uint
                 const float* cam_pos, uint32_t w, uint32_t h, uint32_t spp, uint32_t max_depth, uint32_t rr_start,
                 uint32_t seed, uint32_t frame, uint32_t seed_xor, float* out_rgb, int threads);
 
+/* G-buffer of the reference's formats (EditorLayer.cpp:227-302), host arrays. */
+typedef struct o_gbuffer {
+    const float* depth;     /* D32 clip depth, >= 1 = nothing rasterized */
+    const uint16_t* normal; /* RGBA16F */
+    const int8_t* albedo;   /* RGBA8 SNORM */
+    const uint16_t* rma;    /* RGBA16F */
+    const uint16_t* flags;  /* R16_UINT */
+} o_gbuffer;
+float o_half_to_float(uint16_t h);
+/* o_pt_render with the primary vertex of each pixel read from the G-buffer. */
+int o_pt_render_gbuffer(const float* world, uint32_t ntris, const o_bvh* bvh, const uint32_t* inst5,
+                        uint32_t n_inst, const uint32_t* idx, const float* nrm, const float* mat4s,
+                        const uint32_t* mats16, const uint32_t* emissive, uint32_t n_emissive, const float* view16,
+                        const float* proj16, const float* cam_pos, uint32_t w, uint32_t h, uint32_t spp,
+                        uint32_t max_depth, uint32_t rr_start, uint32_t seed, uint32_t frame, const o_gbuffer* gb,
+                        float* out_rgb, int threads);
+void o_primary_surfaces(const float* world, uint32_t ntris, const o_bvh* bvh, const uint32_t* inst5,
+                        uint32_t n_inst, const uint32_t* idx, const float* nrm, const float* mat4s,
+                        const uint32_t* mats16, const float* view16, const float* proj16, const float* proj_view16,
+                        const float* cam_pos, uint32_t w, uint32_t h, float* depth, float* normal3, float* albedo3,
+                        float* rm2, uint32_t* flags);
+
 /* ---- neural radiance cache (snkpt_oracle_mlp.c), FP64 ----------------------- */
 void o_cache_geom(const float lo[3], const float hi[3], float out_lo_inv[4], uint32_t res[16], uint32_t dense[16]);
 uint32_t o_grid_index(uint32_t x, uint32_t y, uint32_t z, uint32_t res, uint32_t dense);

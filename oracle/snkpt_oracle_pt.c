@@ -19,8 +19,16 @@
  * numbers come from PCG32 (initstate from (seed, frame, pass), stream = pixel)
  * drawn in the same order as the GPU, so most pixels reproduce the GPU's
  * values; sinf/cosf of the platform library may differ in the last ulp.
+ * Shadow rays leave p + ng eps and are aimed at the sampled light point from
+ * there (ShadowTest from biased_pos, RayGen.rgen:101-121,552).
+ * G-buffer mode (o_pt_render_gbuffer): the primary vertex is read from the
+ * reference's G-buffer images (RayGen.rgen:531-552): WorldPosFromDepth
+ * (Util.glsl:1-7), texel normal, SNORM8 albedo, RGBA16F rma, R16 flags, ray bias
+ * 0.01 + max(0.001 |cam - p|, 0.01); an emissive texel's radiance is its albedo.
  * Pinned by: closed forms of the BRDF terms, pdf normalisation, the furnace
- * test (tests/test_oracle_pt.py), and agreement with the GPU in expectation.
+ * test (tests/test_oracle_pt.py), the polygon-formula closed form for G-buffer
+ * texels and an exhaustive binary16 check (tests/test_oracle_gbuffer.py), and
+ * agreement with the GPU in expectation.
  */
 #include <math.h>
 #include <pthread.h>
@@ -484,7 +492,68 @@ typedef struct {
     uint32_t w, h, spp, max_depth, rr_start, seed, frame, seed_xor;
     float* out;
     uint32_t begin, end;
+    const o_gbuffer* gb; /* NULL: camera rays */
 } pt_job;
+
+/* ------------------------------------------------------------- G-buffer --- */
+/* IEEE binary16 -> binary32 (exact). */
+float o_half_to_float(uint16_t hb) {
+    uint32_t sgn = (uint32_t)(hb >> 15) << 31, e = (hb >> 10) & 0x1Fu, m = hb & 0x3FFu, bits;
+    if (e == 0) {
+        bits = m == 0 ? sgn : (f2u((float)m * 5.9604644775390625e-8f) | sgn); /* m * 2^-24 */
+    } else if (e == 31) {
+        bits = sgn | 0x7F800000u | (m << 13);
+    } else {
+        bits = sgn | ((e + 112u) << 23) | (m << 13);
+    }
+    return u2f(bits);
+}
+/* R8_SNORM texel: max(c / 127, -1) (Vulkan spec, fixed-point conversion). */
+static float snorm8_c(int8_t c) { return fmaxf((float)c / 127.0f, -1.0f); }
+static float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+
+/* The primary vertex of pixel px read from a rasterized G-buffer
+ * (RayGen.rgen:531-552): WorldPosFromDepth (Util.glsl:1-7) with the texel's
+ * depth, the texel normal as shading and geometric normal (facing the camera),
+ * material from albedo / rma / flags, ray bias 0.01 + max(0.001 |cam - p|, 0.01). */
+static int gsurface(const pt_job* J, uint32_t px, osurf* s, v3* dir, float* t) {
+    const o_gbuffer* g = J->gb;
+    const float depth = g->depth[px];
+    if (!(depth < 1.0f)) return 0;
+    const uint32_t x = px % J->w, y = px / J->w;
+    const float u = ((float)x + 0.5f) / (float)J->w * 2.0f - 1.0f;
+    const float v = ((float)y + 0.5f) / (float)J->h * 2.0f - 1.0f;
+    const float* m = J->cam;
+    float wp[4];
+    for (int r = 0; r < 4; ++r) wp[r] = ((m[4 * r] * u + m[4 * r + 1] * v) + m[4 * r + 2] * depth) + m[4 * r + 3];
+    const float iw = 1.0f / fmaxf(wp[3], 1e-6f);
+    s->p = V(wp[0] * iw, wp[1] * iw, wp[2] * iw);
+    const v3 d = sub(s->p, J->pos);
+    const float l2 = dotv(d, d);
+    if (!(l2 > 0.0f)) return 0;
+    *t = sqrtf(l2);
+    *dir = scl(d, 1.0f / *t);
+    const uint16_t* nb = g->normal + 4 * (size_t)px;
+    v3 n = V(o_half_to_float(nb[0]), o_half_to_float(nb[1]), o_half_to_float(nb[2]));
+    const float n2 = dotv(n, n);
+    if (!(n2 > 1e-20f)) return 0;
+    n = scl(n, 1.0f / sqrtf(n2));
+    if (dotv(n, *dir) > 0.0f) n = scl(n, -1.0f);
+    s->ng = n;
+    s->ns = n;
+    const int8_t* a = g->albedo + 4 * (size_t)px;
+    s->m.albedo = V(clampf(snorm8_c(a[0]), 0.0f, 1.0f), clampf(snorm8_c(a[1]), 0.0f, 1.0f),
+                    clampf(snorm8_c(a[2]), 0.0f, 1.0f));
+    const uint16_t* rb = g->rma + 4 * (size_t)px;
+    s->m.roughness = clampf(o_half_to_float(rb[0]), 0.02f, 1.0f);
+    s->m.metallic = clampf(o_half_to_float(rb[1]), 0.0f, 1.0f);
+    s->m.flags = g->flags[px];
+    s->emissive = (s->m.flags & 1u) != 0u;
+    s->le = s->m.albedo;
+    s->gid = 0xFFFFFFFFu;
+    s->eps = 0.01f + fmaxf(*t * 0.001f, 0.01f);
+    return 1;
+}
 
 static void* pt_worker(void* arg) {
     pt_job* J = (pt_job*)arg;
@@ -498,12 +567,18 @@ static void* pt_worker(void* arg) {
             v3 thr = V(1, 1, 1);
             float last_pdf = 0.0f;
             for (uint32_t bounce = 0; bounce < J->max_depth; ++bounce) {
-                float ray[8] = {o.x, o.y, o.z, 0.0f, d.x, d.y, d.z, 1e30f};
-                float t, hu, hv;
-                uint32_t hi, hp;
-                o_trace_one(S->bvh, ray, 0, &t, &hi, &hp, &hu, &hv);
-                if (!(t >= 0.0f)) break;
-                osurf s = surface(S, d, hi, hp, hu, hv);
+                osurf s;
+                float t;
+                if (bounce == 0 && J->gb) {
+                    if (!gsurface(J, px, &s, &d, &t)) break;
+                } else {
+                    float ray[8] = {o.x, o.y, o.z, 0.0f, d.x, d.y, d.z, 1e30f};
+                    float hu, hv;
+                    uint32_t hi, hp;
+                    o_trace_one(S->bvh, ray, 0, &t, &hi, &hp, &hu, &hv);
+                    if (!(t >= 0.0f)) break;
+                    s = surface(S, d, hi, hp, hu, hv);
+                }
                 if (s.emissive) {
                     float w = 1.0f;
                     if (bounce > 0) w = power_h(last_pdf, light_pdf_c(S, s.gid, s.ng, d, t));
@@ -525,9 +600,12 @@ static void* pt_worker(void* arg) {
                         float w = last ? 1.0f : power_h(ls.pdf, brdf_pdf_c(&s.m, s.ns, wo, ls.wi));
                         v3 c = scl(mulv(mulv(thr, f), ls.le), cs * w / ls.pdf);
                         if (c.x > 0.0f || c.y > 0.0f || c.z > 0.0f) {
+                            /* from the biased origin, aimed at the light point p + wi dist */
                             v3 so = add(s.p, scl(s.ng, s.eps));
-                            float sray[8] = {so.x, so.y, so.z, 0.0f, ls.wi.x, ls.wi.y, ls.wi.z,
-                                             ls.dist * (1.0f - 1e-4f) - s.eps};
+                            v3 e = sub(add(s.p, scl(ls.wi, ls.dist)), so);
+                            float l = sqrtf(dotv(e, e));
+                            v3 sd = l > 0.0f ? scl(e, 1.0f / l) : ls.wi;
+                            float sray[8] = {so.x, so.y, so.z, 0.0f, sd.x, sd.y, sd.z, l * (1.0f - 1e-4f)};
                             float st, su_, sv_;
                             uint32_t si, sp2;
                             o_trace_one(S->bvh, sray, 1, &st, &si, &sp2, &su_, &sv_);
@@ -568,11 +646,12 @@ static void* pt_worker(void* arg) {
     return NULL;
 }
 
-int o_pt_render(const float* world, uint32_t ntris, const o_bvh* bvh, const uint32_t* inst5, uint32_t n_inst,
-                const uint32_t* idx, const float* nrm, const float* mat4s, const uint32_t* mats16,
-                const uint32_t* emissive, uint32_t n_emissive, const float* view16, const float* proj16,
-                const float* cam_pos, uint32_t w, uint32_t h, uint32_t spp, uint32_t max_depth, uint32_t rr_start,
-                uint32_t seed, uint32_t frame, uint32_t seed_xor, float* out_rgb, int threads) {
+static int pt_render_impl(const float* world, uint32_t ntris, const o_bvh* bvh, const uint32_t* inst5,
+                          uint32_t n_inst, const uint32_t* idx, const float* nrm, const float* mat4s,
+                          const uint32_t* mats16, const uint32_t* emissive, uint32_t n_emissive, const float* view16,
+                          const float* proj16, const float* cam_pos, uint32_t w, uint32_t h, uint32_t spp,
+                          uint32_t max_depth, uint32_t rr_start, uint32_t seed, uint32_t frame, uint32_t seed_xor,
+                          float* out_rgb, int threads, const o_gbuffer* gb) {
     oscene S;
     memset(&S, 0, sizeof(S));
     S.world = world;
@@ -598,7 +677,7 @@ int o_pt_render(const float* world, uint32_t ntris, const o_bvh* bvh, const uint
     uint32_t n = w * h, per = (n + (uint32_t)threads - 1) / (uint32_t)threads;
     for (int k = 0; k < threads; ++k) {
         pt_job j = {&S, cam, V(cam_pos[0], cam_pos[1], cam_pos[2]), w, h, spp ? spp : 1, max_depth ? max_depth : 1,
-                    rr_start, seed, frame, seed_xor, out_rgb, 0, 0};
+                    rr_start, seed, frame, seed_xor, out_rgb, 0, 0, gb};
         j.begin = (uint32_t)k * per < n ? (uint32_t)k * per : n;
         j.end = j.begin + per < n ? j.begin + per : n;
         jobs[k] = j;
@@ -618,4 +697,85 @@ int o_pt_render(const float* world, uint32_t ntris, const o_bvh* bvh, const uint
     free(S.e_pdf);
     free(S.gid2e);
     return 0;
+}
+
+int o_pt_render(const float* world, uint32_t ntris, const o_bvh* bvh, const uint32_t* inst5, uint32_t n_inst,
+                const uint32_t* idx, const float* nrm, const float* mat4s, const uint32_t* mats16,
+                const uint32_t* emissive, uint32_t n_emissive, const float* view16, const float* proj16,
+                const float* cam_pos, uint32_t w, uint32_t h, uint32_t spp, uint32_t max_depth, uint32_t rr_start,
+                uint32_t seed, uint32_t frame, uint32_t seed_xor, float* out_rgb, int threads) {
+    return pt_render_impl(world, ntris, bvh, inst5, n_inst, idx, nrm, mat4s, mats16, emissive, n_emissive, view16,
+                          proj16, cam_pos, w, h, spp, max_depth, rr_start, seed, frame, seed_xor, out_rgb, threads,
+                          NULL);
+}
+
+int o_pt_render_gbuffer(const float* world, uint32_t ntris, const o_bvh* bvh, const uint32_t* inst5,
+                        uint32_t n_inst, const uint32_t* idx, const float* nrm, const float* mat4s,
+                        const uint32_t* mats16, const uint32_t* emissive, uint32_t n_emissive, const float* view16,
+                        const float* proj16, const float* cam_pos, uint32_t w, uint32_t h, uint32_t spp,
+                        uint32_t max_depth, uint32_t rr_start, uint32_t seed, uint32_t frame, const o_gbuffer* gb,
+                        float* out_rgb, int threads) {
+    if (!gb || !gb->depth || !gb->normal || !gb->albedo || !gb->rma || !gb->flags) return -1;
+    return pt_render_impl(world, ntris, bvh, inst5, n_inst, idx, nrm, mat4s, mats16, emissive, n_emissive, view16,
+                          proj16, cam_pos, w, h, spp, max_depth, rr_start, seed, frame, 0u, out_rgb, threads, gb);
+}
+
+/* Primary surfaces of the camera rays, for building test G-buffers: per pixel
+ * clip depth (proj_view * p, z / w), shading normal, albedo, roughness,
+ * metallic, flags; depth 1 and zeros where the ray misses. */
+void o_primary_surfaces(const float* world, uint32_t ntris, const o_bvh* bvh, const uint32_t* inst5,
+                        uint32_t n_inst, const uint32_t* idx, const float* nrm, const float* mat4s,
+                        const uint32_t* mats16, const float* view16, const float* proj16, const float* proj_view16,
+                        const float* cam_pos, uint32_t w, uint32_t h, float* depth, float* normal3, float* albedo3,
+                        float* rm2, uint32_t* flags) {
+    oscene S;
+    memset(&S, 0, sizeof(S));
+    S.world = world;
+    S.ntris = ntris;
+    S.bvh = bvh;
+    S.inst5 = inst5;
+    S.n_inst = n_inst;
+    S.idx = idx;
+    S.nrm = nrm;
+    S.mats16 = mats16;
+    S.tri_off = (uint32_t*)malloc(sizeof(uint32_t) * (n_inst + 1));
+    S.tri_off[0] = 0;
+    for (uint32_t i = 0; i < n_inst; ++i) S.tri_off[i + 1] = S.tri_off[i] + inst5[5 * i + 4] / 3;
+    S.nmat = (float*)malloc(sizeof(float) * 9 * (n_inst ? n_inst : 1));
+    for (uint32_t i = 0; i < n_inst; ++i) normal_matrix(mat4s + 16 * (size_t)inst5[5 * i], S.nmat + 9 * (size_t)i);
+    float cam[16];
+    o_camera_matrix(view16, proj16, cam);
+    const v3 pos = V(cam_pos[0], cam_pos[1], cam_pos[2]);
+    for (uint32_t px = 0; px < w * h; ++px) {
+        v3 d = cam_dir(cam, pos, px % w, px / w, w, h);
+        float ray[8] = {pos.x, pos.y, pos.z, 0.0f, d.x, d.y, d.z, 1e30f};
+        float t, hu, hv;
+        uint32_t hi, hp;
+        o_trace_one(bvh, ray, 0, &t, &hi, &hp, &hu, &hv);
+        float* n3 = normal3 + 3 * (size_t)px;
+        float* a3 = albedo3 + 3 * (size_t)px;
+        if (!(t >= 0.0f)) {
+            depth[px] = 1.0f;
+            n3[0] = n3[1] = n3[2] = a3[0] = a3[1] = a3[2] = rm2[2 * px] = rm2[2 * px + 1] = 0.0f;
+            flags[px] = 0;
+            continue;
+        }
+        osurf s = surface(&S, d, hi, hp, hu, hv);
+        double c[4];
+        for (int r = 0; r < 4; ++r)
+            c[r] = (double)proj_view16[r] * s.p.x + (double)proj_view16[4 + r] * s.p.y +
+                   (double)proj_view16[8 + r] * s.p.z + (double)proj_view16[12 + r];
+        depth[px] = (float)(c[2] / c[3]);
+        n3[0] = s.ns.x;
+        n3[1] = s.ns.y;
+        n3[2] = s.ns.z;
+        a3[0] = s.m.albedo.x;
+        a3[1] = s.m.albedo.y;
+        a3[2] = s.m.albedo.z;
+        rm2[2 * px] = s.m.roughness;
+        rm2[2 * px + 1] = s.m.metallic;
+        flags[px] = s.m.flags;
+    }
+    free(S.tri_off);
+    free(S.nmat);
 }

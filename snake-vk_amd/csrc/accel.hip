@@ -9,7 +9,8 @@
 //   3. k_morton        30-bit Morton code of each triangle's box centre
 //   4. radix sort      LDS-staged stable LSD sort, 4 x 8-bit digits
 //   5. k_karras        binary radix tree (Karras 2012), duplicate keys ordered by index
-//   6. k_bottom_up     boxes, SAH cost and leaf-cluster flags (clusters of <= 3 triangles)
+//   6. k_bottom_level  boxes, SAH cost and leaf-cluster flags (clusters of <= 3 triangles),
+//                      one tree level per launch
 //   7. collapse        level-synchronous greedy collapse to BVH8 (largest-area child
 //                      expanded first), octant slot order, 8-bit quantised child boxes
 // Every integer output and every FP32 box is reproduced bit for bit by the CPU
@@ -75,21 +76,22 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const uint32_t* __re
     if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
-// one block: exclusive scan of nb <= kScanTile block sums, grand total to *total
-__global__ __launch_bounds__(kScanBlock) void k_scan_top(uint32_t* __restrict__ sums, uint32_t nb,
+// one block: exclusive scan of nb <= kScanTile values (in place when in == out),
+// grand total to *total
+__global__ __launch_bounds__(kScanBlock) void k_scan_top(const uint32_t* in, uint32_t* out, uint32_t nb,
                                                          uint32_t* __restrict__ total) {
     __shared__ uint32_t lds[kScanBlock / 64 + 1];
     uint32_t base = threadIdx.x * kScanItems, v[kScanItems], s = 0;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        v[k] = base + k < nb ? sums[base + k] : 0u;
+        v[k] = base + k < nb ? in[base + k] : 0u;
         s += v[k];
     }
     uint32_t tot;
     uint32_t run = block_exclusive_scan(s, lds, &tot);
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        if (base + k < nb) sums[base + k] = run;
+        if (base + k < nb) out[base + k] = run;
         run += v[k];
     }
     if (threadIdx.x == 0 && total) *total = tot;
@@ -266,22 +268,21 @@ __global__ void k_bounds_init(uint32_t* b) {
 __global__ __launch_bounds__(256) void k_bounds(const float4* __restrict__ tris, uint32_t n,
                                                 uint32_t* __restrict__ bounds) {
     __shared__ float red[12][4];
-    uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    // grid-stride: few blocks, so few same-address atomics at the end
     float v[12];
-    if (g < n) {
+    for (int a = 0; a < 3; ++a) {
+        v[a] = v[6 + a] = INFINITY;
+        v[3 + a] = v[9 + a] = -INFINITY;
+    }
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < n; g += gridDim.x * blockDim.x) {
         float lo[3], hi[3];
         tri_box(tris, g, lo, hi);
         for (int a = 0; a < 3; ++a) {
             float c = (lo[a] + hi[a]) * 0.5f;
-            v[a] = c;
-            v[3 + a] = c;
-            v[6 + a] = lo[a];
-            v[9 + a] = hi[a];
-        }
-    } else {
-        for (int a = 0; a < 3; ++a) {
-            v[a] = v[6 + a] = INFINITY;
-            v[3 + a] = v[9 + a] = -INFINITY;
+            v[a] = fminf(v[a], c);
+            v[3 + a] = fmaxf(v[3 + a], c);
+            v[6 + a] = fminf(v[6 + a], lo[a]);
+            v[9 + a] = fmaxf(v[9 + a], hi[a]);
         }
     }
     for (int q = 0; q < 12; ++q) {
@@ -412,55 +413,92 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
     if (i < n) p[i] = v;
 }
 
-__global__ void k_bottom_up(uint32_t n, const uint32_t* __restrict__ child, const uint32_t* __restrict__ count,
-                            const uint32_t* __restrict__ parent_int, const uint32_t* __restrict__ parent_leaf,
-                            uint32_t* __restrict__ flags, float* aabb, float* cost, uint32_t* height,
-                            uint32_t* __restrict__ leafify) {
-    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    const uint32_t n_int = n - 1;
-    uint32_t node = parent_leaf[j];
-    while (node != kNone) {
-        __threadfence();
-        if (atomicAdd(&flags[node], 1u) == 0u) return;
-        __threadfence();
-        float bl[6], br[6], cl, cr;
-        uint32_t hl, hr;
-        uint32_t c0 = child[2 * node], c1 = child[2 * node + 1];
-        {
-            uint32_t k = (c0 & kLeafTag) ? n_int + (c0 & ~kLeafTag) : c0;
-            for (int q = 0; q < 6; ++q) bl[q] = __builtin_nontemporal_load(&aabb[6 * (size_t)k + q]);
-            cl = (c0 & kLeafTag) ? kCostTri * box_area(bl) : __builtin_nontemporal_load(&cost[c0]);
-            hl = (c0 & kLeafTag) ? 1u : __builtin_nontemporal_load(&height[c0]);
+struct BinUp {
+    uint32_t n, n_int;
+    const uint32_t* child;
+    const uint32_t* count;
+    const uint32_t* parent_int;
+    const uint32_t* parent_leaf;
+    uint32_t* flags;
+    float* aabb;
+    float* cost;
+    uint32_t* height;
+    uint32_t* leafify;
+};
+
+// Box, SAH cost, height and leaf decision of internal node `node` from its two
+// children (both already final).
+__device__ __forceinline__ void bin_node(const BinUp& U, uint32_t node) {
+    float bl[6], br[6], cl, cr;
+    uint32_t hl, hr;
+    uint32_t c0 = U.child[2 * node], c1 = U.child[2 * node + 1];
+    {
+        uint32_t k = (c0 & kLeafTag) ? U.n_int + (c0 & ~kLeafTag) : c0;
+        for (int q = 0; q < 6; ++q) bl[q] = U.aabb[6 * (size_t)k + q];
+        cl = (c0 & kLeafTag) ? kCostTri * box_area(bl) : U.cost[c0];
+        hl = (c0 & kLeafTag) ? 1u : U.height[c0];
+    }
+    {
+        uint32_t k = (c1 & kLeafTag) ? U.n_int + (c1 & ~kLeafTag) : c1;
+        for (int q = 0; q < 6; ++q) br[q] = U.aabb[6 * (size_t)k + q];
+        cr = (c1 & kLeafTag) ? kCostTri * box_area(br) : U.cost[c1];
+        hr = (c1 & kLeafTag) ? 1u : U.height[c1];
+    }
+    float b[6];
+    for (int a = 0; a < 3; ++a) {
+        b[a] = fminf(bl[a], br[a]);
+        b[3 + a] = fmaxf(bl[3 + a], br[3 + a]);
+    }
+    float area = box_area(b);
+    float c_split = kCostNode * area + (cl + cr);
+    uint32_t cnt = U.count[node];
+    uint32_t lf = 0;
+    float c = c_split;
+    if (cnt <= kMaxLeafTris) {
+        float c_leaf = (kCostTri * (float)cnt) * area;
+        if (c_leaf <= c_split) {
+            lf = 1;
+            c = c_leaf;
         }
-        {
-            uint32_t k = (c1 & kLeafTag) ? n_int + (c1 & ~kLeafTag) : c1;
-            for (int q = 0; q < 6; ++q) br[q] = __builtin_nontemporal_load(&aabb[6 * (size_t)k + q]);
-            cr = (c1 & kLeafTag) ? kCostTri * box_area(br) : __builtin_nontemporal_load(&cost[c1]);
-            hr = (c1 & kLeafTag) ? 1u : __builtin_nontemporal_load(&height[c1]);
-        }
-        float b[6];
-        for (int a = 0; a < 3; ++a) {
-            b[a] = fminf(bl[a], br[a]);
-            b[3 + a] = fmaxf(bl[3 + a], br[3 + a]);
-        }
-        float area = box_area(b);
-        float c_split = kCostNode * area + (cl + cr);
-        uint32_t cnt = count[node];
-        uint32_t lf = 0;
-        float c = c_split;
-        if (cnt <= kMaxLeafTris) {
-            float c_leaf = (kCostTri * (float)cnt) * area;
-            if (c_leaf <= c_split) {
-                lf = 1;
-                c = c_leaf;
+    }
+    for (int q = 0; q < 6; ++q) U.aabb[6 * (size_t)node + q] = b[q];
+    U.cost[node] = c;
+    U.height[node] = (hl > hr ? hl : hr) + 1u;
+    U.leafify[node] = lf;
+}
+
+// One level of the bottom-up pass per launch.  Each live node (every leaf in
+// the first launch, afterwards the internal nodes the previous launch made)
+// arrives at its parent; the second arrival computes the parent, which is live
+// in the next launch.  Both children of a node are therefore written by
+// earlier launches, and the kernel boundary makes them visible to the thread
+// that reads them: no device-scope fences (each one writes back and
+// invalidates a whole L2) on this many-XCD part.  Counts are read on the
+// device, so the fixed sequence of launches needs no host round trip; the
+// order of the live list does not affect any value.
+constexpr int kUpBlock = 256;
+constexpr uint32_t kMaxBinLevels = 64;  // radix tree over 64-bit (key, index) codes: height <= 65
+
+template <bool FIRST>
+__global__ __launch_bounds__(kUpBlock) void k_bottom_level(BinUp U, const uint32_t* __restrict__ live_in,
+                                                           const uint32_t* __restrict__ cnt_in,
+                                                           uint32_t* __restrict__ live_out,
+                                                           uint32_t* __restrict__ cnt_out) {
+    __shared__ uint32_t lds[kUpBlock / 64 + 1];
+    const uint32_t total = FIRST ? U.n : *cnt_in;
+    for (uint32_t base = blockIdx.x * kUpBlock; base < total; base += gridDim.x * kUpBlock) {
+        const uint32_t i = base + threadIdx.x;
+        bool made = false;
+        uint32_t parent = kNone;
+        if (i < total) {
+            parent = FIRST ? U.parent_leaf[i] : U.parent_int[live_in[i]];
+            if (parent != kNone && atomicAdd(&U.flags[parent], 1u) != 0u) {
+                bin_node(U, parent);
+                made = true;
             }
         }
-        for (int q = 0; q < 6; ++q) aabb[6 * (size_t)node + q] = b[q];
-        cost[node] = c;
-        height[node] = (hl > hr ? hl : hr) + 1u;
-        leafify[node] = lf;
-        node = parent_int[node];
+        const uint32_t slot = block_append<kUpBlock>(made, cnt_out, lds);
+        if (made) live_out[slot] = parent;
     }
 }
 
@@ -488,72 +526,141 @@ struct BinView {
     }
 };
 
-// Children of a BVH8 node made from binary node `b`, in slot order.
-__global__ void k_collapse_expand(BinView bv, const uint32_t* __restrict__ work, uint32_t level_start,
-                                  uint32_t level_count, uint32_t* __restrict__ slot_child,
-                                  uint32_t* __restrict__ n_int, uint32_t* __restrict__ n_tri) {
+// Children of a BVH8 node made from binary node `b`, in slot order.  Every
+// per-child array is indexed only inside fully unrolled loops, so the eight
+// candidates, their areas and centres and the 8x8 octant cost table live in
+// VGPRs (dynamically indexed private arrays go to scratch), and the two
+// children of an expanded node are fetched with independent loads.
+__device__ __forceinline__ void child_info(const BinView& bv, uint32_t t, bool& ex, float& area, float& cx,
+                                           float& cy, float& cz) {
+    const float* bx = bv.box(t);
+    float bb[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) bb[q] = bx[q];
+    ex = bv.expandable(t);
+    area = ex ? box_area(bb) : -1.0f;
+    cx = (bb[0] + bb[3]) * 0.5f;
+    cy = (bb[1] + bb[4]) * 0.5f;
+    cz = (bb[2] + bb[5]) * 0.5f;
+}
+
+__global__ __launch_bounds__(128) void k_collapse_expand(BinView bv, const uint32_t* __restrict__ work,
+                                                         uint32_t level_start, uint32_t level_count,
+                                                         uint32_t* __restrict__ slot_child,
+                                                         uint32_t* __restrict__ n_int, uint32_t* __restrict__ n_tri) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= level_count) return;
-    uint32_t b = work[level_start + i];
+    const uint32_t b = work[level_start + i];
     uint32_t S[8];
-    int ns = 0;
-    if (bv.expandable(b)) {
-        S[0] = bv.child[2 * b];
-        S[1] = bv.child[2 * b + 1];
-        ns = 2;
-        while (ns < 8) {
-            int best = -1;
-            float best_area = -1.0f;
-            for (int k = 0; k < ns; ++k) {
-                if (!bv.expandable(S[k])) continue;
-                float a = box_area(bv.box(S[k]));
-                if (a > best_area) {
-                    best_area = a;
-                    best = k;
-                }
-            }
-            if (best < 0) break;
-            uint32_t e = S[best];
-            S[best] = bv.child[2 * e];
-            S[ns++] = bv.child[2 * e + 1];
-        }
-    } else {
+    float A[8], X[8], Y[8], Z[8];
+    uint32_t em = 0;  // bit k: S[k] is expandable
+    int ns;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        S[k] = kNone;
+        A[k] = X[k] = Y[k] = Z[k] = 0.0f;
+    }
+    {
+        bool ex;
+        child_info(bv, b, ex, A[0], X[0], Y[0], Z[0]);
         S[0] = b;
         ns = 1;
+        if (ex) {
+            // the greedy expansion: repeatedly open the expandable candidate of
+            // largest area (first one on ties) until 8 candidates or none left
+            uint32_t l = bv.child[2 * b], r = bv.child[2 * b + 1];
+            bool el, er;
+            child_info(bv, l, el, A[0], X[0], Y[0], Z[0]);
+            child_info(bv, r, er, A[1], X[1], Y[1], Z[1]);
+            S[0] = l;
+            S[1] = r;
+            em = (el ? 1u : 0u) | (er ? 2u : 0u);
+            ns = 2;
+            while (ns < 8) {
+                int best = -1;
+                float best_area = -1.0f;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    if (k < ns && ((em >> k) & 1u) && A[k] > best_area) {
+                        best_area = A[k];
+                        best = k;
+                    }
+                }
+                if (best < 0) break;
+                uint32_t e = 0;
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (k == best) e = S[k];
+                const uint32_t cl = bv.child[2 * e], cr = bv.child[2 * e + 1];
+                bool xl, xr;
+                float al, ar, lx, ly, lz, rx, ry, rz;
+                child_info(bv, cl, xl, al, lx, ly, lz);
+                child_info(bv, cr, xr, ar, rx, ry, rz);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    if (k == best) {
+                        S[k] = cl;
+                        A[k] = al;
+                        X[k] = lx;
+                        Y[k] = ly;
+                        Z[k] = lz;
+                        em = (em & ~(1u << k)) | (xl ? 1u << k : 0u);
+                    }
+                    if (k == ns) {
+                        S[k] = cr;
+                        A[k] = ar;
+                        X[k] = rx;
+                        Y[k] = ry;
+                        Z[k] = rz;
+                        em = (em & ~(1u << k)) | (xr ? 1u << k : 0u);
+                    }
+                }
+                ++ns;
+            }
+        }
     }
     // octant slot assignment: greedy on cost[c][s] = sum_a (+-1)(centre_c - centre_node)_a
     const float* nb = bv.box(b);
-    float nc[3] = {(nb[0] + nb[3]) * 0.5f, (nb[1] + nb[4]) * 0.5f, (nb[2] + nb[5]) * 0.5f};
+    const float nc0 = (nb[0] + nb[3]) * 0.5f, nc1 = (nb[1] + nb[4]) * 0.5f, nc2 = (nb[2] + nb[5]) * 0.5f;
     float cost[8][8];
-    for (int c = 0; c < ns; ++c) {
-        const float* cb = bv.box(S[c]);
-        float dx = (cb[0] + cb[3]) * 0.5f - nc[0];
-        float dy = (cb[1] + cb[4]) * 0.5f - nc[1];
-        float dz = (cb[2] + cb[5]) * 0.5f - nc[2];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const float dx = X[c] - nc0, dy = Y[c] - nc1, dz = Z[c] - nc2;
+#pragma unroll
         for (int s = 0; s < 8; ++s) cost[c][s] = ((s & 1 ? dx : -dx) + (s & 2 ? dy : -dy)) + (s & 4 ? dz : -dz);
     }
-    uint32_t slot_of_used = 0, child_used = 0;
+    uint32_t slot_used = 0, child_used = 0;
     uint32_t slots[8];
+#pragma unroll
     for (int s = 0; s < 8; ++s) slots[s] = kNone;
     for (int r = 0; r < ns; ++r) {
         int bc = -1, bs = -1;
-        float bv_ = 0.0f;
-        for (int c = 0; c < ns; ++c) {
-            if (child_used & (1u << c)) continue;
+        float bval = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            if (c >= ns || (child_used & (1u << c))) continue;
+#pragma unroll
             for (int s = 0; s < 8; ++s) {
-                if (slot_of_used & (1u << s)) continue;
-                if (bc < 0 || cost[c][s] > bv_) {
-                    bv_ = cost[c][s];
+                if (slot_used & (1u << s)) continue;
+                if (bc < 0 || cost[c][s] > bval) {
+                    bval = cost[c][s];
                     bc = c;
                     bs = s;
                 }
             }
         }
         child_used |= 1u << bc;
-        slot_of_used |= 1u << bs;
-        slots[bs] = S[bc];
+        slot_used |= 1u << bs;
+        uint32_t chosen = kNone;
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (c == bc) chosen = S[c];
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+            if (s == bs) slots[s] = chosen;
     }
     uint32_t ni = 0, nt = 0;
+#pragma unroll
     for (int s = 0; s < 8; ++s) {
         slot_child[8 * (size_t)i + s] = slots[s];
         if (slots[s] == kNone) continue;
@@ -662,9 +769,12 @@ inline uint32_t blocks(uint32_t n, uint32_t b) { return n == 0 ? 1 : (n + b - 1)
 void scan_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* sums, uint32_t* total,
                     hipStream_t st) {
     uint32_t nb = (n + kScanTile - 1) / kScanTile;
-    if (nb == 0) nb = 1;
+    if (nb <= 1) {  // one tile: a single block does it
+        k_scan_top<<<1, kScanBlock, 0, st>>>(in, out, n, total);
+        return;
+    }
     k_scan_reduce<<<nb, kScanBlock, 0, st>>>(in, n, sums);
-    k_scan_top<<<1, kScanBlock, 0, st>>>(sums, nb, total);
+    k_scan_top<<<1, kScanBlock, 0, st>>>(sums, sums, nb, total);
     k_scan_down<<<nb, kScanBlock, 0, st>>>(in, out, n, sums);
 }
 
@@ -728,8 +838,8 @@ snkpt_status build_accel(snkpt_ctx* ctx, hipStream_t st) {
     HIP_CHECK(A.bin_aabb.reserve((2 * nn) * 24));
     HIP_CHECK(A.bin_leafify.reserve(nn * 4));
 
-    DeviceBuffer keys2, ids2, hist, hist_scan, sums, bounds, first, count, pint, pleaf, flags, cost, height,
-        slot_child, nint, ntri, sint, stri, work, totals;
+    DeviceBuffer keys2, ids2, hist, hist_scan, sums, bounds, first, count, pint, pleaf, flags, up_live, up_cnt, cost,
+        height, slot_child, nint, ntri, sint, stri, work, totals;
     const uint32_t nb_sort = blocks(n, kSortTile);
     HIP_CHECK(keys2.reserve(nn * 4));
     HIP_CHECK(ids2.reserve(nn * 4));
@@ -742,6 +852,8 @@ snkpt_status build_accel(snkpt_ctx* ctx, hipStream_t st) {
     HIP_CHECK(pint.reserve(nn * 4));
     HIP_CHECK(pleaf.reserve(nn * 4));
     HIP_CHECK(flags.reserve(nn * 4));
+    HIP_CHECK(up_live.reserve(nn * 8));
+    HIP_CHECK(up_cnt.reserve((kMaxBinLevels + 1) * 4));
     HIP_CHECK(cost.reserve(nn * 4));
     HIP_CHECK(height.reserve(nn * 4));
     HIP_CHECK(slot_child.reserve(nn * 32));
@@ -753,13 +865,15 @@ snkpt_status build_accel(snkpt_ctx* ctx, hipStream_t st) {
     HIP_CHECK(totals.reserve(16));
 
     uint32_t hb[12];
+    const uint32_t cus = ctx->props.multiProcessorCount > 0 ? (uint32_t)ctx->props.multiProcessorCount : 256u;
     if (n > 0) {
         k_flatten<<<blocks(n, 256), 256, 0, st>>>(n, A.inst_tri_offset.as<uint32_t>(), n_inst,
                                                     ctx->d_inst.as<InstDev>(), ctx->d_mat4.as<float>(),
                                                     ctx->d_idx.as<uint32_t>(), ctx->d_pos.as<float>(),
                                                     A.world_tris.as<float4>());
         k_bounds_init<<<1, 64, 0, st>>>(bounds.as<uint32_t>());
-        k_bounds<<<blocks(n, 256), 256, 0, st>>>(A.world_tris.as<float4>(), n, bounds.as<uint32_t>());
+        k_bounds<<<std::min(blocks(n, 256), cus * 2u), 256, 0, st>>>(A.world_tris.as<float4>(), n,
+                                                                   bounds.as<uint32_t>());
         k_morton<<<blocks(n, 256), 256, 0, st>>>(A.world_tris.as<float4>(), n, bounds.as<uint32_t>(),
                                                    A.keys.as<uint32_t>(), A.ids.as<uint32_t>());
         // radix sort, 4 passes, result back in A.keys / A.ids
@@ -785,11 +899,19 @@ snkpt_status build_accel(snkpt_ctx* ctx, hipStream_t st) {
                                                      A.bin_aabb.as<float>());
         if (n >= 2) {
             k_fill_u32<<<blocks(n_int, 256), 256, 0, st>>>(flags.as<uint32_t>(), n_int, 0u);
-            k_bottom_up<<<blocks(n, 256), 256, 0, st>>>(n, A.bin_child.as<uint32_t>(), count.as<uint32_t>(),
-                                                        pint.as<uint32_t>(), pleaf.as<uint32_t>(),
-                                                        flags.as<uint32_t>(), A.bin_aabb.as<float>(),
-                                                        cost.as<float>(), height.as<uint32_t>(),
-                                                        A.bin_leafify.as<uint32_t>());
+            k_fill_u32<<<1, 256, 0, st>>>(up_cnt.as<uint32_t>(), kMaxBinLevels + 1, 0u);
+            BinUp U{n, n_int, A.bin_child.as<uint32_t>(), count.as<uint32_t>(), pint.as<uint32_t>(),
+                    pleaf.as<uint32_t>(), flags.as<uint32_t>(), A.bin_aabb.as<float>(), cost.as<float>(),
+                    height.as<uint32_t>(), A.bin_leafify.as<uint32_t>()};
+            // the live set shrinks level by level: smaller grids (grid-stride
+            // loops, so any grid is correct) keep the late, nearly empty launches cheap
+            auto grid = [&](uint32_t l) { return std::max(16u, std::min(blocks(n, kUpBlock) >> l, cus * 8u)); };
+            uint32_t* live[2] = {up_live.as<uint32_t>(), up_live.as<uint32_t>() + n};
+            uint32_t* cnt = up_cnt.as<uint32_t>();
+            k_bottom_level<true><<<grid(0), kUpBlock, 0, st>>>(U, nullptr, nullptr, live[0], cnt);
+            for (uint32_t l = 1; l < kMaxBinLevels; ++l)
+                k_bottom_level<false><<<grid(l), kUpBlock, 0, st>>>(U, live[(l - 1) & 1], cnt + (l - 1),
+                                                                    live[l & 1], cnt + l);
         }
         // ---- collapse, one level per iteration
         BinView bv{n, n_int, A.bin_child.as<uint32_t>(), first.as<uint32_t>(), count.as<uint32_t>(),
@@ -828,6 +950,7 @@ snkpt_status build_accel(snkpt_ctx* ctx, hipStream_t st) {
             HIP_CHECK(hipMemcpyAsync(&h0, height.ptr, 4, hipMemcpyDeviceToHost, st));
             HIP_CHECK(hipStreamSynchronize(st));
             A.binary_depth = h0;
+            if (h0 > kMaxBinLevels + 1) return fail(ctx, SNKPT_ERR_HIP, "binary tree deeper than the bottom-up pass");
         } else {
             A.binary_depth = 1;
         }

@@ -1099,7 +1099,6 @@ snkpt_status cache_pass(snkpt_ctx* ctx, const CamDev& cd, const snkpt_settings* 
     uint32_t* qc = C.counts.as<uint32_t>();
     unsigned long long* stats = R.stats.as<unsigned long long>();
     // 1. render paths, terminated into the cache at vertex Q
-    generate_camera(ctx, cd, initstate, st);
     k_zero_u32<<<1, 64, 0, st>>>(qc, 4);
     ShadeHooks H;
     H.kind = 1;
@@ -1110,7 +1109,7 @@ snkpt_status cache_pass(snkpt_ctx* ctx, const CamDev& cd, const snkpt_settings* 
     H.q_count = qc;
     // with depth >= Q the vertex before the query keeps its continuation (MIS
     // against the cache vertex's emission); with depth < Q this is plain PT
-    if ((e = pt_bounces(ctx, s, 0, 0, st, &H, std::min(depth, Q))) != SNKPT_OK) return e;
+    if ((e = primary_bounces(ctx, cd, s, initstate, st, &H, std::min(depth, Q))) != SNKPT_OK) return e;
     // 2. query the cache
     const bool avg = s->cache_ema > 0.0f;  // query the weight averages
     k_encode<<<enc_blocks(n), kEncBlock, 0, st>>>(C.q_samples.as<float>(), n, qc, C.geom,

@@ -268,6 +268,24 @@ snkpt_status snkpt_render(snkpt_ctx* ctx, const snkpt_camera* cam, const snkpt_s
     return render(ctx, cam, settings, out_rgba, w, h, out_format, (hipStream_t)stream);
 }
 
+snkpt_status snkpt_render_gbuffer(snkpt_ctx* ctx, const snkpt_camera* cam, const snkpt_settings* settings,
+                                  const snkpt_gbuffer* gbuffer, void* out_rgba, uint32_t w, uint32_t h,
+                                  uint32_t out_format, snkpt_stream stream) {
+    if (!ctx) return SNKPT_ERR_INVALID_ARG;
+    if (!cam || !settings || !gbuffer || !out_rgba || w == 0 || h == 0)
+        return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_render_gbuffer: null camera/settings/G-buffer/output or "
+                                                "empty image");
+    if (!gbuffer->depth || !gbuffer->normal || !gbuffer->albedo || !gbuffer->rma || !gbuffer->mat_flags)
+        return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_render_gbuffer: depth, normal, albedo, rma and mat_flags "
+                                                "images are required (motion is optional)");
+    if ((((uintptr_t)gbuffer->depth | (uintptr_t)gbuffer->albedo | (uintptr_t)gbuffer->motion) & 3u) != 0 ||
+        (((uintptr_t)gbuffer->normal | (uintptr_t)gbuffer->rma) & 7u) != 0 || ((uintptr_t)gbuffer->mat_flags & 1u) != 0)
+        return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_render_gbuffer: images must be aligned to their texel size");
+    if (out_format != SNKPT_FORMAT_RGBA32F && out_format != SNKPT_FORMAT_RGBA16F)
+        return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_render_gbuffer: unknown output format");
+    return render(ctx, cam, settings, out_rgba, w, h, out_format, (hipStream_t)stream, gbuffer);
+}
+
 snkpt_status snkpt_get_frame_stats(snkpt_ctx* ctx, snkpt_frame_stats* out) {
     if (!ctx || !out) return SNKPT_ERR_INVALID_ARG;
     *out = ctx->stats;

@@ -32,12 +32,11 @@ struct VertexOut {
 // and the per-thread kernels consume the stream identically.  rad_now: the
 // path's radiance before this vertex (read for training records only).
 template <int KIND>
-__device__ __forceinline__ void path_vertex(const SceneDev& S, const PtParams& P, const ShadeHooks& H, uint32_t path,
-                                            uint32_t bounce, V3 dir, float t, uint32_t inst, uint32_t prim, float hu,
-                                            float hv, float4 thr4, Rng& rng, float4 rad_now, VertexOut& o) {
+__device__ __forceinline__ void path_vertex_surf(const SceneDev& S, const PtParams& P, const ShadeHooks& H,
+                                                 uint32_t path, uint32_t bounce, V3 dir, float t, const Surf& s,
+                                                 float4 thr4, Rng& rng, float4 rad_now, VertexOut& o) {
     o.thr4 = thr4;
     const V3 thr{thr4.x, thr4.y, thr4.z};
-    Surf s = surface_at(S, dir, inst, prim, hu, hv);
     if (s.emissive) {
         // MIS against the NEE of the previous vertex; a negative stored pdf
         // marks a vertex whose direct light came from ReSTIR (weight 0 here)
@@ -82,9 +81,7 @@ __device__ __forceinline__ void path_vertex(const SceneDev& S, const PtParams& P
             o.sh_c = mul(mul(thr, f), ls.le) * (cs * w / ls.pdf);
             if (o.sh_c.x > 0.0f || o.sh_c.y > 0.0f || o.sh_c.z > 0.0f) {
                 o.sh = true;
-                o.sh_o = s.p + s.ng * s.eps;
-                o.sh_d = ls.wi;
-                o.sh_t = ls.dist * (1.0f - 1e-4f) - s.eps;
+                shadow_segment(s.p, s.ng, s.eps, ls.wi, ls.dist, o.sh_o, o.sh_d, o.sh_t);
             }
         }
     }
@@ -109,6 +106,48 @@ __device__ __forceinline__ void path_vertex(const SceneDev& S, const PtParams& P
                 o.ext_d = wi;
                 o.thr4 = make_float4(nt.x, nt.y, nt.z, pdf);
             }
+        }
+    }
+}
+
+template <int KIND>
+__device__ __forceinline__ void path_vertex(const SceneDev& S, const PtParams& P, const ShadeHooks& H, uint32_t path,
+                                            uint32_t bounce, V3 dir, float t, uint32_t inst, uint32_t prim, float hu,
+                                            float hv, float4 thr4, Rng& rng, float4 rad_now, VertexOut& o) {
+    const Surf s = surface_at(S, dir, inst, prim, hu, hv);
+    path_vertex_surf<KIND>(S, P, H, path, bounce, dir, t, s, thr4, rng, rad_now, o);
+}
+
+// Queue appends of one path vertex: continuation ray, shadow ray and (KIND 1)
+// cache query.  Every thread of the block calls it (block-wide appends).
+template <int KIND, int BLOCK>
+__device__ __forceinline__ void vertex_appends(const VertexOut& o, uint32_t path, uint32_t bounce, const Rng& rng,
+                                               const Queue& out, const ShadowQueue& sq, const PathState& ps,
+                                               uint32_t* __restrict__ counts, uint32_t nxt, const ShadeHooks& H,
+                                               uint32_t* lds_app) {
+    uint32_t slot = block_append<BLOCK>(o.ext, &counts[nxt], lds_app);
+    if (o.ext) {
+        out.org[slot] = make_float4(o.ext_o.x, o.ext_o.y, o.ext_o.z, 0.0f);
+        out.dir[slot] = make_float4(o.ext_d.x, o.ext_d.y, o.ext_d.z, 1e30f);
+        out.path[slot] = path;
+        ps.thr[path] = o.thr4;
+        ps.rng[path] = make_uint2((uint32_t)rng.state, (uint32_t)(rng.state >> 32));
+        ps.bounce[path] = bounce + 1;
+    }
+    uint32_t sslot = block_append<BLOCK>(o.sh, &counts[2], lds_app);
+    if (o.sh) {
+        sq.org[sslot] = make_float4(o.sh_o.x, o.sh_o.y, o.sh_o.z, 0.0f);
+        sq.dir[sslot] = make_float4(o.sh_d.x, o.sh_d.y, o.sh_d.z, o.sh_t);
+        sq.contrib[sslot] = make_float4(o.sh_c.x, o.sh_c.y, o.sh_c.z, __uint_as_float(path));
+    }
+    if (KIND == 1) {
+        uint32_t qslot = block_append<BLOCK>(o.q, H.q_count, lds_app);
+        if (o.q) {
+            float* q = H.q_samples + (size_t)qslot * 16;
+#pragma unroll
+            for (int k = 0; k < 13; ++k) q[k] = o.qs[k];
+            q[13] = q[14] = q[15] = 0.0f;
+            H.q_meta[qslot] = make_float4(o.thr4.x, o.thr4.y, o.thr4.z, __uint_as_float(path));
         }
     }
 }
@@ -144,31 +183,7 @@ __device__ __forceinline__ void shade_step(const SS& ss, const PtParams& P, cons
             }
         }
     }
-    uint32_t slot = block_append<BLOCK>(o.ext, &counts[nxt], lds_app);
-    if (o.ext) {
-        out.org[slot] = make_float4(o.ext_o.x, o.ext_o.y, o.ext_o.z, 0.0f);
-        out.dir[slot] = make_float4(o.ext_d.x, o.ext_d.y, o.ext_d.z, 1e30f);
-        out.path[slot] = path;
-        ps.thr[path] = o.thr4;
-        ps.rng[path] = make_uint2((uint32_t)rng.state, (uint32_t)(rng.state >> 32));
-        ps.bounce[path] = bounce + 1;
-    }
-    uint32_t sslot = block_append<BLOCK>(o.sh, &counts[2], lds_app);
-    if (o.sh) {
-        sq.org[sslot] = make_float4(o.sh_o.x, o.sh_o.y, o.sh_o.z, 0.0f);
-        sq.dir[sslot] = make_float4(o.sh_d.x, o.sh_d.y, o.sh_d.z, o.sh_t);
-        sq.contrib[sslot] = make_float4(o.sh_c.x, o.sh_c.y, o.sh_c.z, __uint_as_float(path));
-    }
-    if (KIND == 1) {
-        uint32_t qslot = block_append<BLOCK>(o.q, H.q_count, lds_app);
-        if (o.q) {
-            float* q = H.q_samples + (size_t)qslot * 16;
-#pragma unroll
-            for (int k = 0; k < 13; ++k) q[k] = o.qs[k];
-            q[13] = q[14] = q[15] = 0.0f;
-            H.q_meta[qslot] = make_float4(o.thr4.x, o.thr4.y, o.thr4.z, __uint_as_float(path));
-        }
-    }
+    vertex_appends<KIND, BLOCK>(o, path, bounce, rng, out, sq, ps, counts, nxt, H, lds_app);
 }
 
 // A whole path per thread (no queues): camera ray of path i through pixel

@@ -96,6 +96,44 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PtParams P, Queue 
     shade_step<KIND, kBlock>(ss, P, in, hit, out, sq, ps, rad, counts, nxt, H, i, i < counts[cur], lds_app);
 }
 
+// Bounce 0 from the G-buffer: the primary vertex of path i (pixel i) read from
+// the rasterized G-buffer instead of a traced camera ray, then the same vertex
+// logic and queue appends as the shade kernel.
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void k_gshade(SceneDev S, GbufDev G, CamDev cam, uint32_t w, uint32_t h,
+                                                   PtParams P, Queue out, ShadowQueue sq, PathState ps,
+                                                   float4* __restrict__ rad, uint32_t* __restrict__ counts,
+                                                   uint32_t nxt, uint64_t initstate, ShadeHooks H) {
+    __shared__ uint32_t lds_app[kBlock / 64 + 1];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    VertexOut o;
+    Rng rng{0, 0};
+    if (i < w * h) {
+        rng = rng_seed(initstate, i);
+        if (KIND == 2) H.nrec[i] = 0;
+        Surf s;
+        V3 dir;
+        float t;
+        if (gbuffer_surface(G, cam, i, w, h, s, dir, t)) {
+            const float4 rn = KIND == 2 ? rad[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            path_vertex_surf<KIND>(S, P, H, i, 0, dir, t, s, make_float4(1.0f, 1.0f, 1.0f, 0.0f), rng, rn, o);
+            if (o.emit) {
+                float4 r = rad[i];
+                rad[i] = make_float4(r.x + o.c_emit.x, r.y + o.c_emit.y, r.z + o.c_emit.z, r.w);
+            }
+        }
+    }
+    vertex_appends<KIND, kBlock>(o, i, 0, rng, out, sq, ps, counts, nxt, H, lds_app);
+}
+
+// extension rays queued by bounce 0 and its shadow rays
+__global__ void k_gbuf_stats(const uint32_t* c, unsigned long long* stats) {
+    if (threadIdx.x == 0) {
+        stats[1] += c[1];
+        stats[2] += c[2];
+    }
+}
+
 // Shadow rays: persistent any-hit traversal with ray refill; each wave owns a
 // contiguous slice of the shadow queue.
 template <int STACK>
@@ -405,17 +443,47 @@ snkpt_status generate_camera(snkpt_ctx* ctx, const CamDev& cam, uint64_t initsta
     return SNKPT_OK;
 }
 
+snkpt_status primary_bounces(snkpt_ctx* ctx, const CamDev& cam, const snkpt_settings* s, uint64_t initstate,
+                             hipStream_t st, const ShadeHooks* hooks, uint32_t max_depth_override) {
+    RenderState& R = *ctx->render;
+    if (!R.use_gb) {
+        generate_camera(ctx, cam, initstate, st);
+        return pt_bounces(ctx, s, 0, 0, st, hooks, max_depth_override);
+    }
+    const uint32_t n = R.w * R.h;
+    SceneDev S = scene_dev(ctx);
+    PtParams P{max_depth_override ? max_depth_override : (s->max_depth ? s->max_depth : 1u), s->rr_start};
+    ShadeHooks H = hooks ? *hooks : ShadeHooks{};
+    float4* rad = H.rad ? H.rad : R.rad.as<float4>();
+    uint32_t* counts = R.counts.as<uint32_t>();
+    unsigned long long* stats = R.stats.as<unsigned long long>();
+    k_bounce_begin<<<1, 64, 0, st>>>(counts, 0u, 1u, stats, 0u, 0u);  // clears queue 1 and the shadow queue
+    if (H.kind == 1)
+        k_gshade<1><<<nblk(n, kBlock), kBlock, 0, st>>>(S, R.gb, cam, R.w, R.h, P, queue_of(R, 1), shadow_of(R),
+                                                        paths_of(R), rad, counts, 1u, initstate, H);
+    else if (H.kind == 2)
+        k_gshade<2><<<nblk(n, kBlock), kBlock, 0, st>>>(S, R.gb, cam, R.w, R.h, P, queue_of(R, 1), shadow_of(R),
+                                                        paths_of(R), rad, counts, 1u, initstate, H);
+    else
+        k_gshade<0><<<nblk(n, kBlock), kBlock, 0, st>>>(S, R.gb, cam, R.w, R.h, P, queue_of(R, 1), shadow_of(R),
+                                                        paths_of(R), rad, counts, 1u, initstate, H);
+    snkpt_status e = launch_shadow(ctx, R, rad, st, n);
+    if (e != SNKPT_OK) return e;
+    k_gbuf_stats<<<1, 64, 0, st>>>(counts, stats);
+    if (P.max_depth > 1) return pt_bounces(ctx, s, 1, 1, st, hooks, max_depth_override, n);
+    return SNKPT_OK;
+}
+
 // One pass of 1 sample per pixel of the plain path tracer, added into R.rad.
 snkpt_status pt_pass(snkpt_ctx* ctx, const CamDev& cam, const snkpt_settings* s, uint64_t initstate,
                      hipStream_t st, uint32_t first_bounce_skip) {
     (void)first_bounce_skip;
-    if (s->flags & SNKPT_FLAG_WHOLE_PATHS) {
+    if ((s->flags & SNKPT_FLAG_WHOLE_PATHS) && !ctx->render->use_gb) {
         RenderState& R = *ctx->render;
         return whole_paths(ctx, cam, R.w, R.h, s, initstate, R.w * R.h, 0, 1, nullptr, R.rad.as<float4>(),
                            R.stats.as<unsigned long long>(), st);
     }
-    generate_camera(ctx, cam, initstate, st);
-    return pt_bounces(ctx, s, 0, 0, st);
+    return primary_bounces(ctx, cam, s, initstate, st);
 }
 
 snkpt_status whole_paths(snkpt_ctx* ctx, const CamDev& cam, uint32_t w, uint32_t h, const snkpt_settings* s,
@@ -478,7 +546,7 @@ __global__ void k_zero_u64(unsigned long long* p, uint32_t n) {
 }
 
 snkpt_status render(snkpt_ctx* ctx, const snkpt_camera* cam, const snkpt_settings* s, void* out, uint32_t w,
-                    uint32_t h, uint32_t fmt, hipStream_t st) {
+                    uint32_t h, uint32_t fmt, hipStream_t st, const snkpt_gbuffer* gb) {
     if (!ctx->accel.built) return fail(ctx, SNKPT_ERR_NOT_BUILT, "snkpt_render: call snkpt_build_accel first");
     if ((uint64_t)w * h > (1ull << 26)) return fail(ctx, SNKPT_ERR_CAPACITY, "snkpt_render: image above 2^26 pixels");
     if (s->mode > SNKPT_MODE_CACHE) return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_render: unknown mode");
@@ -486,6 +554,18 @@ snkpt_status render(snkpt_ctx* ctx, const snkpt_camera* cam, const snkpt_setting
     snkpt_status e = begin_frame(ctx, w, h, st);
     if (e != SNKPT_OK) return e;
     RenderState& R = *ctx->render;
+    // the G-buffer is borrowed for this call only
+    struct GbScope {
+        RenderState& R;
+        ~GbScope() {
+            R.use_gb = false;
+            R.gb = GbufDev{};
+        }
+    } gb_scope{R};
+    if (gb) {
+        R.use_gb = true;
+        R.gb = GbufDev{gb->depth, gb->normal, gb->albedo, gb->rma, gb->mat_flags, gb->motion};
+    }
     k_zero_u64<<<1, 64, 0, st>>>(R.stats.as<unsigned long long>(), 8);
     if (s->flags & SNKPT_FLAG_RESET) {
         k_fill_f4<<<nblk(w * h, kBlock), kBlock, 0, st>>>(R.accum.as<float4>(), w * h, make_float4(0, 0, 0, 0));
@@ -508,7 +588,7 @@ snkpt_status render(snkpt_ctx* ctx, const snkpt_camera* cam, const snkpt_setting
     if (s->flags & SNKPT_FLAG_ACCUMULATE) R.accum_frames += spp;
     hipError_t le = hipGetLastError();
     if (le != hipSuccess) return hip_fail(ctx, le, "render launch");
-    return read_stats(ctx, st, (uint64_t)w * h * spp);
+    return read_stats(ctx, st, gb ? 0ull : (uint64_t)w * h * spp);
 }
 
 snkpt_status oracle_accumulate(snkpt_ctx* ctx, const snkpt_camera* cam, const snkpt_settings* s, double* accum,

@@ -92,6 +92,25 @@ __device__ __forceinline__ Surf surface_at(const SceneDev& S, V3 dir, uint32_t i
     return s;
 }
 
+// Shadow ray of a light sample seen from p: it leaves the biased origin
+// o = p + ng eps and is aimed at the sampled light point p + wi dist, measured
+// from o (ShadowTest from biased_pos, RayGen.rgen:101-121,552), ending 1e-4
+// short of it; with a large bias (G-buffer vertices) the unbiased direction
+// would overshoot into the light's own surface.  Lights at infinity keep wi.
+__device__ __forceinline__ void shadow_segment(V3 p, V3 ng, float eps, V3 wi, float dist, V3& o, V3& d,
+                                               float& tmax) {
+    o = p + ng * eps;
+    if (dist >= 1e29f) {
+        d = wi;
+        tmax = dist * (1.0f - 1e-4f) - eps;
+        return;
+    }
+    const V3 e = (p + wi * dist) - o;
+    const float l = sqrtf(dot(e, e));
+    d = l > 0.0f ? e * (1.0f / l) : wi;
+    tmax = l * (1.0f - 1e-4f);
+}
+
 struct LightSample {
     V3 wi, le;
     float dist, pdf;  // solid-angle pdf; for a delta light the selection probability
@@ -221,35 +240,6 @@ __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t* counter, un
     }
     base = __shfl(base, (int)leader, 64);
     return want ? base + (uint32_t)__popcll(m & lt) : 0xFFFFFFFFu;
-}
-
-// Block-level queue append: every thread of the block calls it (convergent:
-// no thread may have returned before); threads with `want` get consecutive
-// slots from one atomicAdd per block, in wave-then-lane order.  Same-address
-// atomics are serialised by the memory system, so one per block instead of one
-// per wave matters for the big early bounces.  `lds` holds BLOCK / 64 + 1 words
-// and may be reused right after the call.
-template <int BLOCK>
-__device__ __forceinline__ uint32_t block_append(bool want, uint32_t* counter, uint32_t* lds) {
-    constexpr int kWaves = BLOCK / 64;
-    const uint64_t m = __ballot(want);
-    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
-    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    if (lane == 0) lds[wave] = (uint32_t)__popcll(m);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t run = 0;
-        for (int w = 0; w < kWaves; ++w) {
-            uint32_t c = lds[w];
-            lds[w] = run;
-            run += c;
-        }
-        lds[kWaves] = run ? atomicAdd(counter, run) : 0u;
-    }
-    __syncthreads();
-    const uint32_t slot = lds[kWaves] + lds[wave] + (uint32_t)__popcll(m & lt);
-    __syncthreads();
-    return want ? slot : 0xFFFFFFFFu;
 }
 
 }  // namespace snkpt

@@ -22,6 +22,72 @@ __device__ __forceinline__ V3 camera_dir(const CamDev& c, uint32_t x, uint32_t y
     return normalize(p - c.pos);
 }
 
+// Rasterized G-buffer of the frame (device pointers, one texel per pixel):
+// D32 depth, RGBA16F normal, RGBA8 SNORM albedo, RGBA16F roughness /
+// metallic / ao, R16 material flags, optional RG16F motion.
+struct GbufDev {
+    const float* depth = nullptr;
+    const uint16_t* normal = nullptr;
+    const int8_t* albedo = nullptr;
+    const uint16_t* rma = nullptr;
+    const uint16_t* flags = nullptr;
+    const uint16_t* motion = nullptr;
+};
+
+__device__ __forceinline__ float half_bits(uint16_t b) {
+    _Float16 h;
+    __builtin_memcpy(&h, &b, 2);
+    return (float)h;
+}
+// R8_SNORM texel to float: max(c / 127, -1)
+__device__ __forceinline__ float snorm8(int8_t c) { return fmaxf((float)c / 127.0f, -1.0f); }
+
+// The primary vertex of pixel px from the G-buffer (RayGen.rgen:531-552):
+// world position from depth through inverse(view) * inverse(proj)
+// (WorldPosFromDepth, Util.glsl:1-7, the camera-ray unprojection with the
+// texel's depth for 1), the texel's normal as shading and geometric normal
+// (turned to face the camera), material from albedo / rma / flags, and the
+// reference's ray bias 0.01 + max(0.001 |cam - p|, 0.01).  False where no
+// surface was rasterized (depth >= 1) or the normal is zero.
+__device__ __forceinline__ bool gbuffer_surface(const GbufDev& g, const CamDev& c, uint32_t px, uint32_t w,
+                                                uint32_t h, Surf& s, V3& dir, float& t) {
+    const float depth = g.depth[px];
+    if (!(depth < 1.0f)) return false;
+    const uint32_t x = px % w, y = px / w;
+    const float u = ((float)x + 0.5f) / (float)w * 2.0f - 1.0f;
+    const float v = ((float)y + 0.5f) / (float)h * 2.0f - 1.0f;
+    float wp[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) wp[r] = ((c.m[4 * r] * u + c.m[4 * r + 1] * v) + c.m[4 * r + 2] * depth) + c.m[4 * r + 3];
+    const float iw = 1.0f / fmaxf(wp[3], 1e-6f);
+    s.p = V3{wp[0] * iw, wp[1] * iw, wp[2] * iw};
+    const V3 d = s.p - c.pos;
+    const float l2 = dot(d, d);
+    if (!(l2 > 0.0f)) return false;
+    t = sqrtf(l2);
+    dir = d * (1.0f / t);
+    const uint2 nb = *(const uint2*)(g.normal + 4 * (size_t)px);
+    V3 n{half_bits((uint16_t)(nb.x & 0xFFFFu)), half_bits((uint16_t)(nb.x >> 16)), half_bits((uint16_t)(nb.y & 0xFFFFu))};
+    const float n2 = dot(n, n);
+    if (!(n2 > 1e-20f)) return false;
+    n = n * (1.0f / sqrtf(n2));
+    if (dot(n, dir) > 0.0f) n = n * -1.0f;
+    s.ng = n;
+    s.ns = n;
+    const char4 a = *(const char4*)(g.albedo + 4 * (size_t)px);
+    s.m.albedo = V3{fminf(fmaxf(snorm8(a.x), 0.0f), 1.0f), fminf(fmaxf(snorm8(a.y), 0.0f), 1.0f),
+                    fminf(fmaxf(snorm8(a.z), 0.0f), 1.0f)};
+    const uint2 rb = *(const uint2*)(g.rma + 4 * (size_t)px);
+    s.m.roughness = fminf(fmaxf(half_bits((uint16_t)(rb.x & 0xFFFFu)), 0.02f), 1.0f);
+    s.m.metallic = fminf(fmaxf(half_bits((uint16_t)(rb.x >> 16)), 0.0f), 1.0f);
+    s.m.flags = g.flags[px];
+    s.emissive = (s.m.flags & 1u) != 0u;
+    s.le = s.m.albedo;
+    s.gid = 0xFFFFFFFFu;
+    s.eps = 0.01f + fmaxf(t * 0.001f, 0.01f);
+    return true;
+}
+
 struct PtParams {
     uint32_t max_depth, rr_start;
 };
@@ -86,6 +152,8 @@ struct RenderState {
     DeviceBuffer rad, accum, counts, stats;
     uint32_t accum_frames = 0;
     bool history_valid = false;
+    bool use_gb = false;  // this frame's primary vertices come from `gb`
+    GbufDev gb;
     RestirState* restir = nullptr;
     ~RenderState() { restir_state_free(restir); }
 };
@@ -115,6 +183,10 @@ snkpt_status cache_query(snkpt_ctx* ctx, const float* samples, uint32_t n, float
                          hipStream_t st);
 snkpt_status cache_train_frame(snkpt_ctx* ctx, const CamDev& cd, uint32_t w, uint32_t h, const snkpt_settings* s,
                                uint64_t initstate, uint32_t n_paths, unsigned long long* stats, hipStream_t st);
+// Bounces 0.. of one path per pixel: camera rays through the wavefront, or,
+// when the frame has a G-buffer, bounce 0 read from it and bounces 1.. traced.
+snkpt_status primary_bounces(snkpt_ctx* ctx, const CamDev& cam, const snkpt_settings* s, uint64_t initstate,
+                             hipStream_t st, const ShadeHooks* hooks = nullptr, uint32_t max_depth_override = 0);
 // Camera rays for paths j < n_paths through pixels (start + j * stride) mod (w h).
 snkpt_status generate_camera(snkpt_ctx* ctx, const CamDev& cam, uint64_t initstate, hipStream_t st,
                              uint32_t n_paths = 0, uint32_t start = 0, uint32_t stride = 1,

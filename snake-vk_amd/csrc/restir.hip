@@ -30,7 +30,7 @@ namespace snkpt {
 
 struct RestirState {
     uint32_t cap = 0;
-    DeviceBuffer g[2][5];  // per frame parity: p|valid, ns|rough, ng|metal, albedo|flags, wo|t
+    DeviceBuffer g[2][5];  // per frame parity: p|eps (0: no surface), ns|rough, ng|metal, albedo|flags, wo|t
     DeviceBuffer r_init, r_tmp, r_fin;  // float4 (emitter bits, b1, b2, W)
     DeviceBuffer m_init, m_tmp, m_fin;  // M
     int cur = 0;
@@ -52,7 +52,7 @@ struct GView {
 struct GSurf {
     V3 p, ns, ng, wo;
     Mat m;
-    float t;
+    float t, eps;  // eps: ray bias along ng
     bool valid;
 };
 
@@ -61,6 +61,7 @@ __device__ __forceinline__ GSurf load_g(const GView& G, uint32_t i) {
     float4 p = G.p[i], n = G.n[i], g = G.g[i], a = G.a[i], w = G.w[i];
     s.p = V3{p.x, p.y, p.z};
     s.valid = p.w > 0.0f;
+    s.eps = p.w;
     s.ns = V3{n.x, n.y, n.z};
     s.m.roughness = n.w;
     s.ng = V3{g.x, g.y, g.z};
@@ -137,7 +138,11 @@ constexpr uint64_t kPhaseCand = 0x52455354ull, kPhaseTemp = 0x54454d50ull, kPhas
                    kPhaseCont = 0x434f4e54ull;
 
 // Primary vertex: G-buffer, directly seen emission, candidate RIS, continuation ray.
-__global__ __launch_bounds__(kBlock) void k_primary(SceneDev S, GView G, Queue in, HitArrays hit, Queue out,
+// GB: the vertex is read from the caller's rasterized G-buffer (gb, cam, w)
+// instead of the traced camera ray (in, hit).
+template <bool GB>
+__global__ __launch_bounds__(kBlock) void k_primary(SceneDev S, GView G, Queue in, HitArrays hit, GbufDev gb,
+                                                    CamDev cam, uint32_t w, Queue out,
                                                     PathState ps, float4* __restrict__ rad, float4* __restrict__ r_out,
                                                     float* __restrict__ m_out, uint32_t* __restrict__ counts,
                                                     uint32_t n, uint32_t n_cand, uint32_t max_depth,
@@ -151,18 +156,27 @@ __global__ __launch_bounds__(kBlock) void k_primary(SceneDev S, GView G, Queue i
     Rng rc{0, 0};
     if (valid) {
         Res r{0xFFFFFFFFu, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        float4 dq = in.dir[i];
-        V3 dir{dq.x, dq.y, dq.z};
-        float t = hit.t[i];
+        V3 dir;
+        float t;
+        Surf s;
+        bool have;
+        if (GB) {
+            have = gbuffer_surface(gb, cam, i, w, n / w, s, dir, t);
+        } else {
+            const float4 dq = in.dir[i];
+            dir = V3{dq.x, dq.y, dq.z};
+            t = hit.t[i];
+            have = t >= 0.0f;
+            if (have) s = surface_at(S, dir, hit.inst[i], hit.prim[i], hit.u[i], hit.v[i]);
+        }
         float4 gp = make_float4(0, 0, 0, 0);
-        if (t >= 0.0f) {
-            Surf s = surface_at(S, dir, hit.inst[i], hit.prim[i], hit.u[i], hit.v[i]);
+        if (have) {
             if (s.emissive) {
                 float4 o = rad[i];
                 rad[i] = make_float4(o.x + s.le.x, o.y + s.le.y, o.z + s.le.z, o.w);
             } else {
                 V3 wo = dir * -1.0f;
-                gp = make_float4(s.p.x, s.p.y, s.p.z, 1.0f);
+                gp = make_float4(s.p.x, s.p.y, s.p.z, s.eps);  // w > 0: valid
                 G.n[i] = make_float4(s.ns.x, s.ns.y, s.ns.z, s.m.roughness);
                 G.g[i] = make_float4(s.ng.x, s.ng.y, s.ng.z, s.m.metallic);
                 G.a[i] = make_float4(s.m.albedo.x, s.m.albedo.y, s.m.albedo.z, __uint_as_float(s.m.flags));
@@ -173,6 +187,7 @@ __global__ __launch_bounds__(kBlock) void k_primary(SceneDev S, GView G, Queue i
                 g.ng = s.ng;
                 g.wo = wo;
                 g.m = s.m;
+                g.eps = s.eps;
                 Rng rr = rng_seed(initstate ^ kPhaseCand, i);
                 float ph = 0.0f;
                 if (S.n_emit > 0) {
@@ -245,7 +260,8 @@ __global__ __launch_bounds__(kBlock) void k_temporal(SceneDev S, GView G, GView 
                                                      const float* __restrict__ m_in, const float4* __restrict__ r_hist,
                                                      const float* __restrict__ m_hist, float4* __restrict__ r_out,
                                                      float* __restrict__ m_out, uint32_t w, uint32_t h, PrevCam pc,
-                                                     float m_cap, uint64_t initstate) {
+                                                     float m_cap, uint64_t initstate,
+                                                     const uint16_t* __restrict__ motion) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= w * h) return;
     Res rc = load_res(r_in, m_in, i);
@@ -254,21 +270,36 @@ __global__ __launch_bounds__(kBlock) void k_temporal(SceneDev S, GView G, GView 
         store_res(r_out, m_out, i, rc);
         return;
     }
-    // reproject into the previous frame
-    float cl[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) cl[r] = ((pc.pv[r] * g.p.x + pc.pv[4 + r] * g.p.y) + pc.pv[8 + r] * g.p.z) + pc.pv[12 + r];
     int q = -1;
-    if (cl[3] > 1e-6f) {
-        float fx = (cl[0] / cl[3] * 0.5f + 0.5f) * (float)w, fy = (cl[1] / cl[3] * 0.5f + 0.5f) * (float)h;
-        if (fx >= 0.0f && fy >= 0.0f && fx < (float)w && fy < (float)h) q = (int)fy * (int)w + (int)fx;
+    if (motion) {
+        // history texcoord = texcoord + motion (RayGen.rgen:424-428)
+        const uint32_t mb = *(const uint32_t*)(motion + 2 * (size_t)i);
+        const float hx = ((float)(i % w) + 0.5f) / (float)w + half_bits((uint16_t)(mb & 0xFFFFu));
+        const float hy = ((float)(i / w) + 0.5f) / (float)h + half_bits((uint16_t)(mb >> 16));
+        if (hx > 0.0f && hy > 0.0f && hx < 1.0f && hy < 1.0f) {
+            const uint32_t qx = min((uint32_t)(hx * (float)w), w - 1), qy = min((uint32_t)(hy * (float)h), h - 1);
+            q = (int)(qy * w + qx);
+        }
+    } else {
+        // reproject into the previous frame
+        float cl[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            cl[r] = ((pc.pv[r] * g.p.x + pc.pv[4 + r] * g.p.y) + pc.pv[8 + r] * g.p.z) + pc.pv[12 + r];
+        if (cl[3] > 1e-6f) {
+            float fx = (cl[0] / cl[3] * 0.5f + 0.5f) * (float)w, fy = (cl[1] / cl[3] * 0.5f + 0.5f) * (float)h;
+            if (fx >= 0.0f && fy >= 0.0f && fx < (float)w && fy < (float)h) q = (int)fy * (int)w + (int)fx;
+        }
     }
     GSurf gp;
     gp.valid = false;
     if (q >= 0) {
         gp = load_g(Gp, (uint32_t)q);
+        // motion vectors also follow moving geometry: then only the normal test applies
+        // (RayGen.rgen:436); reprojection of a static scene also checks the distance
         float dprev = sqrtf(dot(g.p - pc.pos, g.p - pc.pos));
-        if (!gp.valid || dot(gp.ns, g.ns) < 0.9f || fabsf(dprev - gp.t) > 0.05f * gp.t) gp.valid = false;
+        if (!gp.valid || dot(gp.ns, g.ns) < 0.9f || (!motion && fabsf(dprev - gp.t) > 0.05f * gp.t))
+            gp.valid = false;
     }
     if (!gp.valid) {
         store_res(r_out, m_out, i, rc);
@@ -371,10 +402,7 @@ __global__ __launch_bounds__(kBlock) void k_rshade(SceneDev S, GView G, const fl
                 float p = target(S, g, r.e, r.b1, r.b2, &c, &wi, &dist);
                 if (p > 0.0f) {
                     c = c * r.W;
-                    float am = fmaxf(fmaxf(fabsf(g.p.x), fabsf(g.p.y)), fabsf(g.p.z));
-                    float eps = 1e-4f * (1.0f + am);
-                    o = g.p + g.ng * eps;
-                    tmax = dist * (1.0f - 1e-4f) - eps;
+                    shadow_segment(g.p, g.ng, g.eps, wi, dist, o, wi, tmax);
                     want = true;
                 }
             }
@@ -410,11 +438,7 @@ __global__ __launch_bounds__(kBlock) void k_rshade_analytic(SceneDev S, GView G,
                 if (cs > 0.0f && dot(g.ng, ls.wi) > 0.0f) {
                     c = mul(brdf_eval(g.m, g.ns, g.wo, ls.wi), ls.le) * (cs / ls.pdf);
                     if (c.x > 0.0f || c.y > 0.0f || c.z > 0.0f) {
-                        float am = fmaxf(fmaxf(fabsf(g.p.x), fabsf(g.p.y)), fabsf(g.p.z));
-                        float eps = 1e-4f * (1.0f + am);
-                        o = g.p + g.ng * eps;
-                        wi = ls.wi;
-                        tmax = ls.dist * (1.0f - 1e-4f) - eps;
+                        shadow_segment(g.p, g.ng, g.eps, ls.wi, ls.dist, o, wi, tmax);
                         want = true;
                     }
                 }
@@ -489,18 +513,27 @@ snkpt_status restir_pass(snkpt_ctx* ctx, const snkpt_camera* cam, const CamDev& 
     uint32_t* counts = R.counts.as<uint32_t>();
     unsigned long long* stats = R.stats.as<unsigned long long>();
     const uint32_t max_depth = s->max_depth ? s->max_depth : 1u;
-    // camera rays + primary hits
-    generate_camera(ctx, cd, initstate, st);
-    snkpt_hits hh{R.h_t.as<float>(), R.h_inst.as<uint32_t>(), R.h_prim.as<uint32_t>(), R.h_u.as<float>(),
-                  R.h_v.as<float>()};
-    snkpt_status e = launch_trace(ctx, R.q[0].org.as<float4>(), R.q[0].dir.as<float4>(), n, hh, false, nullptr,
-                                  nullptr, nullptr, st);
-    if (e != SNKPT_OK) return e;
-    k_counts3<<<1, 64, 0, st>>>(counts, n, 0u, 0u);
+    snkpt_status e = SNKPT_OK;
     const uint32_t n_cand = s->restir_candidates ? s->restir_candidates : 32u;
-    k_primary<<<grid_for(n, kBlock), kBlock, 0, st>>>(S, G, queue_of(R, 0), hits_of(R), queue_of(R, 1), paths_of(R),
-                                                      R.rad.as<float4>(), X.r_init.as<float4>(), X.m_init.as<float>(),
-                                                      counts, n, n_cand, max_depth, s->rr_start, initstate, stats);
+    if (R.use_gb) {
+        // primary vertices from the caller's G-buffer
+        k_counts3<<<1, 64, 0, st>>>(counts, n, 0u, 0u);
+        k_primary<true><<<grid_for(n, kBlock), kBlock, 0, st>>>(
+            S, G, queue_of(R, 0), hits_of(R), R.gb, cd, w, queue_of(R, 1), paths_of(R), R.rad.as<float4>(),
+            X.r_init.as<float4>(), X.m_init.as<float>(), counts, n, n_cand, max_depth, s->rr_start, initstate, stats);
+    } else {
+        // camera rays + primary hits
+        generate_camera(ctx, cd, initstate, st);
+        snkpt_hits hh{R.h_t.as<float>(), R.h_inst.as<uint32_t>(), R.h_prim.as<uint32_t>(), R.h_u.as<float>(),
+                      R.h_v.as<float>()};
+        e = launch_trace(ctx, R.q[0].org.as<float4>(), R.q[0].dir.as<float4>(), n, hh, false, nullptr, nullptr,
+                         nullptr, st);
+        if (e != SNKPT_OK) return e;
+        k_counts3<<<1, 64, 0, st>>>(counts, n, 0u, 0u);
+        k_primary<false><<<grid_for(n, kBlock), kBlock, 0, st>>>(
+            S, G, queue_of(R, 0), hits_of(R), R.gb, cd, w, queue_of(R, 1), paths_of(R), R.rad.as<float4>(),
+            X.r_init.as<float4>(), X.m_init.as<float>(), counts, n, n_cand, max_depth, s->rr_start, initstate, stats);
+    }
     const bool temporal = X.have_prev && s->num_temporal_resamples != 0 && !cam->prev_frame_invalid;
     const float4* r_sp_in = X.r_init.as<float4>();
     const float* m_sp_in = X.m_init.as<float>();
@@ -512,7 +545,7 @@ snkpt_status restir_pass(snkpt_ctx* ctx, const snkpt_camera* cam, const CamDev& 
         k_temporal<<<grid_for(n, kBlock), kBlock, 0, st>>>(S, G, Gp, X.r_init.as<float4>(), X.m_init.as<float>(),
                                                            X.r_fin.as<float4>(), X.m_fin.as<float>(),
                                                            X.r_tmp.as<float4>(), X.m_tmp.as<float>(), w, h, pc, m_cap,
-                                                           initstate);
+                                                           initstate, R.use_gb ? R.gb.motion : nullptr);
         r_sp_in = X.r_tmp.as<float4>();
         m_sp_in = X.m_tmp.as<float>();
     }

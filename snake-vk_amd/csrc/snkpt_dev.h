@@ -357,4 +357,33 @@ __device__ __forceinline__ void hit_uv(const float4* __restrict__ tris, uint32_t
     v = W / det;
 }
 
+// Block-level queue append: every thread of the block calls it (convergent:
+// no thread may have returned before); threads with `want` get consecutive
+// slots from one atomicAdd per block, in wave-then-lane order.  Same-address
+// atomics are serialised by the memory system, so one per block instead of one
+// per wave matters for the big early bounces.  `lds` holds BLOCK / 64 + 1 words
+// and may be reused right after the call.
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_append(bool want, uint32_t* counter, uint32_t* lds) {
+    constexpr int kWaves = BLOCK / 64;
+    const uint64_t m = __ballot(want);
+    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    if (lane == 0) lds[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            uint32_t c = lds[w];
+            lds[w] = run;
+            run += c;
+        }
+        lds[kWaves] = run ? atomicAdd(counter, run) : 0u;
+    }
+    __syncthreads();
+    const uint32_t slot = lds[kWaves] + lds[wave] + (uint32_t)__popcll(m & lt);
+    __syncthreads();
+    return want ? slot : 0xFFFFFFFFu;
+}
+
 }  // namespace snkpt

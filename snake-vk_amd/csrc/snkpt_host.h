@@ -127,8 +127,9 @@ snkpt_status launch_trace(snkpt_ctx* ctx, const float4* org_tmin, const float4* 
                           const uint32_t* count_dev, hipStream_t stream);
 
 // render.hip
+// gb: primary vertex from a rasterized G-buffer instead of camera rays (NULL: camera rays)
 snkpt_status render(snkpt_ctx* ctx, const snkpt_camera* cam, const snkpt_settings* s, void* out, uint32_t w,
-                    uint32_t h, uint32_t fmt, hipStream_t stream);
+                    uint32_t h, uint32_t fmt, hipStream_t stream, const snkpt_gbuffer* gb = nullptr);
 snkpt_status oracle_accumulate(snkpt_ctx* ctx, const snkpt_camera* cam, const snkpt_settings* s, double* accum,
                                uint32_t w, uint32_t h, uint32_t spp, hipStream_t stream);
 void render_state_free(RenderState* r);

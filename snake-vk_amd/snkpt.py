@@ -49,6 +49,7 @@ EXPORTS = [
     "snkpt_cache_set_params", "snkpt_scene_view", "snkpt_batch_create", "snkpt_batch_destroy",
     "snkpt_batch_last_error", "snkpt_batch_add_scene", "snkpt_batch_add_view", "snkpt_batch_render",
     "snkpt_batch_autotune", "snkpt_batch_set_tuning", "snkpt_batch_get_tuning", "snkpt_batch_frame_stats",
+    "snkpt_render_gbuffer",
 ]
 CACHE_SAMPLE_FLOATS = 16
 CACHE_MLP_PARAMS = 4 * 4096 + 16 * 64
@@ -103,6 +104,12 @@ class AccelInfo(C.Structure):
 class FrameStats(C.Structure):
     _fields_ = [("camera_rays", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("cache_queries", C.c_uint64), ("cache_train_samples", C.c_uint64)]
+
+
+class Gbuffer(C.Structure):
+    """snkpt_gbuffer: device pointers of the rasterized G-buffer images."""
+    _fields_ = [("depth", C.c_void_p), ("normal", C.c_void_p), ("albedo", C.c_void_p), ("rma", C.c_void_p),
+                ("mat_flags", C.c_void_p), ("motion", C.c_void_p)]
 
 
 class BatchTuning(C.Structure):
@@ -176,6 +183,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "snkpt_batch_set_tuning": (I32, [VP, P(BatchTuning)]),
         "snkpt_batch_get_tuning": (I32, [VP, P(BatchTuning)]),
         "snkpt_batch_frame_stats": (I32, [VP, P(FrameStats), VP]),
+        "snkpt_render_gbuffer": (I32, [VP, P(Camera), P(Settings), P(Gbuffer), VP, U32, U32, U32, VP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -456,6 +464,22 @@ class Context:
                                    _stream(stream))
         self._check(rc, "snkpt_render")
 
+    def render_gbuffer(self, cam: Camera, settings: Settings, gbuffer: dict, out, w: int, h: int,
+                       fmt: int = FORMAT_RGBA32F, stream=None):
+        """One frame with the primary vertices read from a G-buffer: `gbuffer`
+        maps depth (float32 h x w), normal (float16 h x w x 4), albedo (int8
+        h x w x 4, SNORM), rma (float16 h x w x 4), mat_flags (int16/uint16
+        h x w) and optionally motion (float16 h x w x 2) to device tensors."""
+        g = Gbuffer()
+        for k in ("depth", "normal", "albedo", "rma", "mat_flags", "motion"):
+            t = gbuffer.get(k)
+            if t is not None and not t.is_contiguous():
+                raise ValueError(f"G-buffer image {k} must be contiguous")
+            setattr(g, k, None if t is None else t.data_ptr())
+        rc = self.lib.snkpt_render_gbuffer(self._h, C.byref(cam), C.byref(settings), C.byref(g), _ptr(out), w, h,
+                                           fmt, _stream(stream))
+        self._check(rc, "snkpt_render_gbuffer")
+
     def frame_stats(self) -> FrameStats:
         st = FrameStats()
         self._check(self.lib.snkpt_get_frame_stats(self._h, C.byref(st)), "snkpt_get_frame_stats")
@@ -552,8 +576,14 @@ class RT:
         self.output_image = output_image
 
     def RecordRenderCmdBuf(self, stream, output_image, scene: SceneData, settings: Settings, cam: Camera = None,
-                           fmt: int = FORMAT_RGBA32F):
+                           fmt: int = FORMAT_RGBA32F, resources: dict = None):
+        """resources: the G-buffer images of RaytracingResources (RenderCommon.h:6-35)
+        as in Context.render_gbuffer; None traces camera rays instead."""
         if self.ctx is None:
             raise SnkptError(ERR_NOT_BUILT, "RT::Init was not called")
         h, w = output_image.shape[0], output_image.shape[1]
-        self.ctx.render(cam or scene.camera(settings.frame, w, h), settings, output_image, w, h, fmt, stream)
+        cam = cam or scene.camera(settings.frame, w, h)
+        if resources is not None:
+            self.ctx.render_gbuffer(cam, settings, resources, output_image, w, h, fmt, stream)
+        else:
+            self.ctx.render(cam, settings, output_image, w, h, fmt, stream)
