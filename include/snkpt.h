@@ -257,6 +257,20 @@ snkpt_status snkpt_render_gbuffer(snkpt_ctx* ctx, const snkpt_camera* cam, const
                                   const snkpt_gbuffer* gbuffer, void* out_rgba, uint32_t w, uint32_t h,
                                   uint32_t out_format, snkpt_stream stream);
 
+/* TAA_ResolvePass::RecordCommandBuffer (TAA_ResolvePass.cpp:39-56,
+ * TAA_Resolve.comp:37-86): temporal resolve of a rendered frame.  History
+ * texcoord = texcoord + velocity; the bilinear (repeat-addressed) history is
+ * clipped to the 3x3 neighbourhood's mean +- 0.55 sigma box, blended 0.9 with
+ * the current frame, and replaced by the neighbourhood mean where
+ * |previous velocity + velocity| exceeds 0.001 (weight (len - 0.001) * 10).
+ * The context keeps the history and previous-velocity images (the pass's two
+ * blits).  current_rgba and out_rgba: device, w*h RGBA in `format`, may be the
+ * same buffer (every pixel reads the frame before the resolve); velocity:
+ * R16G16_SFLOAT, w*h.  reset != 0 (or a new size / format) starts a new
+ * history: out = current. */
+snkpt_status snkpt_taa_resolve(snkpt_ctx* ctx, const void* current_rgba, void* out_rgba, const uint16_t* velocity,
+                               uint32_t w, uint32_t h, uint32_t format, uint32_t reset, snkpt_stream stream);
+
 /* Rays traced by the last snkpt_render, by kind. */
 typedef struct snkpt_frame_stats {
     uint64_t camera_rays;

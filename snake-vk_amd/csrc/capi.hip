@@ -96,6 +96,7 @@ void snkpt_destroy(snkpt_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     render_state_free(ctx->render);
     cache_state_free(ctx->cache);
+    taa_state_free(ctx->taa);
     delete ctx;
 }
 
@@ -284,6 +285,21 @@ snkpt_status snkpt_render_gbuffer(snkpt_ctx* ctx, const snkpt_camera* cam, const
     if (out_format != SNKPT_FORMAT_RGBA32F && out_format != SNKPT_FORMAT_RGBA16F)
         return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_render_gbuffer: unknown output format");
     return render(ctx, cam, settings, out_rgba, w, h, out_format, (hipStream_t)stream, gbuffer);
+}
+
+snkpt_status snkpt_taa_resolve(snkpt_ctx* ctx, const void* current_rgba, void* out_rgba, const uint16_t* velocity,
+                               uint32_t w, uint32_t h, uint32_t format, uint32_t reset, snkpt_stream stream) {
+    if (!ctx) return SNKPT_ERR_INVALID_ARG;
+    if (!current_rgba || !out_rgba || !velocity || w == 0 || h == 0)
+        return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_taa_resolve: null image or empty size");
+    if (format != SNKPT_FORMAT_RGBA32F && format != SNKPT_FORMAT_RGBA16F)
+        return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_taa_resolve: unknown format");
+    if ((uint64_t)w * h > (1ull << 26) || (h + 15u) / 16u > 65535u || (w + 15u) / 16u > 65535u)
+        return fail(ctx, SNKPT_ERR_CAPACITY, "snkpt_taa_resolve: image above 2^26 pixels or 2^20 rows / columns");
+    const uintptr_t px = format == SNKPT_FORMAT_RGBA32F ? 15u : 7u;
+    if ((((uintptr_t)current_rgba | (uintptr_t)out_rgba) & px) != 0 || ((uintptr_t)velocity & 3u) != 0)
+        return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_taa_resolve: images must be aligned to their texel size");
+    return taa_resolve(ctx, current_rgba, out_rgba, velocity, w, h, format, reset, (hipStream_t)stream);
 }
 
 snkpt_status snkpt_get_frame_stats(snkpt_ctx* ctx, snkpt_frame_stats* out) {

@@ -78,6 +78,7 @@ struct Emitters {
 
 struct RenderState;  // defined in render.hip
 struct CacheState;   // defined in cache.hip
+struct TaaState;     // defined in taa.hip
 
 }  // namespace snkpt
 
@@ -105,6 +106,7 @@ struct snkpt_ctx {
     snkpt::Emitters emit;
     snkpt::RenderState* render = nullptr;
     snkpt::CacheState* cache = nullptr;
+    snkpt::TaaState* taa = nullptr;
     snkpt_frame_stats stats{};
 };
 
@@ -134,6 +136,11 @@ snkpt_status oracle_accumulate(snkpt_ctx* ctx, const snkpt_camera* cam, const sn
                                uint32_t w, uint32_t h, uint32_t spp, hipStream_t stream);
 void render_state_free(RenderState* r);
 void cache_state_free(CacheState* c);
+void taa_state_free(TaaState* t);
+
+// taa.hip
+snkpt_status taa_resolve(snkpt_ctx* ctx, const void* current, void* out, const uint16_t* velocity, uint32_t w,
+                         uint32_t h, uint32_t fmt, uint32_t reset, hipStream_t st);
 
 // cache.hip
 snkpt_status cache_reset(snkpt_ctx* ctx, uint32_t seed, const float* lo, const float* hi, hipStream_t st);

@@ -49,7 +49,7 @@ EXPORTS = [
     "snkpt_cache_set_params", "snkpt_scene_view", "snkpt_batch_create", "snkpt_batch_destroy",
     "snkpt_batch_last_error", "snkpt_batch_add_scene", "snkpt_batch_add_view", "snkpt_batch_render",
     "snkpt_batch_autotune", "snkpt_batch_set_tuning", "snkpt_batch_get_tuning", "snkpt_batch_frame_stats",
-    "snkpt_render_gbuffer",
+    "snkpt_render_gbuffer", "snkpt_taa_resolve",
 ]
 CACHE_SAMPLE_FLOATS = 16
 CACHE_MLP_PARAMS = 4 * 4096 + 16 * 64
@@ -184,6 +184,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "snkpt_batch_get_tuning": (I32, [VP, P(BatchTuning)]),
         "snkpt_batch_frame_stats": (I32, [VP, P(FrameStats), VP]),
         "snkpt_render_gbuffer": (I32, [VP, P(Camera), P(Settings), P(Gbuffer), VP, U32, U32, U32, VP]),
+        "snkpt_taa_resolve": (I32, [VP, VP, VP, VP, U32, U32, U32, U32, VP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -479,6 +480,14 @@ class Context:
         rc = self.lib.snkpt_render_gbuffer(self._h, C.byref(cam), C.byref(settings), C.byref(g), _ptr(out), w, h,
                                            fmt, _stream(stream))
         self._check(rc, "snkpt_render_gbuffer")
+
+    def taa_resolve(self, current, out, velocity, w: int, h: int, fmt: int = FORMAT_RGBA32F, reset: bool = False,
+                    stream=None):
+        """Temporal resolve of `current` into `out` (may be the same tensor);
+        velocity: float16 (h, w, 2) device tensor."""
+        rc = self.lib.snkpt_taa_resolve(self._h, _ptr(current), _ptr(out), _ptr(velocity), w, h, fmt, int(reset),
+                                        _stream(stream))
+        self._check(rc, "snkpt_taa_resolve")
 
     def frame_stats(self) -> FrameStats:
         st = FrameStats()
