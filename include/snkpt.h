@@ -222,6 +222,29 @@ snkpt_status snkpt_trace_counted(snkpt_ctx* ctx, const snkpt_rays* rays, uint32_
                                  const snkpt_hits* hits, int any_hit, uint32_t* node_visits,
                                  uint32_t* tri_visits, snkpt_stream stream);
 
+/* Contact record of a short collision ray (ParticleUpdate.rchit:38-62), one per
+ * ray, device arrays.  Miss: t = -1, zero vectors, ids SNKPT_INVALID_INDEX. */
+typedef struct snkpt_contacts {
+    float* t;          /* gl_RayTmaxEXT at the first hit found */
+    float* normal;     /* float4 (xyz, 0): normalize(transpose(inverse(mat3(M))) * interpolated normal) */
+    float* position;   /* float4: origin + direction * t */
+    float* velocity;   /* float4: (position - M_prev * interpolated object-space point) / timestep */
+    uint32_t* inst;    /* may be NULL */
+    uint32_t* prim;    /* may be NULL */
+} snkpt_contacts;
+
+/* ParticleUpdate.rgen:92-167 (TraceCollisionRay, ConstrainOffsetWithScene,
+ * CalculateGravity): rays traced with gl_RayFlagsTerminateOnFirstHitEXT (the
+ * any-hit traversal of snkpt_trace, same first hit) and the contact record
+ * of ParticleUpdate.rchit.  Rays as for snkpt_trace (org_tmin, dir_tmax). */
+snkpt_status snkpt_trace_contacts(snkpt_ctx* ctx, const snkpt_rays* rays, uint32_t n, const snkpt_contacts* out,
+                                  float timestep, snkpt_stream stream);
+/* transforms_prev_frame (TransformBufferSystem.h:92-122): the previous frame's
+ * column-major mat4s, same count as snkpt_set_instances' (host array, copied).
+ * Until set, and after a new transform set, the current transforms are used
+ * (zero velocity for static geometry). */
+snkpt_status snkpt_set_prev_transforms(snkpt_ctx* ctx, const float* mat4s, uint32_t n_mat4);
+
 /* ---- the pass -------------------------------------------------------------
  * RT::RecordRenderCmdBuf (Raytracing.cpp:227-267): renders one frame of
  * settings->spp samples per pixel into out_rgba (device, w*h RGBA, row-major,

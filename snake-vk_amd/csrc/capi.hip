@@ -191,6 +191,47 @@ snkpt_status snkpt_set_lights(snkpt_ctx* ctx, const void* light_block, size_t by
     return SNKPT_OK;
 }
 
+// Per instance: the previous frame's object->world rows (3 x float4) for
+// contact velocities (transforms_prev_frame, ParticleUpdate.rchit:56-62); the
+// current transforms until snkpt_set_prev_transforms supplies others.
+static snkpt_status upload_prev_xf(snkpt_ctx* ctx) {
+    const std::vector<float>& m = ctx->prev_mat4s.size() == ctx->mat4s.size() ? ctx->prev_mat4s : ctx->mat4s;
+    std::vector<float> xf(12 * std::max<size_t>(1, ctx->inst.size()), 0.0f);
+    const size_t n_mat4 = m.size() / 16;
+    for (size_t i = 0; i < ctx->inst.size(); ++i) {
+        const uint32_t ti = ctx->inst[i].transform_idx;
+        if (ti >= n_mat4) return fail(ctx, SNKPT_ERR_INVALID_ARG, "instance " + std::to_string(i) + ": transform_idx out of range");
+        const float* a = &m[16 * (size_t)ti];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 4; ++c) xf[12 * i + 4 * r + c] = a[4 * c + r];
+    }
+    return upload(ctx, ctx->d_prev_xf, xf.data(), xf.size() * 4);
+}
+
+snkpt_status snkpt_set_prev_transforms(snkpt_ctx* ctx, const float* mat4s, uint32_t n_mat4) {
+    if (!ctx) return SNKPT_ERR_INVALID_ARG;
+    if (n_mat4 && !mat4s) return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_set_prev_transforms: null array");
+    if ((size_t)n_mat4 * 16 != ctx->mat4s.size())
+        return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_set_prev_transforms: count differs from the current transforms");
+    if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, SNKPT_ERR_HIP, "hipSetDevice");
+    ctx->prev_mat4s.assign(mat4s, mat4s + 16 * (size_t)n_mat4);
+    return upload_prev_xf(ctx);
+}
+
+snkpt_status snkpt_trace_contacts(snkpt_ctx* ctx, const snkpt_rays* rays, uint32_t n, const snkpt_contacts* out,
+                                  float timestep, snkpt_stream stream) {
+    if (!ctx) return SNKPT_ERR_INVALID_ARG;
+    if (!rays || !out || (n && (!rays->org_tmin || !rays->dir_tmax || !out->t || !out->normal || !out->position ||
+                                !out->velocity)))
+        return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_trace_contacts: null rays or contact arrays");
+    if (!(timestep > 0.0f)) return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_trace_contacts: timestep must be > 0");
+    if ((((uintptr_t)rays->org_tmin | (uintptr_t)rays->dir_tmax | (uintptr_t)out->normal | (uintptr_t)out->position |
+          (uintptr_t)out->velocity) & 15u) != 0)
+        return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_trace_contacts: float4 arrays must be 16-byte aligned");
+    return trace_contacts(ctx, (const float4*)rays->org_tmin, (const float4*)rays->dir_tmax, n, *out, timestep,
+                          (hipStream_t)stream);
+}
+
 snkpt_status snkpt_build_accel(snkpt_ctx* ctx, snkpt_stream stream) {
     if (!ctx) return SNKPT_ERR_INVALID_ARG;
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, SNKPT_ERR_HIP, "hipSetDevice");
@@ -213,6 +254,8 @@ snkpt_status snkpt_build_accel(snkpt_ctx* ctx, snkpt_stream stream) {
         instance_xf(&ctx->mat4s[16 * (size_t)ctx->inst[i].transform_idx], &xf[24 * i]);
     }
     if ((s = upload(ctx, ctx->d_inst_xf, xf.data(), xf.size() * 4)) != SNKPT_OK) return s;
+    if (ctx->prev_mat4s.size() != ctx->mat4s.size()) ctx->prev_mat4s.clear();  // stale after a new transform set
+    if ((s = upload_prev_xf(ctx)) != SNKPT_OK) return s;
     ctx->dirty = false;
     return build_accel(ctx, st);
 }

@@ -100,6 +100,8 @@ struct snkpt_ctx {
     // device copies
     snkpt::DeviceBuffer d_pos, d_nrm, d_uv, d_idx, d_inst, d_mat4, d_mats;
     snkpt::DeviceBuffer d_inst_xf;  // per instance: 3x4 object->world + 3x3 normal matrix (float x 24)
+    std::vector<float> prev_mat4s;  // previous frame's transforms (contact velocities); empty = current
+    snkpt::DeviceBuffer d_prev_xf;  // per instance: previous 3x4 object->world (float x 12)
     bool dirty = true;
 
     snkpt::Accel accel;
@@ -137,6 +139,10 @@ snkpt_status oracle_accumulate(snkpt_ctx* ctx, const snkpt_camera* cam, const sn
 void render_state_free(RenderState* r);
 void cache_state_free(CacheState* c);
 void taa_state_free(TaaState* t);
+
+// contacts.hip
+snkpt_status trace_contacts(snkpt_ctx* ctx, const float4* org_tmin, const float4* dir_tmax, uint32_t n,
+                            const snkpt_contacts& out, float timestep, hipStream_t st);
 
 // taa.hip
 snkpt_status taa_resolve(snkpt_ctx* ctx, const void* current, void* out, const uint16_t* velocity, uint32_t w,

@@ -15,14 +15,6 @@ namespace {
 
 constexpr int kTraceBlock = 128;
 
-// Spreads consecutive logical blocks over few XCDs: hardware block b runs on
-// XCD b % 8, so logical blocks [x*q + min(x, r), ...) all land on XCD x and
-// neighbouring rays share that XCD's L2.
-__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
-    uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u, j = b >> 3;
-    return x * q + (x < r ? x : r) + j;
-}
-
 template <bool ANY, bool COUNT, int STACK>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(const float4* __restrict__ org_tmin,
                                                        const float4* __restrict__ dir_tmax, uint32_t n,

@@ -49,7 +49,7 @@ EXPORTS = [
     "snkpt_cache_set_params", "snkpt_scene_view", "snkpt_batch_create", "snkpt_batch_destroy",
     "snkpt_batch_last_error", "snkpt_batch_add_scene", "snkpt_batch_add_view", "snkpt_batch_render",
     "snkpt_batch_autotune", "snkpt_batch_set_tuning", "snkpt_batch_get_tuning", "snkpt_batch_frame_stats",
-    "snkpt_render_gbuffer", "snkpt_taa_resolve",
+    "snkpt_render_gbuffer", "snkpt_taa_resolve", "snkpt_trace_contacts", "snkpt_set_prev_transforms",
 ]
 CACHE_SAMPLE_FLOATS = 16
 CACHE_MLP_PARAMS = 4 * 4096 + 16 * 64
@@ -110,6 +110,12 @@ class Gbuffer(C.Structure):
     """snkpt_gbuffer: device pointers of the rasterized G-buffer images."""
     _fields_ = [("depth", C.c_void_p), ("normal", C.c_void_p), ("albedo", C.c_void_p), ("rma", C.c_void_p),
                 ("mat_flags", C.c_void_p), ("motion", C.c_void_p)]
+
+
+class Contacts(C.Structure):
+    """snkpt_contacts: device arrays of the contact records."""
+    _fields_ = [("t", C.c_void_p), ("normal", C.c_void_p), ("position", C.c_void_p), ("velocity", C.c_void_p),
+                ("inst", C.c_void_p), ("prim", C.c_void_p)]
 
 
 class BatchTuning(C.Structure):
@@ -185,6 +191,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "snkpt_batch_frame_stats": (I32, [VP, P(FrameStats), VP]),
         "snkpt_render_gbuffer": (I32, [VP, P(Camera), P(Settings), P(Gbuffer), VP, U32, U32, U32, VP]),
         "snkpt_taa_resolve": (I32, [VP, VP, VP, VP, U32, U32, U32, U32, VP]),
+        "snkpt_trace_contacts": (I32, [VP, P(Rays), U32, P(Contacts), C.c_float, VP]),
+        "snkpt_set_prev_transforms": (I32, [VP, VP, U32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -480,6 +488,31 @@ class Context:
         rc = self.lib.snkpt_render_gbuffer(self._h, C.byref(cam), C.byref(settings), C.byref(g), _ptr(out), w, h,
                                            fmt, _stream(stream))
         self._check(rc, "snkpt_render_gbuffer")
+
+    def trace_contacts(self, org_tmin, dir_tmax, timestep: float = 1.0 / 120.0, stream=None):
+        """Terminate-on-first-hit collision rays with ParticleUpdate.rchit's contact
+        record.  org_tmin / dir_tmax: (n, 4) float32 device tensors.  Returns
+        (t, normal, position, velocity, inst, prim) device tensors."""
+        import torch
+        n = org_tmin.shape[0]
+        dev = org_tmin.device
+        t = torch.empty(n, dtype=torch.float32, device=dev)
+        nrm = torch.empty(n, 4, dtype=torch.float32, device=dev)
+        pos = torch.empty(n, 4, dtype=torch.float32, device=dev)
+        vel = torch.empty(n, 4, dtype=torch.float32, device=dev)
+        inst = torch.empty(n, dtype=torch.int32, device=dev)
+        prim = torch.empty(n, dtype=torch.int32, device=dev)
+        rays = Rays(org_tmin.data_ptr(), dir_tmax.data_ptr())
+        c = Contacts(t.data_ptr(), nrm.data_ptr(), pos.data_ptr(), vel.data_ptr(), inst.data_ptr(), prim.data_ptr())
+        self._check(self.lib.snkpt_trace_contacts(self._h, C.byref(rays), n, C.byref(c), timestep, _stream(stream)),
+                    "snkpt_trace_contacts")
+        return t, nrm, pos, vel, inst, prim
+
+    def set_prev_transforms(self, mat4s):
+        import numpy as np
+        m = np.ascontiguousarray(mat4s, np.float32).reshape(-1)
+        self._check(self.lib.snkpt_set_prev_transforms(self._h, m.ctypes.data_as(C.c_void_p), m.size // 16),
+                    "snkpt_set_prev_transforms")
 
     def taa_resolve(self, current, out, velocity, w: int, h: int, fmt: int = FORMAT_RGBA32F, reset: bool = False,
                     stream=None):
