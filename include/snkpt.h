@@ -245,6 +245,28 @@ snkpt_status snkpt_trace_contacts(snkpt_ctx* ctx, const snkpt_rays* rays, uint32
  * (zero velocity for static geometry). */
 snkpt_status snkpt_set_prev_transforms(snkpt_ctx* ctx, const float* mat4s, uint32_t n_mat4);
 
+/* ---- Vulkan interop ----------------------------------------------------------
+ * The reference's output image and scene buffers are Vulkan memory
+ * (Raytracing.cpp:96-147, S_VkBuffer.cpp:5-22).  Exported with
+ * VK_KHR_external_memory_fd (vkGetMemoryFdKHR, OPAQUE_FD; the allocation made
+ * with VkExportMemoryAllocateInfo), a range of it is imported as a device
+ * pointer for every entry point above and below.  On success the library owns
+ * the fd (do not close it); release the mapping before freeing the Vulkan
+ * memory.  A linear-tiled image's rows are tightly packed only when its row
+ * pitch is w * texel size. */
+typedef struct snkpt_external snkpt_external;
+snkpt_status snkpt_import_memory_fd(snkpt_ctx* ctx, int fd, uint64_t allocation_size, uint64_t offset,
+                                    uint64_t size, void** out_dev_ptr, snkpt_external** out);
+snkpt_status snkpt_release_memory(snkpt_ctx* ctx, snkpt_external* mem);
+/* VK_KHR_external_semaphore_fd (OPAQUE_FD; timeline != 0 for a timeline
+ * semaphore): the frame fence between the Vulkan queue and the HIP stream.
+ * wait / signal are enqueued on the stream (value: timeline semaphores only). */
+typedef struct snkpt_semaphore snkpt_semaphore;
+snkpt_status snkpt_import_semaphore_fd(snkpt_ctx* ctx, int fd, int timeline, snkpt_semaphore** out);
+snkpt_status snkpt_semaphore_wait(snkpt_ctx* ctx, snkpt_semaphore* sem, uint64_t value, snkpt_stream stream);
+snkpt_status snkpt_semaphore_signal(snkpt_ctx* ctx, snkpt_semaphore* sem, uint64_t value, snkpt_stream stream);
+snkpt_status snkpt_release_semaphore(snkpt_ctx* ctx, snkpt_semaphore* sem);
+
 /* ---- the pass -------------------------------------------------------------
  * RT::RecordRenderCmdBuf (Raytracing.cpp:227-267): renders one frame of
  * settings->spp samples per pixel into out_rgba (device, w*h RGBA, row-major,

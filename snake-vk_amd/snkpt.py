@@ -50,6 +50,8 @@ EXPORTS = [
     "snkpt_batch_last_error", "snkpt_batch_add_scene", "snkpt_batch_add_view", "snkpt_batch_render",
     "snkpt_batch_autotune", "snkpt_batch_set_tuning", "snkpt_batch_get_tuning", "snkpt_batch_frame_stats",
     "snkpt_render_gbuffer", "snkpt_taa_resolve", "snkpt_trace_contacts", "snkpt_set_prev_transforms",
+    "snkpt_import_memory_fd", "snkpt_release_memory", "snkpt_import_semaphore_fd", "snkpt_semaphore_wait",
+    "snkpt_semaphore_signal", "snkpt_release_semaphore",
 ]
 CACHE_SAMPLE_FLOATS = 16
 CACHE_MLP_PARAMS = 4 * 4096 + 16 * 64
@@ -193,6 +195,12 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "snkpt_taa_resolve": (I32, [VP, VP, VP, VP, U32, U32, U32, U32, VP]),
         "snkpt_trace_contacts": (I32, [VP, P(Rays), U32, P(Contacts), C.c_float, VP]),
         "snkpt_set_prev_transforms": (I32, [VP, VP, U32]),
+        "snkpt_import_memory_fd": (I32, [VP, C.c_int, C.c_uint64, C.c_uint64, C.c_uint64, P(VP), P(VP)]),
+        "snkpt_release_memory": (I32, [VP, VP]),
+        "snkpt_import_semaphore_fd": (I32, [VP, C.c_int, C.c_int, P(VP)]),
+        "snkpt_semaphore_wait": (I32, [VP, VP, C.c_uint64, VP]),
+        "snkpt_semaphore_signal": (I32, [VP, VP, C.c_uint64, VP]),
+        "snkpt_release_semaphore": (I32, [VP, VP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -507,6 +515,24 @@ class Context:
         self._check(self.lib.snkpt_trace_contacts(self._h, C.byref(rays), n, C.byref(c), timestep, _stream(stream)),
                     "snkpt_trace_contacts")
         return t, nrm, pos, vel, inst, prim
+
+    # ---- Vulkan interop
+    def import_memory_fd(self, fd: int, allocation_size: int, offset: int, size: int):
+        """Imports a range of an exported (OPAQUE_FD) allocation; returns
+        (device pointer as int, handle).  The library owns fd on success."""
+        ptr, h = C.c_void_p(), C.c_void_p()
+        self._check(self.lib.snkpt_import_memory_fd(self._h, fd, allocation_size, offset, size, C.byref(ptr),
+                                                    C.byref(h)), "snkpt_import_memory_fd")
+        return ptr.value, h
+
+    def release_memory(self, handle):
+        self._check(self.lib.snkpt_release_memory(self._h, handle), "snkpt_release_memory")
+
+    def import_semaphore_fd(self, fd: int, timeline: bool = False):
+        h = C.c_void_p()
+        self._check(self.lib.snkpt_import_semaphore_fd(self._h, fd, int(timeline), C.byref(h)),
+                    "snkpt_import_semaphore_fd")
+        return h
 
     def set_prev_transforms(self, mat4s):
         import numpy as np
