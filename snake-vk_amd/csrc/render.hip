@@ -137,7 +137,7 @@ __global__ void k_gbuf_stats(const uint32_t* c, unsigned long long* stats) {
 // Shadow rays: persistent any-hit traversal with ray refill; each wave owns a
 // contiguous slice of the shadow queue.
 template <int STACK>
-__global__ __launch_bounds__(kShadowBlock) void k_shadow(const uint4* __restrict__ nodes,
+__global__ __launch_bounds__(kShadowBlock) __attribute__((amdgpu_waves_per_eu(STACK <= 10 ? 8 : 1, 8))) void k_shadow(const uint4* __restrict__ nodes,
                                                          const float4* __restrict__ tris, ShadowQueue sq,
                                                          const uint32_t* __restrict__ counts,
                                                          float4* __restrict__ rad) {
@@ -376,9 +376,15 @@ snkpt_status launch_shadow(snkpt_ctx* ctx, RenderState& R, float4* rad, hipStrea
     uint32_t need = A.depth > 0 ? A.depth - 1 : 0;
     const uint32_t cus = ctx->props.multiProcessorCount > 0 ? (uint32_t)ctx->props.multiProcessorCount : 256u;
     // persistent grid: as many blocks as the LDS of the stack size lets stay resident
-    const uint32_t resident = need <= 16 ? 10u : need <= 32 ? 5u : 2u;
+    const uint32_t resident = need <= 10 ? 16u : need <= 12 ? 13u : need <= 16 ? 10u : need <= 32 ? 5u : 2u;
     uint32_t grid = std::min(nblk(n_max ? n_max : R.w * R.h, kShadowBlock), cus * resident);
-    if (need <= 16)
+    if (need <= 10)
+        k_shadow<10><<<grid, kShadowBlock, 0, st>>>(A.nodes.as<uint4>(), A.leaf_tris.as<float4>(), shadow_of(R),
+                                                     R.counts.as<uint32_t>(), rad);
+    else if (need <= 12)
+        k_shadow<12><<<grid, kShadowBlock, 0, st>>>(A.nodes.as<uint4>(), A.leaf_tris.as<float4>(), shadow_of(R),
+                                                     R.counts.as<uint32_t>(), rad);
+    else if (need <= 16)
         k_shadow<16><<<grid, kShadowBlock, 0, st>>>(A.nodes.as<uint4>(), A.leaf_tris.as<float4>(), shadow_of(R),
                                                      R.counts.as<uint32_t>(), rad);
     else if (need <= 32)

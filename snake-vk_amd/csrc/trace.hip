@@ -78,7 +78,7 @@ __global__ void k_trace_empty(uint32_t n, const uint32_t* __restrict__ count_dev
 // contiguous slice of the rays and refills its idle lanes from it (see
 // refill_trace).  No atomics: a wave's cursor is uniform and private to it.
 template <bool ANY, bool COUNT, int STACK, int REFILL>
-__global__ __launch_bounds__(kTraceBlock) void k_trace_refill(const float4* __restrict__ org_tmin,
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(STACK <= 10 ? 8 : 1, 8))) void k_trace_refill(const float4* __restrict__ org_tmin,
                                                               const float4* __restrict__ dir_tmax, uint32_t n,
                                                               const uint32_t* __restrict__ count_dev,
                                                               const uint4* __restrict__ nodes,
@@ -128,7 +128,12 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_refill(const float4* __re
 }
 
 constexpr int kRefill = 16;
-constexpr uint32_t kRefillBlocksPerCU = 10;  // LDS of STACK 16: 10 blocks of 128 lanes per CU
+// Resident 128-lane blocks per CU allowed by the 160 KB of LDS: a lane's stack
+// takes STACK x 8 B (10: 16 blocks, 12: 13, 16: 10, 32: 5, 64: 2).  The 10-entry
+// variant is held to 64 VGPRs so that 8 waves per SIMD fit (no spills).
+inline uint32_t refill_blocks_per_cu(uint32_t need) {
+    return need <= 10 ? 16u : need <= 12 ? 13u : need <= 16 ? 10u : need <= 32 ? 5u : 2u;
+}
 
 template <bool ANY, bool COUNT>
 void launch_stack(uint32_t stack, uint32_t grid, uint32_t pgrid, hipStream_t st, const float4* o, const float4* d,
@@ -136,7 +141,13 @@ void launch_stack(uint32_t stack, uint32_t grid, uint32_t pgrid, hipStream_t st,
                   uint32_t* nv, uint32_t* tv) {
     // persistent refill launch, its grid capped by the (maximum) ray count
     const uint32_t g = std::min(pgrid, grid);
-    if (stack <= 16)
+    if (stack <= 10)
+        k_trace_refill<ANY, COUNT, 10, kRefill><<<g, kTraceBlock, 0, st>>>(o, d, n, cnt, nodes, tris, h.t, h.inst,
+                                                                          h.prim, h.u, h.v, nv, tv);
+    else if (stack <= 12)
+        k_trace_refill<ANY, COUNT, 12, kRefill><<<g, kTraceBlock, 0, st>>>(o, d, n, cnt, nodes, tris, h.t, h.inst,
+                                                                          h.prim, h.u, h.v, nv, tv);
+    else if (stack <= 16)
         k_trace_refill<ANY, COUNT, 16, kRefill><<<g, kTraceBlock, 0, st>>>(o, d, n, cnt, nodes, tris, h.t, h.inst,
                                                                           h.prim, h.u, h.v, nv, tv);
     else if (stack <= 32)
@@ -165,7 +176,7 @@ snkpt_status launch_trace(snkpt_ctx* ctx, const float4* org_tmin, const float4* 
         if (need > 64) return fail(ctx, SNKPT_ERR_CAPACITY, "trace: BVH8 deeper than the 64-entry traversal stack");
         bool count = node_visits || tri_visits;
         const uint32_t cus = ctx->props.multiProcessorCount > 0 ? (uint32_t)ctx->props.multiProcessorCount : 256u;
-        const uint32_t pgrid = cus * (need <= 16 ? kRefillBlocksPerCU : need <= 32 ? 5u : 2u);
+        const uint32_t pgrid = cus * refill_blocks_per_cu(need);
         const uint4* nodes = A.nodes.as<uint4>();
         const float4* tris = A.leaf_tris.as<float4>();
         if (any_hit) {
