@@ -118,7 +118,7 @@ __global__ void k_bbounce_end(uint32_t* c, unsigned long long* stats) {
 // Closest hit of the queued rays, each against its own scene's BVH8:
 // persistent waves with ray refill over a contiguous slice each.
 template <int STACK, int TB>
-__global__ __launch_bounds__(TB) void k_btrace(SceneMany ss, const float4* __restrict__ org,
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(STACK <= 10 ? 8 : 1, 8))) void k_btrace(SceneMany ss, const float4* __restrict__ org,
                                                const float4* __restrict__ dir, const uint32_t* __restrict__ q_path,
                                                const uint32_t* __restrict__ count, HitArrays hit) {
     __shared__ uint2 stk[STACK * TB];
@@ -233,7 +233,7 @@ __global__ void k_bq_apply(const float4* __restrict__ meta, const float4* __rest
 }
 
 template <int STACK, int TB>
-__global__ __launch_bounds__(TB) void k_bshadow(SceneMany ss, ShadowQueue sq, const uint32_t* __restrict__ counts,
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(STACK <= 10 ? 8 : 1, 8))) void k_bshadow(SceneMany ss, ShadowQueue sq, const uint32_t* __restrict__ counts,
                                                 float4* __restrict__ rad) {
     __shared__ uint2 stk[STACK * TB];
     const uint32_t n = counts[2];
@@ -345,9 +345,11 @@ snkpt_status bfail(snkpt_batch* B, snkpt_status code, const std::string& msg) {
             return bfail(B, SNKPT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// Smallest traversal stack for the deepest scene: 10 and 12 entries let 16
+// and 13 blocks of 128 lanes share a CU's 160 KB of LDS, against 10 for 16.
 uint32_t stack_for(uint32_t depth) {
     uint32_t need = depth > 0 ? depth - 1 : 0;
-    return need <= 16 ? 16u : need <= 32 ? 32u : 64u;
+    return need <= 10 ? 10u : need <= 12 ? 12u : need <= 16 ? 16u : need <= 32 ? 32u : 64u;
 }
 
 // Scene table, view table (group-relative path offsets), groups, buffers.
@@ -455,7 +457,11 @@ void launch_traces(snkpt_batch* B, const SceneMany& ss, int cur, uint32_t grid, 
 
 template <int TB>
 void launch_traces_tb(snkpt_batch* B, const SceneMany& ss, int cur, uint32_t grid, hipStream_t st, bool shadow) {
-    if (B->stack <= 16)
+    if (B->stack <= 10)
+        launch_traces<10, TB>(B, ss, cur, grid, st, shadow);
+    else if (B->stack <= 12)
+        launch_traces<12, TB>(B, ss, cur, grid, st, shadow);
+    else if (B->stack <= 16)
         launch_traces<16, TB>(B, ss, cur, grid, st, shadow);
     else if (B->stack <= 32)
         launch_traces<32, TB>(B, ss, cur, grid, st, shadow);
