@@ -803,11 +803,17 @@ snkpt_status build_accel(snkpt_ctx* ctx, hipStream_t st) {
             return fail(ctx, SNKPT_ERR_INVALID_ARG, "instance " + std::to_string(i) + ": transform_idx out of range");
         if (in.material_idx >= ctx->mats.size())
             return fail(ctx, SNKPT_ERR_INVALID_ARG, "instance " + std::to_string(i) + ": material_idx out of range");
-        for (uint32_t k = 0; k < in.num_mesh_indices; ++k) {
-            uint64_t v = (uint64_t)ctx->idx[in.mesh_buffer_index_offset + k] + in.mesh_buffer_vertex_offset;
-            if (v >= n_verts)
-                return fail(ctx, SNKPT_ERR_INVALID_ARG, "instance " + std::to_string(i) + ": vertex index out of range");
+        // largest index of the instance's index range, computed once per range
+        // until the meshes change (a per-frame rebuild does not rescan them)
+        const uint64_t key = ((uint64_t)in.mesh_buffer_index_offset << 32) | in.num_mesh_indices;
+        auto it = ctx->range_max.find(key);
+        if (it == ctx->range_max.end()) {
+            uint32_t mx = 0;
+            for (uint32_t k = 0; k < in.num_mesh_indices; ++k) mx = std::max(mx, ctx->idx[in.mesh_buffer_index_offset + k]);
+            it = ctx->range_max.emplace(key, mx).first;
         }
+        if (in.num_mesh_indices && (uint64_t)it->second + in.mesh_buffer_vertex_offset >= n_verts)
+            return fail(ctx, SNKPT_ERR_INVALID_ARG, "instance " + std::to_string(i) + ": vertex index out of range");
         off[i] = (uint32_t)total;
         total += in.num_mesh_indices / 3;
     }
@@ -838,8 +844,11 @@ snkpt_status build_accel(snkpt_ctx* ctx, hipStream_t st) {
     HIP_CHECK(A.bin_aabb.reserve((2 * nn) * 24));
     HIP_CHECK(A.bin_leafify.reserve(nn * 4));
 
-    DeviceBuffer keys2, ids2, hist, hist_scan, sums, bounds, first, count, pint, pleaf, flags, up_live, up_cnt, cost,
-        height, slot_child, nint, ntri, sint, stri, work, totals;
+    DeviceBuffer &keys2 = A.keys2, &ids2 = A.ids2, &hist = A.hist, &hist_scan = A.hist_scan, &sums = A.sums,
+                 &bounds = A.bounds, &first = A.first, &count = A.count, &pint = A.pint, &pleaf = A.pleaf,
+                 &flags = A.flags, &up_live = A.up_live, &up_cnt = A.up_cnt, &cost = A.cost, &height = A.height,
+                 &slot_child = A.slot_child, &nint = A.nint, &ntri = A.ntri, &sint = A.sint, &stri = A.stri,
+                 &work = A.work, &totals = A.totals;
     const uint32_t nb_sort = blocks(n, kSortTile);
     HIP_CHECK(keys2.reserve(nn * 4));
     HIP_CHECK(ids2.reserve(nn * 4));

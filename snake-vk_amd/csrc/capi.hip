@@ -130,6 +130,8 @@ snkpt_status snkpt_upload_mesh(snkpt_ctx* ctx, const float* pos, const float* nr
     if (out_vertex_offset) *out_vertex_offset = (uint32_t)vo;
     if (out_index_offset) *out_index_offset = (uint32_t)io;
     ctx->dirty = true;
+    ctx->mesh_dirty = true;
+    ctx->range_max.clear();
     ctx->accel.built = false;
     return SNKPT_OK;
 }
@@ -144,6 +146,7 @@ snkpt_status snkpt_set_instances(snkpt_ctx* ctx, const snkpt_instance* inst, uin
     ctx->inst.assign(inst, inst + n);
     ctx->mat4s.assign(mat4s, mat4s + 16 * (size_t)n_mat4);
     ctx->dirty = true;
+    ctx->inst_dirty = true;
     ctx->accel.built = false;
     return SNKPT_OK;
 }
@@ -157,6 +160,7 @@ snkpt_status snkpt_set_materials(snkpt_ctx* ctx, const snkpt_material* m, uint32
             return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_set_materials: textures are not supported on this path");
     ctx->mats.assign(m, m + n);
     ctx->dirty = true;
+    ctx->mats_dirty = true;
     ctx->accel.built = false;
     return SNKPT_OK;
 }
@@ -237,15 +241,25 @@ snkpt_status snkpt_build_accel(snkpt_ctx* ctx, snkpt_stream stream) {
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, SNKPT_ERR_HIP, "hipSetDevice");
     hipStream_t st = (hipStream_t)stream;
     snkpt_status s;
-    if ((s = upload(ctx, ctx->d_pos, ctx->pos.data(), ctx->pos.size() * 4)) != SNKPT_OK) return s;
-    if ((s = upload(ctx, ctx->d_nrm, ctx->nrm.data(), ctx->nrm.size() * 4)) != SNKPT_OK) return s;
-    if ((s = upload(ctx, ctx->d_uv, ctx->uv.data(), ctx->uv.size() * 4)) != SNKPT_OK) return s;
-    if ((s = upload(ctx, ctx->d_idx, ctx->idx.data(), ctx->idx.size() * 4)) != SNKPT_OK) return s;
-    if ((s = upload(ctx, ctx->d_inst, ctx->inst.data(), ctx->inst.size() * sizeof(snkpt_instance))) != SNKPT_OK)
-        return s;
-    if ((s = upload(ctx, ctx->d_mat4, ctx->mat4s.data(), ctx->mat4s.size() * 4)) != SNKPT_OK) return s;
-    if ((s = upload(ctx, ctx->d_mats, ctx->mats.data(), ctx->mats.size() * sizeof(snkpt_material))) != SNKPT_OK)
-        return s;
+    // only what changed since the last build is uploaded again (a per-frame
+    // rebuild after new instance transforms moves 64 B per transform, not the meshes)
+    if (ctx->mesh_dirty) {
+        if ((s = upload(ctx, ctx->d_pos, ctx->pos.data(), ctx->pos.size() * 4)) != SNKPT_OK) return s;
+        if ((s = upload(ctx, ctx->d_nrm, ctx->nrm.data(), ctx->nrm.size() * 4)) != SNKPT_OK) return s;
+        if ((s = upload(ctx, ctx->d_uv, ctx->uv.data(), ctx->uv.size() * 4)) != SNKPT_OK) return s;
+        if ((s = upload(ctx, ctx->d_idx, ctx->idx.data(), ctx->idx.size() * 4)) != SNKPT_OK) return s;
+        ctx->mesh_dirty = false;
+    }
+    if (ctx->inst_dirty) {
+        if ((s = upload(ctx, ctx->d_inst, ctx->inst.data(), ctx->inst.size() * sizeof(snkpt_instance))) != SNKPT_OK)
+            return s;
+        if ((s = upload(ctx, ctx->d_mat4, ctx->mat4s.data(), ctx->mat4s.size() * 4)) != SNKPT_OK) return s;
+    }
+    if (ctx->mats_dirty) {
+        if ((s = upload(ctx, ctx->d_mats, ctx->mats.data(), ctx->mats.size() * sizeof(snkpt_material))) != SNKPT_OK)
+            return s;
+        ctx->mats_dirty = false;
+    }
     std::vector<float> xf(24 * ctx->inst.size());
     const uint32_t n_mat4 = (uint32_t)(ctx->mat4s.size() / 16);
     for (size_t i = 0; i < ctx->inst.size(); ++i) {
@@ -256,6 +270,7 @@ snkpt_status snkpt_build_accel(snkpt_ctx* ctx, snkpt_stream stream) {
     if ((s = upload(ctx, ctx->d_inst_xf, xf.data(), xf.size() * 4)) != SNKPT_OK) return s;
     if (ctx->prev_mat4s.size() != ctx->mat4s.size()) ctx->prev_mat4s.clear();  // stale after a new transform set
     if ((s = upload_prev_xf(ctx)) != SNKPT_OK) return s;
+    ctx->inst_dirty = false;
     ctx->dirty = false;
     return build_accel(ctx, st);
 }
@@ -414,6 +429,7 @@ snkpt_status snkpt_scene_view(uint32_t config, uint32_t view, uint32_t* out_vari
 
 snkpt_status snkpt_scene_load(snkpt_ctx* ctx, const snkpt_scene* s) {
     if (!ctx || !s) return SNKPT_ERR_INVALID_ARG;
+    ctx->range_max.clear();
     ctx->pos.clear();
     ctx->nrm.clear();
     ctx->uv.clear();

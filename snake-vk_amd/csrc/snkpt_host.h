@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "snkpt.h"
@@ -60,6 +61,9 @@ struct Accel {
     DeviceBuffer bin_aabb;    // internal then leaf AABBs, 6 floats each
     DeviceBuffer bin_leafify; // u32 per internal node
     DeviceBuffer inst_tri_offset;  // per instance: first global triangle id (n_inst + 1 entries)
+    // build scratch, kept between builds so a per-frame rebuild allocates nothing
+    DeviceBuffer keys2, ids2, hist, hist_scan, sums, bounds, first, count, pint, pleaf, flags, up_live, up_cnt, cost,
+        height, slot_child, nint, ntri, sint, stri, work, totals;
 };
 
 // Emitter table: one entry per emissive triangle, sampled proportionally to power.
@@ -103,6 +107,10 @@ struct snkpt_ctx {
     std::vector<float> prev_mat4s;  // previous frame's transforms (contact velocities); empty = current
     snkpt::DeviceBuffer d_prev_xf;  // per instance: previous 3x4 object->world (float x 12)
     bool dirty = true;
+    // host arrays changed since their last upload (build_accel uploads only these)
+    bool mesh_dirty = true, inst_dirty = true, mats_dirty = true;
+    // largest index per (index offset, count) range, for validating instances; cleared with the meshes
+    std::unordered_map<uint64_t, uint32_t> range_max;
 
     snkpt::Accel accel;
     snkpt::Emitters emit;
