@@ -36,7 +36,16 @@ struct CacheState {
     uint32_t last_n = 0;
     // render integration
     DeviceBuffer q_samples, q_meta, train_vtx, train_meta, t_samples, counts;
+    DeviceBuffer tcounts, tstats;  // the training paths' own counters and ray statistics
     uint32_t rcap = 0;
+    // side stream on which a frame's training paths are traced while the render pass runs
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    ~CacheState() {
+        if (side) (void)hipStreamDestroy(side);
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
+        if (join_ev) (void)hipEventDestroy(join_ev);
+    }
 };
 void cache_state_free(CacheState* c) { delete c; }
 
@@ -819,9 +828,11 @@ snkpt_status cache_params(snkpt_ctx* ctx, uint32_t which, void* dst, size_t byte
         case 6:
         case 7: {
             // query / training records of the last cache-mode pass; counts live on the device
+            // (queries: counts[0]; training samples: tcounts[1])
             uint32_t cnt[2] = {0, 0};
-            if (C.counts.ptr) {
-                HIPC(hipMemcpyAsync(cnt, C.counts.ptr, 8, hipMemcpyDeviceToHost, st));
+            const DeviceBuffer& cb = which == 7 ? C.tcounts : C.counts;
+            if (cb.ptr) {
+                HIPC(hipMemcpyAsync(cnt, cb.ptr, 8, hipMemcpyDeviceToHost, st));
                 HIPC(hipStreamSynchronize(st));
             }
             sz = which == 7 ? (size_t)cnt[1] * 64 : (size_t)cnt[0] * (which == 5 ? 64 : 16);
@@ -1027,14 +1038,20 @@ snkpt_status cache_query(snkpt_ctx* ctx, const float* samples, uint32_t n, float
 // Training half of a cache frame: n_paths full paths through a strided pixel
 // subset of a w x h view of camera cd record their vertices; the thinned
 // vertices become one optimisation step (Adam, weight averages of decay cache_ema).
-snkpt_status cache_train_frame(snkpt_ctx* ctx, const CamDev& cd, uint32_t w, uint32_t h, const snkpt_settings* s,
+// Training paths of a frame and their samples (no network involved, no host
+// synchronisation): full-length unbiased paths through a strided pixel subset,
+// recorded vertices thinned and emitted in path order into t_samples; the
+// sample count lands in tcounts[1].
+snkpt_status cache_train_paths(snkpt_ctx* ctx, const CamDev& cd, uint32_t w, uint32_t h, const snkpt_settings* s,
                                uint64_t initstate, uint32_t n_paths, unsigned long long* stats, hipStream_t st) {
     CacheState& C = *ctx->cache;
     const uint32_t n = w * h;
     const uint32_t depth = s->max_depth ? s->max_depth : 1u;
     const uint32_t NT = std::min<uint32_t>(n_paths, n);
+    HIPC(C.tcounts.reserve(64));
+    uint32_t* qc = C.tcounts.as<uint32_t>();
+    k_train_clear<<<1, 64, 0, st>>>(qc);
     if (NT == 0) return SNKPT_OK;
-    HIPC(C.counts.reserve(64));
     HIPC(C.train_vtx.reserve((size_t)NT * depth * kRecFloats * 4));
     // train_meta: nrec | per-path sample counts | offsets (u32 each) | scan sums | radiance (float4)
     const size_t u32_bytes = ((size_t)NT * 4 + 255) & ~(size_t)255;
@@ -1042,13 +1059,11 @@ snkpt_status cache_train_frame(snkpt_ctx* ctx, const CamDev& cd, uint32_t w, uin
     const size_t trad_off = 3 * u32_bytes + sums_bytes;
     HIPC(C.train_meta.reserve(trad_off + (size_t)NT * 16));
     HIPC(C.t_samples.reserve((size_t)NT * depth * 64));
-    uint32_t* qc = C.counts.as<uint32_t>();
     const uint32_t indirect = (s->flags & SNKPT_FLAG_CACHE_INDIRECT) ? 1u : 0u;
     uint32_t* nrec = C.train_meta.as<uint32_t>();
     float4* trad = (float4*)(C.train_meta.as<uint8_t>() + trad_off);
     const uint32_t stride = std::max<uint32_t>(n / NT, 1u);
     const uint32_t start = (uint32_t)((initstate >> 17) % stride);
-    k_train_clear<<<1, 64, 0, st>>>(qc);
     k_fill4<<<grid_for(NT, 256), 256, 0, st>>>(trad, NT);
     ShadeHooks T;
     T.kind = 2;
@@ -1071,12 +1086,32 @@ snkpt_status cache_train_frame(snkpt_ctx* ctx, const CamDev& cd, uint32_t w, uin
     scan_exclusive(pcount, off, NT, sums, qc + 1, st);
     k_train_emit<<<grid_for(NT, 256), 256, 0, st>>>(C.train_vtx.as<float>(), nrec, depth, trad, NT, kseed, indirect,
                                                      qc, budget, off, C.t_samples.as<float>(), stats + 4);
+    return SNKPT_OK;
+}
+
+// One optimiser step on the samples cache_train_paths emitted (reads their count).
+snkpt_status cache_train_step(snkpt_ctx* ctx, const snkpt_settings* s, hipStream_t st) {
+    CacheState& C = *ctx->cache;
     uint32_t nt = 0;
-    HIPC(hipMemcpyAsync(&nt, qc + 1, 4, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(&nt, C.tcounts.as<uint32_t>() + 1, 4, hipMemcpyDeviceToHost, st));
     HIPC(hipStreamSynchronize(st));
     if (nt == 0) return SNKPT_OK;
     return cache_train_ema(ctx, C.t_samples.as<float>(), nt, s->cache_lr > 0.0f ? s->cache_lr : 1e-2f, 1,
                            std::min(std::max(s->cache_ema, 0.0f), 0.9999f), nullptr, st);
+}
+
+snkpt_status cache_train_frame(snkpt_ctx* ctx, const CamDev& cd, uint32_t w, uint32_t h, const snkpt_settings* s,
+                               uint64_t initstate, uint32_t n_paths, unsigned long long* stats, hipStream_t st) {
+    snkpt_status e = cache_train_paths(ctx, cd, w, h, s, initstate, n_paths, stats, st);
+    if (e != SNKPT_OK) return e;
+    return cache_train_step(ctx, s, st);
+}
+
+__global__ void k_add_u64(unsigned long long* dst, const unsigned long long* src, uint32_t n) {
+    if (threadIdx.x < n) dst[threadIdx.x] += src[threadIdx.x];
+}
+__global__ void k_clear_u64(unsigned long long* p, uint32_t n) {
+    if (threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
 
 snkpt_status cache_pass(snkpt_ctx* ctx, const CamDev& cd, const snkpt_settings* s, uint64_t initstate, uint32_t pass,
@@ -1098,6 +1133,25 @@ snkpt_status cache_pass(snkpt_ctx* ctx, const CamDev& cd, const snkpt_settings* 
     if ((e = ensure_work(ctx, C, n, false)) != SNKPT_OK) return e;
     uint32_t* qc = C.counts.as<uint32_t>();
     unsigned long long* stats = R.stats.as<unsigned long long>();
+    // 0. the training paths need no network: fork them onto the side stream so
+    //    they are traced while the render pass runs (own counters and statistics)
+    const bool train = !(s->flags & SNKPT_FLAG_NO_TRAIN);
+    if (train) {
+        if (!C.side) {
+            HIPC(hipStreamCreateWithFlags(&C.side, hipStreamNonBlocking));
+            HIPC(hipEventCreateWithFlags(&C.fork_ev, hipEventDisableTiming));
+            HIPC(hipEventCreateWithFlags(&C.join_ev, hipEventDisableTiming));
+        }
+        HIPC(C.tstats.reserve(64));
+        HIPC(hipEventRecord(C.fork_ev, st));
+        HIPC(hipStreamWaitEvent(C.side, C.fork_ev, 0));
+        k_clear_u64<<<1, 64, 0, C.side>>>(C.tstats.as<unsigned long long>(), 8);
+        if ((e = cache_train_paths(ctx, cd, R.w, R.h, s, initstate,
+                                   s->cache_train_paths ? s->cache_train_paths : 65536u,
+                                   C.tstats.as<unsigned long long>(), C.side)) != SNKPT_OK)
+            return e;
+        HIPC(hipEventRecord(C.join_ev, C.side));
+    }
     // 1. render paths, terminated into the cache at vertex Q
     k_zero_u32<<<1, 64, 0, st>>>(qc, 4);
     ShadeHooks H;
@@ -1120,10 +1174,11 @@ snkpt_status cache_pass(snkpt_ctx* ctx, const CamDev& cd, const snkpt_settings* 
     k_cache_apply<<<grid_for(n, 256), 256, 0, st>>>(C.q_meta.as<float4>(), C.y.as<float4>(), qc, R.rad.as<float4>(),
                                                       stats + 3);
     (void)pass;
-    if (s->flags & SNKPT_FLAG_NO_TRAIN) return SNKPT_OK;
-    // 3. training paths: full-length unbiased paths through a strided pixel subset
-    return cache_train_frame(ctx, cd, R.w, R.h, s, initstate, s->cache_train_paths ? s->cache_train_paths : 65536u,
-                             stats, st);
+    if (!train) return SNKPT_OK;
+    // 3. join the training paths, then one optimiser step on their samples
+    HIPC(hipStreamWaitEvent(st, C.join_ev, 0));
+    k_add_u64<<<1, 64, 0, st>>>(stats, C.tstats.as<unsigned long long>(), 8);
+    return cache_train_step(ctx, s, st);
 }
 
 }  // namespace snkpt
