@@ -489,6 +489,18 @@ def main():
         extra["primary_mrays_s"] = round(po.shape[0] / (prim_ms * 1e-3) / 1e6, 2)
         extra["shadow_mrays_s"] = round(ns / (sh_ms * 1e-3) / 1e6, 2)
         extra["shadow_rays"] = ns
+        # the same roofline for the primary (closest hit, 56 B + nodes + triangles)
+        # and shadow (any hit, 40 B + nodes + triangles) ray sets, from their own
+        # deterministic visit counts
+        with torch.cuda.stream(stream):
+            *_, pnv, ptv = ctx.trace(po, pd, counted=True, stream=stream)
+            *_, snv, stv = ctx.trace(shadow_o, shadow_d, any_hit=True, counted=True, stream=stream)
+            stream.synchronize()
+        for key, nv_, tv_, base, ms_, cnt in (("primary", pnv, ptv, 56.0, prim_ms, po.shape[0]),
+                                               ("shadow", snv, stv, 40.0, sh_ms, ns)):
+            b = base + 80.0 * float(nv_.double().mean()) + 48.0 * float(tv_.double().mean())
+            extra[f"{key}_bytes_per_ray"] = round(b, 1)
+            extra[f"{key}_roofline_frac"] = round(cnt * b / (ms_ * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)
     extra.update({"incoherent_rays": n, "primary_hit_fraction": round(hit_frac, 4),
                   "secondary_hit_fraction": round(secondary_hit, 4), "mean_nodes_visited": round(n_node, 3),
                   "mean_tris_visited": round(n_tri, 3), "bvh8_nodes": info.num_nodes, "bvh8_depth": info.depth,

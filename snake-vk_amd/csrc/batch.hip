@@ -320,7 +320,18 @@ struct snkpt_batch {
     hipGraphExec_t exec = nullptr;
     uint64_t graph_key = 0;
     hipStream_t graph_stream = nullptr;
-    ~snkpt_batch() { drop_graph(); }
+    // cache form: side streams on which the scenes' training paths run during the render
+    static constexpr int kSide = 4;
+    hipStream_t side[kSide] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[kSide] = {};
+    ~snkpt_batch() {
+        drop_graph();
+        for (int i = 0; i < kSide; ++i) {
+            if (side[i]) (void)hipStreamDestroy(side[i]);
+            if (join_ev[i]) (void)hipEventDestroy(join_ev[i]);
+        }
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
+    }
     void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -572,6 +583,43 @@ snkpt_status record_cache_frame(snkpt_batch* B, const snkpt_settings* s, hipStre
     const uint32_t sgrid = (uint32_t)B->cus * std::max(1u, B->tuning.shade_blocks_per_cu);
     const uint32_t ggrid = (uint32_t)B->cus * 8u;
     k_bzero<<<1, 64, 0, st>>>(stats, 8);
+    // training: full paths through each scene's first view, traced on side
+    // streams (round robin) while the render below runs; they need no network
+    const bool train = !(s->flags & SNKPT_FLAG_NO_TRAIN);
+    std::vector<snkpt_ctx*> trained;
+    if (train) {
+        if (!B->fork_ev) {
+            for (int i = 0; i < snkpt_batch::kSide; ++i) {
+                BHIP(hipStreamCreateWithFlags(&B->side[i], hipStreamNonBlocking));
+                BHIP(hipEventCreateWithFlags(&B->join_ev[i], hipEventDisableTiming));
+            }
+            BHIP(hipEventCreateWithFlags(&B->fork_ev, hipEventDisableTiming));
+        }
+        BHIP(hipEventRecord(B->fork_ev, st));
+        for (int i = 0; i < snkpt_batch::kSide; ++i) BHIP(hipStreamWaitEvent(B->side[i], B->fork_ev, 0));
+        std::vector<uint64_t> px(nsc, 0);
+        std::vector<int64_t> first(nsc, -1);
+        uint64_t total = 0;
+        for (size_t v = 0; v < B->views.size(); ++v) {
+            const auto& V = B->views[v];
+            px[V.scene] += (uint64_t)V.w * V.h;
+            total += (uint64_t)V.w * V.h;
+            if (first[V.scene] < 0) first[V.scene] = (int64_t)v;
+        }
+        const uint64_t budget = s->cache_train_paths ? s->cache_train_paths : 65536u;
+        for (uint32_t k = 0; k < nsc; ++k) {
+            if (first[k] < 0) continue;
+            const auto& V = B->views[(size_t)first[k]];
+            uint64_t nt = std::max<uint64_t>(1024u, budget * px[k] / std::max<uint64_t>(total, 1));
+            nt = std::min<uint64_t>(nt, (uint64_t)V.w * V.h);
+            const CamDev cd = make_cam(&V.cam);
+            snkpt_status e = cache_train_paths_forked(B->scenes[k], cd, V.w, V.h, s, pass_state(V.seed, s->frame, 0),
+                                                      (uint32_t)nt, B->side[trained.size() % snkpt_batch::kSide]);
+            if (e != SNKPT_OK) return bfail(B, e, std::string("batch cache training: ") + B->scenes[k]->last_error);
+            trained.push_back(B->scenes[k]);
+        }
+        for (int i = 0; i < snkpt_batch::kSide; ++i) BHIP(hipEventRecord(B->join_ev[i], B->side[i]));
+    }
     std::vector<uint32_t> hist(nsc), off(nsc);
     for (const auto& g : B->groups) {
         k_bgenerate<<<ggrid, kBBlock, 0, st>>>(B->d_jobs.as<JobD>(), g.j0, g.j1, g.total, B->d_frame.as<FrameD>(),
@@ -636,28 +684,11 @@ snkpt_status record_cache_frame(snkpt_batch* B, const snkpt_settings* s, hipStre
                                               B->rad.as<float4>(), B->accum.as<float4>(), g.total,
                                               B->d_frame.as<FrameD>());
     }
-    if (!(s->flags & SNKPT_FLAG_NO_TRAIN)) {
-        // training: full paths through each scene's first view
-        std::vector<uint64_t> px(nsc, 0);
-        std::vector<int64_t> first(nsc, -1);
-        uint64_t total = 0;
-        for (size_t v = 0; v < B->views.size(); ++v) {
-            const auto& V = B->views[v];
-            px[V.scene] += (uint64_t)V.w * V.h;
-            total += (uint64_t)V.w * V.h;
-            if (first[V.scene] < 0) first[V.scene] = (int64_t)v;
-        }
-        const uint64_t budget = s->cache_train_paths ? s->cache_train_paths : 65536u;
-        for (uint32_t k = 0; k < nsc; ++k) {
-            if (first[k] < 0) continue;
-            const auto& V = B->views[(size_t)first[k]];
-            uint64_t nt = std::max<uint64_t>(1024u, budget * px[k] / std::max<uint64_t>(total, 1));
-            nt = std::min<uint64_t>(nt, (uint64_t)V.w * V.h);
-            const CamDev cd = make_cam(&V.cam);
-            snkpt_status e = cache_train_frame(B->scenes[k], cd, V.w, V.h, s, pass_state(V.seed, s->frame, 0),
-                                               (uint32_t)nt, stats, st);
-            if (e != SNKPT_OK) return bfail(B, e, std::string("batch cache training: ") + B->scenes[k]->last_error);
-        }
+    if (train) {
+        // join the training paths, then one step per scene in scene order
+        for (int i = 0; i < snkpt_batch::kSide; ++i) BHIP(hipStreamWaitEvent(st, B->join_ev[i], 0));
+        snkpt_status e = cache_train_steps(trained.data(), (uint32_t)trained.size(), s, stats, st);
+        if (e != SNKPT_OK) return bfail(B, e, "batch cache training step failed");
     }
     hipError_t le = hipGetLastError();
     if (le != hipSuccess) return bfail(B, SNKPT_ERR_HIP, std::string("batch launch: ") + hipGetErrorString(le));

@@ -1035,9 +1035,17 @@ snkpt_status cache_query(snkpt_ctx* ctx, const float* samples, uint32_t n, float
     return SNKPT_OK;
 }
 
+__global__ void k_add_u64(unsigned long long* dst, const unsigned long long* src, uint32_t n) {
+    if (threadIdx.x < n) dst[threadIdx.x] += src[threadIdx.x];
+}
+__global__ void k_clear_u64(unsigned long long* p, uint32_t n) {
+    if (threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
 // Training half of a cache frame: n_paths full paths through a strided pixel
 // subset of a w x h view of camera cd record their vertices; the thinned
 // vertices become one optimisation step (Adam, weight averages of decay cache_ema).
+
 // Training paths of a frame and their samples (no network involved, no host
 // synchronisation): full-length unbiased paths through a strided pixel subset,
 // recorded vertices thinned and emitted in path order into t_samples; the
@@ -1100,18 +1108,43 @@ snkpt_status cache_train_step(snkpt_ctx* ctx, const snkpt_settings* s, hipStream
                            std::min(std::max(s->cache_ema, 0.0f), 0.9999f), nullptr, st);
 }
 
-snkpt_status cache_train_frame(snkpt_ctx* ctx, const CamDev& cd, uint32_t w, uint32_t h, const snkpt_settings* s,
-                               uint64_t initstate, uint32_t n_paths, unsigned long long* stats, hipStream_t st) {
-    snkpt_status e = cache_train_paths(ctx, cd, w, h, s, initstate, n_paths, stats, st);
-    if (e != SNKPT_OK) return e;
-    return cache_train_step(ctx, s, st);
+// Optimiser steps of several contexts whose training paths were traced
+// (cache_train_paths): one readback of all their sample counts, then each
+// context's step in order on `st`.  The paths' ray statistics (tstats) are
+// added into `stats` first.
+snkpt_status cache_train_steps(snkpt_ctx* const* ctxs, uint32_t n, const snkpt_settings* s,
+                               unsigned long long* stats, hipStream_t st) {
+    std::vector<uint32_t> nts(n, 0u);
+    for (uint32_t k = 0; k < n; ++k) {
+        snkpt_ctx* ctx = ctxs[k];
+        CacheState& C = *ctx->cache;
+        if (stats) k_add_u64<<<1, 64, 0, st>>>(stats, C.tstats.as<unsigned long long>(), 8);
+        HIPC(hipMemcpyAsync(&nts[k], C.tcounts.as<uint32_t>() + 1, 4, hipMemcpyDeviceToHost, st));
+    }
+    if (n) {
+        snkpt_ctx* ctx = ctxs[0];
+        HIPC(hipStreamSynchronize(st));
+    }
+    for (uint32_t k = 0; k < n; ++k) {
+        if (nts[k] == 0) continue;
+        CacheState& C = *ctxs[k]->cache;
+        snkpt_status e = cache_train_ema(ctxs[k], C.t_samples.as<float>(), nts[k],
+                                         s->cache_lr > 0.0f ? s->cache_lr : 1e-2f, 1,
+                                         std::min(std::max(s->cache_ema, 0.0f), 0.9999f), nullptr, st);
+        if (e != SNKPT_OK) return e;
+    }
+    return SNKPT_OK;
 }
 
-__global__ void k_add_u64(unsigned long long* dst, const unsigned long long* src, uint32_t n) {
-    if (threadIdx.x < n) dst[threadIdx.x] += src[threadIdx.x];
-}
-__global__ void k_clear_u64(unsigned long long* p, uint32_t n) {
-    if (threadIdx.x < n) p[threadIdx.x] = 0ull;
+// Training paths of a context forked onto `side` after `st`'s work so far;
+// their statistics go to the context's own tstats (cleared first).
+snkpt_status cache_train_paths_forked(snkpt_ctx* ctx, const CamDev& cd, uint32_t w, uint32_t h,
+                                      const snkpt_settings* s, uint64_t initstate, uint32_t n_paths,
+                                      hipStream_t side) {
+    CacheState& C = *ctx->cache;
+    HIPC(C.tstats.reserve(64));
+    k_clear_u64<<<1, 64, 0, side>>>(C.tstats.as<unsigned long long>(), 8);
+    return cache_train_paths(ctx, cd, w, h, s, initstate, n_paths, C.tstats.as<unsigned long long>(), side);
 }
 
 snkpt_status cache_pass(snkpt_ctx* ctx, const CamDev& cd, const snkpt_settings* s, uint64_t initstate, uint32_t pass,

@@ -181,8 +181,14 @@ snkpt_status whole_paths(snkpt_ctx* ctx, const CamDev& cam, uint32_t w, uint32_t
 snkpt_status cache_ensure(snkpt_ctx* ctx, const snkpt_settings* s, hipStream_t st);
 snkpt_status cache_query(snkpt_ctx* ctx, const float* samples, uint32_t n, float4* y, bool averaged,
                          hipStream_t st);
-snkpt_status cache_train_frame(snkpt_ctx* ctx, const CamDev& cd, uint32_t w, uint32_t h, const snkpt_settings* s,
-                               uint64_t initstate, uint32_t n_paths, unsigned long long* stats, hipStream_t st);
+// Training paths of a context enqueued on `side` (statistics kept per context),
+// then, for several contexts at once, one readback of their sample counts and
+// each one's optimiser step on `st` (their statistics added into `stats`).
+snkpt_status cache_train_paths_forked(snkpt_ctx* ctx, const CamDev& cd, uint32_t w, uint32_t h,
+                                      const snkpt_settings* s, uint64_t initstate, uint32_t n_paths,
+                                      hipStream_t side);
+snkpt_status cache_train_steps(snkpt_ctx* const* ctxs, uint32_t n, const snkpt_settings* s,
+                               unsigned long long* stats, hipStream_t st);
 // Bounces 0.. of one path per pixel: camera rays through the wavefront, or,
 // when the frame has a G-buffer, bounce 0 read from it and bounces 1.. traced.
 snkpt_status primary_bounces(snkpt_ctx* ctx, const CamDev& cam, const snkpt_settings* s, uint64_t initstate,
