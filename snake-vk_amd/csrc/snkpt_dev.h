@@ -295,10 +295,12 @@ __device__ __forceinline__ TraceResult traverse(const uint4* __restrict__ nodes,
 
 // Persistent traversal with ray refill (after Aila & Laine 2009, "Understanding
 // the efficiency of ray traversal on GPUs"), for the calling wave.  The rays
-// [0, n) are split into 64-ray chunks; wave w of nwaves owns chunks w,
-// w + nwaves, w + 2 nwaves, ... (every wave samples the whole stream, so the
-// static split stays balanced; a chunk keeps neighbouring rays together).  Its
-// rays are handed out in order to idle lanes whenever at least REFILL lanes
+// [0, n) are cut into bands of nwaves x 64 rays.  In every whole band wave w
+// owns the 64-ray chunk w (so the waves in flight trace one band of
+// neighbouring rays at a time and share its BVH nodes in L2); the last, partial
+// band is split evenly, wave w taking its [w rest / nwaves, (w+1) rest / nwaves)
+// slice, so no wave gets a whole chunk more than another.  Its rays are handed
+// out in order to idle lanes whenever at least REFILL lanes
 // are idle (or all are), so a wave does not wait for its slowest ray with the
 // other lanes masked off.  load(id, ray, nodes, tris) fetches ray `id` and the
 // BVH it is traced against; done(id, state, nvis, tvis) is called once per ray
@@ -310,9 +312,12 @@ __device__ __forceinline__ void refill_trace(uint32_t n, uint32_t wave, uint32_t
                                              Load load, Done done) {
     const uint32_t lane = __lane_id();
     const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    const uint32_t chunks = (n + 63) / 64;
-    const uint32_t mine = wave < chunks ? (chunks - wave + nwaves - 1) / nwaves : 0u;
-    const uint32_t end = mine * 64u;  // this wave's rays, counted in its own order
+    const uint32_t band = nwaves * 64u;
+    const uint32_t full = n / band;  // whole bands
+    const uint32_t base = full * band, rest = n - base;
+    const uint32_t lo = (uint32_t)(((uint64_t)wave * rest) / nwaves);
+    // this wave's rays, counted in its own order: 64 per whole band, then its slice
+    const uint32_t end = full * 64u + (uint32_t)(((uint64_t)(wave + 1u) * rest) / nwaves) - lo;
     uint32_t cursor = 0;
     bool active = false;
     uint32_t id = 0, nv = 0, tv = 0;
@@ -326,9 +331,8 @@ __device__ __forceinline__ void refill_trace(uint32_t n, uint32_t wave, uint32_t
         if (cursor < end && (nidle >= (uint32_t)REFILL || nidle == 64u)) {
             if (!active) {
                 const uint32_t k = cursor + (uint32_t)__popcll(idle & below);
-                const uint32_t r = (wave + (k >> 6) * nwaves) * 64u + (k & 63u);
-                if (k < end && r < n) {
-                    id = r;
+                if (k < end) {
+                    id = k < full * 64u ? (wave + (k >> 6) * nwaves) * 64u + (k & 63u) : base + lo + (k - full * 64u);
                     Ray ray;
                     load(id, ray, nodes, tris);
                     nv = tv = 0;

@@ -74,9 +74,10 @@ __global__ void k_trace_empty(uint32_t n, const uint32_t* __restrict__ count_dev
     if (tvis_out) tvis_out[i] = 0;
 }
 
-// Persistent closest / any hit over the ray stream: each wave owns a
-// contiguous slice of the rays and refills its idle lanes from it (see
-// refill_trace).  No atomics: a wave's cursor is uniform and private to it.
+// Persistent closest / any hit over the ray stream: each wave owns one 64-ray
+// chunk per band of the rays plus an even slice of the last band, and refills
+// its idle lanes from them (see refill_trace).  No atomics: a wave's cursor is
+// uniform and private to it.
 template <bool ANY, bool COUNT, int STACK, int REFILL>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(STACK <= 10 ? 8 : 1, 8))) void k_trace_refill(const float4* __restrict__ org_tmin,
                                                               const float4* __restrict__ dir_tmax, uint32_t n,
