@@ -128,10 +128,14 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(STA
     refill_trace<ANY, COUNT, STACK, REFILL>(nn, wave, nwaves, &stk[threadIdx.x], kTraceBlock, load, done);
 }
 
-constexpr int kRefill = 16;
+// Idle lanes of a wave are refilled once this many are idle (4: 3.18 Grays/s on
+// the 1.29M incoherent rays, 8: 3.23, 12: 3.24, 16: 3.20, 24: 3.15, 32: 3.09;
+// coherent primary rays favour 8: 6.31 against 6.18 at 16).
+constexpr int kRefill = 8;
 // Resident 128-lane blocks per CU allowed by the 160 KB of LDS: a lane's stack
 // takes STACK x 8 B (10: 16 blocks, 12: 13, 16: 10, 32: 5, 64: 2).  The 10-entry
-// variant is held to 64 VGPRs so that 8 waves per SIMD fit (no spills).
+// variant is held to 64 VGPRs so that 8 waves per SIMD fit (two spilled dwords;
+// 6 waves per SIMD with 80 VGPRs: +3% on incoherent rays, -12% on primary rays).
 inline uint32_t refill_blocks_per_cu(uint32_t need) {
     return need <= 10 ? 16u : need <= 12 ? 13u : need <= 16 ? 10u : need <= 32 ? 5u : 2u;
 }
