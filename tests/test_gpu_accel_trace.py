@@ -261,3 +261,32 @@ def test_rebuild_after_new_transforms_matches_cpu_twin(gpu):
         c = b.trace(rays)
         for j, name in enumerate(["t", "inst", "prim", "u", "v"]):
             assert_bits_equal(g[j], c[j], name)
+
+
+def test_ray_counts_around_band_edges(gpu):
+    """The persistent traversal deals the rays out in bands of (resident waves x
+    64) and splits the last, partial band evenly; ray counts at and around the
+    chunk and band edges (closest and any hit) must each trace every ray once,
+    bit-identical to the CPU twin."""
+    import torch
+
+    import raysets
+    import snkpt
+    scene = snkpt.SceneData(1)
+    ctx = make_ctx(scene)
+    world = scene_world(scene)
+    b = O.Bvh(world)
+    assert b.depth - 1 <= 10  # the 10-entry stack variant: 16 resident blocks of 2 waves per CU
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    band = cus * 16 * 2 * 64
+    counts = sorted({1, 63, 64, 65, 1000, band - 1, band, band + 1, band + 777, 2 * band + 4097})
+    lo, hi = world[:, :9].reshape(-1, 3).min(0), world[:, :9].reshape(-1, 3).max(0)
+    org, dr = raysets.random_rays(counts[-1], lo, hi, seed=21)
+    rays = np.concatenate([org, dr], 1)
+    for any_hit in (False, True):
+        c = b.trace(rays, any_hit=any_hit)
+        for n in counts:
+            _say(f"band edges: n={n} (band {band}) any_hit={any_hit}")
+            g = gpu_trace(ctx, rays[:n], any_hit=any_hit)
+            for k, name in enumerate(["t", "inst", "prim", "u", "v", "nodes visited", "tris visited"]):
+                assert_bits_equal(g[k], c[k][:n], f"n={n} {name}")
