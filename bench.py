@@ -58,19 +58,23 @@ def cpu_baseline(world, rays_np, threads):
                       f"triangles took {build_s:.2f} s"}
 
 
-FRAME_RUNS = [(1, "pt"), (2, "pt"), (3, "restir"), (4, "pt"), (4, "cache"), (4, "cache_indirect")]
+# (config, form); reference_compat: the reference's own estimator (ReSTIR direct light +
+# BounceDiffuse's 8 + up to 16 rays per pixel) on the config 4 room
+FRAME_RUNS = [(1, "pt"), (2, "pt"), (3, "restir"), (4, "pt"), (4, "cache"), (4, "cache_indirect"),
+              (4, "reference_compat")]
 
 
 def frame_rates(stream, frames=10, warmup=3, only=None):
     """Frames/s per configuration (1 spp frames of the render pass, event-timed
     on the launch stream).  Cache frames include the online training step
     (cache: L_o queried at path vertex 3; cache_indirect: NEE at the primary
-    hit and the cache's indirect light there)."""
+    hit and the cache's indirect light there); reference_compat frames run
+    the reference's biased BounceDiffuse estimator."""
     import torch
 
     import snkpt
     modes = {"pt": snkpt.MODE_PT, "restir": snkpt.MODE_RESTIR, "cache": snkpt.MODE_CACHE,
-             "cache_indirect": snkpt.MODE_CACHE}
+             "cache_indirect": snkpt.MODE_CACHE, "reference_compat": snkpt.MODE_REFERENCE_COMPAT}
     res = {}
     for cfg, label in FRAME_RUNS:
         if only is not None and f"c{cfg}_{label}" != only:

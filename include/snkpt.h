@@ -102,6 +102,13 @@ typedef struct snkpt_pointlight {
 #define SNKPT_MODE_PT 0u     /* unbiased path tracing, NEE + MIS */
 #define SNKPT_MODE_RESTIR 1u /* ReSTIR DI for direct light at the primary hit + PT for the rest */
 #define SNKPT_MODE_CACHE 2u  /* PT terminated into the MLP radiance cache, cache trained online */
+/* The reference's own (biased) estimator, for regression against its look: ReSTIR DI at the
+ * primary hit plus BounceDiffuse (RayGen.rgen:139-213): 8 uniform-hemisphere rays from the biased
+ * primary position, each adding albedo * payload * max(cos(dot(rd, n)), 0) / 8, and 2 more from
+ * every non-emissive hit at a quarter weight, all times 2 PI; the payload of a hit is its emission,
+ * or the directional light through its BRDF and a shadow ray (RayClosestHit.rchit:87-142).  Random
+ * numbers are PCG32 instead of the wall-clock gold noise; max_depth and rr_start are ignored. */
+#define SNKPT_MODE_REFERENCE_COMPAT 3u
 
 #define SNKPT_FLAG_ACCUMULATE 1u /* add this frame to the running mean and output the mean */
 #define SNKPT_FLAG_RESET 2u      /* clear the running mean first */

@@ -46,6 +46,9 @@ def load():
     lib.o_pt_render_gbuffer.argtypes = [VP, U32, C.POINTER(OBvh), VP, U32, VP, VP, VP, VP, VP, U32, VP, VP, VP, U32,
                                         U32, U32, U32, U32, U32, U32, C.POINTER(OGbuffer), VP, I]
     lib.o_pt_render_gbuffer.restype = I
+    lib.o_compat_bounce.argtypes = [VP, U32, C.POINTER(OBvh), VP, U32, VP, VP, VP, VP, VP, U32, VP, VP, VP, U32, U32,
+                                    U32, U32, U32, VP, I]
+    lib.o_compat_bounce.restype = I
     lib.o_half_to_float.argtypes = [C.c_uint16]
     lib.o_half_to_float.restype = F
     lib.o_primary_surfaces.argtypes = [VP, U32, C.POINTER(OBvh), VP, U32, VP, VP, VP, VP, VP, VP, VP, VP, U32, U32,
@@ -209,6 +212,27 @@ def pt_render(arrs, bvh: "Bvh", cam, w, h, spp, max_depth, rr_start, seed, frame
     load().o_pt_render(_p(bvh.world), bvh.n, C.byref(bvh.b), _p(inst), inst.shape[0], _p(idx), _p(nrm), _p(m4),
                        _p(mats), _p(em), em.size, _p(view), _p(proj), _p(cpos), w, h, spp, max_depth, rr_start,
                        seed, frame, seed_xor, _p(out), threads)
+    return out
+
+
+def compat_bounce(arrs, bvh: "Bvh", cam, w, h, seed, frame=0, spp=1, dir_lights=None, threads=8):
+    """The reference's BounceDiffuse term (RayGen.rgen:139-213) per pixel, as
+    SNKPT_MODE_REFERENCE_COMPAT adds it to the direct light; dir_lights: rows of
+    (colour rgb, direction towards the light).  Returns (h, w, 3) float32."""
+    pos, idx, inst, m4, mats, nrm, em = arrs
+    inst = np.ascontiguousarray(inst, np.uint32)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    m4 = np.ascontiguousarray(m4, np.float32)
+    mats = np.ascontiguousarray(mats, np.uint32)
+    nrm = np.ascontiguousarray(nrm, np.float32)
+    dl = np.ascontiguousarray(np.zeros((0, 6)) if dir_lights is None else dir_lights, np.float32).reshape(-1, 6)
+    view = np.array(cam.view, np.float32)
+    proj = np.array(cam.proj, np.float32)
+    cpos = np.array(cam.cam_pos[:3], np.float32)
+    out = np.zeros((h, w, 3), np.float32)
+    load().o_compat_bounce(_p(bvh.world), bvh.n, C.byref(bvh.b), _p(inst), inst.shape[0], _p(idx), _p(nrm), _p(m4),
+                           _p(mats), _p(dl) if dl.size else None, dl.shape[0], _p(view), _p(proj), _p(cpos), w, h,
+                           spp, seed, frame, _p(out), threads)
     return out
 
 

@@ -109,6 +109,11 @@ int o_pt_render_gbuffer(const float* world, uint32_t ntris, const o_bvh* bvh, co
                         const float* proj16, const float* cam_pos, uint32_t w, uint32_t h, uint32_t spp,
                         uint32_t max_depth, uint32_t rr_start, uint32_t seed, uint32_t frame, const o_gbuffer* gb,
                         float* out_rgb, int threads);
+int o_compat_bounce(const float* world, uint32_t ntris, const o_bvh* bvh, const uint32_t* inst5, uint32_t n_inst,
+                    const uint32_t* idx, const float* nrm, const float* mat4s, const uint32_t* mats16,
+                    const float* dir_lights, uint32_t n_dir, const float* view16, const float* proj16,
+                    const float* cam_pos, uint32_t w, uint32_t h, uint32_t spp, uint32_t seed, uint32_t frame,
+                    float* out_rgb, int threads);
 void o_primary_surfaces(const float* world, uint32_t ntris, const o_bvh* bvh, const uint32_t* inst5,
                         uint32_t n_inst, const uint32_t* idx, const float* nrm, const float* mat4s,
                         const uint32_t* mats16, const float* view16, const float* proj16, const float* proj_view16,

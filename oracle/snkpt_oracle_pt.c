@@ -21,6 +21,9 @@
  * values; sinf/cosf of the platform library may differ in the last ulp.
  * Shadow rays leave p + ng eps and are aimed at the sampled light point from
  * there (ShadowTest from biased_pos, RayGen.rgen:101-121,552).
+ * o_compat_bounce restates BounceDiffuse itself (RayGen.rgen:123-213, payload
+ * RayClosestHit.rchit:87-142) for the reference-compatibility mode, pinned by
+ * the closed forms of tests/test_oracle_compat.py.
  * G-buffer mode (o_pt_render_gbuffer): the primary vertex is read from the
  * reference's G-buffer images (RayGen.rgen:531-552): WorldPosFromDepth
  * (Util.glsl:1-7), texel normal, SNORM8 albedo, RGBA16F rma, R16 flags, ray bias
@@ -337,6 +340,7 @@ static void build_emitters_c(oscene* S, const uint32_t* em_list, uint32_t n_em) 
 
 typedef struct {
     v3 p, ng, ns;
+    v3 nraw; /* shading normal as interpolated and transformed, not turned to the ray */
     omat m;
     v3 le;
     int emissive;
@@ -364,6 +368,7 @@ static osurf surface(const oscene* S, v3 dir, uint32_t inst, uint32_t prim, floa
               (nm[6] * nl.x + nm[7] * nl.y) + nm[8] * nl.z);
     float l2 = dotv(ns, ns);
     ns = l2 > 1e-20f ? scl(ns, 1.0f / sqrtf(l2)) : ng;
+    s.nraw = ns;
     if (dotv(ng, dir) > 0.0f) ng = scl(ng, -1.0f);
     if (dotv(ns, ng) < 0.0f) ns = scl(ns, -1.0f);
     s.ng = ng;
@@ -541,6 +546,7 @@ static int gsurface(const pt_job* J, uint32_t px, osurf* s, v3* dir, float* t) {
     if (dotv(n, *dir) > 0.0f) n = scl(n, -1.0f);
     s->ng = n;
     s->ns = n;
+    s->nraw = n;
     const int8_t* a = g->albedo + 4 * (size_t)px;
     s->m.albedo = V(clampf(snorm8_c(a[0]), 0.0f, 1.0f), clampf(snorm8_c(a[1]), 0.0f, 1.0f),
                     clampf(snorm8_c(a[2]), 0.0f, 1.0f));
@@ -778,4 +784,162 @@ void o_primary_surfaces(const float* world, uint32_t ntris, const o_bvh* bvh, co
     }
     free(S.tri_off);
     free(S.nmat);
+}
+
+/* ------------------------------------------------- reference compat --- */
+/* BounceDiffuse (RayGen.rgen:123-213) with the payload of RayClosestHit.rchit:
+ * 87-142 / RayMiss.rmiss:15-26, the term SNKPT_MODE_REFERENCE_COMPAT adds to
+ * ReSTIR's direct light.  Random numbers: PCG32, state = the pass's ^
+ * 0x434f4d50, stream = pixel; per outer ray r1, r2, then r1, r2 of each of its
+ * two inner rays when it hit a non-emissive surface. */
+#define COMPAT_PHASE 0x434f4d50ull
+
+typedef struct {
+    const oscene* S;
+    const float* cam;
+    v3 pos;
+    uint32_t w, h, spp, seed, frame, n_dir;
+    const float* dir_lights; /* n_dir x (colour rgb, direction towards the light) */
+    float* out;
+    uint32_t begin, end;
+} compat_job;
+
+static void ref_frame_c(v3 n, v3* nt, v3* nb) {
+    if (fabsf(n.x) > fabsf(n.y))
+        *nt = scl(V(n.z, 0.0f, -n.x), 1.0f / sqrtf(n.x * n.x + n.z * n.z));
+    else
+        *nt = scl(V(0.0f, -n.z, n.y), 1.0f / sqrtf(n.y * n.y + n.z * n.z));
+    *nb = crossv(n, *nt);
+}
+
+static v3 ref_hemi_c(v3 n, v3 nt, v3 nb, float r1, float r2) {
+    const float st = sqrtf(1.0f - r1 * r1), phi = 2.0f * REF_PI * r2;
+    const float x = st * cosf(phi), z = st * sinf(phi);
+    return V(x * nb.x + r1 * n.x + z * nt.x, x * nb.y + r1 * n.y + z * nt.y, x * nb.z + r1 * n.z + z * nt.z);
+}
+
+/* payload of one closest-hit ray; *surf = 1 for a non-emissive hit */
+static v3 compat_payload(const compat_job* J, v3 o, v3 d, int* surf, v3* hp, v3* hn) {
+    *surf = 0;
+    float ray[8] = {o.x, o.y, o.z, 0.01f, d.x, d.y, d.z, 1000.0f};
+    float t, hu, hv;
+    uint32_t hi, hpr;
+    o_trace_one(J->S->bvh, ray, 0, &t, &hi, &hpr, &hu, &hv);
+    if (!(t >= 0.0f)) return V(0, 0, 0);
+    osurf s = surface(J->S, d, hi, hpr, hu, hv);
+    if (s.emissive) return s.le;
+    *surf = 1;
+    *hp = s.p;
+    *hn = s.nraw;
+    v3 c = V(0, 0, 0);
+    const v3 v = scl(d, -1.0f);
+    for (uint32_t k = 0; k < J->n_dir; ++k) {
+        const float* L = J->dir_lights + 6 * (size_t)k;
+        const v3 l = V(L[3], L[4], L[5]);
+        v3 ck = scl(mulv(brdf_f(&s.m, s.nraw, v, l), V(L[0], L[1], L[2])), fmaxf(dotv(s.nraw, l), 0.0f));
+        if (!(ck.x > 0.0f || ck.y > 0.0f || ck.z > 0.0f)) continue;
+        const v3 so = add(s.p, scl(s.nraw, 0.01f));
+        float sray[8] = {so.x, so.y, so.z, 0.001f, l.x, l.y, l.z, 10000.0f};
+        float st, su, sv;
+        uint32_t si, sp;
+        o_trace_one(J->S->bvh, sray, 1, &st, &si, &sp, &su, &sv);
+        if (!(st >= 0.0f)) c = add(c, ck);
+    }
+    return c;
+}
+
+static void* compat_worker(void* arg) {
+    compat_job* J = (compat_job*)arg;
+    for (uint32_t px = J->begin; px < J->end; ++px) {
+        float acc[3] = {0, 0, 0};
+        for (uint32_t k = 0; k < J->spp; ++k) {
+            v3 d = cam_dir(J->cam, J->pos, px % J->w, px / J->w, J->w, J->h);
+            float ray[8] = {J->pos.x, J->pos.y, J->pos.z, 0.0f, d.x, d.y, d.z, 1e30f};
+            float t, hu, hv;
+            uint32_t hi, hp;
+            o_trace_one(J->S->bvh, ray, 0, &t, &hi, &hp, &hu, &hv);
+            if (!(t >= 0.0f)) continue;
+            osurf s = surface(J->S, d, hi, hp, hu, hv);
+            if (s.emissive) continue;
+            const v3 n = s.ns, alb = s.m.albedo;
+            const v3 ro = add(s.p, scl(n, 0.01f + fmaxf(t * 0.001f, 0.01f)));
+            v3 nt, nb;
+            ref_frame_c(n, &nt, &nb);
+            prng rng = rseed(o_pass_state(J->seed, J->frame, k) ^ COMPAT_PHASE, px);
+            v3 sum = V(0, 0, 0);
+            for (int r = 0; r < 8; ++r) {
+                const float r1 = runi(&rng), r2 = runi(&rng);
+                const v3 rd = ref_hemi_c(n, nt, nb, r1, r2);
+                int surf;
+                v3 hpos, hn;
+                v3 col = compat_payload(J, ro, rd, &surf, &hpos, &hn);
+                sum = add(sum, scl(mulv(alb, col), fmaxf(cosf(dotv(rd, n)), 0.0f) / 8.0f));
+                if (!surf) continue;
+                v3 it, ib;
+                ref_frame_c(hn, &it, &ib);
+                for (int b = 0; b < 2; ++b) {
+                    const float q1 = runi(&rng), q2 = runi(&rng);
+                    const v3 rdi = ref_hemi_c(hn, it, ib, q1, q2);
+                    int s2;
+                    v3 p2, n2;
+                    v3 ci = compat_payload(J, hpos, rdi, &s2, &p2, &n2);
+                    sum = add(sum, scl(mulv(alb, ci), 0.25f * fmaxf(cosf(dotv(rdi, hn)), 0.0f) / 8.0f));
+                }
+            }
+            sum = scl(sum, 2.0f * REF_PI);
+            acc[0] += sum.x;
+            acc[1] += sum.y;
+            acc[2] += sum.z;
+        }
+        const float inv = 1.0f / (float)J->spp;
+        J->out[3 * (size_t)px] = acc[0] * inv;
+        J->out[3 * (size_t)px + 1] = acc[1] * inv;
+        J->out[3 * (size_t)px + 2] = acc[2] * inv;
+    }
+    return NULL;
+}
+
+int o_compat_bounce(const float* world, uint32_t ntris, const o_bvh* bvh, const uint32_t* inst5, uint32_t n_inst,
+                    const uint32_t* idx, const float* nrm, const float* mat4s, const uint32_t* mats16,
+                    const float* dir_lights, uint32_t n_dir, const float* view16, const float* proj16,
+                    const float* cam_pos, uint32_t w, uint32_t h, uint32_t spp, uint32_t seed, uint32_t frame,
+                    float* out_rgb, int threads) {
+    oscene S;
+    memset(&S, 0, sizeof(S));
+    S.world = world;
+    S.ntris = ntris;
+    S.bvh = bvh;
+    S.inst5 = inst5;
+    S.n_inst = n_inst;
+    S.idx = idx;
+    S.nrm = nrm;
+    S.mats16 = mats16;
+    S.tri_off = (uint32_t*)malloc(sizeof(uint32_t) * (n_inst + 1));
+    S.tri_off[0] = 0;
+    for (uint32_t i = 0; i < n_inst; ++i) S.tri_off[i + 1] = S.tri_off[i] + inst5[5 * i + 4] / 3;
+    S.nmat = (float*)malloc(sizeof(float) * 9 * (n_inst ? n_inst : 1));
+    for (uint32_t i = 0; i < n_inst; ++i) normal_matrix(mat4s + 16 * (size_t)inst5[5 * i], S.nmat + 9 * (size_t)i);
+    float cam[16];
+    o_camera_matrix(view16, proj16, cam);
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    pthread_t th[256];
+    compat_job jobs[256];
+    uint32_t n = w * h, per = (n + (uint32_t)threads - 1) / (uint32_t)threads;
+    for (int k = 0; k < threads; ++k) {
+        compat_job j = {&S, cam, V(cam_pos[0], cam_pos[1], cam_pos[2]), w, h, spp ? spp : 1, seed, frame,
+                        dir_lights ? n_dir : 0u, dir_lights, out_rgb, 0, 0};
+        j.begin = (uint32_t)k * per < n ? (uint32_t)k * per : n;
+        j.end = j.begin + per < n ? j.begin + per : n;
+        jobs[k] = j;
+        if (pthread_create(&th[k], NULL, compat_worker, &jobs[k]) != 0) {
+            compat_worker(&jobs[k]);
+            th[k] = 0;
+        }
+    }
+    for (int k = 0; k < threads; ++k)
+        if (th[k]) pthread_join(th[k], NULL);
+    free(S.tri_off);
+    free(S.nmat);
+    return 0;
 }

@@ -555,7 +555,7 @@ snkpt_status render(snkpt_ctx* ctx, const snkpt_camera* cam, const snkpt_setting
                     uint32_t h, uint32_t fmt, hipStream_t st, const snkpt_gbuffer* gb) {
     if (!ctx->accel.built) return fail(ctx, SNKPT_ERR_NOT_BUILT, "snkpt_render: call snkpt_build_accel first");
     if ((uint64_t)w * h > (1ull << 26)) return fail(ctx, SNKPT_ERR_CAPACITY, "snkpt_render: image above 2^26 pixels");
-    if (s->mode > SNKPT_MODE_CACHE) return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_render: unknown mode");
+    if (s->mode > SNKPT_MODE_REFERENCE_COMPAT) return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_render: unknown mode");
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, SNKPT_ERR_HIP, "hipSetDevice");
     snkpt_status e = begin_frame(ctx, w, h, st);
     if (e != SNKPT_OK) return e;
@@ -583,7 +583,7 @@ snkpt_status render(snkpt_ctx* ctx, const snkpt_camera* cam, const snkpt_setting
         uint64_t is = pass_state(s->seed, s->frame, k);
         if (s->mode == SNKPT_MODE_PT)
             e = pt_pass(ctx, cd, s, is, st, 0);
-        else if (s->mode == SNKPT_MODE_RESTIR)
+        else if (s->mode == SNKPT_MODE_RESTIR || s->mode == SNKPT_MODE_REFERENCE_COMPAT)
             e = restir_pass(ctx, cam, cd, s, is, k, st);
         else
             e = cache_pass(ctx, cd, s, is, k, st);

@@ -44,6 +44,7 @@ struct SceneDev {
 // Surface data at a hit.
 struct Surf {
     V3 p, ng, ns;  // position, geometric and shading normals (both facing the incoming ray)
+    V3 nraw;       // shading normal as interpolated and transformed (not turned to the ray)
     Mat m;
     V3 le;         // emission (albedo of an emissive material, TexMatBuffers.glsl:3)
     bool emissive;
@@ -76,6 +77,7 @@ __device__ __forceinline__ Surf surface_at(const SceneDev& S, V3 dir, uint32_t i
                (m2.x * nl.x + m2.y * nl.y) + m2.z * nl.z};
     float l2 = dot(ns, ns);
     ns = l2 > 1e-20f ? ns * (1.0f / sqrtf(l2)) : ng;
+    s.nraw = ns;
     if (dot(ng, dir) > 0.0f) ng = ng * -1.0f;  // face the incoming ray
     if (dot(ns, ng) < 0.0f) ns = ns * -1.0f;
     s.ng = ng;

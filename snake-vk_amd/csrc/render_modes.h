@@ -74,6 +74,7 @@ __device__ __forceinline__ bool gbuffer_surface(const GbufDev& g, const CamDev& 
     if (dot(n, dir) > 0.0f) n = n * -1.0f;
     s.ng = n;
     s.ns = n;
+    s.nraw = n;
     const char4 a = *(const char4*)(g.albedo + 4 * (size_t)px);
     s.m.albedo = V3{fminf(fmaxf(snorm8(a.x), 0.0f), 1.0f), fminf(fmaxf(snorm8(a.y), 0.0f), 1.0f),
                     fminf(fmaxf(snorm8(a.z), 0.0f), 1.0f)};

@@ -477,6 +477,137 @@ __global__ void k_counts3(uint32_t* c, uint32_t a, uint32_t b, uint32_t d) {
     }
 }
 
+// ---- reference-compatibility indirect term (SNKPT_MODE_REFERENCE_COMPAT) ----
+//
+// BounceDiffuse (RayGen.rgen:139-213) as the reference computes it, with PCG32
+// numbers (stream = pixel, state = the pass's ^ kPhaseCompat) in place of the
+// wall-clock gold noise: from biased_pos = p + n (0.01 + max(0.001 |cam - p|,
+// 0.01)) (:552) eight rays rd = UniformSampleHemisphere(r1, r2) in the frame of
+// CreateCoordinateSystem(n) (:123-136), tmin 0.01, tmax 1000, each adding
+// albedo * payload * max(cos(dot(rd, n)), 0) / 8; a non-emissive hit at t sends
+// two more from ro + rd t about the hit's own normal, adding
+// albedo * payload * 0.25 * max(cos(dot(rd', n')), 0) / 8; the sum is scaled by
+// 2 PI.  The payload (RayClosestHit.rchit:87-142, RayMiss.rmiss:15-26): 0 on a
+// miss, the albedo of an emissive hit, else the directional lights through the
+// hit's BRDF (its normal as interpolated, v = -rd) times a shadow test from
+// pos + n 0.01 (tmin 0.001, tmax 10000).  Numbers per pixel, in order: r1, r2
+// of each outer ray, followed by r1, r2 of its two inner rays when it spawns them.
+constexpr uint64_t kPhaseCompat = 0x434f4d50ull;
+constexpr int kCompatBlock = 128;
+
+__device__ __forceinline__ void ref_frame(V3 n, V3& nt, V3& nb) {
+    if (fabsf(n.x) > fabsf(n.y))
+        nt = V3{n.z, 0.0f, -n.x} * (1.0f / sqrtf(n.x * n.x + n.z * n.z));
+    else
+        nt = V3{0.0f, -n.z, n.y} * (1.0f / sqrtf(n.y * n.y + n.z * n.z));
+    nb = cross(n, nt);
+}
+
+__device__ __forceinline__ V3 ref_hemisphere(V3 n, V3 nt, V3 nb, float r1, float r2) {
+    const float st = sqrtf(1.0f - r1 * r1), phi = 2.0f * kRefPi * r2;
+    const float x = st * cosf(phi), z = st * sinf(phi);
+    return V3{x * nb.x + r1 * n.x + z * nt.x, x * nb.y + r1 * n.y + z * nt.y, x * nb.z + r1 * n.z + z * nt.z};
+}
+
+struct CompatHit {
+    bool surface;   // a non-emissive hit (it spawns inner rays)
+    V3 colour;      // payload.colour
+    V3 pos, nraw;   // hit position and normal (RayClosestHit.rchit:109-111)
+};
+
+template <int STACK>
+__device__ CompatHit compat_trace(const SceneDev& S, V3 o, V3 d, uint2* stk, int stride, uint32_t& n_sh) {
+    CompatHit h;
+    h.surface = false;
+    h.colour = V3{0.0f, 0.0f, 0.0f};
+    uint32_t nv = 0, tv = 0;
+    const Ray r{o.x, o.y, o.z, 0.01f, d.x, d.y, d.z, 1000.0f};
+    const TraceResult res = traverse<false, false, STACK>(S.nodes, S.leaf, r, stk, stride, nv, tv);
+    if (res.rec == 0xFFFFFFFFu) return h;
+    const float4 r2 = S.leaf[(size_t)res.rec * kTriWords + 2];
+    float hu, hv;
+    hit_uv(S.leaf, res.rec, r, hu, hv);
+    const Surf s = surface_at(S, d, __float_as_uint(r2.y), __float_as_uint(r2.z), hu, hv);
+    if (s.emissive) {
+        h.colour = s.le;
+        return h;
+    }
+    h.surface = true;
+    h.pos = s.p;
+    h.nraw = s.nraw;
+    const V3 v = d * -1.0f;
+    for (uint32_t k = 0; k < S.n_alights; ++k) {
+        const float4* L = S.alights + 4 * (size_t)k;
+        const float4 w0 = L[0], w1 = L[1];
+        if ((uint32_t)w0.w != 0u) continue;  // directional lights only
+        const V3 l{w1.x, w1.y, w1.z};
+        const V3 c = mul(brdf_eval(s.m, s.nraw, v, l), V3{w0.x, w0.y, w0.z}) * fmaxf(dot(s.nraw, l), 0.0f);
+        if (!(c.x > 0.0f || c.y > 0.0f || c.z > 0.0f)) continue;
+        const V3 so = s.p + s.nraw * 0.01f;
+        const Ray sr{so.x, so.y, so.z, 0.001f, l.x, l.y, l.z, 10000.0f};
+        ++n_sh;
+        if (traverse<true, false, STACK>(S.nodes, S.leaf, sr, stk, stride, nv, tv).rec == 0xFFFFFFFFu)
+            h.colour = h.colour + c;
+    }
+    return h;
+}
+
+template <int STACK>
+__global__ __launch_bounds__(kCompatBlock) void k_compat_bounce(SceneDev S, GView G, float4* __restrict__ rad,
+                                                                uint32_t n, uint64_t initstate,
+                                                                unsigned long long* __restrict__ stats) {
+    __shared__ uint2 stk[STACK * kCompatBlock];
+    __shared__ uint32_t red[2];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (threadIdx.x == 0) red[0] = red[1] = 0u;
+    __syncthreads();
+    uint32_t n_ext = 0, n_sh = 0;
+    if (i < n) {
+        const GSurf g = load_g(G, i);
+        if (g.valid && !(g.m.flags & 1u)) {
+            const V3 nrm = g.ns, alb = g.m.albedo;
+            const V3 ro = g.p + nrm * (0.01f + fmaxf(g.t * 0.001f, 0.01f));
+            V3 nt, nb;
+            ref_frame(nrm, nt, nb);
+            Rng rng = rng_seed(initstate ^ kPhaseCompat, i);
+            V3 sum{0.0f, 0.0f, 0.0f};
+            for (int k = 0; k < 8; ++k) {
+                const float r1 = rng.uniform(), r2 = rng.uniform();
+                const V3 rd = ref_hemisphere(nrm, nt, nb, r1, r2);
+                ++n_ext;
+                const CompatHit h = compat_trace<STACK>(S, ro, rd, &stk[threadIdx.x], kCompatBlock, n_sh);
+                sum = sum + mul(alb, h.colour) * (fmaxf(cosf(dot(rd, nrm)), 0.0f) / 8.0f);
+                if (!h.surface) continue;
+                V3 it, ib;
+                ref_frame(h.nraw, it, ib);
+                for (int b = 0; b < 2; ++b) {
+                    const float q1 = rng.uniform(), q2 = rng.uniform();
+                    const V3 rdi = ref_hemisphere(h.nraw, it, ib, q1, q2);
+                    ++n_ext;
+                    const CompatHit hi = compat_trace<STACK>(S, h.pos, rdi, &stk[threadIdx.x], kCompatBlock, n_sh);
+                    sum = sum + mul(alb, hi.colour) * (0.25f * fmaxf(cosf(dot(rdi, h.nraw)), 0.0f) / 8.0f);
+                }
+            }
+            sum = sum * (2.0f * kRefPi);
+            const float4 o = rad[i];
+            rad[i] = make_float4(o.x + sum.x, o.y + sum.y, o.z + sum.z, o.w);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        n_ext += __shfl_xor(n_ext, o, 64);
+        n_sh += __shfl_xor(n_sh, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&red[0], n_ext);
+        atomicAdd(&red[1], n_sh);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && stats && (red[0] | red[1])) {
+        atomicAdd(&stats[1], (unsigned long long)red[0]);
+        atomicAdd(&stats[2], (unsigned long long)red[1]);
+    }
+}
+
 }  // namespace
 
 #define HIPR(expr)                                                \
@@ -512,7 +643,9 @@ snkpt_status restir_pass(snkpt_ctx* ctx, const snkpt_camera* cam, const CamDev& 
     GView G = gview(X.cur), Gp = gview(X.cur ^ 1);
     uint32_t* counts = R.counts.as<uint32_t>();
     unsigned long long* stats = R.stats.as<unsigned long long>();
-    const uint32_t max_depth = s->max_depth ? s->max_depth : 1u;
+    const bool compat = s->mode == SNKPT_MODE_REFERENCE_COMPAT;
+    // the compatibility estimator replaces the path after the primary vertex
+    const uint32_t max_depth = compat ? 1u : (s->max_depth ? s->max_depth : 1u);
     snkpt_status e = SNKPT_OK;
     const uint32_t n_cand = s->restir_candidates ? s->restir_candidates : 32u;
     if (R.use_gb) {
@@ -573,6 +706,23 @@ snkpt_status restir_pass(snkpt_ctx* ctx, const snkpt_camera* cam, const CamDev& 
     X.have_prev = true;
     R.history_valid = true;
     (void)pass;
+    if (compat) {
+        // BounceDiffuse over this frame's primary vertices (written to G before the swap)
+        const Accel& A = ctx->accel;
+        if (A.num_tris == 0) return SNKPT_OK;
+        const uint32_t need = A.depth > 0 ? A.depth - 1 : 0;
+        const uint32_t grid = (n + kCompatBlock - 1) / kCompatBlock;
+        if (need <= 16)
+            k_compat_bounce<16><<<grid, kCompatBlock, 0, st>>>(S, G, R.rad.as<float4>(), n, initstate, stats);
+        else if (need <= 32)
+            k_compat_bounce<32><<<grid, kCompatBlock, 0, st>>>(S, G, R.rad.as<float4>(), n, initstate, stats);
+        else if (need <= 64)
+            k_compat_bounce<64><<<grid, kCompatBlock, 0, st>>>(S, G, R.rad.as<float4>(), n, initstate, stats);
+        else
+            return fail(ctx, SNKPT_ERR_CAPACITY, "compat bounce: BVH8 deeper than the 64-entry traversal stack");
+        HIPR(hipGetLastError());
+        return SNKPT_OK;
+    }
     // remaining bounces with the plain path tracer
     if (max_depth > 1) return pt_bounces(ctx, s, 1, 1, st);
     return SNKPT_OK;

@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsnkpt.so")
@@ -31,6 +32,7 @@ INVALID_INDEX = 0xFFFFFFFF
 MODE_PT = 0
 MODE_RESTIR = 1
 MODE_CACHE = 2
+MODE_REFERENCE_COMPAT = 3  # the reference's biased BounceDiffuse estimator (ReSTIR direct light + 8 + 16 rays)
 FLAG_ACCUMULATE = 1
 FLAG_RESET = 2
 FLAG_NO_TRAIN = 4
@@ -153,6 +155,11 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build() "
                            "(make -C snake-vk_amd/csrc); there is no CPU fallback")
+    if "torch" in sys.modules:
+        # libsnkpt shares torch's HIP runtime: let torch initialise it first (its
+        # lazy initialisation raised "No HIP GPUs are available" when a snkpt
+        # context had been created before the first torch.cuda call)
+        sys.modules["torch"].cuda.is_available()
     lib = C.CDLL(path)
     P, U32, I32, VP, SZ = C.POINTER, C.c_uint32, C.c_int32, C.c_void_p, C.c_size_t
     sig = {
