@@ -463,10 +463,11 @@ def main():
 
     bytes_per_ray = 56.0 + 80.0 * n_node + 48.0 * n_tri
     achieved = n * bytes_per_ray / (launch_ms * 1e-3) / 1e9
-    traffic = None
+    traffic, prof = None, {}
     if os.path.exists(TRAFFIC_FILE):
         try:
-            traffic = json.load(open(TRAFFIC_FILE)).get("hbm_bytes_per_launch")
+            prof = json.load(open(TRAFFIC_FILE))
+            traffic = prof.get("hbm_bytes_per_launch")
         except Exception:
             traffic = None
 
@@ -543,6 +544,12 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_ray": round(bytes_per_ray, 1),
+                     # compulsory bytes only (ray 32 + path id 4 + hit 20 per ray, no BVH reads)
+                     "compulsory_frac": round(n * 56.0 / (launch_ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+                     # rocprofv3 counters of the same launch (profiles/): HBM bytes over this run's
+                     # launch time, and the L2 / L1 hit rates
+                     "measured_hbm_gbs": (round(traffic / (launch_ms * 1e-3) / 1e9, 1) if traffic else None),
+                     "l2_hit_rate": prof.get("l2_hit_rate"), "l1_hit_rate": prof.get("l1_hit_rate"),
                      "note": "algorithmic bytes = 56 + 80*nodes + 48*tris per ray; the BVH (~70 MB) is "
                              "cache-resident, so frac > 1 would mean cache reuse"},
         "cpu_baseline": cpu,
