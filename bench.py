@@ -130,6 +130,129 @@ def frame_rates(stream, frames=10, warmup=3, only=None):
     return res
 
 
+def f_rows(stream, frames=10, warmup=3):
+    """The rows beside the hot path, on one GPU: (1) frames whose primary
+    vertices come from a G-buffer in the reference's formats (config 4 room,
+    1080p, PT depth 8; the G-buffer is made on the GPU from traced camera rays:
+    D32 clip depth, geometric normal RGBA16F, SNORM8 albedo, RGBA16F rma, R16
+    flags) beside the same frames from camera rays; (2) the TAA resolve of a
+    1080p RGBA32F frame; (3) short particle collision rays with contact records
+    on the 1.02M-triangle scene.  Event-timed on the launch stream."""
+    import numpy as np
+    import torch
+
+    import raysets
+    import snkpt
+    res = {}
+
+    def timed(fn, n, w_up):
+        for k in range(w_up):
+            fn(k)
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n)]
+        for k, (a, b) in enumerate(evs):
+            a.record(stream)
+            fn(w_up + k)
+            b.record(stream)
+        stream.synchronize()
+        return sum(a.elapsed_time(b) for a, b in evs) / n
+
+    # (1) G-buffer input
+    scene = snkpt.SceneData(4)
+    ctx = snkpt.Context(0)
+    ctx.load_scene(scene)
+    w, h = scene.s.width, scene.s.height
+    with torch.cuda.stream(stream):
+        ctx.build_accel(stream)
+        cam = scene.camera(0, w, h)
+        porg, pdr = raysets.primary_rays(cam, w, h)
+        po, pd = torch.from_numpy(porg).cuda(), torch.from_numpy(pdr).cuda()
+        t, inst, prim, _, _ = ctx.trace(po, pd, stream=stream)
+        stream.synchronize()
+        world = torch.from_numpy(np.frombuffer(ctx.export(6).tobytes(), np.float32).reshape(-1, 12).copy()).cuda()
+        _, _, inst5, _, mats = scene.arrays()
+        off = torch.from_numpy(np.concatenate([[0], np.cumsum(inst5[:, 4] // 3)]).astype(np.int64)).cuda()
+        hit = t >= 0
+        ii = torch.where(hit, inst, torch.zeros_like(inst)).long()
+        g = off[ii] + torch.where(hit, prim, torch.zeros_like(prim)).long()
+        tri = world[g]
+        v0, v1, v2 = tri[:, 0:3], tri[:, 3:6], tri[:, 6:9]
+        nrm = torch.linalg.cross(v1 - v0, v2 - v0)
+        nrm = nrm / nrm.norm(dim=1, keepdim=True).clamp_min(1e-30)
+        d3 = pd[:, :3]
+        nrm = torch.where(((nrm * d3).sum(1) > 0)[:, None], -nrm, nrm)
+        p = po[:, :3] + d3 * t[:, None]
+        pv = torch.tensor(np.array(cam.proj_view, np.float32).reshape(4, 4).T.copy()).cuda()
+        clip = p @ pv[:, :3].T + pv[:, 3]
+        depth = torch.where(hit, clip[:, 2] / clip[:, 3], torch.ones_like(t))
+        m = torch.from_numpy(mats.view(np.float32).copy()).cuda()[inst5[:, 3].astype(np.int64)][ii]
+        mflags = torch.from_numpy(mats[:, 12].astype(np.int32)).cuda()[torch.from_numpy(inst5[:, 3].astype(np.int64)).cuda()][ii]
+        gb = {"depth": depth.reshape(h, w).contiguous(),
+              "normal": torch.cat([nrm, torch.ones_like(t)[:, None]], 1).half().reshape(h, w, 4).contiguous(),
+              "albedo": torch.cat([torch.round(m[:, 8:11].clamp(-1, 1) * 127.0), torch.full_like(t, 127.0)[:, None]],
+                                  1).to(torch.int8).reshape(h, w, 4).contiguous(),
+              "rma": torch.stack([m[:, 5], m[:, 6], torch.ones_like(t), torch.ones_like(t)], 1).half()
+                          .reshape(h, w, 4).contiguous(),
+              "mat_flags": torch.where(hit, mflags, torch.zeros_like(mflags)).to(torch.int16).reshape(h, w)
+                                .contiguous()}
+        out = torch.zeros(h, w, 4, device="cuda")
+
+        def st(f):
+            return snkpt.default_settings(max_depth=scene.s.max_depth, rr_start=scene.s.rr_start, seed=scene.s.seed,
+                                          frame=f)
+        gb_ms = timed(lambda f: ctx.render_gbuffer(cam, st(f), gb, out, w, h, stream=stream), frames, warmup)
+        cam_ms = timed(lambda f: ctx.render(cam, st(f), out, w, h, stream=stream), frames, warmup)
+        # (2) TAA resolve of the last frame (zero motion), ping-pong history inside the library
+        cur = out.clone()
+        vel = torch.zeros(h, w, 2, dtype=torch.float16, device="cuda")
+        taa_out = torch.empty_like(cur)
+        ctx.taa_resolve(cur, taa_out, vel, w, h, reset=True, stream=stream)
+        taa_ms = timed(lambda f: ctx.taa_resolve(cur, taa_out, vel, w, h, stream=stream), 50, 5)
+    ctx.close()
+    # per pixel: current 16 B + history 16 B + motion 4 B + previous velocity 4 B read, output 16 B +
+    # next history 16 B + next velocity 4 B written (neighbourhood reads served from LDS)
+    taa_bytes = 76.0 * w * h
+    res["gbuffer_input_c4_1080p"] = {
+        "ms_per_frame": round(gb_ms, 3), "camera_rays_ms_per_frame": round(cam_ms, 3),
+        "note": "PT depth 8, RR from 3; primary vertices from the G-buffer instead of traced camera rays"}
+    res["taa_resolve_1080p"] = {"us": round(taa_ms * 1e3, 1), "bytes_per_pixel": 76,
+                                "hbm_frac": round(taa_bytes / (taa_ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)}
+    # (3) contact rays on the 1.02M-triangle scene
+    scene = snkpt.SceneData(2)
+    ctx = snkpt.Context(0)
+    ctx.load_scene(scene)
+    n = 1 << 21
+    rng = np.random.default_rng(7)
+    pts = np.stack([rng.uniform(-50, 50, n), rng.uniform(0.0, 12.0, n), rng.uniform(-50, 50, n)], 1)
+    dv = rng.normal(size=(n, 3))
+    dv /= np.linalg.norm(dv, axis=1, keepdims=True)
+    r = rng.uniform(0.01, 0.08, n)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3] = pts - dv * r[:, None]
+    rays[:, 4:7] = dv
+    rays[:, 7] = rng.uniform(0.0, 30.0, n) / 120.0 + 2 * r
+    with torch.cuda.stream(stream):
+        ctx.build_accel(stream)
+        o = torch.from_numpy(np.ascontiguousarray(rays[:, :4])).cuda()
+        d = torch.from_numpy(np.ascontiguousarray(rays[:, 4:])).cuda()
+        *_, cnv, ctv = ctx.trace(o, d, any_hit=True, counted=True, stream=stream)
+        ct = ctx.trace_contacts(o, d, 1.0 / 120.0, stream=stream)
+        stream.synchronize()
+        c_ms = timed(lambda f: ctx.trace_contacts(o, d, 1.0 / 120.0, stream=stream), frames, warmup)
+    ctx.close()
+    nn_, nt_ = float(cnv.double().mean()), float(ctv.double().mean())
+    # ray 32 B + nodes + triangles + contact record 60 B (t, normal, point, velocity, instance, primitive)
+    cb = 32.0 + 80.0 * nn_ + 48.0 * nt_ + 60.0
+    res["contacts_c2"] = {"rays": n, "mrays_s": round(n / (c_ms * 1e-3) / 1e6, 1), "us": round(c_ms * 1e3, 1),
+                          "hit_fraction": round(float((ct[0] >= 0).float().mean()), 4),
+                          "bytes_per_ray": round(cb, 1), "hbm_frac": round(n * cb / (c_ms * 1e-3) /
+                                                                            (HBM_PEAK_GBS * 1e9), 4),
+                          "note": "particle-style rays (origin behind a particle of radius 0.01-0.08, "
+                                  "tmax = speed / 120 + 2 r), terminate on first hit"}
+    log(f"bench: f rows: G-buffer frame {gb_ms:.3f} ms (camera rays {cam_ms:.3f}), TAA {taa_ms * 1e3:.1f} us, "
+        f"contacts {res['contacts_c2']['mrays_s']} Mrays/s")
+    return res
+
+
 def batched_config5(stream, frames=10, warmup=2):
     """Config 5: 16 scenes (10k-2M triangles) x 64 views (128^2-1024^2) in one
     batch, depth 6, RR from bounce 3, 1 spp.  Reports frames/s of the whole
@@ -345,7 +468,7 @@ def main():
     ap.add_argument("--no-rmse", action="store_true", help="skip the time-to-RMSE race (PT vs cache)")
     ap.add_argument("--leg", default=None,
                     help="run one workload only and print its record (profiling): "
-                         + ", ".join(f"c{c}_{l}" for c, l in FRAME_RUNS) + ", c5_batched, rmse")
+                         + ", ".join(f"c{c}_{l}" for c, l in FRAME_RUNS) + ", c5_batched, rmse, f_rows")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -378,6 +501,8 @@ def main():
             rec = batched_config5(stream)
         elif args.leg == "rmse":
             rec = time_to_rmse(stream)
+        elif args.leg == "f_rows":
+            rec = f_rows(stream)
         else:
             rec = frame_rates(stream, frames=args.steps, warmup=args.warmup, only=args.leg)
             if not rec:
@@ -516,6 +641,7 @@ def main():
         extra["frames"] = frame_rates(stream)
     if not args.only_trace and not args.no_frames:
         extra["frames"]["c5_batched_pt"] = batched_config5(stream)
+        extra["f_rows"] = f_rows(stream)
     if not args.only_trace and not args.no_rmse:
         extra["time_to_rmse"] = time_to_rmse(stream)
 
