@@ -483,8 +483,8 @@ def main():
         print(json.dumps({"copy_calib_bytes": n * 4, "reps": 3}))
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every box has one MI355X; "
-              "the multi-GPU sharding design is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the path is built and "
+              "measured on one MI355X; how it would shard across GPUs is a design, not code)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
