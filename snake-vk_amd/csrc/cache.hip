@@ -231,41 +231,82 @@ __device__ __forceinline__ void load_x0_tile(const _Float16* __restrict__ x0, ui
 
 constexpr int kInferWaves = 4;
 
+// Inference form of layer_hidden, in place: the product is taken transposed,
+// act^T = W * act^T (A = weight rows, B = activation rows), so a lane's four
+// accumulators are four consecutive outputs of one sample and leave as one 8 B
+// LDS store (16 per lane and layer instead of 64 two-byte stores).  All of the
+// wave's activation fragments are read before the first MFMA, which the stores
+// depend on, so one tile per wave suffices.  Each output is the same 64-term
+// dot product, so the values are those of layer_hidden.
+__device__ __forceinline__ void layer_hidden_t(_Float16* act, const _Float16* __restrict__ W, int lane) {
+    f32x4_t acc[4][4];  // [output tile n][sample tile m]
+#pragma unroll
+    for (int n = 0; n < 4; ++n)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) acc[n][m] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    half8_t x[2][4];
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) x[kb][m] = a_frag(act, m, kb, lane);
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+#pragma unroll
+        for (int n = 0; n < 4; ++n) {
+            const half8_t wn = w_frag(W, n, kb, lane, kWidth);
+#pragma unroll
+            for (int m = 0; m < 4; ++m) acc[n][m] = mfma16(wn, x[kb][m], acc[n][m]);
+        }
+    }
+    typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+#pragma unroll
+    for (int n = 0; n < 4; ++n)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            half4_t o;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) o[e] = (_Float16)fmaxf(acc[n][m][e], 0.0f);
+            const int sample = m * 16 + (lane & 15), out = n * 16 + 4 * (lane >> 4);
+            *(half4_t*)(act + sample * kRow + out) = o;
+        }
+}
+
 // Inference: y[q] = MLP(x0[q]) for 64 samples per wave.
 __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const _Float16* __restrict__ x0, uint32_t n,
                                                                 const uint32_t* __restrict__ n_dev,
                                                                 const _Float16* __restrict__ W,
                                                                 float4* __restrict__ y) {
-    __shared__ _Float16 buf[kInferWaves][2][64 * kRow];
+    __shared__ __attribute__((aligned(16))) _Float16 buf[kInferWaves][64 * kRow];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (n_dev) n = *n_dev;
     if (blockIdx.x * kInferWaves * 64 >= n) return;  // the whole block is past the end
     const uint32_t base = (blockIdx.x * kInferWaves + wv) * 64;  // waves past n run on zero rows
-    _Float16* a0 = buf[wv][0];
-    _Float16* a1 = buf[wv][1];
+    _Float16* a0 = buf[wv];
     load_x0_tile(x0, base, n, a0, lane);
     __syncthreads();
-    layer_hidden(a0, W + 0 * kLayerParams, a1, lane);
-    __syncthreads();
-    layer_hidden(a1, W + 1 * kLayerParams, a0, lane);
-    __syncthreads();
-    layer_hidden(a0, W + 2 * kLayerParams, a1, lane);
-    __syncthreads();
-    layer_hidden(a1, W + 3 * kLayerParams, a0, lane);
-    __syncthreads();
+#pragma unroll
+    for (int l = 0; l < kHidden; ++l) {
+        layer_hidden_t(a0, W + l * kLayerParams, lane);
+        __syncthreads();
+    }
+    // output layer, transposed as well: lanes 0..15 hold outputs 0..3 of samples
+    // m * 16 + lane in acc[m][0..3]
     f32x4_t acc[4];
-    layer_out(a0, W + 4 * kLayerParams, acc, lane);
-    const int c = lane & 15;
-    // gather the three channels per sample through LDS (reuse a1 as fp32 scratch)
-    float* sc = (float*)a1;  // 64 x 4 floats = 1 KB
-    if (c < 4)
 #pragma unroll
-        for (int m = 0; m < 4; ++m)
+    for (int m = 0; m < 4; ++m) acc[m] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-            for (int e = 0; e < 4; ++e) sc[(m * 16 + 4 * (lane >> 4) + e) * 4 + c] = c < kOut ? acc[m][e] : 0.0f;
-    __syncthreads();
-    uint32_t q = base + lane;
-    if (q < n) y[q] = make_float4(sc[lane * 4], sc[lane * 4 + 1], sc[lane * 4 + 2], 0.0f);
+    for (int kb = 0; kb < 2; ++kb) {
+        const half8_t wn = w_frag(W + 4 * kLayerParams, 0, kb, lane, kWidth);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) acc[m] = mfma16(wn, a_frag(a0, m, kb, lane), acc[m]);
+    }
+    if (lane < 16) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const uint32_t q = base + m * 16 + lane;
+            if (q < n) y[q] = make_float4(acc[m][0], acc[m][1], acc[m][2], 0.0f);
+        }
+    }
 }
 
 // dX tile (64 samples x 64 inputs) = dZ (64 x K) * W (K x 64), via W^T rows.
