@@ -172,37 +172,6 @@ __device__ __forceinline__ half8_t w_frag(const _Float16* __restrict__ W, int n,
     return *(const half8_t*)(W + (n * 16 + (lane & 15)) * row_len + kb * 32 + 8 * (lane >> 4));
 }
 
-// Hidden layer: act_out = relu(act_in * W^T), 64 samples x 64 outputs.
-__device__ __forceinline__ void layer_hidden(const _Float16* act_in, const _Float16* __restrict__ W, _Float16* act_out,
-                                             int lane) {
-    f32x4_t acc[4][4];
-#pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-        for (int n = 0; n < 4; ++n) acc[m][n] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int kb = 0; kb < 2; ++kb) {
-        half8_t b[4];
-#pragma unroll
-        for (int n = 0; n < 4; ++n) b[n] = w_frag(W, n, kb, lane, kWidth);
-#pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            half8_t a = a_frag(act_in, m, kb, lane);
-#pragma unroll
-            for (int n = 0; n < 4; ++n) acc[m][n] = mfma16(a, b[n], acc[m][n]);
-        }
-    }
-#pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-        for (int n = 0; n < 4; ++n)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                int row = m * 16 + 4 * (lane >> 4) + e, col = n * 16 + (lane & 15);
-                act_out[row * kRow + col] = (_Float16)fmaxf(acc[m][n][e], 0.0f);
-            }
-}
-
 // Output layer: 64 samples x 16 (3 used) outputs, no activation.  acc[m][e] is
 // sample m*16 + 4(lane/16) + e, channel lane%16.
 __device__ __forceinline__ void layer_out(const _Float16* act_in, const _Float16* __restrict__ W4, f32x4_t acc[4],
@@ -231,14 +200,14 @@ __device__ __forceinline__ void load_x0_tile(const _Float16* __restrict__ x0, ui
 
 constexpr int kInferWaves = 4;
 
-// Inference form of layer_hidden, in place: the product is taken transposed,
-// act^T = W * act^T (A = weight rows, B = activation rows), so a lane's four
+// Hidden layer: out = relu(in * W^T), 64 samples x 64 outputs, computed as
+// out^T = W * in^T (A = weight rows, B = activation rows): a lane's four
 // accumulators are four consecutive outputs of one sample and leave as one 8 B
-// LDS store (16 per lane and layer instead of 64 two-byte stores).  All of the
-// wave's activation fragments are read before the first MFMA, which the stores
-// depend on, so one tile per wave suffices.  Each output is the same 64-term
-// dot product, so the values are those of layer_hidden.
-__device__ __forceinline__ void layer_hidden_t(_Float16* act, const _Float16* __restrict__ W, int lane) {
+// LDS store (16 per lane and layer, not 64 two-byte stores).  All of the wave's
+// input fragments are read before the first MFMA, which the stores depend on,
+// so out may be in (one tile per wave).
+__device__ __forceinline__ void layer_hidden(const _Float16* in, const _Float16* __restrict__ W, _Float16* out,
+                                             int lane) {
     f32x4_t acc[4][4];  // [output tile n][sample tile m]
 #pragma unroll
     for (int n = 0; n < 4; ++n)
@@ -248,7 +217,7 @@ __device__ __forceinline__ void layer_hidden_t(_Float16* act, const _Float16* __
 #pragma unroll
     for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
-        for (int m = 0; m < 4; ++m) x[kb][m] = a_frag(act, m, kb, lane);
+        for (int m = 0; m < 4; ++m) x[kb][m] = a_frag(in, m, kb, lane);
 #pragma unroll
     for (int kb = 0; kb < 2; ++kb) {
 #pragma unroll
@@ -266,8 +235,8 @@ __device__ __forceinline__ void layer_hidden_t(_Float16* act, const _Float16* __
             half4_t o;
 #pragma unroll
             for (int e = 0; e < 4; ++e) o[e] = (_Float16)fmaxf(acc[n][m][e], 0.0f);
-            const int sample = m * 16 + (lane & 15), out = n * 16 + 4 * (lane >> 4);
-            *(half4_t*)(act + sample * kRow + out) = o;
+            const int sample = m * 16 + (lane & 15), col = n * 16 + 4 * (lane >> 4);
+            *(half4_t*)(out + sample * kRow + col) = o;
         }
 }
 
@@ -286,7 +255,7 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const _Float16* 
     __syncthreads();
 #pragma unroll
     for (int l = 0; l < kHidden; ++l) {
-        layer_hidden_t(a0, W + l * kLayerParams, lane);
+        layer_hidden(a0, W + l * kLayerParams, a0, lane);
         __syncthreads();
     }
     // output layer, transposed as well: lanes 0..15 hold outputs 0..3 of samples
@@ -373,8 +342,8 @@ __global__ __launch_bounds__(64) void k_mlp_train(const _Float16* __restrict__ x
                                                   const _Float16* __restrict__ WT, float grad_scale,
                                                   float* __restrict__ partial, float* __restrict__ dx0,
                                                   float* __restrict__ loss_part) {
-    __shared__ _Float16 act[5][64 * kRow];
-    __shared__ _Float16 gb[2][64 * kRow];
+    __shared__ __attribute__((aligned(16))) _Float16 act[5][64 * kRow];
+    __shared__ __attribute__((aligned(16))) _Float16 gb[2][64 * kRow];
     const int lane = threadIdx.x;
     const uint32_t base = blockIdx.x * 64;
     load_x0_tile(x0, base, n, act[0], lane);
