@@ -553,7 +553,7 @@ __device__ CompatHit compat_trace(const SceneDev& S, V3 o, V3 d, uint2* stk, int
 }
 
 template <int STACK>
-__global__ __launch_bounds__(kCompatBlock) void k_compat_bounce(SceneDev S, GView G, float4* __restrict__ rad,
+__global__ __launch_bounds__(kCompatBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_compat_bounce(SceneDev S, GView G, float4* __restrict__ rad,
                                                                 uint32_t n, uint64_t initstate,
                                                                 unsigned long long* __restrict__ stats) {
     __shared__ uint2 stk[STACK * kCompatBlock];
