@@ -33,7 +33,7 @@
 #define LEAF_TAG 0x80000000u
 #define NONE 0xFFFFFFFFu
 #define COST_NODE 1.0f
-#define COST_TRI 0.3f
+#define COST_TRI 2.0f /* a leaf triangle test costs two node visits (accel.hip kCostTri) */
 #define MAX_LEAF_TRIS 3u
 #define BOX_ROBUST 1.0000152587890625f
 
