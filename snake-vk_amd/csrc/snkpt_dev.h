@@ -213,10 +213,12 @@ __device__ __forceinline__ bool trav_tri(const float4* __restrict__ tris, TravSt
 // Leaf triangles a lane may test in one traversal step before it goes on to the
 // next node.  Every triangle of a node is still tested before the next node, so
 // a ray's sequence of tests (and its culling) does not depend on this; it only
-// bounds how long the other lanes of the wave wait for a lane with a full leaf
-// (all of a node's triangles per step: 2.82 Grays/s on the 1.29M incoherent
-// rays; one: 3.09; two: 3.22; three: 3.19; four: 3.09).
-constexpr int kTrisPerStep = 2;
+// bounds how long the other lanes of the wave wait for a lane with a full leaf.
+// 1.29M incoherent rays, Grays/s: with leaves of up to three triangles (SAH
+// triangle cost 0.3) all of a node's triangles per step 2.82, one 3.09, two
+// 3.22, three 3.19, four 3.09; with leaves of mostly one triangle (cost 2.0)
+// two 3.57, one 3.72.
+constexpr int kTrisPerStep = 1;
 
 // One traversal step: up to kTrisPerStep pending leaf triangles, then, when
 // none is left, the next node of the current group (its 8 child boxes, which
