@@ -434,7 +434,12 @@ def time_to_rmse(stream, ref_spp=4096, max_frames=768, warm_frames=256):
                  "cache_q1_indirect": dict(cache_query_depth=1, flags=snkpt.FLAG_CACHE_INDIRECT),
                  "cache_q3_full_warm": dict(cache_query_depth=3, warm=warm_frames),
                  "cache_q1_indirect_warm": dict(cache_query_depth=1, flags=snkpt.FLAG_CACHE_INDIRECT,
-                                                warm=warm_frames)}
+                                                warm=warm_frames),
+                 "cache_q1_indirect_mis": dict(cache_query_depth=1,
+                                               flags=snkpt.FLAG_CACHE_INDIRECT | snkpt.FLAG_CACHE_MIS),
+                 "cache_q1_indirect_mis_warm": dict(cache_query_depth=1,
+                                                    flags=snkpt.FLAG_CACHE_INDIRECT | snkpt.FLAG_CACHE_MIS,
+                                                    warm=warm_frames)}
         caches = {}
         for name, kw in forms.items():
             caches[name] = race(snkpt.MODE_CACHE, **kw)
@@ -451,7 +456,8 @@ def time_to_rmse(stream, ref_spp=4096, max_frames=768, warm_frames=256):
     ctx.close()
     return {"workload": "config 4: room at 1920x1080, 1 spp frames, depth 8, RR from bounce 3, 65,536 training "
                         "paths/frame, Adam lr 1e-2. cache_q3_full: the cache answers L_o at path vertex 3 (after 2 "
-                        "bounces); cache_q1_indirect: NEE at the primary hit, the cache answers its indirect light",
+                        "bounces); cache_q1_indirect: NEE at the primary hit, the cache answers its indirect light; "
+                        "_mis: that vertex's direct light is NEE + a traced BRDF sample under the power heuristic",
             "oracle": {"spp": ref_spp, "mean_radiance": round(ref_mean, 6),
                        "rel_noise_est": round(pt["first_frame_rel_rmse"] / ref_spp ** 0.5, 5)},
             "pt": pt, **caches}

@@ -116,6 +116,10 @@ typedef struct snkpt_pointlight {
 #define SNKPT_FLAG_WHOLE_PATHS 16u   /* plain PT: one thread runs each whole path (no queues); same image */
 #define SNKPT_FLAG_CACHE_INDIRECT 8u /* cache mode: NEE at the query vertex, the cache learns and answers
                                         only the indirect radiance there */
+#define SNKPT_FLAG_CACHE_MIS 32u     /* with SNKPT_FLAG_CACHE_INDIRECT: the query vertex's direct light is NEE
+                                        and one traced BRDF sample under the power heuristic (one more
+                                        closest-hit ray per query) instead of NEE alone, which bounds the
+                                        1/r^2 outliers of shading points next to an emitter (snkpt_render) */
 
 #define SNKPT_FORMAT_RGBA32F 0u
 #define SNKPT_FORMAT_RGBA16F 1u

@@ -1201,12 +1201,15 @@ snkpt_status cache_pass(snkpt_ctx* ctx, const CamDev& cd, const snkpt_settings* 
     H.kind = 1;
     H.query_at = Q - 1;  // bounce index of that vertex
     H.indirect = (s->flags & SNKPT_FLAG_CACHE_INDIRECT) ? 1u : 0u;
+    H.mis = (H.indirect && (s->flags & SNKPT_FLAG_CACHE_MIS)) ? 1u : 0u;
     H.q_samples = C.q_samples.as<float>();
     H.q_meta = C.q_meta.as<float4>();
     H.q_count = qc;
     // with depth >= Q the vertex before the query keeps its continuation (MIS
-    // against the cache vertex's emission); with depth < Q this is plain PT
-    if ((e = primary_bounces(ctx, cd, s, initstate, st, &H, std::min(depth, Q))) != SNKPT_OK) return e;
+    // against the cache vertex's emission); with depth < Q this is plain PT.
+    // The MIS probe of an indirect query vertex is one more bounce.
+    const uint32_t bounces = (H.mis && depth >= Q) ? Q + 1 : std::min(depth, Q);
+    if ((e = primary_bounces(ctx, cd, s, initstate, st, &H, bounces)) != SNKPT_OK) return e;
     // 2. query the cache
     const bool avg = s->cache_ema > 0.0f;  // query the weight averages
     k_encode<<<enc_blocks(n), kEncBlock, 0, st>>>(C.q_samples.as<float>(), n, qc, C.geom,

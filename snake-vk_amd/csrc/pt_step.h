@@ -22,6 +22,7 @@ struct VertexOut {
     V3 c_emit, ext_o, ext_d, sh_o, sh_d, sh_c;
     float sh_t = 0.0f;
     float4 thr4;        // throughput | pdf after the vertex (unchanged when the path ends here)
+    float4 qthr;        // throughput arriving at a cache query vertex
     float qs[13];
 };
 
@@ -46,17 +47,21 @@ __device__ __forceinline__ void path_vertex_surf(const SceneDev& S, const PtPara
         o.emit = true;
         return;
     }
+    // the BRDF probe of an indirect query vertex (H.mis) only looks for emission
+    if (KIND == 1 && H.mis && bounce == H.query_at + 1) return;
     const V3 wo = dir * -1.0f;
     const bool query = KIND == 1 && bounce == H.query_at;
     if (query) {
         o.q = true;
+        o.qthr = thr4;
         float q[13] = {s.p.x, s.p.y, s.p.z, wo.x, wo.y, wo.z, s.ns.x, s.ns.y, s.ns.z, s.m.roughness,
                        s.m.albedo.x, s.m.albedo.y, s.m.albedo.z};
 #pragma unroll
         for (int k = 0; k < 13; ++k) o.qs[k] = q[k];
         // terminate into the radiance cache: L_o(x, wo) from the network
         if (!H.indirect) return;
-        // indirect split: direct light here by NEE alone, the cache answers the rest
+        // indirect split: direct light here by NEE alone (or, with H.mis, NEE and a
+        // traced BRDF sample under the power heuristic), the cache answers the rest
     }
     if (KIND == 2 && bounce < H.max_rec) {
         // training record: sample fields, radiance so far, throughput at arrival
@@ -70,7 +75,9 @@ __device__ __forceinline__ void path_vertex_surf(const SceneDev& S, const PtPara
     }
     float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform(), u3 = rng.uniform();
     float ul = rng.uniform(), us1 = rng.uniform(), us2 = rng.uniform(), urr = rng.uniform();
-    const bool last = bounce + 1 >= P.max_depth || query;
+    // an indirect query vertex with H.mis continues one BRDF sample as the probe
+    const bool probe = KIND == 1 && query && H.mis;
+    const bool last = (bounce + 1 >= P.max_depth || query) && !probe;
     // next-event estimation
     LightSample ls = sample_direct(S, s.p, u0, u1, u2, u3);
     if (ls.ok) {
@@ -147,7 +154,7 @@ __device__ __forceinline__ void vertex_appends(const VertexOut& o, uint32_t path
 #pragma unroll
             for (int k = 0; k < 13; ++k) q[k] = o.qs[k];
             q[13] = q[14] = q[15] = 0.0f;
-            H.q_meta[qslot] = make_float4(o.thr4.x, o.thr4.y, o.thr4.z, __uint_as_float(path));
+            H.q_meta[qslot] = make_float4(o.qthr.x, o.qthr.y, o.qthr.z, __uint_as_float(path));
         }
     }
 }

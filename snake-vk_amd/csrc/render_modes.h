@@ -127,6 +127,7 @@ struct ShadeHooks {
     uint32_t kind = 0;
     uint32_t query_at = 0xFFFFFFFFu;
     uint32_t indirect = 0;
+    uint32_t mis = 0;            // with indirect: NEE and a traced BRDF probe under MIS at the query vertex
     float* q_samples = nullptr;  // 16 floats per query
     float4* q_meta = nullptr;    // throughput rgb, path (bits)
     uint32_t* q_count = nullptr;

@@ -108,9 +108,39 @@ def test_zero_indirect_cache_equals_pt_to_query_depth(gpu):
     assert rel < 0.05
 
 
-def test_cache_term_is_throughput_times_network_output(gpu):
+@pytest.mark.parametrize("q", [1, 2])
+def test_zero_indirect_cache_with_mis_probe_equals_pt_to_query_depth(gpu, q):
+    """Indirect split with the MIS probe, zero network: the query vertex's
+    direct light is NEE plus a traced BRDF sample under the power heuristic, so
+    E[image] = PT with max_depth Q, as without the probe."""
+    import snkpt
+    w = h = 32
+    scene, ctx, cam = _room(w, h)
+    _zero_net(ctx)
+    ref = _oracle(ctx, cam, w, h, 2048, q)
+    acc = np.zeros((h, w, 3))
+    frames = 128
+    for f in range(frames):
+        acc += _render(ctx, cam, w, h, mode=snkpt.MODE_CACHE, max_depth=6, cache_query_depth=q, frame=f, seed=6,
+                       flags=snkpt.FLAG_NO_TRAIN | snkpt.FLAG_CACHE_INDIRECT | snkpt.FLAG_CACHE_MIS)
+    mean = acc / frames
+    st = ctx.frame_stats()
+    tr = mean.reshape(4, 8, 4, 8, 3).mean((1, 3))
+    to = ref.reshape(4, 8, 4, 8, 3).mean((1, 3))
+    rel = np.abs(tr - to).sum() / to.sum()
+    _say(f"zero indirect cache + MIS probe, Q={q}: mean {mean.mean():.5f} vs PT depth {q} {ref.mean():.5f}, "
+         f"tile L1 rel {rel:.4f}, extension rays {st.extension_rays}")
+    assert st.extension_rays > 0
+    assert abs(mean.mean() - ref.mean()) < 0.02 * ref.mean()
+    assert rel < 0.05
+
+
+@pytest.mark.parametrize("form", ["full", "indirect_mis"])
+def test_cache_term_is_throughput_times_network_output(gpu, form):
     """render(W) - render(0) with the same seed = scatter of thr * max(y, 0)
-    over the exported queries, y from snkpt_cache_infer on those queries."""
+    over the exported queries, y from snkpt_cache_infer on those queries (also
+    with the indirect split and its MIS probe, whose continuation must not
+    change the throughput recorded for the query)."""
     import torch
 
     import snkpt
@@ -119,7 +149,10 @@ def test_cache_term_is_throughput_times_network_output(gpu):
     ctx.cache_reset(5)
     W = ctx.cache_params(0).copy()
     W[4 * 4096:] *= 8.0  # a larger output layer so the term is well above rounding
-    kw = dict(mode=snkpt.MODE_CACHE, max_depth=5, cache_query_depth=2, seed=21, frame=4, flags=snkpt.FLAG_NO_TRAIN)
+    fl = snkpt.FLAG_NO_TRAIN
+    if form == "indirect_mis":
+        fl |= snkpt.FLAG_CACHE_INDIRECT | snkpt.FLAG_CACHE_MIS
+    kw = dict(mode=snkpt.MODE_CACHE, max_depth=5, cache_query_depth=2, seed=21, frame=4, flags=fl)
     ctx.cache_set_params(0, W)
     img_w = _render(ctx, cam, w, h, **kw)
     q = ctx.cache_params(5).reshape(-1, 16)
