@@ -31,7 +31,7 @@ constexpr uint32_t kLeafTag = 0x80000000u;
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 // SAH costs of a BVH node visit and of a leaf triangle test.  A triangle costs
 // two node visits: leaves then hold mostly one triangle, and the traversal
-// (one node or up to two triangles per step) spends fewer steps per ray.  On
+// (one node or one triangle per step) spends fewer steps per ray.  On
 // the 1.29M incoherent rays of the 1.02M-triangle scene (nodes / triangles
 // visited per ray, Grays/s): 0.3 -> 12.63 / 5.18, 3.33; 1.0 -> 13.10 / 3.00,
 // 3.53; 2.0 -> 13.41 / 2.26, 3.59; 3.0 -> 13.42 / 2.26, 3.62; hits unchanged.
