@@ -36,7 +36,7 @@ __global__ __launch_bounds__(kBlock) void k_contacts(const float4* __restrict__ 
                                                      uint32_t* __restrict__ prim_out) {
     __shared__ uint2 stk[STACK * kBlock];
     const uint32_t nwaves = gridDim.x * (kBlock / 64);
-    const uint32_t wave = xcd_block(blockIdx.x, gridDim.x) * (kBlock / 64) + (threadIdx.x >> 6);
+    const uint32_t wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
     auto load = [&](uint32_t id, Ray& ray, const uint4*& nd, const float4*& tr) {
         const float4 a = org_tmin[id], d = dir_tmax[id];
         ray = Ray{a.x, a.y, a.z, a.w, d.x, d.y, d.z, d.w};

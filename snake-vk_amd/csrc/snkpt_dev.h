@@ -427,12 +427,4 @@ __device__ __forceinline__ uint32_t block_append(bool want, uint32_t* counter, u
     return want ? slot : 0xFFFFFFFFu;
 }
 
-// Spreads consecutive logical blocks over few XCDs: hardware block b runs on
-// XCD b % 8, so logical blocks [x*q + min(x, r), ...) all land on XCD x and
-// neighbouring rays share that XCD's L2.
-__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
-    uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u, j = b >> 3;
-    return x * q + (x < r ? x : r) + j;
-}
-
 }  // namespace snkpt

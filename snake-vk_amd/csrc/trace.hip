@@ -26,8 +26,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(const float4* __restrict_
                                                        float* __restrict__ v_out, uint32_t* __restrict__ nvis_out,
                                                        uint32_t* __restrict__ tvis_out) {
     __shared__ uint2 stk[STACK * kTraceBlock];
-    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t i = lb * kTraceBlock + threadIdx.x;
+    const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
     const uint32_t nn = count_dev ? *count_dev : n;
     if (i >= nn) return;
     float4 a = org_tmin[i], d = dir_tmax[i];
@@ -91,7 +90,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(STA
     __shared__ uint2 stk[STACK * kTraceBlock];
     const uint32_t nn = count_dev ? *count_dev : n;
     const uint32_t nwaves = gridDim.x * (kTraceBlock / 64);
-    const uint32_t wave = xcd_block(blockIdx.x, gridDim.x) * (kTraceBlock / 64) + (threadIdx.x >> 6);
+    const uint32_t wave = blockIdx.x * (kTraceBlock / 64) + (threadIdx.x >> 6);
     auto load = [&](uint32_t id, Ray& ray, const uint4*& nd, const float4*& tr) {
         float4 a = org_tmin[id], d = dir_tmax[id];
         ray = Ray{a.x, a.y, a.z, a.w, d.x, d.y, d.z, d.w};
