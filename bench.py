@@ -91,14 +91,17 @@ def frame_rates(stream, frames=10, warmup=3, only=None):
             if mode == snkpt.MODE_CACHE:
                 ctx.cache_reset(1, stream=stream)
 
-            def frame(f):
+            def settings(f):
                 s = snkpt.default_settings(mode=mode, max_depth=scene.s.max_depth, rr_start=scene.s.rr_start,
                                            seed=scene.s.seed, frame=f, cache_query_depth=3)
                 if label == "cache_indirect":
                     s.cache_query_depth, s.flags = 1, snkpt.FLAG_CACHE_INDIRECT
                 if cfg == 3:
                     s.num_temporal_resamples, s.num_spatial_resamples = 1, 5
-                ctx.render(cam, s, out, w, h, stream=stream)
+                return s
+
+            def frame(f):
+                ctx.render(cam, settings(f), out, w, h, stream=stream)
             for f in range(warmup):
                 frame(f)
             evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
@@ -108,6 +111,26 @@ def frame_rates(stream, frames=10, warmup=3, only=None):
                 frame(warmup + k)
                 b.record(stream)
             stream.synchronize()
+            gms = None
+            if label == "pt":
+                # the same frame recorded into a HIP graph (no end-of-frame readback)
+                # and replayed: the launch-bound part of a small frame goes away
+                gs = settings(warmup)
+                gs.flags |= snkpt.FLAG_NO_STATS
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g, stream=stream):
+                    ctx.render(cam, gs, out, w, h, stream=stream)
+                for _ in range(2):
+                    g.replay()
+                gev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                       for _ in range(frames)]
+                for a, b in gev:
+                    a.record(stream)
+                    g.replay()
+                    b.record(stream)
+                stream.synchronize()
+                gms = sum(a.elapsed_time(b) for a, b in gev) / frames
+                del g
         ms = sum(a.elapsed_time(b) for a, b in evs) / frames
         st = ctx.frame_stats()
         key = f"c{cfg}_{label}"
@@ -115,6 +138,8 @@ def frame_rates(stream, frames=10, warmup=3, only=None):
                     "depth": scene.s.max_depth, "rays_per_frame": int(st.camera_rays + st.extension_rays +
                                                                       st.shadow_rays)}
         res[key]["mrays_s"] = round(res[key]["rays_per_frame"] / (ms * 1e-3) / 1e6, 1)
+        if gms is not None:
+            res[key]["graph_ms_per_frame"] = round(gms, 3)
         # compulsory bytes of a frame (BVH node/triangle reads excluded): 56 B per closest-hit ray
         # (ray 32 + path id 4 + hit 20), 40 B per shadow ray, 370 B per shaded path vertex
         # (hit, ray, path state, indices + normals, instance, material, inverse transform read;

@@ -120,6 +120,12 @@ typedef struct snkpt_pointlight {
                                         and one traced BRDF sample under the power heuristic (one more
                                         closest-hit ray per query) instead of NEE alone, which bounds the
                                         1/r^2 outliers of shading points next to an emitter (snkpt_render) */
+#define SNKPT_FLAG_NO_STATS 64u      /* snkpt_render returns without reading the frame's ray counts back
+                                        (snkpt_get_frame_stats keeps the previous values) and without
+                                        waiting for the GPU.  With it a SNKPT_MODE_PT frame can be recorded
+                                        into a HIP graph by stream capture, once an uncaptured frame of the
+                                        same size has sized the context's buffers (no counterpart in the
+                                        reference, whose command buffers are re-recorded per frame) */
 
 #define SNKPT_FORMAT_RGBA32F 0u
 #define SNKPT_FORMAT_RGBA16F 1u
@@ -282,7 +288,10 @@ snkpt_status snkpt_release_semaphore(snkpt_ctx* ctx, snkpt_semaphore* sem);
  * RT::RecordRenderCmdBuf (Raytracing.cpp:227-267): renders one frame of
  * settings->spp samples per pixel into out_rgba (device, w*h RGBA, row-major,
  * row 0 at the top).  With SNKPT_FLAG_ACCUMULATE the output is the running mean
- * over all accumulated frames. */
+ * over all accumulated frames.  On a stream under capture only SNKPT_MODE_PT
+ * frames with SNKPT_FLAG_NO_STATS are accepted, at a size no larger than an
+ * earlier uncaptured frame's (SNKPT_ERR_INVALID_ARG otherwise); a replay renders
+ * the captured camera, seed and frame index again. */
 snkpt_status snkpt_render(snkpt_ctx* ctx, const snkpt_camera* cam, const snkpt_settings* settings,
                           void* out_rgba, uint32_t w, uint32_t h, uint32_t out_format,
                           snkpt_stream stream);

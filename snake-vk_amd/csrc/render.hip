@@ -557,6 +557,19 @@ snkpt_status render(snkpt_ctx* ctx, const snkpt_camera* cam, const snkpt_setting
     if ((uint64_t)w * h > (1ull << 26)) return fail(ctx, SNKPT_ERR_CAPACITY, "snkpt_render: image above 2^26 pixels");
     if (s->mode > SNKPT_MODE_REFERENCE_COMPAT) return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_render: unknown mode");
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, SNKPT_ERR_HIP, "hipSetDevice");
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cap) != hipSuccess) return fail(ctx, SNKPT_ERR_HIP, "hipStreamIsCapturing");
+    if (cap != hipStreamCaptureStatusNone) {
+        // a captured frame may launch kernels only: no allocation, no readback,
+        // no host-side state that a replay would not repeat
+        if (s->mode != SNKPT_MODE_PT || !(s->flags & SNKPT_FLAG_NO_STATS))
+            return fail(ctx, SNKPT_ERR_INVALID_ARG,
+                        "snkpt_render: under stream capture only SNKPT_MODE_PT with SNKPT_FLAG_NO_STATS");
+        if (!ctx->render || ctx->render->cap < w * h || ctx->render->counts.ptr == nullptr ||
+            ctx->render->stats.ptr == nullptr)
+            return fail(ctx, SNKPT_ERR_INVALID_ARG,
+                        "snkpt_render: render one uncaptured frame of this size before capturing");
+    }
     snkpt_status e = begin_frame(ctx, w, h, st);
     if (e != SNKPT_OK) return e;
     RenderState& R = *ctx->render;
@@ -594,6 +607,7 @@ snkpt_status render(snkpt_ctx* ctx, const snkpt_camera* cam, const snkpt_setting
     if (s->flags & SNKPT_FLAG_ACCUMULATE) R.accum_frames += spp;
     hipError_t le = hipGetLastError();
     if (le != hipSuccess) return hip_fail(ctx, le, "render launch");
+    if (s->flags & SNKPT_FLAG_NO_STATS) return SNKPT_OK;
     return read_stats(ctx, st, gb ? 0ull : (uint64_t)w * h * spp);
 }
 

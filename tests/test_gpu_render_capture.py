@@ -1,0 +1,121 @@
+"""SNKPT_FLAG_NO_STATS: snkpt_render without its end-of-frame readback.  The
+frame is the same image; a plain PT frame recorded into a HIP graph by stream
+capture renders that image again on every replay, with uncaptured frames of
+the same context run between the replays; what cannot be captured is refused
+before anything is launched."""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import make_ctx
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _settings(**kw):
+    import snkpt
+    kw.setdefault("max_depth", 4)
+    return snkpt.default_settings(**kw)
+
+
+def test_no_stats_frame_is_the_same_image_and_keeps_the_stats(gpu):
+    import torch
+
+    import snkpt
+    _say("capture: no-stats frame, room 64x48")
+    scene = snkpt.SceneData(1)
+    ctx = make_ctx(scene)
+    w, h = 64, 48
+    cam = scene.camera(0, w, h)
+    a = torch.zeros(h, w, 4, device="cuda")
+    b = torch.zeros_like(a)
+    ctx.render(cam, _settings(seed=7), a, w, h)
+    torch.cuda.synchronize()
+    st = ctx.frame_stats()
+    assert st.camera_rays == w * h and st.extension_rays > 0
+    ctx.render(cam, _settings(seed=9, max_depth=2), b, w, h)
+    torch.cuda.synchronize()
+    st2 = ctx.frame_stats()
+    ctx.render(cam, _settings(seed=7, flags=snkpt.FLAG_NO_STATS), a.new_zeros(a.shape), w, h)
+    c = a.new_zeros(a.shape)
+    ctx.render(cam, _settings(seed=7, flags=snkpt.FLAG_NO_STATS), c, w, h)
+    torch.cuda.synchronize()
+    # the no-stats frames left the last readback (seed 9, depth 2) in place
+    st3 = ctx.frame_stats()
+    assert (st3.camera_rays, st3.extension_rays, st3.shadow_rays) == \
+        (st2.camera_rays, st2.extension_rays, st2.shadow_rays)
+    assert torch.equal(a, c)
+
+
+@pytest.mark.parametrize("whole_paths", [False, True])
+def test_captured_pt_frame_replays_match_the_uncaptured_frame(gpu, whole_paths):
+    import torch
+
+    import snkpt
+    _say(f"capture: PT graph replays, whole_paths={whole_paths}")
+    scene = snkpt.SceneData(1)
+    ctx = make_ctx(scene)
+    w, h = 96, 64
+    cam = scene.camera(0, w, h)
+    fl = snkpt.FLAG_NO_STATS | (snkpt.FLAG_WHOLE_PATHS if whole_paths else 0)
+    s = _settings(seed=21, frame=3, flags=fl)
+    want = torch.zeros(h, w, 4, device="cuda")
+    ctx.render(cam, s, want, w, h)  # uncaptured: sizes the context's buffers
+    torch.cuda.synchronize()
+    assert float(want[..., :3].sum()) > 0.0
+    got = torch.zeros_like(want)
+    other = torch.zeros_like(want)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        ctx.render(cam, s, got, w, h)
+    torch.cuda.synchronize()
+    for k in range(4):
+        got.zero_()
+        g.replay()
+        torch.cuda.synchronize()
+        ok = torch.equal(got, want)
+        _say(f"capture: replay {k} equal={ok}")
+        assert ok
+        # uncaptured work on the same context between replays: another seed
+        ctx.render(cam, _settings(seed=100 + k), other, w, h)
+        torch.cuda.synchronize()
+        assert not torch.equal(other, want)
+
+
+def test_capture_refuses_other_modes_and_unsized_frames(gpu):
+    import torch
+
+    import snkpt
+    _say("capture: refusals")
+    scene = snkpt.SceneData(1)
+    ctx = make_ctx(scene)
+    w, h = 32, 32
+    cam = scene.camera(0, w, h)
+    out = torch.zeros(h, w, 4, device="cuda")
+    ctx.render(cam, _settings(seed=1, flags=snkpt.FLAG_NO_STATS), out, w, h)
+    torch.cuda.synchronize()
+    big = torch.zeros(2 * h, 2 * w, 4, device="cuda")
+    cases = [
+        (_settings(seed=1), out, w, h, "without the no-stats flag"),
+        (_settings(seed=1, mode=snkpt.MODE_RESTIR, max_depth=1, flags=snkpt.FLAG_NO_STATS), out, w, h, "ReSTIR"),
+        (_settings(seed=1, flags=snkpt.FLAG_NO_STATS), big, 2 * w, 2 * h, "larger than any uncaptured frame"),
+    ]
+    for s, o, ww, hh, what in cases:
+        _say(f"capture: refuse {what}")
+        g = torch.cuda.CUDAGraph()
+        with pytest.raises(snkpt.SnkptError):
+            with torch.cuda.graph(g):
+                ctx.render(cam, s, o, ww, hh)
+        torch.cuda.synchronize()
+    # the context still renders uncaptured afterwards
+    again = torch.zeros_like(out)
+    ctx.render(cam, _settings(seed=1, flags=snkpt.FLAG_NO_STATS), again, w, h)
+    torch.cuda.synchronize()
+    assert torch.equal(again, out)
+    assert np.isfinite(again.cpu().numpy()).all()
