@@ -119,7 +119,8 @@ typedef struct snkpt_pointlight {
 #define SNKPT_FLAG_CACHE_MIS 32u     /* with SNKPT_FLAG_CACHE_INDIRECT: the query vertex's direct light is NEE
                                         and one traced BRDF sample under the power heuristic (one more
                                         closest-hit ray per query) instead of NEE alone, which bounds the
-                                        1/r^2 outliers of shading points next to an emitter (snkpt_render) */
+                                        1/r^2 outliers of shading points next to an emitter (snkpt_render,
+                                        snkpt_batch_render) */
 #define SNKPT_FLAG_NO_STATS 64u      /* snkpt_render returns without reading the frame's ray counts back
                                         (snkpt_get_frame_stats keeps the previous values) and without
                                         waiting for the GPU.  With it a SNKPT_MODE_PT frame can be recorded

@@ -562,8 +562,10 @@ snkpt_status record_cache_frame(snkpt_batch* B, const snkpt_settings* s, hipStre
     }
     BHIP(B->hist.reserve((size_t)nsc * 8));
     const bool avg = s->cache_ema > 0.0f;
-    // with depth >= Q the vertex before the query keeps its continuation; with depth < Q this is plain PT
-    PtParams P{std::min(depth, Q), s->rr_start};
+    // with depth >= Q the vertex before the query keeps its continuation; with depth < Q this is plain PT.
+    // The MIS probe of an indirect query vertex is one more bounce.
+    const bool mis = (s->flags & SNKPT_FLAG_CACHE_INDIRECT) && (s->flags & SNKPT_FLAG_CACHE_MIS);
+    PtParams P{(mis && depth >= Q) ? Q + 1 : std::min(depth, Q), s->rr_start};
     SceneMany ss{B->d_scenes.as<SceneDev>(), B->d_jobs.as<JobD>(), B->path_job.as<uint32_t>()};
     PathState ps{B->p_thr.as<float4>(), B->p_rng.as<uint2>(), B->p_bounce.as<uint32_t>()};
     ShadowQueue sq{B->s_org.as<float4>(), B->s_dir.as<float4>(), B->s_contrib.as<float4>()};
@@ -575,6 +577,7 @@ snkpt_status record_cache_frame(snkpt_batch* B, const snkpt_settings* s, hipStre
     H.kind = 1;
     H.query_at = Q - 1;
     H.indirect = (s->flags & SNKPT_FLAG_CACHE_INDIRECT) ? 1u : 0u;
+    H.mis = mis ? 1u : 0u;
     H.q_samples = B->q_samples.as<float>();
     H.q_meta = B->q_meta.as<float4>();
     H.q_count = counts + 3;

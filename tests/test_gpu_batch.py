@@ -183,8 +183,8 @@ def _contexts(scene_ids):
     return out
 
 
-@pytest.mark.parametrize("indirect", [False, True])
-def test_batch_cache_form_equals_single_view_cache_renders(gpu, indirect):
+@pytest.mark.parametrize("form", ["full", "indirect", "indirect_mis"])
+def test_batch_cache_form_equals_single_view_cache_renders(gpu, form):
     """Cache form, one view per scene, training on: each batched view equals
     the single-context cache render of that view (own networks, same seeds),
     frame after frame, bit for bit."""
@@ -207,8 +207,10 @@ def test_batch_cache_form_equals_single_view_cache_renders(gpu, indirect):
         b.add_view(si, cam, w, h, seed, o)
         outs.append(o)
         cams.append(cam)
+    indirect = form != "full"
+    fl = (snkpt.FLAG_CACHE_INDIRECT if indirect else 0) | (snkpt.FLAG_CACHE_MIS if form == "indirect_mis" else 0)
     kw = dict(mode=snkpt.MODE_CACHE, max_depth=4, rr_start=3, cache_train_paths=1 << 20,
-              cache_query_depth=1 if indirect else 2, flags=snkpt.FLAG_CACHE_INDIRECT if indirect else 0)
+              cache_query_depth=1 if indirect else 2, flags=fl)
     for frame in range(3):
         b.render(_settings(frame=frame, **kw))
         got = _grab(outs)
