@@ -44,10 +44,18 @@ struct Ray {
 struct RayPre {
     float ox, oy, oz;
     float ix, iy, iz;  // 1/d, NaN on a parallel axis
-    int kx, ky, kz;
     float Sx, Sy, Sz;
-    uint32_t oct;  // bit a set: the ray runs towards -a
-    uint32_t par;  // bit a set: axis a is parallel
+    // kx | ky << 2 | kz << 4 | oct << 6 | par << 9 in one register: the traversal
+    // loop keeps all of RayPre live, and as five registers they made the 64-VGPR
+    // closest-hit loop spill one register to scratch on every step
+    uint32_t bits;
+    __device__ __forceinline__ int kx() const { return (int)(bits & 3u); }
+    __device__ __forceinline__ int ky() const { return (int)((bits >> 2) & 3u); }
+    __device__ __forceinline__ int kz() const { return (int)((bits >> 4) & 3u); }
+    // bit a set: the ray runs towards -a
+    __device__ __forceinline__ uint32_t oct() const { return (bits >> 6) & 7u; }
+    // bit a set: axis a is parallel
+    __device__ __forceinline__ uint32_t par() const { return (bits >> 9) & 7u; }
 };
 
 __device__ __forceinline__ float sel3(float x, float y, float z, int k) { return k == 0 ? x : (k == 1 ? y : z); }
@@ -81,15 +89,13 @@ __device__ __forceinline__ RayPre ray_setup(const Ray& r) {
         kx = ky;
         ky = t;
     }
-    p.kx = kx;
-    p.ky = ky;
-    p.kz = kz;
     p.Sx = sel3(r.dx, r.dy, r.dz, kx) / dkz;
     p.Sy = sel3(r.dx, r.dy, r.dz, ky) / dkz;
     p.Sz = 1.0f / dkz;
-    p.oct = (p.ix < 0.0f ? 1u : 0u) | (p.iy < 0.0f ? 2u : 0u) | (p.iz < 0.0f ? 4u : 0u);
-    p.par = (fabsf(r.dx) < kParallelEps ? 1u : 0u) | (fabsf(r.dy) < kParallelEps ? 2u : 0u) |
-            (fabsf(r.dz) < kParallelEps ? 4u : 0u);
+    const uint32_t oct = (p.ix < 0.0f ? 1u : 0u) | (p.iy < 0.0f ? 2u : 0u) | (p.iz < 0.0f ? 4u : 0u);
+    const uint32_t par = (fabsf(r.dx) < kParallelEps ? 1u : 0u) | (fabsf(r.dy) < kParallelEps ? 2u : 0u) |
+                         (fabsf(r.dz) < kParallelEps ? 4u : 0u);
+    p.bits = (uint32_t)kx | (uint32_t)ky << 2 | (uint32_t)kz << 4 | oct << 6 | par << 9;
     return p;
 }
 
@@ -103,13 +109,13 @@ __device__ __forceinline__ bool intersect_tri(const RayPre& p, float v0x, float 
     float Ax_ = v0x - p.ox, Ay_ = v0y - p.oy, Az_ = v0z - p.oz;
     float Bx_ = v1x - p.ox, By_ = v1y - p.oy, Bz_ = v1z - p.oz;
     float Cx_ = v2x - p.ox, Cy_ = v2y - p.oy, Cz_ = v2z - p.oz;
-    float Akz = sel3(Ax_, Ay_, Az_, p.kz), Bkz = sel3(Bx_, By_, Bz_, p.kz), Ckz = sel3(Cx_, Cy_, Cz_, p.kz);
-    float Ax = sel3(Ax_, Ay_, Az_, p.kx) - p.Sx * Akz;
-    float Ay = sel3(Ax_, Ay_, Az_, p.ky) - p.Sy * Akz;
-    float Bx = sel3(Bx_, By_, Bz_, p.kx) - p.Sx * Bkz;
-    float By = sel3(Bx_, By_, Bz_, p.ky) - p.Sy * Bkz;
-    float Cx = sel3(Cx_, Cy_, Cz_, p.kx) - p.Sx * Ckz;
-    float Cy = sel3(Cx_, Cy_, Cz_, p.ky) - p.Sy * Ckz;
+    float Akz = sel3(Ax_, Ay_, Az_, p.kz()), Bkz = sel3(Bx_, By_, Bz_, p.kz()), Ckz = sel3(Cx_, Cy_, Cz_, p.kz());
+    float Ax = sel3(Ax_, Ay_, Az_, p.kx()) - p.Sx * Akz;
+    float Ay = sel3(Ax_, Ay_, Az_, p.ky()) - p.Sy * Akz;
+    float Bx = sel3(Bx_, By_, Bz_, p.kx()) - p.Sx * Bkz;
+    float By = sel3(Bx_, By_, Bz_, p.ky()) - p.Sy * Bkz;
+    float Cx = sel3(Cx_, Cy_, Cz_, p.kx()) - p.Sx * Ckz;
+    float Cy = sel3(Cx_, Cy_, Cz_, p.ky()) - p.Sy * Ckz;
     U = Cx * By - Cy * Bx;
     V = Ax * Cy - Ay * Cx;
     W = Bx * Ay - By * Ax;
@@ -145,11 +151,11 @@ __device__ __forceinline__ uint32_t parallel_miss_axis(float o, float pa, uint32
 }
 __device__ __forceinline__ uint32_t parallel_miss(const RayPre& p, uint4 w0, uint4 w2, uint4 w3, uint4 w4) {
     uint32_t miss = 0;
-    if (p.par & 1u)
+    if (p.par() & 1u)
         miss |= parallel_miss_axis(p.ox, __uint_as_float(w0.x), w0.w & 0xFFu, w2.x, w2.y, w3.z, w3.w);
-    if (p.par & 2u)
+    if (p.par() & 2u)
         miss |= parallel_miss_axis(p.oy, __uint_as_float(w0.y), (w0.w >> 8) & 0xFFu, w2.z, w2.w, w4.x, w4.y);
-    if (p.par & 4u)
+    if (p.par() & 4u)
         miss |= parallel_miss_axis(p.oz, __uint_as_float(w0.z), (w0.w >> 16) & 0xFFu, w3.x, w3.y, w4.z, w4.w);
     return miss;
 }
@@ -181,7 +187,7 @@ __device__ __forceinline__ bool trav_init(const Ray& ray, TravState& s) {
     s.sp = 0;
     // start with a node group holding the root in slot 0
     s.ng_base = 0;
-    s.ng_hits = (1u << (24 + (0u ^ s.p.oct))) | 1u;
+    s.ng_hits = (1u << (24 + (0u ^ s.p.oct()))) | 1u;
     s.tg_base = 0;
     s.tg_hits = 0;
     return true;
@@ -237,7 +243,7 @@ __device__ __forceinline__ bool trav_step(const uint4* __restrict__ nodes, const
         const bool nx = p.ix < 0.0f, ny = p.iy < 0.0f, nz = p.iz < 0.0f;
         uint32_t k = __builtin_ctz(st.ng_hits >> 24);
         st.ng_hits &= ~(1u << (24 + k));
-        uint32_t s = k ^ p.oct;
+        uint32_t s = k ^ p.oct();
         uint32_t node = st.ng_base + __builtin_popcount(st.ng_hits & 0xFFu & ((1u << s) - 1u));
         if (st.ng_hits & 0xFF000000u) {
             stk[st.sp * stride] = make_uint2(st.ng_base, st.ng_hits);
@@ -262,7 +268,7 @@ __device__ __forceinline__ bool trav_step(const uint4* __restrict__ nodes, const
         uint32_t fz0 = nz ? w3.x : w4.z, fz1 = nz ? w3.y : w4.w;
         // parallel axes: exact origin-in-slab test on the decoded planes
         uint32_t pmiss = 0;
-        if (p.par) pmiss = parallel_miss(p, w0, w2, w3, w4);
+        if (p.par()) pmiss = parallel_miss(p, w0, w2, w3, w4);
         uint32_t int_hits = 0, tri_hits = 0;
 #pragma unroll
         for (int s2 = 0; s2 < 8; ++s2) {
@@ -282,7 +288,7 @@ __device__ __forceinline__ bool trav_step(const uint4* __restrict__ nodes, const
             uint32_t meta = (metaw >> (8 * b)) & 0xFFu;
             if (meta != 0u && tn <= tf && !((pmiss >> s2) & 1u)) {
                 if ((imask >> s2) & 1u)
-                    int_hits |= 1u << (24 + (s2 ^ p.oct));
+                    int_hits |= 1u << (24 + (s2 ^ p.oct()));
                 else
                     tri_hits |= (meta >> 5) << (meta & 31u);
             }
