@@ -49,6 +49,8 @@ def load():
     lib.o_compat_bounce.argtypes = [VP, U32, C.POINTER(OBvh), VP, U32, VP, VP, VP, VP, VP, U32, VP, VP, VP, U32, U32,
                                     U32, U32, U32, VP, I]
     lib.o_compat_bounce.restype = I
+    lib.o_emitter_sample.argtypes = [VP, U32, VP, U32, VP, VP, U32, C.c_uint64, U32, U32, VP, VP, VP]
+    lib.o_emitter_sample.restype = U32
     lib.o_half_to_float.argtypes = [C.c_uint16]
     lib.o_half_to_float.restype = F
     lib.o_primary_surfaces.argtypes = [VP, U32, C.POINTER(OBvh), VP, U32, VP, VP, VP, VP, VP, VP, VP, VP, U32, U32,
@@ -234,6 +236,23 @@ def compat_bounce(arrs, bvh: "Bvh", cam, w, h, seed, frame=0, spp=1, dir_lights=
                            _p(mats), _p(dl) if dl.size else None, dl.shape[0], _p(view), _p(proj), _p(cpos), w, h,
                            spp, seed, frame, _p(out), threads)
     return out
+
+
+def emitter_sample(arrs, n, seed=1):
+    """The oracle's emitter table for a scene (pt_render's arrs tuple) and n
+    draws of its sampler: per emitter (world triangle id, pdf, draw count)."""
+    pos, idx, inst, m4, mats, nrm, em = arrs
+    world = flatten(pos, idx, inst, m4)
+    inst = np.ascontiguousarray(inst, np.uint32)
+    mats = np.ascontiguousarray(mats, np.uint32)
+    em = np.ascontiguousarray(em, np.uint32)
+    cap = int(inst[em, 4].sum() // 3) if em.size else 0
+    gid = np.zeros(max(cap, 1), np.uint32)
+    pdf = np.zeros(max(cap, 1), np.float32)
+    cnt = np.zeros(max(cap, 1), np.uint32)
+    ne = load().o_emitter_sample(_p(world), world.shape[0], _p(inst), inst.shape[0], _p(mats), _p(em), em.size, seed,
+                                 n, cap, _p(gid), _p(pdf), _p(cnt))
+    return gid[:ne], pdf[:ne], cnt[:ne]
 
 
 def half_to_float(h: int) -> float:

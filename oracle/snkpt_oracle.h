@@ -86,6 +86,10 @@ void o_brdf_eval(const float albedo[3], float roughness, float metallic, uint32_
                  const float wo[3], const float wi[3], float out_f[3], float* out_pdf);
 int o_brdf_sample(const float albedo[3], float roughness, float metallic, uint32_t flags, const float n[3],
                   const float wo[3], float ul, float u1, float u2, float out_wi[3]);
+/* Emitter table + `n` draws of the emitter sampler (per emitter: triangle id, pdf, count). */
+uint32_t o_emitter_sample(const float* world, uint32_t ntris, const uint32_t* inst5, uint32_t n_inst,
+                          const uint32_t* mats16, const uint32_t* emissive, uint32_t n_emissive, uint64_t seed,
+                          uint32_t n, uint32_t cap, uint32_t* out_gid, float* out_pdf, uint32_t* out_count);
 /* Mean of `spp` samples per pixel (pass k uses o_pass_state(seed ^ seed_xor, frame, k)). */
 int o_pt_render(const float* world, uint32_t ntris, const o_bvh* bvh, const uint32_t* inst5, uint32_t n_inst,
                 const uint32_t* idx, const float* nrm, const float* mat4s, const uint32_t* mats16,

@@ -391,13 +391,19 @@ typedef struct {
     int ok;
 } olight;
 
+/* Alias-table draw: u0 picks a column, u1 against its probability keeps it or takes its alias. */
+static uint32_t pick_emitter(const oscene* S, float u0, float u1) {
+    uint32_t e = (uint32_t)(u0 * (float)S->n_emit);
+    if (e >= S->n_emit) e = S->n_emit - 1;
+    if (u1 >= S->e_prob[e]) e = S->e_alias[e];
+    return e;
+}
+
 static olight sample_light_c(const oscene* S, v3 p, float u0, float u1, float u2, float u3) {
     olight ls;
     memset(&ls, 0, sizeof(ls));
     if (S->n_emit == 0) return ls;
-    uint32_t e = (uint32_t)(u0 * (float)S->n_emit);
-    if (e >= S->n_emit) e = S->n_emit - 1;
-    if (u1 >= S->e_prob[e]) e = S->e_alias[e];
+    uint32_t e = pick_emitter(S, u0, u1);
     const float* t = S->e_tri + 9 * (size_t)e;
     float su = sqrtf(u2);
     float w1 = su * (1.0f - u3), w2 = su * u3, w0 = 1.0f - su;
@@ -428,6 +434,48 @@ static float light_pdf_c(const oscene* S, uint32_t gid, v3 ng, v3 dir, float t) 
     float cosl = fabsf(dotv(ng, dir));
     if (cosl <= 0.0f) return 0.0f;
     return S->e_pdf[e] * t * t / (S->e_info[4 * (size_t)e + 3] * cosl);
+}
+
+/* The emitter table as o_pt_render builds it, and `n` draws of its emitter
+ * choice from PCG32(seed, 0x51).  Writes, per
+ * emitter in table order: its world triangle id, its selection pdf, and how
+ * often pick_emitter (the sampler o_pt_render uses) drew it.  Returns the emitter count (<= cap). */
+uint32_t o_emitter_sample(const float* world, uint32_t ntris, const uint32_t* inst5, uint32_t n_inst,
+                          const uint32_t* mats16, const uint32_t* emissive, uint32_t n_emissive, uint64_t seed,
+                          uint32_t n, uint32_t cap, uint32_t* out_gid, float* out_pdf, uint32_t* out_count) {
+    oscene S;
+    memset(&S, 0, sizeof(S));
+    S.world = world;
+    S.ntris = ntris;
+    S.inst5 = inst5;
+    S.n_inst = n_inst;
+    S.mats16 = mats16;
+    S.tri_off = (uint32_t*)malloc(sizeof(uint32_t) * (n_inst + 1));
+    S.tri_off[0] = 0;
+    for (uint32_t i = 0; i < n_inst; ++i) S.tri_off[i + 1] = S.tri_off[i] + inst5[5 * i + 4] / 3;
+    build_emitters_c(&S, emissive, n_emissive);
+    uint32_t ne = S.n_emit < cap ? S.n_emit : cap;
+    for (uint32_t e = 0; e < ne; ++e) {
+        out_pdf[e] = S.e_pdf[e];
+        out_count[e] = 0;
+    }
+    for (uint32_t g = 0; g < ntris; ++g)
+        if (S.gid2e[g] < ne) out_gid[S.gid2e[g]] = g;
+    prng r = rseed(seed, 0x51u);
+    for (uint32_t k = 0; k < n && S.n_emit; ++k) {
+        float u0 = runi(&r), u1 = runi(&r);
+        uint32_t e = pick_emitter(&S, u0, u1);
+        if (e < ne) out_count[e]++;
+    }
+    uint32_t total = S.n_emit;
+    free(S.tri_off);
+    free(S.e_tri);
+    free(S.e_info);
+    free(S.e_prob);
+    free(S.e_alias);
+    free(S.e_pdf);
+    free(S.gid2e);
+    return total;
 }
 
 /* --------------------------------------------------------------- camera --- */

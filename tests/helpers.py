@@ -161,3 +161,55 @@ def floor_light_scene(L=10.0, albedo=0.5, side=1.0, height=2.0):
     sc.quad([-h, height, -h], [side, 0, 0], [0, 0, side], [L] * 3, 1)
     verts = [[-h, height, -h], [h, height, -h], [h, height, h], [-h, height, h]]
     return sc, verts
+
+
+class ArraysScene:
+    """A scene given as the path tracer's input tuple (pos, idx, inst5, mat4s,
+    mats16, nrm, emissive_instances), loadable into a context."""
+
+    def __init__(self, arrs):
+        self.arrs = tuple(np.ascontiguousarray(a) for a in arrs)
+
+    def arrays(self):
+        return self.arrs
+
+    def world(self):
+        pos, idx, inst, m4 = self.arrs[:4]
+        return O.flatten(pos, idx, inst, m4)
+
+    def load_into(self, ctx, lights=None):
+        import snkpt
+        pos, idx, inst, m4, mats, nrm, em = self.arrs
+        ctx.upload_mesh(pos, idx, nrm=nrm)
+        ctx.set_instances(inst, m4)
+        ctx.set_materials([snkpt.Material.from_buffer_copy(m.tobytes()) for m in mats])
+        ctx.set_lights(lights, em)
+        ctx.build_accel()
+
+
+def ggx_room(seed=5):
+    """The config-1 room with config 2's material distribution: every
+    non-emissive material GGX (flags 0), roughness U[0.05, 1], metallic
+    Bernoulli(0.3), albedo U[0.1, 0.9]^3; the first two are forced to a
+    rough dielectric and a glossy metal so both lobes always appear."""
+    import snkpt
+    scene = snkpt.SceneData(1)
+    pos, idx, inst, m4, mats, nrm, em = O.scene_arrays(scene)
+    mats = mats.copy()
+    rng = np.random.default_rng(seed)
+    k = 0
+    for m in mats:
+        if m[12] & 1:
+            continue
+        f = m[5:12].view(np.float32)
+        f[0] = rng.uniform(0.05, 1.0)
+        f[1] = float(rng.random() < 0.3)
+        f[3:6] = rng.uniform(0.1, 0.9, 3)
+        if k == 0:
+            f[0], f[1] = 0.6, 0.0
+        elif k == 1:
+            f[0], f[1] = 0.15, 1.0
+        m[12] = 0
+        k += 1
+    cam_scene = scene
+    return ArraysScene((pos.copy(), idx.copy(), inst.copy(), m4.copy(), mats, nrm, em)), cam_scene

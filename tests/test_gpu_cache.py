@@ -62,6 +62,21 @@ def test_encode_and_forward_match_oracle(gpu):
     assert np.sqrt((err ** 2).mean()) < 5e-3 * np.sqrt((want ** 2).mean()) + 1e-3
 
 
+def test_forward_closed_form_on_routing_weights(gpu):
+    """The MFMA forward on hand-set weights that route inputs through the
+    ReLU layers gives the closed-form outputs (FP16 inputs and activations)."""
+    from test_oracle_pins import routing_net
+    import snkpt
+    g, W, grid, s, want = routing_net(1000)
+    ctx = snkpt.Context(0)
+    ctx.cache_reset(1, [0, 0, 0], [1, 1, 1])
+    ctx.cache_set_params(0, W)
+    ctx.cache_set_params(1, grid)
+    y = ctx.cache_infer(_dev(s)).cpu().numpy()[:, :3].astype(np.float64)
+    err = np.abs(y - want)
+    assert np.all(err <= 4e-3 * np.abs(want) + 4e-3), err.max()
+
+
 def test_gradients_match_oracle(gpu):
     ctx, W, grid = _ctx_with_params(3)
     g = O.CacheGeom(LO, HI)
