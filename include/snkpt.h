@@ -369,6 +369,17 @@ snkpt_status snkpt_cache_reset(snkpt_ctx* ctx, uint32_t seed, const float* lo3, 
 /* out_rgba: n x 4 floats (radiance rgb, 0). */
 snkpt_status snkpt_cache_infer(snkpt_ctx* ctx, const float* samples, uint32_t n, float* out_rgba,
                                snkpt_stream stream);
+/* snkpt_cache_infer for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Same kernels and bit-identical outputs, but the
+ * encoded queries go to a static 8 MiB device workspace instead of buffers the
+ * context grows: n <= 65,536 per call (larger n: SNKPT_ERR_CAPACITY), and no
+ * hipMalloc / hipFree or allocating library call is made.  Every call on one GPU
+ * shares that workspace, so calls that overlap on one GPU (from any context or
+ * stream) race; serialise them.  The cache must have been set up with
+ * snkpt_cache_reset (which does allocate). */
+snkpt_status snkpt_cache_infer_noalloc(snkpt_ctx* ctx, const float* samples, uint32_t n, float* out_rgba,
+                                       snkpt_stream stream);
 /* One optimisation step on n samples (relative L2 loss).  apply_adam = 0 only
  * computes the gradients (readable through snkpt_cache_params 2 and 3). */
 snkpt_status snkpt_cache_train(snkpt_ctx* ctx, const float* samples, uint32_t n, float lr, int apply_adam,

@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "cache_nn.h"
@@ -749,6 +750,58 @@ snkpt_status cache_infer(snkpt_ctx* ctx, const float* samples, uint32_t n, float
     C.last_n = n;
     hipError_t le = hipGetLastError();
     if (le != hipSuccess) return hip_fail(ctx, le, "cache_infer launch");
+    return SNKPT_OK;
+}
+
+// ---------------------------------------------- queries without allocation ----
+// Workspace of cache_infer_noalloc: the encoded query rows (64 FP16 network
+// inputs = 128 B per query) for at most kNoAllocQueries queries; 8 MiB held on
+// every GPU the library runs on until the process ends.  Every call on a GPU
+// shares that GPU's copy, so calls that overlap on one GPU (any context, any
+// stream) race.  Nothing in it has to start at zero: k_encode writes every row
+// below n before k_mlp_infer reads rows below n only.
+static __device__ __align__(256) _Float16 scratch[(size_t)kNoAllocQueries * 64];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+_Float16* g_scratch[kMaxScratchDevices] = {};
+}  // namespace
+
+// Looks up this device's copy of `scratch` (the array's name is only a host
+// stand-in); called by snkpt_create with `device` current.
+snkpt_status cache_scratch_init(int device) {
+    if (device < 0 || device >= kMaxScratchDevices) return SNKPT_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device]) return SNKPT_OK;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return SNKPT_ERR_HIP;
+    g_scratch[device] = (_Float16*)p;
+    return SNKPT_OK;
+}
+
+// cache_infer with the encoded rows in `scratch`: no allocation, same kernels,
+// bit-identical outputs.
+snkpt_status cache_infer_noalloc(snkpt_ctx* ctx, const float* samples, uint32_t n, float* out, hipStream_t st) {
+    CacheState* Cp = ctx->cache;
+    if (!Cp || !Cp->ready) return fail(ctx, SNKPT_ERR_NOT_BUILT, "cache_infer_noalloc: call snkpt_cache_reset first");
+    if (n > kNoAllocQueries)
+        return fail(ctx, SNKPT_ERR_CAPACITY, "cache_infer_noalloc: more than 65,536 queries per call");
+    if (n == 0) return SNKPT_OK;
+    _Float16* x0 = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (ctx->device >= 0 && ctx->device < kMaxScratchDevices) x0 = g_scratch[ctx->device];
+    }
+    if (!x0) return fail(ctx, SNKPT_ERR_HIP, "cache_infer_noalloc: workspace address not set up for this device");
+    CacheState& C = *Cp;
+    k_encode<<<enc_blocks(n), kEncBlock, 0, st>>>(samples, n, nullptr, C.geom, C.grid16.as<__half2>(), x0);
+    const uint32_t waves = (n + 63) / 64;
+    k_mlp_infer<<<(waves + kInferWaves - 1) / kInferWaves, 64 * kInferWaves, 0, st>>>(x0, n, nullptr,
+                                                                                      C.mlp16.as<_Float16>(),
+                                                                                      (float4*)out);
+    hipError_t le = hipGetLastError();
+    if (le != hipSuccess) return hip_fail(ctx, le, "cache_infer_noalloc launch");
     return SNKPT_OK;
 }
 

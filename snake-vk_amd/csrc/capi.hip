@@ -83,7 +83,8 @@ snkpt_status snkpt_create(int hip_device, snkpt_ctx** out) {
     snkpt_ctx* ctx = new (std::nothrow) snkpt_ctx();
     if (!ctx) return SNKPT_ERR_OOM;
     ctx->device = hip_device;
-    if (hipSetDevice(hip_device) != hipSuccess || hipGetDeviceProperties(&ctx->props, hip_device) != hipSuccess) {
+    if (hipSetDevice(hip_device) != hipSuccess || hipGetDeviceProperties(&ctx->props, hip_device) != hipSuccess ||
+        cache_scratch_init(hip_device) != SNKPT_OK) {
         delete ctx;
         return SNKPT_ERR_HIP;
     }
@@ -388,6 +389,14 @@ snkpt_status snkpt_cache_infer(snkpt_ctx* ctx, const float* samples, uint32_t n,
     if (n && (!samples || !out_rgba)) return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_cache_infer: null pointer");
     if (n > (1u << 26)) return fail(ctx, SNKPT_ERR_CAPACITY, "snkpt_cache_infer: more than 2^26 samples");
     return cache_infer(ctx, samples, n, out_rgba, (hipStream_t)stream);
+}
+
+snkpt_status snkpt_cache_infer_noalloc(snkpt_ctx* ctx, const float* samples, uint32_t n, float* out_rgba,
+                                       snkpt_stream stream) {
+    if (!ctx) return SNKPT_ERR_INVALID_ARG;
+    if (n && (!samples || !out_rgba))
+        return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_cache_infer_noalloc: null pointer");
+    return cache_infer_noalloc(ctx, samples, n, out_rgba, (hipStream_t)stream);
 }
 
 snkpt_status snkpt_cache_train(snkpt_ctx* ctx, const float* samples, uint32_t n, float lr, int apply_adam,

@@ -159,6 +159,10 @@ snkpt_status taa_resolve(snkpt_ctx* ctx, const void* current, void* out, const u
 // cache.hip
 snkpt_status cache_reset(snkpt_ctx* ctx, uint32_t seed, const float* lo, const float* hi, hipStream_t st);
 snkpt_status cache_infer(snkpt_ctx* ctx, const float* samples, uint32_t n, float* out, hipStream_t st);
+// queries whose workspace is a static device array (no allocation); at most kNoAllocQueries
+constexpr uint32_t kNoAllocQueries = 65536;
+snkpt_status cache_scratch_init(int device);
+snkpt_status cache_infer_noalloc(snkpt_ctx* ctx, const float* samples, uint32_t n, float* out, hipStream_t st);
 snkpt_status cache_train(snkpt_ctx* ctx, const float* samples, uint32_t n, float lr, int apply, float* loss_out,
                          hipStream_t st);
 snkpt_status cache_train_ema(snkpt_ctx* ctx, const float* samples, uint32_t n, float lr, int apply, float ema,

@@ -55,7 +55,7 @@ EXPORTS = [
     "snkpt_batch_autotune", "snkpt_batch_set_tuning", "snkpt_batch_get_tuning", "snkpt_batch_frame_stats",
     "snkpt_render_gbuffer", "snkpt_taa_resolve", "snkpt_trace_contacts", "snkpt_set_prev_transforms",
     "snkpt_import_memory_fd", "snkpt_release_memory", "snkpt_import_semaphore_fd", "snkpt_semaphore_wait",
-    "snkpt_semaphore_signal", "snkpt_release_semaphore",
+    "snkpt_semaphore_signal", "snkpt_release_semaphore", "snkpt_cache_infer_noalloc",
 ]
 CACHE_SAMPLE_FLOATS = 16
 CACHE_MLP_PARAMS = 4 * 4096 + 16 * 64
@@ -186,6 +186,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "snkpt_scene_load": (I32, [VP, P(Scene)]),
         "snkpt_cache_reset": (I32, [VP, U32, VP, VP, VP]),
         "snkpt_cache_infer": (I32, [VP, VP, U32, VP, VP]),
+        "snkpt_cache_infer_noalloc": (I32, [VP, VP, U32, VP, VP]),
         "snkpt_cache_train": (I32, [VP, VP, U32, C.c_float, C.c_int, P(C.c_float), VP]),
         "snkpt_cache_params": (I32, [VP, U32, VP, SZ, P(SZ)]),
         "snkpt_cache_set_params": (I32, [VP, U32, VP, SZ]),
@@ -577,6 +578,13 @@ class Context:
         out = torch.empty(n, 4, dtype=torch.float32, device=samples.device)
         self._check(self.lib.snkpt_cache_infer(self._h, _ptr(samples), n, _ptr(out), _stream(stream)),
                     "snkpt_cache_infer")
+        return out
+
+    def cache_infer_noalloc(self, samples, out, stream=None):
+        """snkpt_cache_infer into a caller-given (n, 4) float32 device tensor,
+        with the library's static workspace (n <= 65,536; no allocation)."""
+        self._check(self.lib.snkpt_cache_infer_noalloc(self._h, _ptr(samples), samples.shape[0], _ptr(out),
+                                                       _stream(stream)), "snkpt_cache_infer_noalloc")
         return out
 
     def cache_train(self, samples, lr: float = 1e-2, apply: bool = True, stream=None) -> float:
