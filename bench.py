@@ -101,7 +101,9 @@ def frame_rates(stream, frames=10, warmup=3, only=None):
                 return s
 
             def frame(f):
-                ctx.render(cam, settings(f), out, w, h, stream=stream)
+                # config 3 renders its seeded 64-frame camera orbit (temporal reuse under motion)
+                c = scene.camera(f, w, h) if cfg == 3 else cam
+                ctx.render(c, settings(f), out, w, h, stream=stream)
             for f in range(warmup):
                 frame(f)
             evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
@@ -147,6 +149,9 @@ def frame_rates(stream, frames=10, warmup=3, only=None):
         hits = int(st.camera_rays + st.extension_rays)
         comp = (56.0 + 370.0) * hits + 40.0 * int(st.shadow_rays)
         res[key]["hbm_frac_compulsory"] = round(comp / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)
+        if mode == snkpt.MODE_RESTIR:
+            res[key]["camera"] = "seeded 64-frame orbit" if cfg == 3 else "static"
+            res[key]["history_reused_frac"] = round(int(st.restir_history_reused) / (w * h), 4)
         if mode == snkpt.MODE_CACHE:
             res[key]["cache_queries"] = int(st.cache_queries)
             res[key]["cache_train_samples"] = int(st.cache_train_samples)

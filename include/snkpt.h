@@ -290,9 +290,12 @@ snkpt_status snkpt_release_semaphore(snkpt_ctx* ctx, snkpt_semaphore* sem);
  * settings->spp samples per pixel into out_rgba (device, w*h RGBA, row-major,
  * row 0 at the top).  With SNKPT_FLAG_ACCUMULATE the output is the running mean
  * over all accumulated frames.  On a stream under capture only SNKPT_MODE_PT
- * frames with SNKPT_FLAG_NO_STATS are accepted, at a size no larger than an
- * earlier uncaptured frame's (SNKPT_ERR_INVALID_ARG otherwise); a replay renders
- * the captured camera, seed and frame index again. */
+ * frames with SNKPT_FLAG_NO_STATS are accepted, at the width and height of the
+ * last uncaptured frame (SNKPT_ERR_INVALID_ARG otherwise); a replay renders the
+ * captured camera, seed and frame index again, and is valid until the context's
+ * generation (snkpt_get_generation) changes: a larger or differently shaped
+ * frame, any scene-input call, snkpt_build_accel, snkpt_cache_reset or
+ * snkpt_destroy invalidates it. */
 snkpt_status snkpt_render(snkpt_ctx* ctx, const snkpt_camera* cam, const snkpt_settings* settings,
                           void* out_rgba, uint32_t w, uint32_t h, uint32_t out_format,
                           snkpt_stream stream);
@@ -337,6 +340,17 @@ snkpt_status snkpt_render_gbuffer(snkpt_ctx* ctx, const snkpt_camera* cam, const
 snkpt_status snkpt_taa_resolve(snkpt_ctx* ctx, const void* current_rgba, void* out_rgba, const uint16_t* velocity,
                                uint32_t w, uint32_t h, uint32_t format, uint32_t reset, snkpt_stream stream);
 
+/* State generation of a context: changes whenever a call changes its scene data
+ * or may move a device buffer that launched work reads (snkpt_upload_mesh,
+ * _set_instances, _set_materials, _set_lights, _set_prev_transforms,
+ * _build_accel, _cache_reset, and a render whose frame size differs from the
+ * last one or grows the render buffers).  A frame captured into a HIP graph
+ * (SNKPT_FLAG_NO_STATS) replays correctly only while the generation is the one
+ * read right after the capture; a batch compares its scenes' generations and
+ * rebuilds its tables and graph when one changed.  No reference counterpart
+ * (the reference re-records its command buffer every frame). */
+snkpt_status snkpt_get_generation(const snkpt_ctx* ctx, uint64_t* out);
+
 /* Rays traced by the last snkpt_render, by kind. */
 typedef struct snkpt_frame_stats {
     uint64_t camera_rays;
@@ -344,6 +358,7 @@ typedef struct snkpt_frame_stats {
     uint64_t shadow_rays;
     uint64_t cache_queries;
     uint64_t cache_train_samples;
+    uint64_t restir_history_reused; /* SNKPT_MODE_RESTIR: pixels whose reprojected history was accepted */
 } snkpt_frame_stats;
 snkpt_status snkpt_get_frame_stats(snkpt_ctx* ctx, snkpt_frame_stats* out);
 
@@ -440,7 +455,10 @@ snkpt_status snkpt_scene_view(uint32_t config, uint32_t view, uint32_t* out_vari
  * bit-identical to snkpt_render of that view alone with the view's seed
  * (plain path tracing, 1 spp per frame).  Scenes are borrowed contexts with a
  * built acceleration structure; they and the output buffers must outlive the
- * batch.  A batch is externally synchronised like a context. */
+ * batch.  A scene may be rebuilt, re-lit or re-uploaded between batch frames:
+ * the next snkpt_batch_render sees its changed generation (snkpt_get_generation)
+ * and rebuilds the batch's tables and graph first.  A batch is externally
+ * synchronised like a context. */
 typedef struct snkpt_batch snkpt_batch;
 typedef struct snkpt_batch_tuning {
     uint32_t trace_block;         /* lanes per traversal block: 64 or 128 (extension and shadow rays) */

@@ -299,6 +299,7 @@ struct snkpt_batch {
         uint32_t j0, j1, total;
     };
     std::vector<snkpt_ctx*> scenes;
+    std::vector<uint64_t> scene_gen;  // each scene context's generation when the tables were built
     std::vector<View> views;
     std::vector<Group> groups;
     snkpt_batch_tuning tuning{128, 16, 8, 0, 0.0f, {0, 0, 0}};
@@ -369,7 +370,9 @@ snkpt_status prepare(snkpt_batch* B, hipStream_t st) {
     if (B->views.empty()) return bfail(B, SNKPT_ERR_INVALID_ARG, "batch: no views");
     std::vector<SceneDev> sd;
     uint32_t depth = 0;
+    B->scene_gen.clear();
     for (snkpt_ctx* c : B->scenes) {
+        B->scene_gen.push_back(c->generation);
         if (!c->accel.built) return bfail(B, SNKPT_ERR_NOT_BUILT, "batch: a scene's acceleration structure is not built");
         if (c->accel.num_tris == 0) return bfail(B, SNKPT_ERR_INVALID_ARG, "batch: a scene has no triangles");
         sd.push_back(scene_dev(c));
@@ -723,6 +726,10 @@ snkpt_status batch_render(snkpt_batch* B, const snkpt_settings* s, int use_graph
     if (use_graph && st == nullptr)
         return bfail(B, SNKPT_ERR_INVALID_ARG, "batch_render: graph capture needs a created (non-null) stream");
     if (hipSetDevice(B->device) != hipSuccess) return bfail(B, SNKPT_ERR_HIP, "hipSetDevice");
+    // a scene rebuilt, re-lit or re-uploaded since the tables were made: its device
+    // pointers, emitter counts or BVH depth may have changed
+    for (size_t k = 0; k < B->scenes.size(); ++k)
+        if (k >= B->scene_gen.size() || B->scenes[k]->generation != B->scene_gen[k]) B->tables_dirty = true;
     const bool rebuilt = B->tables_dirty;
     snkpt_status e = prepare(B, st);
     if (e != SNKPT_OK) return e;

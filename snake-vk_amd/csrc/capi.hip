@@ -107,6 +107,7 @@ snkpt_status snkpt_upload_mesh(snkpt_ctx* ctx, const float* pos, const float* nr
                                uint32_t n_verts, const uint32_t* idx, uint32_t n_idx, uint32_t* out_vertex_offset,
                                uint32_t* out_index_offset) {
     if (!ctx) return SNKPT_ERR_INVALID_ARG;
+    ++ctx->generation;
     if ((n_verts && !pos) || (n_idx && !idx) || n_idx % 3 != 0)
         return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_upload_mesh: null arrays or index count not a multiple of 3");
     for (uint32_t i = 0; i < n_idx; ++i)
@@ -140,6 +141,7 @@ snkpt_status snkpt_upload_mesh(snkpt_ctx* ctx, const float* pos, const float* nr
 snkpt_status snkpt_set_instances(snkpt_ctx* ctx, const snkpt_instance* inst, uint32_t n, const float* mat4s,
                                  uint32_t n_mat4) {
     if (!ctx) return SNKPT_ERR_INVALID_ARG;
+    ++ctx->generation;
     if ((n && !inst) || (n_mat4 && !mat4s)) return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_set_instances: null arrays");
     if (n > SNKPT_MAX_INSTANCES) return fail(ctx, SNKPT_ERR_CAPACITY, "snkpt_set_instances: more than 4096 instances");
     if (n_mat4 > SNKPT_MAX_TRANSFORMS)
@@ -154,6 +156,7 @@ snkpt_status snkpt_set_instances(snkpt_ctx* ctx, const snkpt_instance* inst, uin
 
 snkpt_status snkpt_set_materials(snkpt_ctx* ctx, const snkpt_material* m, uint32_t n) {
     if (!ctx) return SNKPT_ERR_INVALID_ARG;
+    ++ctx->generation;
     if (n && !m) return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_set_materials: null array");
     if (n > SNKPT_MAX_MATERIALS) return fail(ctx, SNKPT_ERR_CAPACITY, "snkpt_set_materials: more than 4096 materials");
     for (uint32_t i = 0; i < n; ++i)
@@ -169,6 +172,7 @@ snkpt_status snkpt_set_materials(snkpt_ctx* ctx, const snkpt_material* m, uint32
 snkpt_status snkpt_set_lights(snkpt_ctx* ctx, const void* light_block, size_t bytes,
                               const uint32_t* emissive_instances, uint32_t n_emissive) {
     if (!ctx) return SNKPT_ERR_INVALID_ARG;
+    ++ctx->generation;
     if (light_block) {
         if (bytes < 112) return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_set_lights: light block shorter than its header");
         uint32_t np, ns;
@@ -215,6 +219,7 @@ static snkpt_status upload_prev_xf(snkpt_ctx* ctx) {
 
 snkpt_status snkpt_set_prev_transforms(snkpt_ctx* ctx, const float* mat4s, uint32_t n_mat4) {
     if (!ctx) return SNKPT_ERR_INVALID_ARG;
+    ++ctx->generation;
     if (n_mat4 && !mat4s) return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_set_prev_transforms: null array");
     if ((size_t)n_mat4 * 16 != ctx->mat4s.size())
         return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_set_prev_transforms: count differs from the current transforms");
@@ -239,6 +244,7 @@ snkpt_status snkpt_trace_contacts(snkpt_ctx* ctx, const snkpt_rays* rays, uint32
 
 snkpt_status snkpt_build_accel(snkpt_ctx* ctx, snkpt_stream stream) {
     if (!ctx) return SNKPT_ERR_INVALID_ARG;
+    ++ctx->generation;
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, SNKPT_ERR_HIP, "hipSetDevice");
     hipStream_t st = (hipStream_t)stream;
     snkpt_status s;
@@ -361,6 +367,12 @@ snkpt_status snkpt_taa_resolve(snkpt_ctx* ctx, const void* current_rgba, void* o
     return taa_resolve(ctx, current_rgba, out_rgba, velocity, w, h, format, reset, (hipStream_t)stream);
 }
 
+snkpt_status snkpt_get_generation(const snkpt_ctx* ctx, uint64_t* out) {
+    if (!ctx || !out) return SNKPT_ERR_INVALID_ARG;
+    *out = ctx->generation;
+    return SNKPT_OK;
+}
+
 snkpt_status snkpt_get_frame_stats(snkpt_ctx* ctx, snkpt_frame_stats* out) {
     if (!ctx || !out) return SNKPT_ERR_INVALID_ARG;
     *out = ctx->stats;
@@ -378,6 +390,7 @@ snkpt_status snkpt_oracle_accumulate(snkpt_ctx* ctx, const snkpt_camera* cam, co
 snkpt_status snkpt_cache_reset(snkpt_ctx* ctx, uint32_t seed, const float* lo3, const float* hi3,
                                snkpt_stream stream) {
     if (!ctx) return SNKPT_ERR_INVALID_ARG;
+    ++ctx->generation;
     if ((lo3 == nullptr) != (hi3 == nullptr)) return fail(ctx, SNKPT_ERR_INVALID_ARG, "snkpt_cache_reset: lo3/hi3");
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, SNKPT_ERR_HIP, "hipSetDevice");
     return cache_reset(ctx, seed, lo3, hi3, (hipStream_t)stream);

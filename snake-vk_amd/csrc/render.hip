@@ -343,12 +343,14 @@ snkpt_status ensure_state(snkpt_ctx* ctx, uint32_t w, uint32_t h, hipStream_t st
         HIPR(R.rad.reserve((size_t)n * 16));
         HIPR(R.accum.reserve((size_t)n * 16));
         R.cap = n;
+        ++ctx->generation;
     }
     HIPR(R.counts.reserve(64));
     HIPR(R.stats.reserve(64));
     if (R.w != w || R.h != h) {
         R.w = w;
         R.h = h;
+        ++ctx->generation;
         k_fill_f4<<<nblk(n, kBlock), kBlock, 0, st>>>(R.accum.as<float4>(), n, make_float4(0, 0, 0, 0));
         R.accum_frames = 0;
         R.history_valid = false;
@@ -544,6 +546,7 @@ static snkpt_status read_stats(snkpt_ctx* ctx, hipStream_t st, uint64_t camera_r
     ctx->stats.shadow_rays = h[2];
     ctx->stats.cache_queries = h[3];
     ctx->stats.cache_train_samples = h[4];
+    ctx->stats.restir_history_reused = h[5];
     return SNKPT_OK;
 }
 
@@ -565,10 +568,10 @@ snkpt_status render(snkpt_ctx* ctx, const snkpt_camera* cam, const snkpt_setting
         if (s->mode != SNKPT_MODE_PT || !(s->flags & SNKPT_FLAG_NO_STATS))
             return fail(ctx, SNKPT_ERR_INVALID_ARG,
                         "snkpt_render: under stream capture only SNKPT_MODE_PT with SNKPT_FLAG_NO_STATS");
-        if (!ctx->render || ctx->render->cap < w * h || ctx->render->counts.ptr == nullptr ||
+        if (!ctx->render || ctx->render->w != w || ctx->render->h != h || ctx->render->counts.ptr == nullptr ||
             ctx->render->stats.ptr == nullptr)
             return fail(ctx, SNKPT_ERR_INVALID_ARG,
-                        "snkpt_render: render one uncaptured frame of this size before capturing");
+                        "snkpt_render: render one uncaptured frame of this size (the last one) before capturing");
     }
     snkpt_status e = begin_frame(ctx, w, h, st);
     if (e != SNKPT_OK) return e;

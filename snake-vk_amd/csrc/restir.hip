@@ -255,20 +255,19 @@ struct PrevCam {
     V3 pos;
 };
 
-// Temporal reuse (WriteInitialReservoirs, RayGen.rgen:411-468) with 1/Z weights.
-__global__ __launch_bounds__(kBlock) void k_temporal(SceneDev S, GView G, GView Gp, const float4* __restrict__ r_in,
-                                                     const float* __restrict__ m_in, const float4* __restrict__ r_hist,
-                                                     const float* __restrict__ m_hist, float4* __restrict__ r_out,
-                                                     float* __restrict__ m_out, uint32_t w, uint32_t h, PrevCam pc,
-                                                     float m_cap, uint64_t initstate,
-                                                     const uint16_t* __restrict__ motion) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= w * h) return;
+// Temporal reuse (WriteInitialReservoirs, RayGen.rgen:411-468) with 1/Z weights,
+// for pixel i; true when the pixel's history was found and combined.
+__device__ __forceinline__ bool temporal_pixel(const SceneDev& S, const GView& G, const GView& Gp, uint32_t i,
+                                               const float4* __restrict__ r_in, const float* __restrict__ m_in,
+                                               const float4* __restrict__ r_hist, const float* __restrict__ m_hist,
+                                               float4* __restrict__ r_out, float* __restrict__ m_out, uint32_t w,
+                                               uint32_t h, const PrevCam& pc, float m_cap, uint64_t initstate,
+                                               const uint16_t* __restrict__ motion) {
     Res rc = load_res(r_in, m_in, i);
     GSurf g = load_g(G, i);
     if (!g.valid) {
         store_res(r_out, m_out, i, rc);
-        return;
+        return false;
     }
     int q = -1;
     if (motion) {
@@ -303,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void k_temporal(SceneDev S, GView G, GView 
     }
     if (!gp.valid) {
         store_res(r_out, m_out, i, rc);
-        return;
+        return false;
     }
     Res rh = load_res(r_hist, m_hist, (uint32_t)q);
     if (rh.M > m_cap * rc.M) rh.M = m_cap * rc.M;
@@ -327,6 +326,27 @@ __global__ __launch_bounds__(kBlock) void k_temporal(SceneDev S, GView G, GView 
     s.M = rc.M + rh.M;
     s.W = ph > 0.0f ? s.wsum / ph : 0.0f;
     store_res(r_out, m_out, i, s);
+    return true;
+}
+
+// stats[5] += pixels whose history was reused (one atomic per block)
+__global__ __launch_bounds__(kBlock) void k_temporal(SceneDev S, GView G, GView Gp, const float4* __restrict__ r_in,
+                                                     const float* __restrict__ m_in, const float4* __restrict__ r_hist,
+                                                     const float* __restrict__ m_hist, float4* __restrict__ r_out,
+                                                     float* __restrict__ m_out, uint32_t w, uint32_t h, PrevCam pc,
+                                                     float m_cap, uint64_t initstate,
+                                                     const uint16_t* __restrict__ motion,
+                                                     unsigned long long* __restrict__ stats) {
+    __shared__ uint32_t red;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (threadIdx.x == 0) red = 0u;
+    __syncthreads();
+    const bool reused = i < w * h && temporal_pixel(S, G, Gp, i, r_in, m_in, r_hist, m_hist, r_out, m_out, w, h, pc,
+                                                    m_cap, initstate, motion);
+    const uint64_t b = __ballot(reused);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(&red, (uint32_t)__popcll(b));
+    __syncthreads();
+    if (threadIdx.x == 0 && stats && red) atomicAdd(&stats[5], (unsigned long long)red);
 }
 
 // Spatial reuse (ComputeAndWriteSpatialReservoirReuse, RayGen.rgen:470-506) with 1/Z weights.
@@ -678,7 +698,7 @@ snkpt_status restir_pass(snkpt_ctx* ctx, const snkpt_camera* cam, const CamDev& 
         k_temporal<<<grid_for(n, kBlock), kBlock, 0, st>>>(S, G, Gp, X.r_init.as<float4>(), X.m_init.as<float>(),
                                                            X.r_fin.as<float4>(), X.m_fin.as<float>(),
                                                            X.r_tmp.as<float4>(), X.m_tmp.as<float>(), w, h, pc, m_cap,
-                                                           initstate, R.use_gb ? R.gb.motion : nullptr);
+                                                           initstate, R.use_gb ? R.gb.motion : nullptr, stats);
         r_sp_in = X.r_tmp.as<float4>();
         m_sp_in = X.m_tmp.as<float>();
     }

@@ -118,6 +118,11 @@ struct snkpt_ctx {
     snkpt::CacheState* cache = nullptr;
     snkpt::TaaState* taa = nullptr;
     snkpt_frame_stats stats{};
+    // State generation: advanced by every call that changes scene data or may move
+    // a device buffer that launched work reads (scene input, build, render-buffer
+    // growth, cache reset).  Batches compare it to rebuild their tables; callers
+    // that capture frames re-capture when it changes.
+    uint64_t generation = 1;
 };
 
 namespace snkpt {

@@ -55,7 +55,7 @@ EXPORTS = [
     "snkpt_batch_autotune", "snkpt_batch_set_tuning", "snkpt_batch_get_tuning", "snkpt_batch_frame_stats",
     "snkpt_render_gbuffer", "snkpt_taa_resolve", "snkpt_trace_contacts", "snkpt_set_prev_transforms",
     "snkpt_import_memory_fd", "snkpt_release_memory", "snkpt_import_semaphore_fd", "snkpt_semaphore_wait",
-    "snkpt_semaphore_signal", "snkpt_release_semaphore", "snkpt_cache_infer_noalloc",
+    "snkpt_semaphore_signal", "snkpt_release_semaphore", "snkpt_cache_infer_noalloc", "snkpt_get_generation",
 ]
 CACHE_SAMPLE_FLOATS = 16
 CACHE_MLP_PARAMS = 4 * 4096 + 16 * 64
@@ -109,7 +109,8 @@ class AccelInfo(C.Structure):
 
 class FrameStats(C.Structure):
     _fields_ = [("camera_rays", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
-                ("cache_queries", C.c_uint64), ("cache_train_samples", C.c_uint64)]
+                ("cache_queries", C.c_uint64), ("cache_train_samples", C.c_uint64),
+                ("restir_history_reused", C.c_uint64)]
 
 
 class Gbuffer(C.Structure):
@@ -187,6 +188,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "snkpt_cache_reset": (I32, [VP, U32, VP, VP, VP]),
         "snkpt_cache_infer": (I32, [VP, VP, U32, VP, VP]),
         "snkpt_cache_infer_noalloc": (I32, [VP, VP, U32, VP, VP]),
+        "snkpt_get_generation": (I32, [VP, VP]),
         "snkpt_cache_train": (I32, [VP, VP, U32, C.c_float, C.c_int, P(C.c_float), VP]),
         "snkpt_cache_params": (I32, [VP, U32, VP, SZ, P(SZ)]),
         "snkpt_cache_set_params": (I32, [VP, U32, VP, SZ]),
@@ -557,6 +559,13 @@ class Context:
         rc = self.lib.snkpt_taa_resolve(self._h, _ptr(current), _ptr(out), _ptr(velocity), w, h, fmt, int(reset),
                                         _stream(stream))
         self._check(rc, "snkpt_taa_resolve")
+
+    def generation(self) -> int:
+        """The context's state generation (changes whenever a captured frame
+        or a batch would have to be recorded again)."""
+        g = C.c_uint64()
+        self._check(self.lib.snkpt_get_generation(self._h, C.byref(g)), "snkpt_get_generation")
+        return g.value
 
     def frame_stats(self) -> FrameStats:
         st = FrameStats()

@@ -245,3 +245,102 @@ def test_batch_cache_form_frozen_many_views(gpu):
         single[si][1].render(cams[k], _settings(frame=1, seed=seed, **kw), o, w, h)
         torch.cuda.synchronize()
         assert np.array_equal(got[k], o.cpu().numpy()), k
+
+
+def test_full_config5_batch_matches_single_views_and_oracle(gpu):
+    """The whole config-5 workload (16 scenes of 10k-2M triangles, 64 views of
+    128^2-1024^2, depth 6, RR from bounce 3, 1 spp): one eager frame and one
+    graph-replayed frame, every view bit-identical to its own snkpt_render,
+    and the smallest scene's smallest view per pixel against the CPU oracle."""
+    import torch
+
+    import snkpt
+    from oracle import binding as O
+    scenes, ctxs = [], []
+    for v in range(16):
+        sd = snkpt.SceneData(5, v)
+        c = snkpt.Context(0)
+        c.load_scene(sd)
+        c.build_accel()
+        scenes.append(sd)
+        ctxs.append(c)
+    tris = [int(sd.s.n_idx // 3) for sd in scenes]
+    _say(f"config 5: 16 scenes, {min(tris)}..{max(tris)} triangles")
+    b = snkpt.Batch(0)
+    for c in ctxs:
+        b.add_scene(c)
+    views, outs = [], []
+    for view in range(64):
+        var, w, h = snkpt.scene_view(5, view)
+        cam = scenes[var].camera(view, w, h)
+        out = torch.zeros(h, w, 4, device="cuda")
+        b.add_view(var, cam, w, h, 0x5000 + view, out)
+        views.append((var, w, h, cam))
+        outs.append(out)
+    assert sum(w * h for _, w, h, _ in views) > 10_000_000
+    st = torch.cuda.Stream()
+    for frame, graph in ((0, False), (1, True)):
+        for o in outs:
+            o.zero_()
+        torch.cuda.synchronize()
+        b.render(_settings(frame=frame), use_graph=graph, stream=st)
+        st.synchronize()
+        got = _grab(outs)
+        for k, (var, w, h, cam) in enumerate(views):
+            o = torch.zeros(h, w, 4, device="cuda")
+            ctxs[var].render(cam, _settings(seed=0x5000 + k, frame=frame), o, w, h)
+            want = o.cpu().numpy()
+            assert np.array_equal(got[k], want), (frame, graph, k, np.abs(got[k] - want).max())
+        _say(f"frame {frame} (graph={graph}): 64 views bit-identical to single-view renders")
+        assert all(np.isfinite(g).all() for g in got)
+    k = min(range(64), key=lambda k: (tris[views[k][0]], views[k][1] * views[k][2]))
+    var, w, h, cam = views[k]
+    arrs = O.scene_arrays(scenes[var])
+    c = O.pt_render(arrs, O.Bvh(O.flatten(*arrs[:4])), cam, w, h, 1, 6, 3, 0x5000 + k, frame=1)
+    g = got[k][..., :3]
+    frac = (np.abs(g - c) <= 1e-3 * np.abs(c) + 1e-5).all(-1).mean()
+    _say(f"view {k} ({w}x{h}, scene {var}, {tris[var]} tris) vs oracle: {frac:.4f} of the pixels agree")
+    assert frac >= 0.97, frac
+    b.close()
+
+
+def test_batch_follows_scene_rebuilds_between_frames(gpu):
+    """A scene context rebuilt with moved instances and given lights between
+    two graph-replayed batch frames: the batch sees the new generation,
+    rebuilds its tables and graph, and every view still equals its own
+    snkpt_render of the changed scene."""
+    import torch
+
+    import snkpt
+    scenes, ctxs, b, outs, cams = _setup()
+    st = torch.cuda.Stream()
+    b.render(_settings(frame=0), use_graph=True, stream=st)
+    st.synchronize()
+    want0 = _single(ctxs, cams, 0)
+    got0 = _grab(outs)
+    for k in range(len(VIEWS)):
+        assert np.array_equal(got0[k], want0[k]), k
+    g0 = ctxs[0].generation()
+    pos, idx, inst, m4, mats = scenes[0].arrays()
+    moved = np.asarray(m4, np.float32).reshape(-1, 16).copy()
+    moved[:, 12] += 0.15  # column-major x translation of every transform
+    ctxs[0].set_instances(inst, moved)
+    ctxs[0].set_lights(snkpt.light_block(points=[((4.0, 3.0, 2.0), (0.2, 1.6, 0.1), (0.0, 0.0, 1.0))]), None)
+    ctxs[0].build_accel()
+    ctxs[2].set_lights(snkpt.light_block(dir_colour=(0.5, 0.5, 0.5), dir_to_light=(0.3, 1.0, 0.2)), None)
+    ctxs[2].build_accel()
+    assert ctxs[0].generation() != g0
+    for frame in (1, 2):
+        for o in outs:
+            o.zero_()
+        torch.cuda.synchronize()
+        b.render(_settings(frame=frame), use_graph=True, stream=st)
+        st.synchronize()
+        got = _grab(outs)
+        want = _single(ctxs, cams, frame)
+        for k in range(len(VIEWS)):
+            assert np.array_equal(got[k], want[k]), (frame, k, np.abs(got[k] - want[k]).max())
+    moved_views = [k for k, v in enumerate(VIEWS) if v[0] in (0, 2)]
+    assert any(not np.array_equal(want0[k], _single(ctxs, cams, 0)[k]) for k in moved_views)
+    _say("batch after scene rebuilds: views bit-identical to single-view renders")
+    b.close()

@@ -105,6 +105,9 @@ def test_capture_refuses_other_modes_and_unsized_frames(gpu):
         (_settings(seed=1), out, w, h, "without the no-stats flag"),
         (_settings(seed=1, mode=snkpt.MODE_RESTIR, max_depth=1, flags=snkpt.FLAG_NO_STATS), out, w, h, "ReSTIR"),
         (_settings(seed=1, flags=snkpt.FLAG_NO_STATS), big, 2 * w, 2 * h, "larger than any uncaptured frame"),
+        (_settings(seed=1, flags=snkpt.FLAG_NO_STATS, frame=0), big, w // 2, h, "smaller than the last frame"),
+        (_settings(seed=1, flags=snkpt.FLAG_NO_STATS | snkpt.FLAG_ACCUMULATE), big, w, h // 2,
+         "another shape of the same size"),
     ]
     for s, o, ww, hh, what in cases:
         _say(f"capture: refuse {what}")
@@ -119,3 +122,42 @@ def test_capture_refuses_other_modes_and_unsized_frames(gpu):
     torch.cuda.synchronize()
     assert torch.equal(again, out)
     assert np.isfinite(again.cpu().numpy()).all()
+
+
+def test_generation_advances_on_every_state_change(gpu):
+    """snkpt_get_generation changes after each call that may move or change what
+    a captured frame or a batch reads (scene input, build, render-buffer
+    growth, cache reset), and stays put across frames of one size."""
+    import torch
+
+    import snkpt
+    from helpers import floor_light_scene
+    scene = snkpt.SceneData(1)
+    ctx = make_ctx(scene)
+    w, h = 32, 32
+    cam = scene.camera(0, w, h)
+    out = torch.zeros(2 * h, 2 * w, 4, device="cuda")
+    g = [ctx.generation()]
+
+    def step(what, fn):
+        fn()
+        torch.cuda.synchronize()
+        g.append(ctx.generation())
+        _say(f"generation after {what}: {g[-1]}")
+        assert g[-1] != g[-2], what
+    step("first frame", lambda: ctx.render(cam, _settings(seed=1), out, w, h))
+    before = ctx.generation()
+    ctx.render(cam, _settings(seed=2), out, w, h)
+    ctx.render(cam, _settings(seed=3, flags=snkpt.FLAG_NO_STATS), out, w, h)
+    torch.cuda.synchronize()
+    assert ctx.generation() == before, "frames of one size keep the generation"
+    step("a larger frame", lambda: ctx.render(cam, _settings(seed=1), out, 2 * w, 2 * h))
+    pos, idx, inst, m4, mats = scene.arrays()
+    step("set_instances", lambda: ctx.set_instances(inst, m4))
+    step("build_accel", lambda: ctx.build_accel())
+    ms = [snkpt.Material.from_buffer_copy(m.tobytes()) for m in mats]
+    step("set_materials", lambda: ctx.set_materials(ms))
+    step("set_lights", lambda: ctx.set_lights(snkpt.light_block(points=[((1, 1, 1), (0, 1.5, 0), (0, 0, 1))]), None))
+    sc, _ = floor_light_scene()
+    step("upload_mesh", lambda: ctx.upload_mesh(*sc.arrays()[:2]))
+    step("cache_reset", lambda: ctx.cache_reset(1, [-1, 0, -1], [1, 2, 1]))
