@@ -350,12 +350,29 @@ __global__ __launch_bounds__(kBlock) void k_temporal(SceneDev S, GView G, GView 
 }
 
 // Spatial reuse (ComputeAndWriteSpatialReservoirReuse, RayGen.rgen:470-506) with 1/Z weights.
+// A block shades one kSpatTile x kSpatTile pixel tile, so the neighbours its
+// pixels read (within `radius` pixels) overlap in L2 far more than for a run of
+// one row.  Each input surface is read once: the balance-heuristic sums run with
+// the surfaces in the outer loop and the inputs' samples in the inner one (the
+// same additions in the same order as sample-major loops, so the same bits).
+constexpr uint32_t kSpatTile = 16;
+static_assert(kSpatTile * kSpatTile == kBlock, "one pixel per thread");
+
+__device__ __forceinline__ bool neighbour_ok(const GView& G, uint32_t q, const GSurf& g) {
+    // the reuse test needs the validity, shading normal and distance only (3 of 5 words)
+    const float4 p = G.p[q], n = G.n[q], wv = G.w[q];
+    const V3 ns{n.x, n.y, n.z};
+    return p.w > 0.0f && !(dot(ns, g.ns) < 0.9f) && !(fabsf(wv.w - g.t) > 0.1f * g.t);
+}
+
 __global__ __launch_bounds__(kBlock) void k_spatial(SceneDev S, GView G, const float4* __restrict__ r_in,
                                                     const float* __restrict__ m_in, float4* __restrict__ r_out,
                                                     float* __restrict__ m_out, uint32_t w, uint32_t h, uint32_t k,
                                                     float radius, uint64_t initstate) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= w * h) return;
+    const uint32_t x = blockIdx.x * kSpatTile + (threadIdx.x % kSpatTile);
+    const uint32_t y = blockIdx.y * kSpatTile + (threadIdx.x / kSpatTile);
+    if (x >= w || y >= h) return;
+    const uint32_t i = y * w + x;
     Res r0 = load_res(r_in, m_in, i);
     GSurf g = load_g(G, i);
     if (!g.valid || k == 0) {
@@ -363,7 +380,6 @@ __global__ __launch_bounds__(kBlock) void k_spatial(SceneDev S, GView G, const f
         return;
     }
     Rng rr = rng_seed(initstate ^ kPhaseSpat, i);
-    const int x = (int)(i % w), y = (int)(i / w);
     // inputs: this pixel's reservoir and up to k accepted neighbours
     uint32_t px[kMaxSpatial + 1];
     px[0] = i;
@@ -372,34 +388,53 @@ __global__ __launch_bounds__(kBlock) void k_spatial(SceneDev S, GView G, const f
     for (uint32_t j = 0; j < k; ++j) {
         float ur = rr.uniform(), ua = rr.uniform();
         float rad = radius * sqrtf(ur), ang = 2.0f * kPi * ua;
-        int nx = x + (int)lrintf(rad * cosf(ang)), ny = y + (int)lrintf(rad * sinf(ang));
-        if (nx < 0 || ny < 0 || nx >= (int)w || ny >= (int)h || (nx == x && ny == y)) continue;
+        int nx = (int)x + (int)lrintf(rad * cosf(ang)), ny = (int)y + (int)lrintf(rad * sinf(ang));
+        if (nx < 0 || ny < 0 || nx >= (int)w || ny >= (int)h || (nx == (int)x && ny == (int)y)) continue;
         uint32_t q = (uint32_t)ny * w + (uint32_t)nx;
-        GSurf gq = load_g(G, q);
-        if (!gq.valid || dot(gq.ns, g.ns) < 0.9f || fabsf(gq.t - g.t) > 0.1f * g.t) continue;
+        if (!neighbour_ok(G, q, g)) continue;
         px[ni++] = q;
     }
-    float Ms[kMaxSpatial + 1];
-    for (int l = 0; l < ni; ++l) Ms[l] = m_in[px[l]];
+    // the inputs' samples and their uniforms (drawn in input order)
+    float Ms[kMaxSpatial + 1], uw[kMaxSpatial + 1], Wl[kMaxSpatial + 1], b1[kMaxSpatial + 1], b2[kMaxSpatial + 1];
+    uint32_t el[kMaxSpatial + 1];
+    float den[kMaxSpatial + 1], p_self[kMaxSpatial + 1], p_l[kMaxSpatial + 1];
+#pragma unroll
+    for (int l = 0; l < (int)kMaxSpatial + 1; ++l) {
+        if (l >= ni) break;
+        const Res rl = load_res(r_in, m_in, px[l]);
+        Ms[l] = rl.M;
+        uw[l] = rr.uniform();
+        el[l] = (rl.e == 0xFFFFFFFFu || !(rl.W > 0.0f)) ? 0xFFFFFFFFu : rl.e;
+        b1[l] = rl.b1;
+        b2[l] = rl.b2;
+        Wl[l] = rl.W;
+        den[l] = p_self[l] = p_l[l] = 0.0f;
+    }
+    // balance heuristic over the inputs' targets, each on its own surface
+#pragma unroll
+    for (int j = 0; j < (int)kMaxSpatial + 1; ++j) {
+        if (j >= ni) break;
+        const GSurf gj = j == 0 ? g : load_g(G, px[j]);
+#pragma unroll
+        for (int l = 0; l < (int)kMaxSpatial + 1; ++l) {
+            if (l >= ni) break;
+            if (el[l] == 0xFFFFFFFFu) continue;
+            const float pj = target(S, gj, el[l], b1[l], b2[l], nullptr, nullptr, nullptr);
+            if (j == 0) p_self[l] = pj;
+            if (j == l) p_l[l] = pj;
+            den[l] += Ms[j] * pj;
+        }
+    }
     Res s{0xFFFFFFFFu, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     float ph = 0.0f;
     s.M = 0.0f;
-    for (int l = 0; l < ni; ++l) {
+#pragma unroll
+    for (int l = 0; l < (int)kMaxSpatial + 1; ++l) {
+        if (l >= ni) break;
         s.M += Ms[l];
-        Res rl = load_res(r_in, m_in, px[l]);
-        float uw = rr.uniform();
-        if (rl.e == 0xFFFFFFFFu || !(rl.W > 0.0f)) continue;
-        // balance heuristic over the inputs' targets, each on its own surface
-        float p_self = 0.0f, p_l = 0.0f, den = 0.0f;
-        for (int j = 0; j < ni; ++j) {
-            GSurf gj = j == 0 ? g : load_g(G, px[j]);
-            float pj = target(S, gj, rl.e, rl.b1, rl.b2, nullptr, nullptr, nullptr);
-            if (j == 0) p_self = pj;
-            if (j == l) p_l = pj;
-            den += Ms[j] * pj;
-        }
-        float m = den > 0.0f ? Ms[l] * p_l / den : 0.0f;
-        if (res_update(s, rl.e, rl.b1, rl.b2, m * p_self * rl.W, uw)) ph = p_self;
+        if (el[l] == 0xFFFFFFFFu) continue;
+        const float m = den[l] > 0.0f ? Ms[l] * p_l[l] / den[l] : 0.0f;
+        if (res_update(s, el[l], b1[l], b2[l], m * p_self[l] * Wl[l], uw[l])) ph = p_self[l];
     }
     s.W = ph > 0.0f ? s.wsum / ph : 0.0f;
     store_res(r_out, m_out, i, s);
@@ -703,7 +738,8 @@ snkpt_status restir_pass(snkpt_ctx* ctx, const snkpt_camera* cam, const CamDev& 
         m_sp_in = X.m_tmp.as<float>();
     }
     const uint32_t k = s->num_spatial_resamples > 0 ? (uint32_t)s->num_spatial_resamples : 0u;
-    k_spatial<<<grid_for(n, kBlock), kBlock, 0, st>>>(S, G, r_sp_in, m_sp_in, X.r_fin.as<float4>(),
+    k_spatial<<<dim3((w + kSpatTile - 1) / kSpatTile, (h + kSpatTile - 1) / kSpatTile), kBlock, 0, st>>>(
+        S, G, r_sp_in, m_sp_in, X.r_fin.as<float4>(),
                                                       X.m_fin.as<float>(), w, h, k,
                                                       s->restir_radius > 0.0f ? s->restir_radius : 30.0f, initstate);
     k_rshade<<<grid_for(n, kBlock), kBlock, 0, st>>>(S, G, X.r_fin.as<float4>(), X.m_fin.as<float>(), shadow_of(R),
