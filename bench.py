@@ -42,26 +42,98 @@ def time_launches(fn, reps, stream):
     return sum(a.elapsed_time(b) for a, b in evs) / reps
 
 
-def cpu_baseline(world, rays_np, threads):
-    """The CPU restatement (oracle BVH8 traversal) on the same incoherent rays."""
+def cpu_baseline(world, rays_np, threads, gpu_hits=None):
+    """The CPU restatement (oracle BVH8 traversal) on the same incoherent rays;
+    with gpu_hits = (t, inst, prim) of the timed GPU trace, also how many of the
+    rays got the same hit bit for bit."""
+    import numpy as np
+
     from oracle import binding as O
     t0 = time.perf_counter()
     bvh = O.Bvh(world)
     build_s = time.perf_counter() - t0
     n = rays_np.shape[0]
     t0 = time.perf_counter()
-    bvh.trace(rays_np, threads=threads)
+    ct, ci, cp, *_ = bvh.trace(rays_np, threads=threads)
     dt = time.perf_counter() - t0
-    return {"value": round(n / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"all {n} incoherent secondary rays of the 1080p ray set, closest hit, CPU BVH8 "
-                      f"restatement (oracle), {threads} threads; CPU LBVH->BVH8 build of the 1.02M "
-                      f"triangles took {build_s:.2f} s"}
+    rec = {"value": round(n / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+           "sample": f"all {n} incoherent secondary rays of the 1080p ray set, closest hit, CPU BVH8 "
+                     f"restatement (oracle), {threads} threads; CPU LBVH->BVH8 build of the 1.02M "
+                     f"triangles took {build_s:.2f} s"}
+    match = None
+    if gpu_hits is not None:
+        gt, gi, gp = (np.ascontiguousarray(a) for a in gpu_hits)
+        same = (gt.view(np.uint32) == ct.view(np.uint32)) & (gi.view(np.uint32) == ci) & (gp.view(np.uint32) == cp)
+        match = {"rays": int(n), "equal": int(same.sum()), "frac": round(float(same.mean()), 6),
+                 "what": "hit t (bits), instance id and primitive id of every ray of the timed GPU trace "
+                         "against the CPU BVH8 traversal"}
+    return rec, match
 
 
 # (config, form); reference_compat: the reference's own estimator (ReSTIR direct light +
 # BounceDiffuse's 8 + up to 16 rays per pixel) on the config 4 room
 FRAME_RUNS = [(1, "pt"), (2, "pt"), (3, "restir"), (4, "pt"), (4, "cache"), (4, "cache_indirect"),
               (4, "reference_compat")]
+
+
+# Algorithmic bytes per unit (the scope table's measurement figures, with this build's
+# record sizes): closest-hit ray 56 + 80 N_node + 48 N_tri, shadow ray 40 + 80 N + 48 N,
+# shaded path vertex 370; ReSTIR per pixel: candidates 2,790, temporal 476, spatial
+# 3,164, shade 212 (profiles/restir_c3_1080p_pmc.md); cache query: 16 levels x 8
+# corners x 2 x 2 B of hash-grid features = 512.
+SHADE_BYTES = 370.0
+RESTIR_BYTES = {"primary": 2790.0, "temporal": 476.0, "spatial": 3164.0, "shade": 212.0}
+CACHE_QUERY_BYTES = 512.0
+
+
+def visit_means(ctx, scene, w, h, stream):
+    """Mean BVH8 nodes and triangles visited per ray (deterministic, counted by
+    the traversal itself) for the configuration's camera rays, one cosine bounce
+    from their hits and shadow rays from their hits to the emitters; returns the
+    algorithmic bytes per ray of each kind."""
+    import numpy as np
+    import torch
+
+    import raysets
+    dev = torch.device("cuda:0")
+    with torch.cuda.stream(stream):
+        porg, pdr = raysets.primary_rays(scene.camera(0, w, h), w, h)
+        po = torch.from_numpy(porg).to(dev)
+        pd = torch.from_numpy(pdr).to(dev)
+        pt, pinst, pprim, _, _, pnv, ptv = ctx.trace(po, pd, counted=True, stream=stream)
+        world = torch.from_numpy(np.frombuffer(ctx.export(6).tobytes(), np.float32).reshape(-1, 12).copy()).to(dev)
+        inst5 = scene.arrays()[2]
+        off = torch.from_numpy(np.concatenate([[0], np.cumsum(inst5[:, 4] // 3)]).astype(np.int64)).to(dev)
+        ext = float((world[:, :9].reshape(-1, 3).max(0).values - world[:, :9].reshape(-1, 3).min(0).values).max())
+        so, sd = raysets.secondary_rays_torch(po, pd, pt, pinst, pprim, world, off, ext, seed=1)
+        *_, snv, stv = ctx.trace(so, sd, counted=True, stream=stream)
+        em = np.ctypeslib.as_array(scene.s.emissive_instances, (scene.s.n_emissive,))
+        off_np = off.cpu().numpy()
+        gids = np.concatenate([np.arange(off_np[i], off_np[i + 1]) for i in em]) if len(em) else np.zeros(0, np.int64)
+        res = {"primary": 56.0 + 80.0 * float(pnv.double().mean()) + 48.0 * float(ptv.double().mean()),
+               "extension": 56.0 + 80.0 * float(snv.double().mean()) + 48.0 * float(stv.double().mean())}
+        if gids.size:
+            et = world[torch.from_numpy(gids).to(dev), :9]
+            ea = torch.linalg.cross(et[:, 3:6] - et[:, 0:3], et[:, 6:9] - et[:, 0:3]).norm(dim=1)
+            sho, shd = raysets.shadow_rays_torch(po, pd, pt, et, ea / ea.sum(), seed=2, scale=ext)
+            *_, hnv, htv = ctx.trace(sho, shd, any_hit=True, counted=True, stream=stream)
+            res["shadow"] = 40.0 + 80.0 * float(hnv.double().mean()) + 48.0 * float(htv.double().mean())
+        else:
+            res["shadow"] = 40.0
+        stream.synchronize()
+    return res
+
+
+def frame_algorithmic_bytes(st, B, mode, pixels):
+    """Algorithmic bytes of one frame from its ray counts and per-ray bytes B."""
+    import snkpt
+    b = (st.camera_rays * B["primary"] + st.extension_rays * B["extension"] + st.shadow_rays * B["shadow"] +
+         (st.camera_rays + st.extension_rays) * SHADE_BYTES)
+    if mode == snkpt.MODE_RESTIR:
+        b += pixels * sum(RESTIR_BYTES.values())
+    if mode == snkpt.MODE_CACHE:
+        b += (st.cache_queries + st.cache_train_samples) * CACHE_QUERY_BYTES
+    return float(b)
 
 
 def frame_rates(stream, frames=10, warmup=3, only=None):
@@ -149,6 +221,13 @@ def frame_rates(stream, frames=10, warmup=3, only=None):
         hits = int(st.camera_rays + st.extension_rays)
         comp = (56.0 + 370.0) * hits + 40.0 * int(st.shadow_rays)
         res[key]["hbm_frac_compulsory"] = round(comp / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)
+        # algorithmic bytes: rays x (56 + 80 N_node + 48 N_tri) with N counted on this
+        # configuration's camera, bounce and shadow ray sets, + shaded vertices, ReSTIR, cache
+        B = visit_means(ctx, scene, w, h, stream)
+        alg = frame_algorithmic_bytes(st, B, mode, w * h)
+        res[key]["bytes_per_ray"] = {k: round(v, 1) for k, v in B.items()}
+        res[key]["algorithmic_bytes_per_frame"] = int(alg)
+        res[key]["hbm_frac_algorithmic"] = round(alg / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)
         if mode == snkpt.MODE_RESTIR:
             res[key]["camera"] = "seeded 64-frame orbit" if cfg == 3 else "static"
             res[key]["history_reused_frac"] = round(int(st.restir_history_reused) / (w * h), 4)
@@ -681,12 +760,14 @@ def main():
     if not args.only_trace and not args.no_rmse:
         extra["time_to_rmse"] = time_to_rmse(stream)
 
-    cpu = None
+    cpu = hits_match = None
     if not args.no_cpu and not args.only_trace:
         threads = max(1, min(16, os.cpu_count() or 1))
         rays_np = torch.cat([so, sd], 1).cpu().numpy()
         log(f"bench: CPU baseline on {threads} threads")
-        cpu = cpu_baseline(world_np, rays_np, threads)
+        cpu, hits_match = cpu_baseline(world_np, rays_np, threads,
+                                       (t.cpu().numpy(), hi.cpu().numpy(), hp.cpu().numpy()))
+        log(f"bench: GPU hits equal to the CPU traversal: {hits_match['equal']} of {hits_match['rays']}")
 
     out = {
         "metric": "Mrays/s incoherent secondary (closest hit), 1.02M-triangle scene, 1080p, 1 GPU",
@@ -715,6 +796,7 @@ def main():
                      "note": "algorithmic bytes = 56 + 80*nodes + 48*tris per ray; the BVH (~70 MB) is "
                              "cache-resident, so frac > 1 would mean cache reuse"},
         "cpu_baseline": cpu,
+        "hits_match": hits_match,
         "extra": extra,
     }
     print(json.dumps(out), flush=True)
