@@ -102,12 +102,17 @@ typedef struct snkpt_pointlight {
 #define SNKPT_MODE_PT 0u     /* unbiased path tracing, NEE + MIS */
 #define SNKPT_MODE_RESTIR 1u /* ReSTIR DI for direct light at the primary hit + PT for the rest */
 #define SNKPT_MODE_CACHE 2u  /* PT terminated into the MLP radiance cache, cache trained online */
-/* The reference's own (biased) estimator, for regression against its look: ReSTIR DI at the
- * primary hit plus BounceDiffuse (RayGen.rgen:139-213): 8 uniform-hemisphere rays from the biased
- * primary position, each adding albedo * payload * max(cos(dot(rd, n)), 0) / 8, and 2 more from
- * every non-emissive hit at a quarter weight, all times 2 PI; the payload of a hit is its emission,
- * or the directional light through its BRDF and a shadow ray (RayClosestHit.rchit:87-142).  Random
- * numbers are PCG32 instead of the wall-clock gold noise; max_depth and rr_start are ignored. */
+/* The reference's own (biased) estimator, for regression against its look: its ReSTIR DI at the
+ * primary hit (RayGen.rgen:215-528: 32 point-light candidates drawn uniformly and 32 emissive
+ * triangles with pdf 1 / (N_inst N_tri area), target = length of the unshadowed contribution
+ * without the emitter cosine, temporal and spatial combines weighted p_hat M W with visibility and
+ * no MIS, a fresh reservoir without history discarded, each combine counting 1 into M; the
+ * directional light shadowed) plus BounceDiffuse (RayGen.rgen:139-213): 8 uniform-hemisphere rays
+ * from the biased primary position, each adding albedo * payload * max(cos(dot(rd, n)), 0) / 8, and
+ * 2 more from every non-emissive hit at a quarter weight, all times 2 PI; the payload of a hit is
+ * its emission, or the directional light through its BRDF and a shadow ray
+ * (RayClosestHit.rchit:87-142).  Random numbers are PCG32 instead of the wall-clock gold noise;
+ * max_depth and rr_start are ignored. */
 #define SNKPT_MODE_REFERENCE_COMPAT 3u
 
 #define SNKPT_FLAG_ACCUMULATE 1u /* add this frame to the running mean and output the mean */
@@ -121,6 +126,7 @@ typedef struct snkpt_pointlight {
                                         closest-hit ray per query) instead of NEE alone, which bounds the
                                         1/r^2 outliers of shading points next to an emitter (snkpt_render,
                                         snkpt_batch_render) */
+#define SNKPT_FLAG_NO_BOUNCE 128u   /* SNKPT_MODE_REFERENCE_COMPAT: direct light only (no BounceDiffuse term) */
 #define SNKPT_FLAG_NO_STATS 64u      /* snkpt_render returns without reading the frame's ray counts back
                                         (snkpt_get_frame_stats keeps the previous values) and without
                                         waiting for the GPU.  With it a SNKPT_MODE_PT frame can be recorded

@@ -49,6 +49,9 @@ def load():
     lib.o_compat_bounce.argtypes = [VP, U32, C.POINTER(OBvh), VP, U32, VP, VP, VP, VP, VP, U32, VP, VP, VP, U32, U32,
                                     U32, U32, U32, VP, I]
     lib.o_compat_bounce.restype = I
+    lib.o_compat_direct.argtypes = [VP, U32, C.POINTER(OBvh), VP, U32, VP, VP, VP, VP, VP, U32, VP, C.c_size_t, VP,
+                                    VP, VP, VP, U32, U32, U32, U32, U32, F, U32, VP, I]
+    lib.o_compat_direct.restype = I
     lib.o_emitter_sample.argtypes = [VP, U32, VP, U32, VP, VP, U32, C.c_uint64, U32, U32, VP, VP, VP]
     lib.o_emitter_sample.restype = U32
     lib.o_half_to_float.argtypes = [C.c_uint16]
@@ -253,6 +256,31 @@ def emitter_sample(arrs, n, seed=1):
     ne = load().o_emitter_sample(_p(world), world.shape[0], _p(inst), inst.shape[0], _p(mats), _p(em), em.size, seed,
                                  n, cap, _p(gid), _p(pdf), _p(cnt))
     return gid[:ne], pdf[:ne], cnt[:ne]
+
+
+def compat_direct(arrs, bvh: "Bvh", cam, w, h, frames, light_block=None, temporal=1, spatial=3, radius=30.0,
+                  seed=1, threads=8):
+    """The reference's own ReSTIR DI (SNKPT_MODE_REFERENCE_COMPAT with
+    SNKPT_FLAG_NO_BOUNCE) over `frames` frames of the static camera `cam`;
+    returns the last frame (h, w, 3): emission of directly seen emitters plus
+    the direct light."""
+    pos, idx, inst, m4, mats, nrm, em = arrs
+    inst = np.ascontiguousarray(inst, np.uint32)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    m4 = np.ascontiguousarray(m4, np.float32)
+    mats = np.ascontiguousarray(mats, np.uint32)
+    nrm = np.ascontiguousarray(nrm, np.float32)
+    em = np.ascontiguousarray(em, np.uint32)
+    lb = np.frombuffer(light_block, np.uint8) if light_block else np.zeros(0, np.uint8)
+    view = np.array(cam.view, np.float32)
+    proj = np.array(cam.proj, np.float32)
+    pv = np.array(cam.proj_view, np.float32)
+    cpos = np.array(cam.cam_pos[:3], np.float32)
+    out = np.zeros((h, w, 3), np.float32)
+    load().o_compat_direct(_p(bvh.world), bvh.n, C.byref(bvh.b), _p(inst), inst.shape[0], _p(idx), _p(nrm), _p(m4),
+                           _p(mats), _p(em), em.size, _p(lb) if lb.size else None, lb.size, _p(view), _p(proj),
+                           _p(pv), _p(cpos), w, h, frames, temporal, spatial, radius, seed, _p(out), threads)
+    return out
 
 
 def half_to_float(h: int) -> float:

@@ -90,6 +90,14 @@ int o_brdf_sample(const float albedo[3], float roughness, float metallic, uint32
 uint32_t o_emitter_sample(const float* world, uint32_t ntris, const uint32_t* inst5, uint32_t n_inst,
                           const uint32_t* mats16, const uint32_t* emissive, uint32_t n_emissive, uint64_t seed,
                           uint32_t n, uint32_t cap, uint32_t* out_gid, float* out_pdf, uint32_t* out_count);
+/* The reference's own ReSTIR DI (SNKPT_MODE_REFERENCE_COMPAT's direct light) over `frames`
+ * frames of a static camera; writes the last frame's emission + direct light. */
+int o_compat_direct(const float* world, uint32_t ntris, const o_bvh* bvh, const uint32_t* inst5, uint32_t n_inst,
+                    const uint32_t* idx, const float* nrm, const float* mat4s, const uint32_t* mats16,
+                    const uint32_t* emissive, uint32_t n_emissive, const uint8_t* light_block, size_t block_bytes,
+                    const float* view16, const float* proj16, const float* proj_view16, const float* cam_pos,
+                    uint32_t w, uint32_t h, uint32_t frames, uint32_t temporal, uint32_t spatial, float radius,
+                    uint32_t seed, float* out_rgb, int threads);
 /* Mean of `spp` samples per pixel (pass k uses o_pass_state(seed ^ seed_xor, frame, k)). */
 int o_pt_render(const float* world, uint32_t ntris, const o_bvh* bvh, const uint32_t* inst5, uint32_t n_inst,
                 const uint32_t* idx, const float* nrm, const float* mat4s, const uint32_t* mats16,

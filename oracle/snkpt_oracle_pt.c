@@ -991,3 +991,519 @@ int o_compat_bounce(const float* world, uint32_t ntris, const o_bvh* bvh, const 
     free(S.nmat);
     return 0;
 }
+
+/* ------------------------------ reference-compat direct light (Q10) --- */
+/* The reference's own ReSTIR DI (RayGen.rgen:215-528) as SNKPT_MODE_REFERENCE_
+ * COMPAT computes it: DoInitialWRS (32 uniform point-light candidates, weight
+ * length(CalcPointlight) N_pl; 32 emissive candidates: instance uniform in the
+ * emissive list, triangle uniform in it, folded uniform barycentrics, pdf
+ * 1 / (N_inst N_tri area), weight length(ComputeTriangleRadiance) / pdf, which
+ * has no emitter cosine; W = visibility w_sum / M / p_hat, the point light's ray
+ * aimed at pos - biased_pos / dist as written at :345), WriteInitialReservoirs
+ * (history with normal dot > 0.95, M capped at 20 M with w_sum scaled, the
+ * fresh and history reservoirs combined with weights p_hat M W where p_hat
+ * includes visibility, no MIS, M = M_fresh + M_hist; without a history the
+ * fresh reservoir is discarded), ComputeAndWriteSpatialReservoirReuse (own +
+ * k neighbours in a +-radius box redrawn until inside, skipped when the
+ * previous frame's normal there has dot < 0.95; each combine also counts 1 into
+ * M) and CalculateRadianceFromReservoirs plus the shadowed directional light.
+ * Every UpdateReservoir adds 1 to M.  Random numbers: PCG32 (state =
+ * o_pass_state(seed, frame, 0) ^ phase, stream = pixel); pass A (phase
+ * 0x51314141): u_idx, u_wrs per point-light candidate, u_inst, u_tri, u_b1,
+ * u_b2, u_wrs per emissive candidate, then u_pl, u_em per combine; pass B
+ * (0x51314242): u_pl, u_em of the own combine, per neighbour (ux, uy) per try
+ * (at most 16) and u_pl, u_em when it is combined.  The emissive point is
+ * interpolated from the world-space triangle, whose area is used for the pdf
+ * (the reference's det(M) x object-space area, equal for rigid transforms); the
+ * history pixel is found by reprojection into the previous camera.  The camera
+ * is static: frames 0..frames-1 are rendered and the last is returned
+ * (emission of directly seen emitters + this direct light). */
+#define QA_PHASE 0x51314141ull
+#define QB_PHASE 0x51314242ull
+#define QINV 0xFFFFFFFFu
+
+typedef struct {
+    float wsum_pl, w_pl;
+    uint32_t pl, m_pl, inst, tri;
+    float b1, b2, wsum_em, w_em;
+    uint32_t m_em;
+} qres;
+
+typedef struct {
+    int valid;
+    v3 p, n, v, f0, biased;
+    omat m;
+    float t;
+    v3 emit; /* radiance of a directly seen emitter */
+} qvert;
+
+typedef struct {
+    const oscene* S;
+    const float* pl; /* 12 floats per point light: colour, pad, position, pad, (c, l, e), pad */
+    uint32_t n_pl;
+    const uint32_t* em;
+    uint32_t n_em;
+    v3 dir_colour, dir_l;
+    uint32_t w, h, k, temporal, have_prev;
+    float radius;
+    uint64_t state;
+    float pv[16];
+    const qvert* G;
+    const qvert* Gp;
+    qres* hist;
+    qres* tmp;
+    float* out;
+    uint32_t begin, end;
+    int pass;
+} qjob;
+
+static qres q_empty(void) {
+    qres r = {0.0f, 0.0f, QINV, 0u, QINV, QINV, 0.0f, 0.0f, 0.0f, 0.0f, 0u};
+    return r;
+}
+
+static float vlen_c(v3 a) { return sqrtf(dotv(a, a)); }
+
+static v3 nrmz_q(v3 a) {
+    float l = sqrtf(dotv(a, a));
+    return l > 0.0f ? V(a.x / l, a.y / l, a.z / l) : a;
+}
+
+/* CookTorranceBRDF (LightBuffers.glsl:103-124): (kd / PI + spec) n.l, kd without albedo */
+static v3 ct_ref_c(v3 lpos, v3 sp, v3 v, v3 f0, v3 n, float rough, float metal) {
+    const v3 l = nrmz_q(sub(lpos, sp));
+    const v3 hh = nrmz_q(add(v, l));
+    const float c1 = fminf(fmaxf(1.0f - fmaxf(dotv(hh, v), 0.0f), 0.0f), 1.0f), c2 = c1 * c1;
+    const float fw = (c2 * c2) * c1;
+    const v3 F = V(f0.x + (1.0f - f0.x) * fw, f0.y + (1.0f - f0.y) * fw, f0.z + (1.0f - f0.z) * fw);
+    const float a = rough * rough;
+    const float D = ggx_d_ref(fmaxf(dotv(n, hh), 0.0f), a * a);
+    const float r = rough + 1.0f, k = (r * r) / 8.0f;
+    const float nv = fmaxf(dotv(n, v), 0.0f), nl = fmaxf(dotv(n, l), 0.0f);
+    const float G = g1s(nv, k) * g1s(nl, k);
+    const float den = 4.0f * nv * nl + 0.0001f;
+    const float km = 1.0f - metal;
+    const v3 spec = scl(F, D * G / den);
+    const v3 kd = V((1.0f - F.x) * km, (1.0f - F.y) * km, (1.0f - F.z) * km);
+    return scl(add(scl(kd, 1.0f / REF_PI), spec), nl);
+}
+
+/* CalcPointlight (LightBuffers.glsl:145-157) */
+static v3 calc_point_c(const qjob* J, uint32_t k, const qvert* x) {
+    const float* L = J->pl + 12 * (size_t)k;
+    const v3 lp = V(L[4], L[5], L[6]);
+    const float dist = vlen_c(sub(lp, x->p));
+    const float atten = (L[8] + L[9] * dist) + L[10] * (dist * dist);
+    const v3 brdf = ct_ref_c(lp, x->p, x->v, x->f0, x->n, x->m.roughness, x->m.metallic);
+    return scl(mulv(mulv(brdf, x->m.albedo), V(L[0], L[1], L[2])), 1.0f / atten);
+}
+
+/* CalcDirectionalLight (LightBuffers.glsl:127-143) */
+static v3 calc_dir_c(const qjob* J, const qvert* x) {
+    const v3 l = J->dir_l, v = x->v, n = x->n, f0 = x->f0;
+    const v3 hh = nrmz_q(add(v, l));
+    const float c1 = fminf(fmaxf(1.0f - fmaxf(dotv(hh, v), 0.0f), 0.0f), 1.0f), c2 = c1 * c1;
+    const float fw = (c2 * c2) * c1;
+    const v3 F = V(f0.x + (1.0f - f0.x) * fw, f0.y + (1.0f - f0.y) * fw, f0.z + (1.0f - f0.z) * fw);
+    const float a = x->m.roughness * x->m.roughness;
+    const float D = ggx_d_ref(fmaxf(dotv(n, hh), 0.0f), a * a);
+    const float r = x->m.roughness + 1.0f, k = (r * r) / 8.0f;
+    const float nv = fmaxf(dotv(n, v), 0.0f), nl = fmaxf(dotv(n, l), 0.0f);
+    const float G = g1s(nv, k) * g1s(nl, k);
+    const float den = 4.0f * nv * nl + 0.0001f;
+    const float km = 1.0f - x->m.metallic;
+    const v3 spec = scl(F, D * G / den);
+    const v3 kd = V((1.0f - F.x) * km * x->m.albedo.x, (1.0f - F.y) * km * x->m.albedo.y,
+                    (1.0f - F.z) * km * x->m.albedo.z);
+    return mulv(scl(add(scl(kd, 1.0f / REF_PI), spec), nl), J->dir_colour);
+}
+
+typedef struct {
+    v3 p, n, le;
+    float area;
+} qtri;
+
+static qtri tri_point_c(const oscene* S, uint32_t inst, uint32_t tri, float b1, float b2) {
+    const uint32_t* in = S->inst5 + 5 * inst;
+    const float* w = S->world + 12 * (size_t)(S->tri_off[inst] + tri);
+    const v3 v0 = V(w[0], w[1], w[2]), v1 = V(w[3], w[4], w[5]), v2 = V(w[6], w[7], w[8]);
+    const float b3 = 1.0f - b1 - b2;
+    qtri q;
+    q.p = add(add(scl(v0, b1), scl(v1, b2)), scl(v2, b3));
+    const uint32_t* ix = S->idx + in[1] + 3 * tri;
+    const float* n0 = S->nrm + 3 * (size_t)(ix[0] + in[2]);
+    const float* n1 = S->nrm + 3 * (size_t)(ix[1] + in[2]);
+    const float* n2 = S->nrm + 3 * (size_t)(ix[2] + in[2]);
+    const v3 nl = add(add(scl(V(n0[0], n0[1], n0[2]), b1), scl(V(n1[0], n1[1], n1[2]), b2)),
+                      scl(V(n2[0], n2[1], n2[2]), b3));
+    const float* nm = S->nmat + 9 * (size_t)inst;
+    q.n = nrmz_q(V((nm[0] * nl.x + nm[1] * nl.y) + nm[2] * nl.z, (nm[3] * nl.x + nm[4] * nl.y) + nm[5] * nl.z,
+                   (nm[6] * nl.x + nm[7] * nl.y) + nm[8] * nl.z));
+    q.area = 0.5f * vlen_c(crossv(sub(v1, v0), sub(v1, v2)));
+    const uint32_t* mt = S->mats16 + 16 * (size_t)in[3];
+    q.le = V(u2f(mt[8]), u2f(mt[9]), u2f(mt[10]));
+    return q;
+}
+
+/* ShadowTest (:101-121) */
+static float qshadow_c(const oscene* S, v3 ro, v3 rd, float maxd) {
+    float ray[8] = {ro.x, ro.y, ro.z, 0.001f, rd.x, rd.y, rd.z, maxd};
+    float t, u, v;
+    uint32_t i, p;
+    o_trace_one(S->bvh, ray, 1, &t, &i, &p, &u, &v);
+    return t >= 0.0f ? 0.0f : 1.0f;
+}
+
+/* ComputeTriangleRadiance (:258-282) */
+static v3 tri_radiance_c(const oscene* S, uint32_t inst, uint32_t tri, float b1, float b2, const qvert* x,
+                         int shadow) {
+    const qtri q = tri_point_c(S, inst, tri, b1, b2);
+    const v3 brdf = ct_ref_c(q.p, x->p, x->v, x->f0, x->n, x->m.roughness, x->m.metallic);
+    const v3 d = sub(x->p, q.p);
+    v3 rad = scl(mulv(mulv(q.le, x->m.albedo), brdf), 1.0f / fabsf(dotv(d, d)));
+    if (shadow) {
+        const v3 tgt = add(q.p, scl(q.n, 0.01f));
+        const float pd = vlen_c(sub(x->biased, tgt));
+        rad = scl(rad, qshadow_c(S, x->biased, scl(sub(tgt, x->biased), 1.0f / pd), pd));
+    }
+    return rad;
+}
+
+static int q_update_c(float* wsum, uint32_t* M, float weight, float u) {
+    *M += 1u;
+    *wsum += weight;
+    return *wsum > 0.0f && u < weight / *wsum;
+}
+
+/* CombinePixelReservoir (:361-386) */
+static void q_combine_c(const qjob* J, const qvert* x, qres* r, const qres* nb, float* ph_pl, float* ph_em, prng* rng) {
+    const float u_pl = runi(rng), u_em = runi(rng);
+    if (nb->pl != QINV) {
+        const float* L = J->pl + 12 * (size_t)nb->pl;
+        const v3 lp = V(L[4], L[5], L[6]);
+        const float ld = vlen_c(sub(lp, x->biased));
+        const float ph = vlen_c(calc_point_c(J, nb->pl, x)) *
+                         qshadow_c(J->S, x->biased, scl(sub(lp, x->biased), 1.0f / ld), ld - 0.001f);
+        const float weight = ph * (float)nb->m_pl * nb->w_pl;
+        if (q_update_c(&r->wsum_pl, &r->m_pl, weight, u_pl)) {
+            r->pl = nb->pl;
+            *ph_pl = ph;
+        }
+    }
+    if (nb->inst != QINV) {
+        const float ph = vlen_c(tri_radiance_c(J->S, nb->inst, nb->tri, nb->b1, nb->b2, x, 1));
+        const float weight = ph * nb->w_em * (float)nb->m_em;
+        if (q_update_c(&r->wsum_em, &r->m_em, weight, u_em)) {
+            r->inst = nb->inst;
+            r->tri = nb->tri;
+            r->b1 = nb->b1;
+            r->b2 = nb->b2;
+            *ph_em = ph;
+        }
+    }
+}
+
+static void q_pass_a(const qjob* J, uint32_t i) {
+    const qvert* x = &J->G[i];
+    qres out = q_empty();
+    if (x->valid) {
+        prng rng = rseed(J->state ^ QA_PHASE, i);
+        qres r = q_empty();
+        float ph_pl = 0.0f, ph_em = 0.0f;
+        for (uint32_t c = 0; c < 32u; ++c) {
+            const float u = runi(&rng), uw = runi(&rng);
+            uint32_t k = 0;
+            float ph = 0.0f, weight = 0.0f;
+            if (J->n_pl > 0) {
+                k = (uint32_t)(u * (float)J->n_pl);
+                if (k > J->n_pl - 1u) k = J->n_pl - 1u;
+                ph = vlen_c(calc_point_c(J, k, x));
+                weight = ph / (1.0f / (float)J->n_pl);
+            }
+            if (q_update_c(&r.wsum_pl, &r.m_pl, weight, uw)) {
+                r.pl = k;
+                ph_pl = ph;
+            }
+        }
+        for (uint32_t c = 0; c < 32u; ++c) {
+            const float ui = runi(&rng), ut = runi(&rng), ub1 = runi(&rng), ub2 = runi(&rng), uw = runi(&rng);
+            if (J->n_em == 0) {
+                r.m_em += 1u;
+                continue;
+            }
+            uint32_t ei = (uint32_t)(ui * (float)J->n_em);
+            if (ei > J->n_em - 1u) ei = J->n_em - 1u;
+            const uint32_t inst = J->em[ei];
+            const uint32_t ntri = J->S->inst5[5 * inst + 4] / 3u;
+            if (ntri == 0) {
+                r.m_em += 1u;
+                continue;
+            }
+            uint32_t tri = (uint32_t)(ut * (float)ntri);
+            if (tri > ntri - 1u) tri = ntri - 1u;
+            float b1 = ub1, b2 = ub2;
+            if (b1 + b2 >= 1.0f) {
+                b1 = 1.0f - b1;
+                b2 = 1.0f - b2;
+            }
+            const qtri q = tri_point_c(J->S, inst, tri, b1, b2);
+            const float ph = vlen_c(tri_radiance_c(J->S, inst, tri, b1, b2, x, 0));
+            const float pdf = 1.0f / (((float)J->n_em * (float)ntri) * q.area);
+            const float weight = ph / pdf;
+            if (q_update_c(&r.wsum_em, &r.m_em, weight, uw)) {
+                r.inst = inst;
+                r.tri = tri;
+                r.b1 = b1;
+                r.b2 = b2;
+                ph_em = ph;
+            }
+        }
+        if (r.pl != QINV) {
+            const float* L = J->pl + 12 * (size_t)r.pl;
+            const v3 lp = V(L[4], L[5], L[6]);
+            const float d = vlen_c(sub(lp, x->biased));
+            r.w_pl = qshadow_c(J->S, x->biased, sub(lp, scl(x->biased, 1.0f / d)), d) * ((r.wsum_pl / (float)r.m_pl) / ph_pl);
+        }
+        if (r.inst != QINV) {
+            const qtri q = tri_point_c(J->S, r.inst, r.tri, r.b1, r.b2);
+            const v3 tgt = add(q.p, scl(q.n, 0.01f));
+            const float pd = vlen_c(sub(x->biased, tgt));
+            r.w_em = qshadow_c(J->S, x->biased, scl(sub(tgt, x->biased), 1.0f / pd), pd) *
+                     ((r.wsum_em / (float)r.m_em) / ph_em);
+        }
+        int q = -1;
+        if (J->temporal) {
+            float cl[4];
+            for (int k = 0; k < 4; ++k)
+                cl[k] = ((J->pv[k] * x->p.x + J->pv[4 + k] * x->p.y) + J->pv[8 + k] * x->p.z) + J->pv[12 + k];
+            if (cl[3] > 1e-6f) {
+                const float fx = (cl[0] / cl[3] * 0.5f + 0.5f) * (float)J->w;
+                const float fy = (cl[1] / cl[3] * 0.5f + 0.5f) * (float)J->h;
+                if (fx > 0.0f && fy > 0.0f && fx < (float)J->w && fy < (float)J->h)
+                    q = (int)fy * (int)J->w + (int)fx;
+            }
+        }
+        if (q >= 0 && (!J->Gp[q].valid || !(dotv(J->Gp[q].n, x->n) > 0.95f))) q = -1;
+        if (q >= 0) {
+            qres hh = J->hist[q];
+            if ((float)hh.m_pl > 20.0f * (float)r.m_pl) {
+                hh.wsum_pl *= 20.0f * (float)r.m_pl / (float)hh.m_pl;
+                hh.m_pl = 20u * r.m_pl;
+            }
+            if ((float)hh.m_em > 20.0f * (float)r.m_em) {
+                hh.wsum_em *= 20.0f * (float)r.m_em / (float)hh.m_em;
+                hh.m_em = 20u * r.m_em;
+            }
+            float tp_pl = 0.0f, tp_em = 0.0f;
+            q_combine_c(J, x, &out, &r, &tp_pl, &tp_em, &rng);
+            q_combine_c(J, x, &out, &hh, &tp_pl, &tp_em, &rng);
+            out.m_em = hh.m_em + r.m_em;
+            out.m_pl = hh.m_pl + r.m_pl;
+            out.w_em = tp_em > 0.0f ? (out.wsum_em / (float)out.m_em) / tp_em : 0.0f;
+            out.w_pl = tp_pl > 0.0f ? (out.wsum_pl / (float)out.m_pl) / tp_pl : 0.0f;
+        }
+    }
+    J->tmp[i] = out;
+}
+
+static void q_pass_b(const qjob* J, uint32_t i) {
+    const qvert* x = &J->G[i];
+    qres r = q_empty();
+    float* o = J->out + 3 * (size_t)i;
+    o[0] = x->emit.x;
+    o[1] = x->emit.y;
+    o[2] = x->emit.z;
+    if (x->valid) {
+        prng rng = rseed(J->state ^ QB_PHASE, i);
+        float ph_pl = 0.0f, ph_em = 0.0f;
+        const qres own = J->tmp[i];
+        q_combine_c(J, x, &r, &own, &ph_pl, &ph_em, &rng);
+        r.m_em += own.m_em;
+        r.m_pl += own.m_pl;
+        const float cx0 = (float)(i % J->w) / (float)J->w, cy0 = (float)(i / J->w) / (float)J->h;
+        for (uint32_t s = 0; s < J->k; ++s) {
+            float cx = -1.0f, cy = -1.0f;
+            for (int t = 0; t < 16; ++t) {
+                const float ux = runi(&rng), uy = runi(&rng);
+                const float ox = (ux - 0.5f) * 2.0f * J->radius / (float)J->w;
+                const float oy = (uy - 0.5f) * 2.0f * J->radius / (float)J->h;
+                const float tx = cx0 + ox, ty = cy0 + oy;
+                if (tx >= 0.0f && ty >= 0.0f && tx <= 1.0f && ty <= 1.0f) {
+                    cx = tx;
+                    cy = ty;
+                    break;
+                }
+            }
+            if (cx < 0.0f || !J->have_prev) continue;
+            uint32_t nx = (uint32_t)((float)J->w * cx), ny = (uint32_t)((float)J->h * cy);
+            if (nx > J->w - 1u) nx = J->w - 1u;
+            if (ny > J->h - 1u) ny = J->h - 1u;
+            const uint32_t q = ny * J->w + nx;
+            if (!J->Gp[q].valid || dotv(J->Gp[q].n, x->n) < 0.95f) continue;
+            const qres nb = J->tmp[q];
+            q_combine_c(J, x, &r, &nb, &ph_pl, &ph_em, &rng);
+            r.m_em += nb.m_em;
+            r.m_pl += nb.m_pl;
+        }
+        r.w_em = ph_em > 0.0f ? r.wsum_em / ph_em / (float)r.m_em : 0.0f;
+        r.w_pl = ph_pl > 0.0f ? r.wsum_pl / ph_pl / (float)r.m_pl : 0.0f;
+        v3 light = calc_dir_c(J, x);
+        if (light.x > 0.0f || light.y > 0.0f || light.z > 0.0f)
+            light = scl(light, qshadow_c(J->S, x->biased, J->dir_l, 1000.0f));
+        if (J->n_pl > 0 && r.pl != QINV) {
+            const float* L = J->pl + 12 * (size_t)r.pl;
+            const v3 lp = V(L[4], L[5], L[6]);
+            const float ld = vlen_c(sub(lp, x->biased));
+            const v3 c = scl(calc_point_c(J, r.pl, x),
+                             qshadow_c(J->S, x->biased, scl(sub(lp, x->biased), 1.0f / ld), ld));
+            light = add(light, scl(c, r.w_pl));
+        }
+        if (J->n_em > 0 && r.inst != QINV) {
+            const v3 c = tri_radiance_c(J->S, r.inst, r.tri, r.b1, r.b2, x, 1);
+            light = add(light, scl(c, r.w_em));
+        }
+        o[0] = o[0] + light.x;
+        o[1] = o[1] + light.y;
+        o[2] = o[2] + light.z;
+    }
+    J->hist[i] = r;
+}
+
+static void* q_worker(void* arg) {
+    const qjob* J = (const qjob*)arg;
+    for (uint32_t i = J->begin; i < J->end; ++i) {
+        if (J->pass == 0)
+            q_pass_a(J, i);
+        else
+            q_pass_b(J, i);
+    }
+    return NULL;
+}
+
+static void q_run(qjob* base, int threads, int pass) {
+    pthread_t th[256];
+    qjob jobs[256];
+    uint32_t n = base->w * base->h, per = (n + (uint32_t)threads - 1) / (uint32_t)threads;
+    for (int k = 0; k < threads; ++k) {
+        jobs[k] = *base;
+        jobs[k].pass = pass;
+        jobs[k].begin = (uint32_t)k * per < n ? (uint32_t)k * per : n;
+        jobs[k].end = jobs[k].begin + per < n ? jobs[k].begin + per : n;
+        if (pthread_create(&th[k], NULL, q_worker, &jobs[k]) != 0) {
+            q_worker(&jobs[k]);
+            th[k] = 0;
+        }
+    }
+    for (int k = 0; k < threads; ++k)
+        if (th[k]) pthread_join(th[k], NULL);
+}
+
+int o_compat_direct(const float* world, uint32_t ntris, const o_bvh* bvh, const uint32_t* inst5, uint32_t n_inst,
+                    const uint32_t* idx, const float* nrm, const float* mat4s, const uint32_t* mats16,
+                    const uint32_t* emissive, uint32_t n_emissive, const uint8_t* light_block, size_t block_bytes,
+                    const float* view16, const float* proj16, const float* proj_view16, const float* cam_pos,
+                    uint32_t w, uint32_t h, uint32_t frames, uint32_t temporal, uint32_t spatial, float radius,
+                    uint32_t seed, float* out_rgb, int threads) {
+    oscene S;
+    memset(&S, 0, sizeof(S));
+    S.world = world;
+    S.ntris = ntris;
+    S.bvh = bvh;
+    S.inst5 = inst5;
+    S.n_inst = n_inst;
+    S.idx = idx;
+    S.nrm = nrm;
+    S.mats16 = mats16;
+    S.tri_off = (uint32_t*)malloc(sizeof(uint32_t) * (n_inst + 1));
+    S.tri_off[0] = 0;
+    for (uint32_t i = 0; i < n_inst; ++i) S.tri_off[i + 1] = S.tri_off[i] + inst5[5 * i + 4] / 3;
+    S.nmat = (float*)malloc(sizeof(float) * 9 * (n_inst ? n_inst : 1));
+    for (uint32_t i = 0; i < n_inst; ++i) normal_matrix(mat4s + 16 * (size_t)inst5[5 * i], S.nmat + 9 * (size_t)i);
+    /* the light block as the shaders read it (LightBuffers.glsl:1-63) */
+    uint32_t n_pl = 0;
+    float* pl = NULL;
+    qjob J;
+    memset(&J, 0, sizeof(J));
+    if (light_block && block_bytes >= 112) {
+        float f[8];
+        memcpy(f, light_block, 32);
+        J.dir_colour = V(f[0], f[1], f[2]);
+        J.dir_l = V(f[4], f[5], f[6]);
+        memcpy(&n_pl, light_block + 96, 4);
+        pl = (float*)calloc(12 * (size_t)(n_pl ? n_pl : 1), sizeof(float));
+        for (uint32_t i = 0; i < n_pl && 112 + 48 * (size_t)(i + 1) <= block_bytes; ++i) {
+            float e[12];
+            memcpy(e, light_block + 112 + 48 * (size_t)i, 48);
+            float* d = pl + 12 * (size_t)i;
+            d[0] = e[0], d[1] = e[1], d[2] = e[2];
+            d[4] = e[4], d[5] = e[5], d[6] = e[6];
+            d[8] = e[10], d[9] = e[9], d[10] = e[8]; /* (constant, linear, exp) */
+        }
+    }
+    const uint32_t n = w * h;
+    qvert* G = (qvert*)calloc(n, sizeof(qvert));
+    qvert* Gp = (qvert*)calloc(n, sizeof(qvert));
+    qres* hist = (qres*)malloc(sizeof(qres) * n);
+    qres* tmp = (qres*)malloc(sizeof(qres) * n);
+    for (uint32_t i = 0; i < n; ++i) hist[i] = tmp[i] = q_empty();
+    float cam[16];
+    o_camera_matrix(view16, proj16, cam);
+    const v3 pos = V(cam_pos[0], cam_pos[1], cam_pos[2]);
+    /* the primary surfaces (the same every frame: the camera does not move) */
+    for (uint32_t px = 0; px < n; ++px) {
+        qvert* x = &G[px];
+        v3 d = cam_dir(cam, pos, px % w, px / w, w, h);
+        float ray[8] = {pos.x, pos.y, pos.z, 0.0f, d.x, d.y, d.z, 1e30f};
+        float t, hu, hv;
+        uint32_t hi, hp;
+        o_trace_one(bvh, ray, 0, &t, &hi, &hp, &hu, &hv);
+        if (!(t >= 0.0f)) continue;
+        osurf s = surface(&S, d, hi, hp, hu, hv);
+        if (s.emissive) {
+            x->emit = s.le;
+            continue;
+        }
+        x->valid = 1;
+        x->p = s.p;
+        x->n = s.ns;
+        x->v = scl(d, -1.0f);
+        x->m = s.m;
+        x->t = t;
+        x->f0 = V(0.04f + (s.m.albedo.x - 0.04f) * s.m.metallic, 0.04f + (s.m.albedo.y - 0.04f) * s.m.metallic,
+                  0.04f + (s.m.albedo.z - 0.04f) * s.m.metallic);
+        x->biased = add(s.p, scl(s.ns, 0.01f + fmaxf(t * 0.001f, 0.01f)));
+    }
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    J.S = &S;
+    J.pl = pl;
+    J.n_pl = n_pl;
+    J.em = emissive;
+    J.n_em = n_emissive;
+    J.w = w;
+    J.h = h;
+    J.k = spatial;
+    J.radius = radius;
+    memcpy(J.pv, proj_view16, sizeof(J.pv));
+    J.G = G;
+    J.Gp = Gp;
+    J.hist = hist;
+    J.tmp = tmp;
+    J.out = out_rgb;
+    for (uint32_t f = 0; f < frames; ++f) {
+        J.state = o_pass_state(seed, f, 0);
+        J.have_prev = f > 0;
+        J.temporal = f > 0 && temporal != 0;
+        q_run(&J, threads, 0);
+        q_run(&J, threads, 1);
+        memcpy(Gp, G, sizeof(qvert) * n);
+    }
+    free(G);
+    free(Gp);
+    free(hist);
+    free(tmp);
+    free(pl);
+    free(S.tri_off);
+    free(S.nmat);
+    return 0;
+}

@@ -7,6 +7,7 @@
 // variance for the many-light configuration; the emissive instance list
 // (UpdateEmissiveIdxBuffer, RaytracingBufferSystem.cpp:105-130) still decides
 // which instances are emitters.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -161,6 +162,26 @@ snkpt_status build_emitters(snkpt_ctx* ctx, hipStream_t st) {
             push(w);
         }
     }
+    // the block as the reference's shaders read it: all point lights (black ones
+    // too: num_pointlights is their count) and the directional light unchanged
+    std::vector<float> rp;
+    std::fill(E.ref_dir, E.ref_dir + 8, 0.0f);
+    if (lb.size() >= 112) {
+        for (int k = 0; k < 3; ++k) {
+            E.ref_dir[k] = f32(4 * (size_t)k);
+            E.ref_dir[4 + k] = f32(16 + 4 * (size_t)k);
+        }
+        uint32_t np;
+        std::memcpy(&np, lb.data() + 96, 4);
+        for (uint32_t i = 0; i < np; ++i) {
+            const size_t o = 112 + 48 * (size_t)i;
+            const float w[12] = {f32(o), f32(o + 4), f32(o + 8), 0.0f, f32(o + 16), f32(o + 20), f32(o + 24), 0.0f,
+                                 f32(o + 40), f32(o + 36), f32(o + 32), 0.0f};
+            rp.insert(rp.end(), w, w + 12);
+        }
+    }
+    E.n_ref_pl = (uint32_t)(rp.size() / 12);
+    E.n_em_list = (uint32_t)list.size();
     E.n_analytic = (uint32_t)(al.size() / 16);
     E.q_tri = ne == 0 ? (E.n_analytic ? 0.0f : 1.0f) : (E.n_analytic ? 0.5f : 1.0f);
     snkpt_status s;
@@ -170,6 +191,8 @@ snkpt_status build_emitters(snkpt_ctx* ctx, hipStream_t st) {
     if ((s = up(E.alias, alias_packed.data(), alias_packed.size() * 4)) != SNKPT_OK) return s;
     if ((s = up(E.pdf, pdf.data(), pdf.size() * 4)) != SNKPT_OK) return s;
     if ((s = up(E.gid_to_emitter, gid_to_e.data(), gid_to_e.size() * 4)) != SNKPT_OK) return s;
+    if ((s = up(E.ref_pl, rp.data(), rp.size() * 4)) != SNKPT_OK) return s;
+    if ((s = up(E.em_list, list.data(), list.size() * 4)) != SNKPT_OK) return s;
     hipError_t e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
     E.count = ne;

@@ -33,6 +33,7 @@ struct RestirState {
     DeviceBuffer g[2][5];  // per frame parity: p|eps (0: no surface), ns|rough, ng|metal, albedo|flags, wo|t
     DeviceBuffer r_init, r_tmp, r_fin;  // float4 (emitter bits, b1, b2, W)
     DeviceBuffer m_init, m_tmp, m_fin;  // M
+    DeviceBuffer q_tmp, q_hist;  // reference-compatible reservoirs, 3 float4 per pixel (pass A out, pass B out)
     int cur = 0;
     bool have_prev = false;
     float prev_pv[16] = {0};
@@ -663,6 +664,439 @@ __global__ __launch_bounds__(kCompatBlock) __attribute__((amdgpu_waves_per_eu(4,
     }
 }
 
+
+// ---- the reference's own ReSTIR DI (Q10) for SNKPT_MODE_REFERENCE_COMPAT ----
+//
+// DoInitialWRS, WriteInitialReservoirs, ComputeAndWriteSpatialReservoirReuse and
+// CalculateRadianceFromReservoirs (RayGen.rgen:215-528) as the reference computes
+// them, for the pass's point lights and emissive triangles:
+//   candidates  32 point lights drawn uniformly (weight length(CalcPointlight) N_pl)
+//               and 32 emissive triangles (instance uniform in the emissive list,
+//               triangle uniform in it, folded uniform barycentrics; pdf
+//               1 / (N_inst N_tri area)), weight length(ComputeTriangleRadiance)
+//               / pdf: emission albedo CookTorranceBRDF / d^2, no emitter cosine;
+//               W = visibility w_sum / M / p_hat (the point light's ray aimed at
+//               pos - biased_pos / dist, as written at :345)
+//   temporal    the history (normal test > 0.95, M capped at 20 M with w_sum
+//               scaled) and the fresh reservoir are combined with weights
+//               p_hat (visibility included) M W, no MIS; M = M_fresh + M_hist;
+//               without a history the fresh reservoir is discarded (an empty one
+//               is written)
+//   spatial     own reservoir + k neighbours in a +-radius pixel box (redrawn until
+//               inside the image), skipped when the previous frame's normal there
+//               has dot < 0.95; every combine also counts 1 into M
+//   shade       directional light (shadowed), the point light and the emissive
+//               point times their W (shadowed)
+// Each UpdateReservoir adds 1 to M, as in the reference.  Deviations: PCG32 in
+// place of the wall-clock gold noise (numbers per pixel, stream = pixel; pass A
+// state ^ kPhaseQA: per point-light candidate u_idx, u_wrs; per emissive
+// candidate u_inst, u_tri, u_b1, u_b2, u_wrs; then per combine u_pl, u_em; pass B
+// state ^ kPhaseQB: u_pl, u_em of the own combine, per neighbour (ux, uy) per try
+// (at most 16 tries) and u_pl, u_em when it is combined); the emissive point is
+// interpolated from the world-space triangle and its area is the world-space
+// area (the reference's det(M) x object area, equal for rigid transforms); the
+// history is found by reprojection into the previous camera (no motion vectors)
+// and its normal is read at the nearest texel; surfaces of the previous frame
+// without a surface never pass the normal tests.
+constexpr uint64_t kPhaseQA = 0x51314141ull, kPhaseQB = 0x51314242ull;
+constexpr int kQBlock = 128;
+constexpr uint32_t kQInvalid = 0xFFFFFFFFu;
+
+struct RefLights {
+    const float4* pl;          // per point light, 3 float4: colour | position | (constant, linear, exp)
+    uint32_t n_pl;
+    const uint32_t* em_inst;   // emissive instance list
+    uint32_t n_em;
+    float4 dir_colour, dir_l;  // the block's directional light (its direction as stored)
+};
+
+struct QRes {
+    float wsum_pl, w_pl;
+    uint32_t pl, m_pl;
+    uint32_t inst, tri;
+    float b1, b2;
+    float wsum_em, w_em;
+    uint32_t m_em;
+};
+__device__ __forceinline__ QRes qres_empty() {
+    return QRes{0.0f, 0.0f, kQInvalid, 0u, kQInvalid, kQInvalid, 0.0f, 0.0f, 0.0f, 0.0f, 0u};
+}
+__device__ __forceinline__ QRes qres_load(const float4* r, uint32_t i) {
+    const float4 a = r[3 * (size_t)i], b = r[3 * (size_t)i + 1], c = r[3 * (size_t)i + 2];
+    return QRes{a.x, a.y, __float_as_uint(a.z), __float_as_uint(a.w), __float_as_uint(b.x), __float_as_uint(b.y),
+                b.z, b.w, c.x, c.y, __float_as_uint(c.z)};
+}
+__device__ __forceinline__ void qres_store(float4* r, uint32_t i, const QRes& x) {
+    r[3 * (size_t)i] = make_float4(x.wsum_pl, x.w_pl, __uint_as_float(x.pl), __uint_as_float(x.m_pl));
+    r[3 * (size_t)i + 1] = make_float4(__uint_as_float(x.inst), __uint_as_float(x.tri), x.b1, x.b2);
+    r[3 * (size_t)i + 2] = make_float4(x.wsum_em, x.w_em, __uint_as_float(x.m_em), 0.0f);
+}
+
+__device__ __forceinline__ float vlen(V3 a) { return sqrtf(dot(a, a)); }
+
+// CookTorranceBRDF (LightBuffers.glsl:103-124): (kd / PI + spec) n.l, kd without albedo
+__device__ __forceinline__ V3 ct_ref(V3 lpos, V3 sp, V3 v, V3 f0, V3 n, float rough, float metal) {
+    const V3 l = normalize(lpos - sp);
+    const V3 hh = normalize(v + l);
+    const float c1 = fminf(fmaxf(1.0f - fmaxf(dot(hh, v), 0.0f), 0.0f), 1.0f), c2 = c1 * c1;
+    const float fw = (c2 * c2) * c1;
+    const V3 F{f0.x + (1.0f - f0.x) * fw, f0.y + (1.0f - f0.y) * fw, f0.z + (1.0f - f0.z) * fw};
+    const float a = rough * rough;
+    const float D = ggx_d(fmaxf(dot(n, hh), 0.0f), a * a);
+    const float r = rough + 1.0f, k = (r * r) / 8.0f;
+    const float nv = fmaxf(dot(n, v), 0.0f), nl = fmaxf(dot(n, l), 0.0f);
+    const float G = g1_schlick(nv, k) * g1_schlick(nl, k);
+    const float den = 4.0f * nv * nl + 0.0001f;
+    const float km = 1.0f - metal;
+    const V3 spec = F * (D * G / den);
+    const V3 kd{(1.0f - F.x) * km, (1.0f - F.y) * km, (1.0f - F.z) * km};
+    return (kd * (1.0f / kRefPi) + spec) * nl;
+}
+
+// CalcPointlight (LightBuffers.glsl:145-157)
+__device__ __forceinline__ V3 calc_point(const RefLights& L, uint32_t k, V3 v, V3 f0, V3 wp, V3 n, const Mat& m) {
+    const float4 c = L.pl[3 * (size_t)k], p = L.pl[3 * (size_t)k + 1], at = L.pl[3 * (size_t)k + 2];
+    const V3 lp{p.x, p.y, p.z};
+    const float dist = vlen(lp - wp);
+    const float atten = (at.x + at.y * dist) + at.z * (dist * dist);
+    const V3 brdf = ct_ref(lp, wp, v, f0, n, m.roughness, m.metallic);
+    return mul(mul(brdf, m.albedo), V3{c.x, c.y, c.z}) * (1.0f / atten);
+}
+
+// CalcDirectionalLight (LightBuffers.glsl:127-143)
+__device__ __forceinline__ V3 calc_dir(const RefLights& L, V3 v, V3 f0, V3 n, const Mat& m) {
+    const V3 l{L.dir_l.x, L.dir_l.y, L.dir_l.z};
+    const V3 hh = normalize(v + l);
+    const float c1 = fminf(fmaxf(1.0f - fmaxf(dot(hh, v), 0.0f), 0.0f), 1.0f), c2 = c1 * c1;
+    const float fw = (c2 * c2) * c1;
+    const V3 F{f0.x + (1.0f - f0.x) * fw, f0.y + (1.0f - f0.y) * fw, f0.z + (1.0f - f0.z) * fw};
+    const float a = m.roughness * m.roughness;
+    const float D = ggx_d(fmaxf(dot(n, hh), 0.0f), a * a);
+    const float r = m.roughness + 1.0f, k = (r * r) / 8.0f;
+    const float nv = fmaxf(dot(n, v), 0.0f), nl = fmaxf(dot(n, l), 0.0f);
+    const float G = g1_schlick(nv, k) * g1_schlick(nl, k);
+    const float den = 4.0f * nv * nl + 0.0001f;
+    const float km = 1.0f - m.metallic;
+    const V3 spec = F * (D * G / den);
+    const V3 kd{(1.0f - F.x) * km * m.albedo.x, (1.0f - F.y) * km * m.albedo.y, (1.0f - F.z) * km * m.albedo.z};
+    return mul((kd * (1.0f / kRefPi) + spec) * nl, V3{L.dir_colour.x, L.dir_colour.y, L.dir_colour.z});
+}
+
+// Point, normal (normalised, not turned), emission and world-space area of the
+// emissive triangle `tri` of instance `inst` at barycentrics (b1, b2, 1 - b1 - b2)
+// of its vertices 0, 1, 2 (TriangleShadowTest / ComputeTriangleRadiance, :241-282).
+struct QTri {
+    V3 p, n, le;
+    float area;
+};
+__device__ __forceinline__ QTri tri_point(const SceneDev& S, uint32_t inst, uint32_t tri, float b1, float b2) {
+    const InstD in = S.inst[inst];
+    const float4* w = S.world + 3 * (size_t)(S.tri_off[inst] + tri);
+    const float4 r0 = w[0], r1 = w[1], r2 = w[2];
+    const V3 v0{r0.x, r0.y, r0.z}, v1{r0.w, r1.x, r1.y}, v2{r1.z, r1.w, r2.x};
+    const float b3 = 1.0f - b1 - b2;
+    QTri q;
+    q.p = (v0 * b1 + v1 * b2) + v2 * b3;
+    const uint32_t* ix = S.idx + in.index_offset + 3 * tri;
+    const V3 n0 = ld3(S.nrm + 3 * (size_t)(ix[0] + in.vertex_offset));
+    const V3 n1 = ld3(S.nrm + 3 * (size_t)(ix[1] + in.vertex_offset));
+    const V3 n2 = ld3(S.nrm + 3 * (size_t)(ix[2] + in.vertex_offset));
+    const V3 nl = (n0 * b1 + n1 * b2) + n2 * b3;
+    const float4* x = S.xf + 6 * (size_t)inst;
+    const float4 m0 = x[3], m1 = x[4], m2 = x[5];
+    q.n = normalize(V3{(m0.x * nl.x + m0.y * nl.y) + m0.z * nl.z, (m1.x * nl.x + m1.y * nl.y) + m1.z * nl.z,
+                       (m2.x * nl.x + m2.y * nl.y) + m2.z * nl.z});
+    q.area = 0.5f * vlen(cross(v1 - v0, v1 - v2));
+    const MatD& md = S.mats[in.material_idx];
+    q.le = V3{md.albedo[0], md.albedo[1], md.albedo[2]};
+    return q;
+}
+
+// ShadowTest (:101-121): 1 when nothing lies on (ro + t rd, t in [0.001, maxd])
+template <int STACK>
+__device__ __forceinline__ float qshadow(const SceneDev& S, V3 ro, V3 rd, float maxd, uint2* stk, uint32_t& n_sh) {
+    uint32_t nv = 0, tv = 0;
+    ++n_sh;
+    const Ray r{ro.x, ro.y, ro.z, 0.001f, rd.x, rd.y, rd.z, maxd};
+    return traverse<true, false, STACK>(S.nodes, S.leaf, r, stk, kQBlock, nv, tv).rec == 0xFFFFFFFFu ? 1.0f : 0.0f;
+}
+
+// ComputeTriangleRadiance (:258-282)
+template <int STACK>
+__device__ __forceinline__ V3 tri_radiance(const SceneDev& S, uint32_t inst, uint32_t tri, float b1, float b2, V3 v,
+                                           V3 f0, V3 wp, V3 n, const Mat& m, V3 biased, bool shadow, uint2* stk,
+                                           uint32_t& n_sh) {
+    const QTri q = tri_point(S, inst, tri, b1, b2);
+    const V3 brdf = ct_ref(q.p, wp, v, f0, n, m.roughness, m.metallic);
+    const V3 d = wp - q.p;
+    V3 rad = mul(mul(q.le, m.albedo), brdf) * (1.0f / fabsf(dot(d, d)));
+    if (shadow) {
+        const V3 tgt = q.p + q.n * 0.01f;
+        const float pd = vlen(biased - tgt);
+        rad = rad * qshadow<STACK>(S, biased, (tgt - biased) * (1.0f / pd), pd, stk, n_sh);
+    }
+    return rad;
+}
+
+__device__ __forceinline__ bool q_update(float& wsum, uint32_t& M, float weight, float u) {
+    M += 1u;
+    wsum += weight;
+    return wsum > 0.0f && u < weight / wsum;
+}
+
+// Primary-vertex inputs of the reference's shading (RayGen.rgen:547-553)
+struct QVert {
+    V3 p, n, v, f0, biased;
+    Mat m;
+};
+__device__ __forceinline__ QVert qvert(const GSurf& g) {
+    QVert q;
+    q.p = g.p;
+    q.n = g.ns;
+    q.v = g.wo;
+    q.m = g.m;
+    q.f0 = V3{0.04f + (g.m.albedo.x - 0.04f) * g.m.metallic, 0.04f + (g.m.albedo.y - 0.04f) * g.m.metallic,
+              0.04f + (g.m.albedo.z - 0.04f) * g.m.metallic};
+    q.biased = g.p + g.ns * (0.01f + fmaxf(g.t * 0.001f, 0.01f));
+    return q;
+}
+
+// CombinePixelReservoir (:361-386)
+template <int STACK>
+__device__ __forceinline__ void q_combine(const SceneDev& S, const RefLights& L, const QVert& x, QRes& r,
+                                          const QRes& nb, float& ph_pl, float& ph_em, Rng& rng, uint2* stk,
+                                          uint32_t& n_sh) {
+    const float u_pl = rng.uniform(), u_em = rng.uniform();
+    if (nb.pl != kQInvalid) {
+        const float4 pp = L.pl[3 * (size_t)nb.pl + 1];
+        const V3 lp{pp.x, pp.y, pp.z};
+        const float ld = vlen(lp - x.biased);
+        const float ph = vlen(calc_point(L, nb.pl, x.v, x.f0, x.p, x.n, x.m)) *
+                         qshadow<STACK>(S, x.biased, (lp - x.biased) * (1.0f / ld), ld - 0.001f, stk, n_sh);
+        const float weight = ph * (float)nb.m_pl * nb.w_pl;
+        if (q_update(r.wsum_pl, r.m_pl, weight, u_pl)) {
+            r.pl = nb.pl;
+            ph_pl = ph;
+        }
+    }
+    if (nb.inst != kQInvalid) {
+        const float ph = vlen(tri_radiance<STACK>(S, nb.inst, nb.tri, nb.b1, nb.b2, x.v, x.f0, x.p, x.n, x.m,
+                                                  x.biased, true, stk, n_sh));
+        const float weight = ph * nb.w_em * (float)nb.m_em;
+        if (q_update(r.wsum_em, r.m_em, weight, u_em)) {
+            r.inst = nb.inst;
+            r.tri = nb.tri;
+            r.b1 = nb.b1;
+            r.b2 = nb.b2;
+            ph_em = ph;
+        }
+    }
+}
+
+// Pass A: DoInitialWRS + WriteInitialReservoirs (:296-357, :411-468) -> tmp
+template <int STACK>
+__global__ __launch_bounds__(kQBlock) void k_q_initial(SceneDev S, RefLights L, GView G, GView Gp,
+                                                       const float4* __restrict__ hist, float4* __restrict__ tmp,
+                                                       uint32_t w, uint32_t h, PrevCam pc, int temporal,
+                                                       uint64_t initstate, unsigned long long* __restrict__ stats) {
+    __shared__ uint2 stk[STACK * kQBlock];
+    __shared__ uint32_t red;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (threadIdx.x == 0) red = 0u;
+    __syncthreads();
+    uint32_t n_sh = 0;
+    if (i < w * h) {
+        const GSurf g = load_g(G, i);
+        QRes out = qres_empty();
+        if (g.valid) {
+            const QVert x = qvert(g);
+            Rng rng = rng_seed(initstate ^ kPhaseQA, i);
+            uint2* sk = &stk[threadIdx.x];
+            QRes r = qres_empty();
+            float ph_pl = 0.0f, ph_em = 0.0f;
+            for (uint32_t c = 0; c < 32u; ++c) {
+                const float u = rng.uniform(), uw = rng.uniform();
+                uint32_t k = 0;
+                float ph = 0.0f, weight = 0.0f;
+                if (L.n_pl > 0) {
+                    k = min((uint32_t)(u * (float)L.n_pl), L.n_pl - 1u);
+                    ph = vlen(calc_point(L, k, x.v, x.f0, x.p, x.n, x.m));
+                    weight = ph / (1.0f / (float)L.n_pl);
+                }
+                if (q_update(r.wsum_pl, r.m_pl, weight, uw)) {
+                    r.pl = k;
+                    ph_pl = ph;
+                }
+            }
+            for (uint32_t c = 0; c < 32u; ++c) {
+                const float ui = rng.uniform(), ut = rng.uniform(), ub1 = rng.uniform(), ub2 = rng.uniform(),
+                            uw = rng.uniform();
+                if (L.n_em == 0) {
+                    r.m_em += 1u;
+                    continue;
+                }
+                const uint32_t inst = L.em_inst[min((uint32_t)(ui * (float)L.n_em), L.n_em - 1u)];
+                const uint32_t ntri = S.inst[inst].num_indices / 3u;
+                if (ntri == 0) {
+                    r.m_em += 1u;
+                    continue;
+                }
+                const uint32_t tri = min((uint32_t)(ut * (float)ntri), ntri - 1u);
+                float b1 = ub1, b2 = ub2;
+                if (b1 + b2 >= 1.0f) {
+                    b1 = 1.0f - b1;
+                    b2 = 1.0f - b2;
+                }
+                const QTri q = tri_point(S, inst, tri, b1, b2);
+                const V3 rad = tri_radiance<STACK>(S, inst, tri, b1, b2, x.v, x.f0, x.p, x.n, x.m, x.biased, false,
+                                                   sk, n_sh);
+                const float ph = vlen(rad);
+                const float pdf = 1.0f / (((float)L.n_em * (float)ntri) * q.area);
+                const float weight = ph / pdf;
+                if (q_update(r.wsum_em, r.m_em, weight, uw)) {
+                    r.inst = inst;
+                    r.tri = tri;
+                    r.b1 = b1;
+                    r.b2 = b2;
+                    ph_em = ph;
+                }
+            }
+            if (r.pl != kQInvalid) {
+                const float4 pp = L.pl[3 * (size_t)r.pl + 1];
+                const V3 lp{pp.x, pp.y, pp.z};
+                const float d = vlen(lp - x.biased);
+                r.w_pl = qshadow<STACK>(S, x.biased, lp - x.biased * (1.0f / d), d, sk, n_sh) *
+                         ((r.wsum_pl / (float)r.m_pl) / ph_pl);
+            }
+            if (r.inst != kQInvalid) {
+                const QTri q = tri_point(S, r.inst, r.tri, r.b1, r.b2);
+                const V3 tgt = q.p + q.n * 0.01f;
+                const float pd = vlen(x.biased - tgt);
+                r.w_em = qshadow<STACK>(S, x.biased, (tgt - x.biased) * (1.0f / pd), pd, sk, n_sh) *
+                         ((r.wsum_em / (float)r.m_em) / ph_em);
+            }
+            // temporal resample: history by reprojection into the previous camera
+            int q = -1;
+            if (temporal) {
+                float cl[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    cl[k] = ((pc.pv[k] * g.p.x + pc.pv[4 + k] * g.p.y) + pc.pv[8 + k] * g.p.z) + pc.pv[12 + k];
+                if (cl[3] > 1e-6f) {
+                    const float fx = (cl[0] / cl[3] * 0.5f + 0.5f) * (float)w;
+                    const float fy = (cl[1] / cl[3] * 0.5f + 0.5f) * (float)h;
+                    if (fx > 0.0f && fy > 0.0f && fx < (float)w && fy < (float)h) q = (int)fy * (int)w + (int)fx;
+                }
+            }
+            if (q >= 0) {
+                const float4 pq = Gp.p[q], nq = Gp.n[q];
+                if (!(pq.w > 0.0f) || !(dot(V3{nq.x, nq.y, nq.z}, x.n) > 0.95f)) q = -1;
+            }
+            if (q >= 0) {
+                QRes hh = qres_load(hist, (uint32_t)q);
+                if ((float)hh.m_pl > 20.0f * (float)r.m_pl) {
+                    hh.wsum_pl *= 20.0f * (float)r.m_pl / (float)hh.m_pl;
+                    hh.m_pl = 20u * r.m_pl;
+                }
+                if ((float)hh.m_em > 20.0f * (float)r.m_em) {
+                    hh.wsum_em *= 20.0f * (float)r.m_em / (float)hh.m_em;
+                    hh.m_em = 20u * r.m_em;
+                }
+                float tp_pl = 0.0f, tp_em = 0.0f;
+                q_combine<STACK>(S, L, x, out, r, tp_pl, tp_em, rng, sk, n_sh);
+                q_combine<STACK>(S, L, x, out, hh, tp_pl, tp_em, rng, sk, n_sh);
+                out.m_em = hh.m_em + r.m_em;
+                out.m_pl = hh.m_pl + r.m_pl;
+                out.w_em = tp_em > 0.0f ? (out.wsum_em / (float)out.m_em) / tp_em : 0.0f;
+                out.w_pl = tp_pl > 0.0f ? (out.wsum_pl / (float)out.m_pl) / tp_pl : 0.0f;
+            }
+        }
+        qres_store(tmp, i, out);
+    }
+    for (int o = 32; o > 0; o >>= 1) n_sh += __shfl_xor(n_sh, o, 64);
+    if ((threadIdx.x & 63) == 0 && n_sh) atomicAdd(&red, n_sh);
+    __syncthreads();
+    if (threadIdx.x == 0 && stats && red) atomicAdd(&stats[2], (unsigned long long)red);
+}
+
+// Pass B: ComputeAndWriteSpatialReservoirReuse + CalculateRadianceFromReservoirs
+// + the directional light (:470-528, :560-565): tmp -> hist, radiance into rad
+template <int STACK>
+__global__ __launch_bounds__(kQBlock) void k_q_spatial(SceneDev S, RefLights L, GView G, GView Gp, int have_prev,
+                                                       const float4* __restrict__ tmp, float4* __restrict__ hist,
+                                                       float4* __restrict__ rad, uint32_t w, uint32_t h, uint32_t k,
+                                                       float radius, uint64_t initstate,
+                                                       unsigned long long* __restrict__ stats) {
+    __shared__ uint2 stk[STACK * kQBlock];
+    __shared__ uint32_t red;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (threadIdx.x == 0) red = 0u;
+    __syncthreads();
+    uint32_t n_sh = 0;
+    if (i < w * h) {
+        const GSurf g = load_g(G, i);
+        QRes r = qres_empty();
+        if (g.valid) {
+            const QVert x = qvert(g);
+            Rng rng = rng_seed(initstate ^ kPhaseQB, i);
+            uint2* sk = &stk[threadIdx.x];
+            float ph_pl = 0.0f, ph_em = 0.0f;
+            const QRes own = qres_load(tmp, i);
+            q_combine<STACK>(S, L, x, r, own, ph_pl, ph_em, rng, sk, n_sh);
+            r.m_em += own.m_em;
+            r.m_pl += own.m_pl;
+            const float cx0 = (float)(i % w) / (float)w, cy0 = (float)(i / w) / (float)h;
+            for (uint32_t s = 0; s < k; ++s) {
+                float cx = -1.0f, cy = -1.0f;
+                for (int t = 0; t < 16; ++t) {
+                    const float ux = rng.uniform(), uy = rng.uniform();
+                    const float ox = (ux - 0.5f) * 2.0f * radius / (float)w, oy = (uy - 0.5f) * 2.0f * radius / (float)h;
+                    const float tx = cx0 + ox, ty = cy0 + oy;
+                    if (tx >= 0.0f && ty >= 0.0f && tx <= 1.0f && ty <= 1.0f) {
+                        cx = tx;
+                        cy = ty;
+                        break;
+                    }
+                }
+                if (cx < 0.0f || !have_prev) continue;
+                const uint32_t nx = min((uint32_t)((float)w * cx), w - 1u), ny = min((uint32_t)((float)h * cy), h - 1u);
+                const uint32_t q = ny * w + nx;
+                const float4 pq = Gp.p[q], nq = Gp.n[q];
+                if (!(pq.w > 0.0f) || dot(V3{nq.x, nq.y, nq.z}, x.n) < 0.95f) continue;
+                const QRes nb = qres_load(tmp, q);
+                q_combine<STACK>(S, L, x, r, nb, ph_pl, ph_em, rng, sk, n_sh);
+                r.m_em += nb.m_em;
+                r.m_pl += nb.m_pl;
+            }
+            r.w_em = ph_em > 0.0f ? r.wsum_em / ph_em / (float)r.m_em : 0.0f;
+            r.w_pl = ph_pl > 0.0f ? r.wsum_pl / ph_pl / (float)r.m_pl : 0.0f;
+            // the pixel's light
+            V3 light = calc_dir(L, x.v, x.f0, x.n, x.m);
+            if (light.x > 0.0f || light.y > 0.0f || light.z > 0.0f)
+                light = light * qshadow<STACK>(S, x.biased, V3{L.dir_l.x, L.dir_l.y, L.dir_l.z}, 1000.0f, sk, n_sh);
+            if (L.n_pl > 0 && r.pl != kQInvalid) {
+                const float4 pp = L.pl[3 * (size_t)r.pl + 1];
+                const V3 lp{pp.x, pp.y, pp.z};
+                const float ld = vlen(lp - x.biased);
+                const V3 c = calc_point(L, r.pl, x.v, x.f0, x.p, x.n, x.m) *
+                             qshadow<STACK>(S, x.biased, (lp - x.biased) * (1.0f / ld), ld, sk, n_sh);
+                light = light + c * r.w_pl;
+            }
+            if (L.n_em > 0 && r.inst != kQInvalid) {
+                const V3 c = tri_radiance<STACK>(S, r.inst, r.tri, r.b1, r.b2, x.v, x.f0, x.p, x.n, x.m, x.biased,
+                                                 true, sk, n_sh);
+                light = light + c * r.w_em;
+            }
+            const float4 o = rad[i];
+            rad[i] = make_float4(o.x + light.x, o.y + light.y, o.z + light.z, o.w);
+        }
+        qres_store(hist, i, r);
+    }
+    for (int o = 32; o > 0; o >>= 1) n_sh += __shfl_xor(n_sh, o, 64);
+    if ((threadIdx.x & 63) == 0 && n_sh) atomicAdd(&red, n_sh);
+    __syncthreads();
+    if (threadIdx.x == 0 && stats && red) atomicAdd(&stats[2], (unsigned long long)red);
+}
 }  // namespace
 
 #define HIPR(expr)                                                \
@@ -670,6 +1104,74 @@ __global__ __launch_bounds__(kCompatBlock) __attribute__((amdgpu_waves_per_eu(4,
         hipError_t e_ = (expr);                                   \
         if (e_ != hipSuccess) return hip_fail(ctx, e_, #expr);    \
     } while (0)
+
+// The camera and primary surfaces of this frame become the next frame's history.
+static void remember_history(RestirState& X, RenderState& R, const snkpt_camera* cam) {
+    std::memcpy(X.prev_pv, cam->proj_view, sizeof(X.prev_pv));
+    X.prev_pos[0] = cam->cam_pos[0];
+    X.prev_pos[1] = cam->cam_pos[1];
+    X.prev_pos[2] = cam->cam_pos[2];
+    X.cur ^= 1;
+    X.have_prev = true;
+    R.history_valid = true;
+}
+
+// SNKPT_MODE_REFERENCE_COMPAT after the primary vertices: the reference's own
+// ReSTIR passes A and B (direct light) and BounceDiffuse (unless SNKPT_FLAG_NO_BOUNCE).
+static snkpt_status compat_pass(snkpt_ctx* ctx, const snkpt_settings* s, const SceneDev& S, const GView& G,
+                                const GView& Gp, bool temporal, uint64_t initstate, hipStream_t st) {
+    RenderState& R = *ctx->render;
+    RestirState& X = *R.restir;
+    const uint32_t w = R.w, h = R.h, n = w * h;
+    const Accel& A = ctx->accel;
+    if (A.num_tris == 0) return SNKPT_OK;
+    const uint32_t need = A.depth > 0 ? A.depth - 1 : 0;
+    if (need > 64) return fail(ctx, SNKPT_ERR_CAPACITY, "compat: BVH8 deeper than the 64-entry traversal stack");
+    unsigned long long* stats = R.stats.as<unsigned long long>();
+    const Emitters& E = ctx->emit;
+    RefLights L;
+    L.pl = E.ref_pl.as<float4>();
+    L.n_pl = E.n_ref_pl;
+    L.em_inst = E.em_list.as<uint32_t>();
+    L.n_em = E.n_em_list;
+    L.dir_colour = make_float4(E.ref_dir[0], E.ref_dir[1], E.ref_dir[2], 0.0f);
+    L.dir_l = make_float4(E.ref_dir[4], E.ref_dir[5], E.ref_dir[6], 0.0f);
+    PrevCam pc;
+    std::memcpy(pc.pv, X.prev_pv, sizeof(pc.pv));
+    pc.pos = V3{X.prev_pos[0], X.prev_pos[1], X.prev_pos[2]};
+    const uint32_t k = s->num_spatial_resamples > 0 ? std::min((uint32_t)s->num_spatial_resamples, 64u) : 0u;
+    const float radius = s->restir_radius > 0.0f ? s->restir_radius : 30.0f;
+    const uint32_t qg = (n + kQBlock - 1) / kQBlock;
+    float4* q_tmp = X.q_tmp.as<float4>();
+    float4* q_hist = X.q_hist.as<float4>();
+    const int have_prev = X.have_prev ? 1 : 0;
+#define Q_LAUNCH(ST)                                                                                              \
+    do {                                                                                                          \
+        k_q_initial<ST><<<qg, kQBlock, 0, st>>>(S, L, G, Gp, q_hist, q_tmp, w, h, pc, temporal ? 1 : 0, initstate, \
+                                                stats);                                                           \
+        k_q_spatial<ST><<<qg, kQBlock, 0, st>>>(S, L, G, Gp, have_prev, q_tmp, q_hist, R.rad.as<float4>(), w, h,  \
+                                                k, radius, initstate, stats);                                     \
+    } while (0)
+    if (need <= 16)
+        Q_LAUNCH(16);
+    else if (need <= 32)
+        Q_LAUNCH(32);
+    else
+        Q_LAUNCH(64);
+#undef Q_LAUNCH
+    if (!(s->flags & SNKPT_FLAG_NO_BOUNCE)) {
+        // BounceDiffuse over this frame's primary vertices
+        const uint32_t grid = (n + kCompatBlock - 1) / kCompatBlock;
+        if (need <= 16)
+            k_compat_bounce<16><<<grid, kCompatBlock, 0, st>>>(S, G, R.rad.as<float4>(), n, initstate, stats);
+        else if (need <= 32)
+            k_compat_bounce<32><<<grid, kCompatBlock, 0, st>>>(S, G, R.rad.as<float4>(), n, initstate, stats);
+        else
+            k_compat_bounce<64><<<grid, kCompatBlock, 0, st>>>(S, G, R.rad.as<float4>(), n, initstate, stats);
+    }
+    HIPR(hipGetLastError());
+    return SNKPT_OK;
+}
 
 snkpt_status restir_pass(snkpt_ctx* ctx, const snkpt_camera* cam, const CamDev& cd, const snkpt_settings* s,
                          uint64_t initstate, uint32_t pass, hipStream_t st) {
@@ -686,6 +1188,8 @@ snkpt_status restir_pass(snkpt_ctx* ctx, const snkpt_camera* cam, const CamDev& 
         HIPR(X.m_init.reserve((size_t)n * 4));
         HIPR(X.m_tmp.reserve((size_t)n * 4));
         HIPR(X.m_fin.reserve((size_t)n * 4));
+        HIPR(X.q_tmp.reserve((size_t)n * 48));
+        HIPR(X.q_hist.reserve((size_t)n * 48));
         X.cap = n;
         X.have_prev = false;
     }
@@ -702,7 +1206,8 @@ snkpt_status restir_pass(snkpt_ctx* ctx, const snkpt_camera* cam, const CamDev& 
     // the compatibility estimator replaces the path after the primary vertex
     const uint32_t max_depth = compat ? 1u : (s->max_depth ? s->max_depth : 1u);
     snkpt_status e = SNKPT_OK;
-    const uint32_t n_cand = s->restir_candidates ? s->restir_candidates : 32u;
+    // the reference-compatible mode runs its own candidates (pass A below)
+    const uint32_t n_cand = compat ? 0u : (s->restir_candidates ? s->restir_candidates : 32u);
     if (R.use_gb) {
         // primary vertices from the caller's G-buffer
         k_counts3<<<1, 64, 0, st>>>(counts, n, 0u, 0u);
@@ -723,6 +1228,11 @@ snkpt_status restir_pass(snkpt_ctx* ctx, const snkpt_camera* cam, const CamDev& 
             X.r_init.as<float4>(), X.m_init.as<float>(), counts, n, n_cand, max_depth, s->rr_start, initstate, stats);
     }
     const bool temporal = X.have_prev && s->num_temporal_resamples != 0 && !cam->prev_frame_invalid;
+    if (compat) {
+        if ((e = compat_pass(ctx, s, S, G, Gp, temporal, initstate, st)) != SNKPT_OK) return e;
+        remember_history(X, R, cam);
+        return SNKPT_OK;
+    }
     const float4* r_sp_in = X.r_init.as<float4>();
     const float* m_sp_in = X.m_init.as<float>();
     if (temporal) {
@@ -753,32 +1263,8 @@ snkpt_status restir_pass(snkpt_ctx* ctx, const snkpt_camera* cam, const CamDev& 
         if ((e = launch_shadow(ctx, R, R.rad.as<float4>(), st)) != SNKPT_OK) return e;
         k_restir_stats2<<<1, 64, 0, st>>>(counts, stats);
     }
-    // history for the next frame
-    std::memcpy(X.prev_pv, cam->proj_view, sizeof(X.prev_pv));
-    X.prev_pos[0] = cam->cam_pos[0];
-    X.prev_pos[1] = cam->cam_pos[1];
-    X.prev_pos[2] = cam->cam_pos[2];
-    X.cur ^= 1;
-    X.have_prev = true;
-    R.history_valid = true;
+    remember_history(X, R, cam);
     (void)pass;
-    if (compat) {
-        // BounceDiffuse over this frame's primary vertices (written to G before the swap)
-        const Accel& A = ctx->accel;
-        if (A.num_tris == 0) return SNKPT_OK;
-        const uint32_t need = A.depth > 0 ? A.depth - 1 : 0;
-        const uint32_t grid = (n + kCompatBlock - 1) / kCompatBlock;
-        if (need <= 16)
-            k_compat_bounce<16><<<grid, kCompatBlock, 0, st>>>(S, G, R.rad.as<float4>(), n, initstate, stats);
-        else if (need <= 32)
-            k_compat_bounce<32><<<grid, kCompatBlock, 0, st>>>(S, G, R.rad.as<float4>(), n, initstate, stats);
-        else if (need <= 64)
-            k_compat_bounce<64><<<grid, kCompatBlock, 0, st>>>(S, G, R.rad.as<float4>(), n, initstate, stats);
-        else
-            return fail(ctx, SNKPT_ERR_CAPACITY, "compat bounce: BVH8 deeper than the 64-entry traversal stack");
-        HIPR(hipGetLastError());
-        return SNKPT_OK;
-    }
     // remaining bounces with the plain path tracer
     if (max_depth > 1) return pt_bounces(ctx, s, 1, 1, st);
     return SNKPT_OK;

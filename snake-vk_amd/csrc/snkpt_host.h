@@ -78,6 +78,12 @@ struct Emitters {
     DeviceBuffer analytic;        // directional / point / spot lights, 4 float4 each
     uint32_t n_analytic = 0;
     float q_tri = 1.0f;           // NEE: probability of sampling the triangles rather than an analytic light
+    // the light block as the reference's shaders read it (SNKPT_MODE_REFERENCE_COMPAT)
+    DeviceBuffer ref_pl;          // every point light of the block, 3 float4: colour | position | (c, l, e)
+    uint32_t n_ref_pl = 0;
+    DeviceBuffer em_list;         // the emissive instance list
+    uint32_t n_em_list = 0;
+    float ref_dir[8] = {0};       // directional light colour, direction (as stored)
 };
 
 struct RenderState;  // defined in render.hip

@@ -40,6 +40,7 @@ FLAG_CACHE_INDIRECT = 8
 FLAG_CACHE_MIS = 32  # with FLAG_CACHE_INDIRECT: NEE + a traced BRDF probe under MIS at the query vertex
 FLAG_WHOLE_PATHS = 16
 FLAG_NO_STATS = 64  # no end-of-frame ray-count readback: PT frames become capturable into a HIP graph
+FLAG_NO_BOUNCE = 128  # reference-compat mode: the reference's ReSTIR direct light only, no BounceDiffuse
 FORMAT_RGBA32F = 0
 FORMAT_RGBA16F = 1
 

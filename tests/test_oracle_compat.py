@@ -96,3 +96,105 @@ def test_no_emitters_and_no_lights_give_zero_and_misses_are_skipped():
     sc2 = ArrayScene()
     sc2.quad([-5, 0, -5], [0, 0, 10], [10, 0, 0], [0.5, 0.5, 0.5], 1)
     assert np.all(_bounce(sc2, cam) == 0.0)
+
+
+# ---- the reference's own ReSTIR DI (Q10), RayGen.rgen:215-528 ----
+#
+# With one light the reservoirs' arithmetic has a closed form: every
+# UpdateReservoir adds 1 to M, so after the spatial pass's combine of the
+# pixel's own reservoir W = T.M T.W / (T.M + 1), and the temporal pass carries
+# w_sum along: in frame f >= 1 (before the M cap at 20 x 32 bites, f < 20) the
+# running w_sum holds 32 f candidates against M = 33 f, so the estimate is
+# 32/33 of the light (exactly for one point light, in expectation over the
+# candidates for one emissive triangle).  Frame 0 has no history, and the
+# reference discards a fresh reservoir without one: only the directional
+# light is left.
+
+
+def _lamp_scene(albedo):
+    sc = ArrayScene()
+    sc.quad([-5, 0, -5], [0, 0, 10], [10, 0, 0], albedo, 0, roughness=1.0, metallic=0.0)
+    return sc
+
+
+def test_q10_single_point_light_is_32_33_of_calcpointlight():
+    """One point light straight above a rough dielectric floor seen straight
+    down (v = l = n): CalcPointlight = (0.96 / PI + 0.04 / (4.0001 PI)) albedo
+    colour / (c + l d + e d^2), and the reservoirs deliver 32/33 of it."""
+    import snkpt
+    alb = np.array([0.7, 0.5, 0.3])
+    sc = _lamp_scene(alb)
+    col, hl = np.array([2.0, 3.0, 4.0]), 1.5
+    att = (0.5, 0.2, 1.0)  # exp, linear, constant
+    lb = snkpt.light_block(points=[(tuple(col), (0.2, hl, -0.1), att)])
+    cam = look_at_camera([0.2, 3.0, -0.1], [0.2, 0.0, -0.1], fov=0.2)
+    arrs = sc.arrays()
+    b = O.Bvh(sc.world())
+    brdf = 0.96 / REF_PI + 0.04 / (4.0001 * REF_PI)
+    light = brdf * alb * col / (att[2] + att[1] * hl + att[0] * hl * hl)
+    assert np.allclose(O.compat_direct(arrs, b, cam, 1, 1, 1, lb, 1, 0)[0, 0], 0.0)
+    for frames in (2, 5, 12):
+        got = O.compat_direct(arrs, b, cam, 1, 1, frames, lb, 1, 0)[0, 0]
+        assert np.allclose(got, 32.0 / 33.0 * light, rtol=2e-4), (frames, got, light)
+    # without temporal reuse every fresh reservoir is discarded (no history)
+    assert np.allclose(O.compat_direct(arrs, b, cam, 1, 1, 5, lb, 0, 0)[0, 0], 0.0)
+
+
+def test_q10_directional_light_is_calcdirectionallight_shadowed():
+    """The directional light alone, every frame: (kd albedo / PI + spec) n.l
+    colour with v = n, l = n at roughness 1 (h = n, F = 0.04, D = 1 / PI, G = 1)."""
+    import snkpt
+    alb = np.array([0.7, 0.5, 0.3])
+    sc = _lamp_scene(alb)
+    col = np.array([1.5, 1.0, 0.5])
+    lb = snkpt.light_block(dir_colour=tuple(col), dir_to_light=(0.0, 1.0, 0.0))
+    cam = look_at_camera([0.2, 3.0, -0.1], [0.2, 0.0, -0.1], fov=0.2)
+    b = O.Bvh(sc.world())
+    want = (0.96 * alb / REF_PI + 0.04 / (4.0001 * REF_PI)) * col
+    for frames in (1, 3):
+        got = O.compat_direct(sc.arrays(), b, cam, 1, 1, frames, lb, 1, 2)[0, 0]
+        assert np.allclose(got, want, rtol=2e-4), (got, want)
+
+
+def test_q10_single_emissive_triangle_expectation():
+    """One emissive triangle over a white rough floor: the pdf the candidates
+    divide by is 1 / (N_inst N_tri area) and the target has no emitter cosine,
+    so frames 1..16 average 32/33 of the integral of Le f n.l / d^2 over the
+    triangle (quadrature, f from the oracle's BRDF, which equals the
+    reference's CookTorranceBRDF x albedo for a white floor)."""
+    sc = _lamp_scene([1.0, 1.0, 1.0])
+    le = np.array([6.0, 4.0, 2.0])
+    tri = np.array([[-0.6, 1.2, -0.4], [0.7, 1.4, -0.3], [0.1, 1.1, 0.8]])
+    n = np.cross(tri[1] - tri[0], tri[2] - tri[0])
+    sc.add(tri, [0, 1, 2], [n / np.linalg.norm(n)] * 3, le, 1)
+    w = h = 6
+    cam = look_at_camera([1.8, 0.8, 0.3], [0.1, 0.0, 0.0], fov=30.0, w=w, h=h)  # below the emitter
+    arrs = sc.arrays()
+    b = O.Bvh(sc.world())
+    frames = [O.compat_direct(arrs, b, cam, w, h, f, None, 1, 0, seed=3) for f in range(2, 18)]
+    got = np.mean(frames, axis=0).reshape(-1, 3)
+    import raysets
+    org, dr = raysets.primary_rays(cam, w, h)
+    xs = org[:, :3].astype(np.float64) + dr[:, :3] * (-org[:, 1] / dr[:, 1])[:, None]
+    # Gauss-Legendre on the unit square mapped onto the triangle (Duffy)
+    g, gw = np.polynomial.legendre.leggauss(24)
+    g, gw = 0.5 * (g + 1), 0.5 * gw
+    area2 = np.linalg.norm(np.cross(tri[1] - tri[0], tri[2] - tri[0]))
+    want = np.zeros((w * h, 3))
+    for p_i, x in enumerate(xs):
+        wo = org[p_i, :3] - x
+        wo /= np.linalg.norm(wo)
+        acc = np.zeros(3)
+        for a, wa in zip(g, gw):
+            for c, wc in zip(g, gw):
+                s1, s2 = a, a * c  # (1 - s1) v0 + (s1 - s2) v1 + s2 v2
+                q = (1 - s1) * tri[0] + (s1 - s2) * tri[1] + s2 * tri[2]
+                d = q - x
+                r2 = d @ d
+                wi = d / np.sqrt(r2)
+                f, _ = O.brdf_eval([1.0, 1.0, 1.0], 1.0, 0.0, 0, [0, 1.0, 0], wo, wi)
+                acc += wa * wc * a * le * f * max(wi[1], 0.0) / r2
+        want[p_i] = acc * area2
+    ratio = got.sum(0) / (32.0 / 33.0 * want.sum(0))
+    assert np.allclose(ratio, 1.0, atol=0.02), ratio
+    assert np.allclose(got, 32.0 / 33.0 * want, rtol=0.1)
