@@ -155,7 +155,11 @@ typedef struct snkpt_settings {
     float cache_lr;                 /* cache mode: Adam learning rate */
     uint32_t flags;                 /* SNKPT_FLAG_* */
     float cache_ema;                /* cache mode: decay of the weight average used for queries (0 = raw weights) */
-    uint32_t reserved[4];
+    uint32_t cache_train_steps;     /* cache mode: optimiser steps per frame on its training samples (0 = 1, <= 16) */
+    float accum_weight;             /* SNKPT_FLAG_ACCUMULATE (snkpt_render): weight of this frame's samples in the
+                                       running mean (0 = 1); rising weights let a still-learning cache's early frames
+                                       fade out of the mean */
+    uint32_t reserved[2];
 } snkpt_settings;
 
 /* Ray and hit streams for snkpt_trace: structure-of-arrays, device pointers.

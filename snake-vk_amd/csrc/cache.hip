@@ -640,9 +640,11 @@ static void init_mlp(std::vector<float>& w, uint32_t seed) {
     next();
     state += 0x5EEDull;
     next();
+    // Glorot-uniform hidden layers; the output layer 100x smaller, so a fresh
+    // cache answers ~0 (the missing indirect light) rather than O(1) noise
     for (int l = 0; l <= kHidden; ++l) {
         int outs = l < kHidden ? kWidth : kOut;
-        float lim = std::sqrt(6.0f / (float)(kWidth + (l < kHidden ? kWidth : kOut)));
+        float lim = std::sqrt(6.0f / (float)(kWidth + (l < kHidden ? kWidth : kOut))) * (l < kHidden ? 1.0f : 0.01f);
         for (int o = 0; o < outs; ++o)
             for (int i = 0; i < kWidth; ++i)
                 w[l * kLayerParams + o * kWidth + i] = ((float)(next() >> 8) * (1.0f / 16777216.0f) * 2.0f - 1.0f) * lim;
@@ -1160,15 +1162,24 @@ snkpt_status cache_train_paths(snkpt_ctx* ctx, const CamDev& cd, uint32_t w, uin
     return SNKPT_OK;
 }
 
-// One optimiser step on the samples cache_train_paths emitted (reads their count).
+// Optimiser steps per frame on the frame's training samples (settings field; 0 = 1, at most 16).
+static uint32_t train_steps(const snkpt_settings* s) {
+    return s->cache_train_steps == 0 ? 1u : std::min(s->cache_train_steps, 16u);
+}
+
+// The frame's optimiser steps on the samples cache_train_paths emitted (reads their count).
 snkpt_status cache_train_step(snkpt_ctx* ctx, const snkpt_settings* s, hipStream_t st) {
     CacheState& C = *ctx->cache;
     uint32_t nt = 0;
     HIPC(hipMemcpyAsync(&nt, C.tcounts.as<uint32_t>() + 1, 4, hipMemcpyDeviceToHost, st));
     HIPC(hipStreamSynchronize(st));
     if (nt == 0) return SNKPT_OK;
-    return cache_train_ema(ctx, C.t_samples.as<float>(), nt, s->cache_lr > 0.0f ? s->cache_lr : 1e-2f, 1,
-                           std::min(std::max(s->cache_ema, 0.0f), 0.9999f), nullptr, st);
+    for (uint32_t k = 0; k < train_steps(s); ++k) {
+        snkpt_status e = cache_train_ema(ctx, C.t_samples.as<float>(), nt, s->cache_lr > 0.0f ? s->cache_lr : 1e-2f, 1,
+                                         std::min(std::max(s->cache_ema, 0.0f), 0.9999f), nullptr, st);
+        if (e != SNKPT_OK) return e;
+    }
+    return SNKPT_OK;
 }
 
 // Optimiser steps of several contexts whose training paths were traced
@@ -1191,10 +1202,12 @@ snkpt_status cache_train_steps(snkpt_ctx* const* ctxs, uint32_t n, const snkpt_s
     for (uint32_t k = 0; k < n; ++k) {
         if (nts[k] == 0) continue;
         CacheState& C = *ctxs[k]->cache;
-        snkpt_status e = cache_train_ema(ctxs[k], C.t_samples.as<float>(), nts[k],
-                                         s->cache_lr > 0.0f ? s->cache_lr : 1e-2f, 1,
-                                         std::min(std::max(s->cache_ema, 0.0f), 0.9999f), nullptr, st);
-        if (e != SNKPT_OK) return e;
+        for (uint32_t j = 0; j < train_steps(s); ++j) {
+            snkpt_status e = cache_train_ema(ctxs[k], C.t_samples.as<float>(), nts[k],
+                                             s->cache_lr > 0.0f ? s->cache_lr : 1e-2f, 1,
+                                             std::min(std::max(s->cache_ema, 0.0f), 0.9999f), nullptr, st);
+            if (e != SNKPT_OK) return e;
+        }
     }
     return SNKPT_OK;
 }

@@ -187,15 +187,17 @@ __global__ __launch_bounds__(kMegaBlock) void k_pt_mega(SceneDev S, CamDev cam, 
     }
 }
 
+// Output of the frame: its mean radiance, or with `accumulate` the weighted
+// running mean (this frame's samples count `weight` each).
 __global__ void k_resolve(const float4* __restrict__ rad, float4* __restrict__ accum, void* __restrict__ out,
-                          uint32_t n, float spp, uint32_t accumulate, uint32_t fmt) {
+                          uint32_t n, float spp, uint32_t accumulate, uint32_t fmt, float weight) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float4 r = rad[i];
     float4 o;
     if (accumulate) {
         float4 a = accum[i];
-        a = make_float4(a.x + r.x, a.y + r.y, a.z + r.z, a.w + spp);
+        a = make_float4(a.x + weight * r.x, a.y + weight * r.y, a.z + weight * r.z, a.w + weight * spp);
         accum[i] = a;
         float inv = a.w > 0.0f ? 1.0f / a.w : 0.0f;
         o = make_float4(a.x * inv, a.y * inv, a.z * inv, 1.0f);
@@ -606,7 +608,8 @@ snkpt_status render(snkpt_ctx* ctx, const snkpt_camera* cam, const snkpt_setting
         if (e != SNKPT_OK) return e;
     }
     k_resolve<<<nblk(w * h, kBlock), kBlock, 0, st>>>(R.rad.as<float4>(), R.accum.as<float4>(), out, w * h,
-                                                      (float)spp, (s->flags & SNKPT_FLAG_ACCUMULATE) ? 1u : 0u, fmt);
+                                                      (float)spp, (s->flags & SNKPT_FLAG_ACCUMULATE) ? 1u : 0u, fmt,
+                                                      s->accum_weight > 0.0f ? s->accum_weight : 1.0f);
     if (s->flags & SNKPT_FLAG_ACCUMULATE) R.accum_frames += spp;
     hipError_t le = hipGetLastError();
     if (le != hipSuccess) return hip_fail(ctx, le, "render launch");

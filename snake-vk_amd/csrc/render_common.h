@@ -155,18 +155,15 @@ __device__ __forceinline__ LightSample sample_light(const SceneDev& S, V3 p, flo
     return ls;
 }
 
-// Analytic light k of the light block, uniformly chosen by u (LightBuffers.glsl:
+// Analytic light k of the light block (LightBuffers.glsl:
 // CalcDirectionalLight, CalcPointlight, CalcSpotlight): radiance arriving from
 // a delta light, attenuation c + l d + e d^2, spot falloff
 // 1 - (1 - cos) / (1 - aperture) inside the aperture, nothing beyond it or where
-// the attenuation exceeds 5000.  pdf = selection probability 1 / n.
-__device__ __forceinline__ LightSample sample_analytic(const SceneDev& S, V3 p, float u) {
+// the attenuation exceeds 5000.  pdf = 1 / n, its probability under a uniform choice.
+__device__ __forceinline__ LightSample analytic_light(const SceneDev& S, uint32_t k, V3 p) {
     LightSample ls;
     ls.ok = false;
     ls.delta = true;
-    if (S.n_alights == 0) return ls;
-    uint32_t k = (uint32_t)(u * (float)S.n_alights);
-    if (k >= S.n_alights) k = S.n_alights - 1;
     const float4* L = S.alights + 4 * (size_t)k;
     const float4 w0 = L[0], w1 = L[1];
     const uint32_t type = (uint32_t)w0.w;
@@ -198,6 +195,17 @@ __device__ __forceinline__ LightSample sample_analytic(const SceneDev& S, V3 p, 
     ls.dist = dist;
     ls.ok = ls.le.x > 0.0f || ls.le.y > 0.0f || ls.le.z > 0.0f;
     return ls;
+}
+__device__ __forceinline__ LightSample sample_analytic(const SceneDev& S, V3 p, float u) {
+    if (S.n_alights == 0) {
+        LightSample ls;
+        ls.ok = false;
+        ls.delta = true;
+        return ls;
+    }
+    uint32_t k = (uint32_t)(u * (float)S.n_alights);
+    if (k >= S.n_alights) k = S.n_alights - 1;
+    return analytic_light(S, k, p);
 }
 
 // Next-event estimation: the emissive triangles with probability q_tri (u0

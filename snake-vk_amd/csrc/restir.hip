@@ -33,6 +33,7 @@ struct RestirState {
     DeviceBuffer g[2][5];  // per frame parity: p|eps (0: no surface), ns|rough, ng|metal, albedo|flags, wo|t
     DeviceBuffer r_init, r_tmp, r_fin;  // float4 (emitter bits, b1, b2, W)
     DeviceBuffer m_init, m_tmp, m_fin;  // M
+    DeviceBuffer a_init, a_tmp, a_fin;  // analytic-light reservoirs: float4 (light bits, W, M, 0)
     DeviceBuffer q_tmp, q_hist;  // reference-compatible reservoirs, 3 float4 per pixel (pass A out, pass B out)
     int cur = 0;
     bool have_prev = false;
@@ -107,6 +108,43 @@ __device__ __forceinline__ float target(const SceneDev& S, const GSurf& g, uint3
     return luminance(cc);
 }
 
+// Unshadowed contribution of analytic light k (point, spot or directional) at
+// surface g: the target of the analytic-light reservoir (the reference's
+// point-light reservoir, RayGen.rgen:300-310, with this build's light kinds).
+__device__ __forceinline__ float atarget(const SceneDev& S, const GSurf& g, uint32_t k, V3* c_out, V3* wi_out,
+                                         float* dist_out) {
+    const LightSample ls = analytic_light(S, k, g.p);
+    if (!ls.ok) return 0.0f;
+    const float cs = dot(g.ns, ls.wi);
+    if (cs <= 0.0f || dot(g.ng, ls.wi) <= 0.0f) return 0.0f;
+    const V3 cc = mul(brdf_eval(g.m, g.ns, g.wo, ls.wi), ls.le) * cs;
+    if (c_out) *c_out = cc;
+    if (wi_out) *wi_out = ls.wi;
+    if (dist_out) *dist_out = ls.dist;
+    return luminance(cc);
+}
+
+struct ResA {
+    uint32_t k;
+    float W, M, wsum;
+};
+__device__ __forceinline__ ResA load_resa(const float4* a, uint32_t i) {
+    const float4 v = a[i];
+    return ResA{__float_as_uint(v.x), v.y, v.z, 0.0f};
+}
+__device__ __forceinline__ void store_resa(float4* a, uint32_t i, const ResA& r) {
+    a[i] = make_float4(__uint_as_float(r.k), r.W, r.M, 0.0f);
+}
+__device__ __forceinline__ bool resa_update(ResA& r, uint32_t k, float w, float u) {
+    if (!(w > 0.0f)) return false;
+    r.wsum += w;
+    if (u * r.wsum < w) {
+        r.k = k;
+        return true;
+    }
+    return false;
+}
+
 __device__ __forceinline__ bool res_update(Res& r, uint32_t e, float b1, float b2, float w, float u) {
     if (!(w > 0.0f)) return false;
     r.wsum += w;
@@ -145,7 +183,8 @@ template <bool GB>
 __global__ __launch_bounds__(kBlock) void k_primary(SceneDev S, GView G, Queue in, HitArrays hit, GbufDev gb,
                                                     CamDev cam, uint32_t w, Queue out,
                                                     PathState ps, float4* __restrict__ rad, float4* __restrict__ r_out,
-                                                    float* __restrict__ m_out, uint32_t* __restrict__ counts,
+                                                    float* __restrict__ m_out, float4* __restrict__ a_out,
+                                                    uint32_t* __restrict__ counts,
                                                     uint32_t n, uint32_t n_cand, uint32_t max_depth,
                                                     uint32_t rr_start, uint64_t initstate,
                                                     unsigned long long* __restrict__ stats) {
@@ -157,6 +196,7 @@ __global__ __launch_bounds__(kBlock) void k_primary(SceneDev S, GView G, Queue i
     Rng rc{0, 0};
     if (valid) {
         Res r{0xFFFFFFFFu, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        ResA ra{0xFFFFFFFFu, 0.0f, 0.0f, 0.0f};
         V3 dir;
         float t;
         Surf s;
@@ -208,6 +248,19 @@ __global__ __launch_bounds__(kBlock) void k_primary(SceneDev S, GView G, Queue i
                 }
                 r.M = (float)n_cand;
                 r.W = ph > 0.0f ? r.wsum / (r.M * ph) : 0.0f;
+                // analytic lights: n_cand candidates drawn uniformly (source pdf 1 / n_alights)
+                if (S.n_alights > 0 && n_cand > 0) {
+                    float pa = 0.0f;
+                    for (uint32_t c = 0; c < n_cand; ++c) {
+                        const float u = rr.uniform(), uw = rr.uniform();
+                        uint32_t k = (uint32_t)(u * (float)S.n_alights);
+                        if (k >= S.n_alights) k = S.n_alights - 1;
+                        const float p = atarget(S, g, k, nullptr, nullptr, nullptr);
+                        if (resa_update(ra, k, p * (float)S.n_alights, uw)) pa = p;
+                    }
+                    ra.M = (float)n_cand;
+                    ra.W = pa > 0.0f ? ra.wsum / (ra.M * pa) : 0.0f;
+                }
                 // continuation into bounce 1 (direct light at this vertex is ReSTIR's)
                 if (max_depth > 1) {
                     rc = rng_seed(initstate ^ kPhaseCont, i);
@@ -238,6 +291,7 @@ __global__ __launch_bounds__(kBlock) void k_primary(SceneDev S, GView G, Queue i
         }
         G.p[i] = gp;
         store_res(r_out, m_out, i, r);
+        store_resa(a_out, i, ra);
     }
     __shared__ uint32_t lds_app[kBlock / 64 + 1];
     uint32_t slot = block_append<kBlock>(want_ext, &counts[1], lds_app);
@@ -263,11 +317,14 @@ __device__ __forceinline__ bool temporal_pixel(const SceneDev& S, const GView& G
                                                const float4* __restrict__ r_hist, const float* __restrict__ m_hist,
                                                float4* __restrict__ r_out, float* __restrict__ m_out, uint32_t w,
                                                uint32_t h, const PrevCam& pc, float m_cap, uint64_t initstate,
-                                               const uint16_t* __restrict__ motion) {
+                                               const uint16_t* __restrict__ motion, const float4* __restrict__ a_in,
+                                               const float4* __restrict__ a_hist, float4* __restrict__ a_out) {
     Res rc = load_res(r_in, m_in, i);
     GSurf g = load_g(G, i);
+    const bool alights = S.n_alights > 0;
     if (!g.valid) {
         store_res(r_out, m_out, i, rc);
+        if (alights) a_out[i] = a_in[i];
         return false;
     }
     int q = -1;
@@ -303,6 +360,7 @@ __device__ __forceinline__ bool temporal_pixel(const SceneDev& S, const GView& G
     }
     if (!gp.valid) {
         store_res(r_out, m_out, i, rc);
+        if (alights) a_out[i] = a_in[i];
         return false;
     }
     Res rh = load_res(r_hist, m_hist, (uint32_t)q);
@@ -327,6 +385,31 @@ __device__ __forceinline__ bool temporal_pixel(const SceneDev& S, const GView& G
     s.M = rc.M + rh.M;
     s.W = ph > 0.0f ? s.wsum / ph : 0.0f;
     store_res(r_out, m_out, i, s);
+    if (alights) {
+        // the analytic-light reservoirs, combined the same way
+        const ResA ac = load_resa(a_in, i);
+        ResA ah = load_resa(a_hist, (uint32_t)q);
+        if (ah.M > m_cap * ac.M) ah.M = m_cap * ac.M;
+        ResA sa{0xFFFFFFFFu, 0.0f, 0.0f, 0.0f};
+        float pa = 0.0f;
+        if (ac.k != 0xFFFFFFFFu && ac.W > 0.0f) {
+            const float p_c = atarget(S, g, ac.k, nullptr, nullptr, nullptr);
+            const float p_p = atarget(S, gp, ac.k, nullptr, nullptr, nullptr);
+            const float den = ac.M * p_c + ah.M * p_p;
+            const float m = den > 0.0f ? ac.M * p_c / den : 0.0f;
+            if (resa_update(sa, ac.k, m * p_c * ac.W, rr.uniform())) pa = p_c;
+        }
+        if (ah.k < S.n_alights && ah.W > 0.0f) {
+            const float p_c = atarget(S, g, ah.k, nullptr, nullptr, nullptr);
+            const float p_p = atarget(S, gp, ah.k, nullptr, nullptr, nullptr);
+            const float den = ac.M * p_c + ah.M * p_p;
+            const float m = den > 0.0f ? ah.M * p_p / den : 0.0f;
+            if (resa_update(sa, ah.k, m * p_c * ah.W, rr.uniform())) pa = p_c;
+        }
+        sa.M = ac.M + ah.M;
+        sa.W = pa > 0.0f ? sa.wsum / pa : 0.0f;
+        store_resa(a_out, i, sa);
+    }
     return true;
 }
 
@@ -337,13 +420,15 @@ __global__ __launch_bounds__(kBlock) void k_temporal(SceneDev S, GView G, GView 
                                                      float* __restrict__ m_out, uint32_t w, uint32_t h, PrevCam pc,
                                                      float m_cap, uint64_t initstate,
                                                      const uint16_t* __restrict__ motion,
-                                                     unsigned long long* __restrict__ stats) {
+                                                     unsigned long long* __restrict__ stats,
+                                                     const float4* __restrict__ a_in,
+                                                     const float4* __restrict__ a_hist, float4* __restrict__ a_out) {
     __shared__ uint32_t red;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (threadIdx.x == 0) red = 0u;
     __syncthreads();
     const bool reused = i < w * h && temporal_pixel(S, G, Gp, i, r_in, m_in, r_hist, m_hist, r_out, m_out, w, h, pc,
-                                                    m_cap, initstate, motion);
+                                                    m_cap, initstate, motion, a_in, a_hist, a_out);
     const uint64_t b = __ballot(reused);
     if ((threadIdx.x & 63) == 0 && b) atomicAdd(&red, (uint32_t)__popcll(b));
     __syncthreads();
@@ -441,6 +526,78 @@ __global__ __launch_bounds__(kBlock) void k_spatial(SceneDev S, GView G, const f
     store_res(r_out, m_out, i, s);
 }
 
+// Spatial reuse of the analytic-light reservoirs (the reference combines its
+// point-light reservoirs in the same pass, RayGen.rgen:470-506): the same
+// neighbour tests and balance-heuristic weights as k_spatial, neighbours drawn
+// from a stream of their own.
+constexpr uint64_t kPhaseSpatA = 0x53504141ull;
+__global__ __launch_bounds__(kBlock) void k_spatial_analytic(SceneDev S, GView G, const float4* __restrict__ a_in,
+                                                             float4* __restrict__ a_out, uint32_t w, uint32_t h,
+                                                             uint32_t k, float radius, uint64_t initstate) {
+    const uint32_t x = blockIdx.x * kSpatTile + (threadIdx.x % kSpatTile);
+    const uint32_t y = blockIdx.y * kSpatTile + (threadIdx.x / kSpatTile);
+    if (x >= w || y >= h) return;
+    const uint32_t i = y * w + x;
+    const GSurf g = load_g(G, i);
+    if (!g.valid || k == 0) {
+        a_out[i] = a_in[i];
+        return;
+    }
+    Rng rr = rng_seed(initstate ^ kPhaseSpatA, i);
+    uint32_t px[kMaxSpatial + 1];
+    px[0] = i;
+    int ni = 1;
+    if (k > kMaxSpatial) k = kMaxSpatial;
+    for (uint32_t j = 0; j < k; ++j) {
+        const float ur = rr.uniform(), ua = rr.uniform();
+        const float rad = radius * sqrtf(ur), ang = 2.0f * kPi * ua;
+        const int nx = (int)x + (int)lrintf(rad * cosf(ang)), ny = (int)y + (int)lrintf(rad * sinf(ang));
+        if (nx < 0 || ny < 0 || nx >= (int)w || ny >= (int)h || (nx == (int)x && ny == (int)y)) continue;
+        const uint32_t q = (uint32_t)ny * w + (uint32_t)nx;
+        if (!neighbour_ok(G, q, g)) continue;
+        px[ni++] = q;
+    }
+    float Ms[kMaxSpatial + 1], uw[kMaxSpatial + 1], Wl[kMaxSpatial + 1];
+    uint32_t kl[kMaxSpatial + 1];
+    float den[kMaxSpatial + 1], p_self[kMaxSpatial + 1], p_l[kMaxSpatial + 1];
+#pragma unroll
+    for (int l = 0; l < (int)kMaxSpatial + 1; ++l) {
+        if (l >= ni) break;
+        const ResA rl = load_resa(a_in, px[l]);
+        Ms[l] = rl.M;
+        uw[l] = rr.uniform();
+        kl[l] = (rl.k < S.n_alights && rl.W > 0.0f) ? rl.k : 0xFFFFFFFFu;
+        Wl[l] = rl.W;
+        den[l] = p_self[l] = p_l[l] = 0.0f;
+    }
+#pragma unroll
+    for (int j = 0; j < (int)kMaxSpatial + 1; ++j) {
+        if (j >= ni) break;
+        const GSurf gj = j == 0 ? g : load_g(G, px[j]);
+#pragma unroll
+        for (int l = 0; l < (int)kMaxSpatial + 1; ++l) {
+            if (l >= ni) break;
+            if (kl[l] == 0xFFFFFFFFu) continue;
+            const float pj = atarget(S, gj, kl[l], nullptr, nullptr, nullptr);
+            if (j == 0) p_self[l] = pj;
+            if (j == l) p_l[l] = pj;
+            den[l] += Ms[j] * pj;
+        }
+    }
+    ResA s{0xFFFFFFFFu, 0.0f, 0.0f, 0.0f};
+    float ph = 0.0f;
+#pragma unroll
+    for (int l = 0; l < (int)kMaxSpatial + 1; ++l) {
+        if (l >= ni) break;
+        s.M += Ms[l];
+        if (kl[l] == 0xFFFFFFFFu) continue;
+        const float m = den[l] > 0.0f ? Ms[l] * p_l[l] / den[l] : 0.0f;
+        if (resa_update(s, kl[l], m * p_self[l] * Wl[l], uw[l])) ph = p_self[l];
+    }
+    s.W = ph > 0.0f ? s.wsum / ph : 0.0f;
+    store_resa(a_out, i, s);
+}
+
 // CalculateRadianceFromReservoirs (RayGen.rgen:508-528): f Le G W, visibility by shadow ray.
 __global__ __launch_bounds__(kBlock) void k_rshade(SceneDev S, GView G, const float4* __restrict__ r_in,
                                                    const float* __restrict__ m_in, ShadowQueue sq,
@@ -473,31 +630,25 @@ __global__ __launch_bounds__(kBlock) void k_rshade(SceneDev S, GView G, const fl
     }
 }
 
-// Analytic lights at the primary hit (ReSTIR resamples the emissive triangles
-// only): one light per pixel, uniformly chosen from a stream of its own, as
-// NEE with a shadow ray (delta lights: no MIS partner).
-__global__ __launch_bounds__(kBlock) void k_rshade_analytic(SceneDev S, GView G, ShadowQueue sq,
-                                                            uint32_t* __restrict__ counts, uint32_t n,
-                                                            uint64_t initstate) {
+// The analytic-light reservoir's sample at the primary hit (the reference's
+// point-light half of CalculateRadianceFromReservoirs, RayGen.rgen:513-519):
+// f Le cos W, visibility by shadow ray.
+__global__ __launch_bounds__(kBlock) void k_rshade_analytic(SceneDev S, GView G, const float4* __restrict__ a_in,
+                                                            ShadowQueue sq, uint32_t* __restrict__ counts,
+                                                            uint32_t n) {
     __shared__ uint32_t lds_app[kBlock / 64 + 1];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool want = false;
     V3 c, wi, o;
-    float tmax = 0.0f;
+    float tmax = 0.0f, dist = 0.0f;
     if (i < n) {
-        GSurf g = load_g(G, i);
+        const GSurf g = load_g(G, i);
         if (g.valid) {
-            Rng r = rng_seed(initstate ^ 0x414E414C59544943ull, i);
-            LightSample ls = sample_analytic(S, g.p, r.uniform());
-            if (ls.ok) {
-                float cs = dot(g.ns, ls.wi);
-                if (cs > 0.0f && dot(g.ng, ls.wi) > 0.0f) {
-                    c = mul(brdf_eval(g.m, g.ns, g.wo, ls.wi), ls.le) * (cs / ls.pdf);
-                    if (c.x > 0.0f || c.y > 0.0f || c.z > 0.0f) {
-                        shadow_segment(g.p, g.ng, g.eps, ls.wi, ls.dist, o, wi, tmax);
-                        want = true;
-                    }
-                }
+            const ResA r = load_resa(a_in, i);
+            if (r.k < S.n_alights && r.W > 0.0f && atarget(S, g, r.k, &c, &wi, &dist) > 0.0f) {
+                c = c * r.W;
+                shadow_segment(g.p, g.ng, g.eps, wi, dist, o, wi, tmax);
+                want = true;
             }
         }
     }
@@ -1188,6 +1339,9 @@ snkpt_status restir_pass(snkpt_ctx* ctx, const snkpt_camera* cam, const CamDev& 
         HIPR(X.m_init.reserve((size_t)n * 4));
         HIPR(X.m_tmp.reserve((size_t)n * 4));
         HIPR(X.m_fin.reserve((size_t)n * 4));
+        HIPR(X.a_init.reserve((size_t)n * 16));
+        HIPR(X.a_tmp.reserve((size_t)n * 16));
+        HIPR(X.a_fin.reserve((size_t)n * 16));
         HIPR(X.q_tmp.reserve((size_t)n * 48));
         HIPR(X.q_hist.reserve((size_t)n * 48));
         X.cap = n;
@@ -1213,7 +1367,8 @@ snkpt_status restir_pass(snkpt_ctx* ctx, const snkpt_camera* cam, const CamDev& 
         k_counts3<<<1, 64, 0, st>>>(counts, n, 0u, 0u);
         k_primary<true><<<grid_for(n, kBlock), kBlock, 0, st>>>(
             S, G, queue_of(R, 0), hits_of(R), R.gb, cd, w, queue_of(R, 1), paths_of(R), R.rad.as<float4>(),
-            X.r_init.as<float4>(), X.m_init.as<float>(), counts, n, n_cand, max_depth, s->rr_start, initstate, stats);
+            X.r_init.as<float4>(), X.m_init.as<float>(), X.a_init.as<float4>(), counts, n, n_cand, max_depth,
+            s->rr_start, initstate, stats);
     } else {
         // camera rays + primary hits
         generate_camera(ctx, cd, initstate, st);
@@ -1225,7 +1380,8 @@ snkpt_status restir_pass(snkpt_ctx* ctx, const snkpt_camera* cam, const CamDev& 
         k_counts3<<<1, 64, 0, st>>>(counts, n, 0u, 0u);
         k_primary<false><<<grid_for(n, kBlock), kBlock, 0, st>>>(
             S, G, queue_of(R, 0), hits_of(R), R.gb, cd, w, queue_of(R, 1), paths_of(R), R.rad.as<float4>(),
-            X.r_init.as<float4>(), X.m_init.as<float>(), counts, n, n_cand, max_depth, s->rr_start, initstate, stats);
+            X.r_init.as<float4>(), X.m_init.as<float>(), X.a_init.as<float4>(), counts, n, n_cand, max_depth,
+            s->rr_start, initstate, stats);
     }
     const bool temporal = X.have_prev && s->num_temporal_resamples != 0 && !cam->prev_frame_invalid;
     if (compat) {
@@ -1234,6 +1390,7 @@ snkpt_status restir_pass(snkpt_ctx* ctx, const snkpt_camera* cam, const CamDev& 
         return SNKPT_OK;
     }
     const float4* r_sp_in = X.r_init.as<float4>();
+    const float4* a_sp_in = X.a_init.as<float4>();
     const float* m_sp_in = X.m_init.as<float>();
     if (temporal) {
         PrevCam pc;
@@ -1243,8 +1400,11 @@ snkpt_status restir_pass(snkpt_ctx* ctx, const snkpt_camera* cam, const CamDev& 
         k_temporal<<<grid_for(n, kBlock), kBlock, 0, st>>>(S, G, Gp, X.r_init.as<float4>(), X.m_init.as<float>(),
                                                            X.r_fin.as<float4>(), X.m_fin.as<float>(),
                                                            X.r_tmp.as<float4>(), X.m_tmp.as<float>(), w, h, pc, m_cap,
-                                                           initstate, R.use_gb ? R.gb.motion : nullptr, stats);
+                                                           initstate, R.use_gb ? R.gb.motion : nullptr, stats,
+                                                           X.a_init.as<float4>(), X.a_fin.as<float4>(),
+                                                           X.a_tmp.as<float4>());
         r_sp_in = X.r_tmp.as<float4>();
+        a_sp_in = X.a_tmp.as<float4>();
         m_sp_in = X.m_tmp.as<float>();
     }
     const uint32_t k = s->num_spatial_resamples > 0 ? (uint32_t)s->num_spatial_resamples : 0u;
@@ -1259,7 +1419,11 @@ snkpt_status restir_pass(snkpt_ctx* ctx, const snkpt_camera* cam, const CamDev& 
     if (S.n_alights > 0) {
         // the shadow queue is reused for the analytic lights of the primary hits
         k_restir_stats2<<<1, 64, 0, st>>>(counts, nullptr);
-        k_rshade_analytic<<<grid_for(n, kBlock), kBlock, 0, st>>>(S, G, shadow_of(R), counts, n, initstate);
+        k_spatial_analytic<<<dim3((w + kSpatTile - 1) / kSpatTile, (h + kSpatTile - 1) / kSpatTile), kBlock, 0,
+                             st>>>(S, G, a_sp_in, X.a_fin.as<float4>(), w, h, k,
+                                   s->restir_radius > 0.0f ? s->restir_radius : 30.0f, initstate);
+        k_rshade_analytic<<<grid_for(n, kBlock), kBlock, 0, st>>>(S, G, X.a_fin.as<float4>(), shadow_of(R), counts,
+                                                                  n);
         if ((e = launch_shadow(ctx, R, R.rad.as<float4>(), st)) != SNKPT_OK) return e;
         k_restir_stats2<<<1, 64, 0, st>>>(counts, stats);
     }

@@ -152,3 +152,23 @@ def test_whole_path_kernel_matches_wavefront(gpu, config, w, h):
         outs.append((o.cpu().numpy(), st.extension_rays, st.shadow_rays))
     assert np.array_equal(outs[0][0], outs[1][0])
     assert outs[0][1:] == outs[1][1:]
+
+
+def test_weighted_accumulation_is_the_weighted_mean(gpu):
+    """SNKPT_FLAG_ACCUMULATE with accum_weight w_f: the output is
+    sum w_f x_f / sum w_f over the accumulated frames."""
+    import snkpt
+    scene = snkpt.SceneData(1)
+    ctx = snkpt.Context(0)
+    ctx.load_scene(scene)
+    ctx.build_accel()
+    w = h = 32
+    cam = scene.camera(0, w, h)
+    wts = [1.0, 4.0, 9.0]
+    singles = [_render(ctx, cam, w, h, spp=1, max_depth=4, seed=5, frame=f)[..., :3].astype(np.float64)
+               for f in range(3)]
+    for f, wt in enumerate(wts):
+        fl = snkpt.FLAG_ACCUMULATE | (snkpt.FLAG_RESET if f == 0 else 0)
+        got = _render(ctx, cam, w, h, spp=1, max_depth=4, seed=5, frame=f, flags=fl, accum_weight=wt)[..., :3]
+    want = sum(wt * x for wt, x in zip(wts, singles)) / sum(wts)
+    assert np.allclose(got, want, rtol=1e-5, atol=1e-6)
