@@ -736,7 +736,7 @@ def main():
     extra.update({"incoherent_rays": n, "primary_hit_fraction": round(hit_frac, 4),
                   "secondary_hit_fraction": round(secondary_hit, 4), "mean_nodes_visited": round(n_node, 3),
                   "mean_tris_visited": round(n_tri, 3), "bvh8_nodes": info.num_nodes, "bvh8_depth": info.depth,
-                  "gpu_build_ms": round(min(build_ms), 3),
+                  "gpu_build_ms": round(min(build_ms), 3), "trace_tuning": ctx.trace_tuning().as_dict(),
                   "transform_update_rebuild_ms_wall": round(min(dyn_ms[1:]), 3), "launch_ms": round(launch_ms, 4)})
 
     if not args.only_trace and not args.no_frames:

@@ -193,6 +193,25 @@ typedef struct snkpt_accel_info {
     double build_ms;          /* device time of the last build */
 } snkpt_accel_info;
 
+/* Launch shape of the context's persistent traversal kernel (snkpt_trace,
+ * snkpt_trace_counted and the extension rays of snkpt_render), [0] closest hit,
+ * [1] any hit: a wave hands new rays to its idle lanes once refill_lanes of them
+ * are idle, taking steps_per_check traversal steps between two such checks, with
+ * blocks_per_cu resident 128-lane blocks per CU.  snkpt_build_accel picks it with
+ * an autotuner (timed on 2^20 bounce-like rays leaving the scene's triangles; kept per device,
+ * stack size and scene size, so rebuilds of similar scenes reuse it) unless the
+ * caller fixed it with snkpt_set_trace_tuning (NULL hands it back to the
+ * autotuner).  The shape never changes a hit.  No reference counterpart (the
+ * driver schedules traceRaysKHR). */
+typedef struct snkpt_trace_tuning {
+    uint32_t refill_lanes[2];    /* 1..64 */
+    uint32_t steps_per_check[2]; /* 3 or 4 */
+    uint32_t blocks_per_cu[2];   /* 1..64 (capped by the LDS the stack needs) */
+    float ms[2];                 /* the autotuner's time for its 2^20-ray sample (0: set by hand) */
+    uint32_t fixed;              /* 1: set by snkpt_set_trace_tuning */
+    uint32_t reserved[3];
+} snkpt_trace_tuning;
+
 /* ---- context ------------------------------------------------------------- */
 snkpt_status snkpt_create(int hip_device, snkpt_ctx** out);
 void snkpt_destroy(snkpt_ctx* ctx);
@@ -226,6 +245,8 @@ snkpt_status snkpt_set_lights(snkpt_ctx* ctx, const void* light_block, size_t by
  * 8-bit quantised children), plus the emitter table.  Synchronises `stream`. */
 snkpt_status snkpt_build_accel(snkpt_ctx* ctx, snkpt_stream stream);
 snkpt_status snkpt_get_accel_info(const snkpt_ctx* ctx, snkpt_accel_info* out);
+snkpt_status snkpt_get_trace_tuning(const snkpt_ctx* ctx, snkpt_trace_tuning* out);
+snkpt_status snkpt_set_trace_tuning(snkpt_ctx* ctx, const snkpt_trace_tuning* tuning);
 
 /* Copies one build array to host memory for bit-exact checks.  `which`:
  *   0 sorted Morton keys (u32 / triangle)     1 sorted triangle ids (u32 / triangle)

@@ -279,7 +279,40 @@ snkpt_status snkpt_build_accel(snkpt_ctx* ctx, snkpt_stream stream) {
     if ((s = upload_prev_xf(ctx)) != SNKPT_OK) return s;
     ctx->inst_dirty = false;
     ctx->dirty = false;
-    return build_accel(ctx, st);
+    if ((s = build_accel(ctx, st)) != SNKPT_OK) return s;
+    return autotune_trace(ctx, st);
+}
+
+snkpt_status snkpt_get_trace_tuning(const snkpt_ctx* ctx, snkpt_trace_tuning* out) {
+    if (!ctx || !out) return SNKPT_ERR_INVALID_ARG;
+    std::memset(out, 0, sizeof(*out));
+    const Accel& A = ctx->accel;
+    for (int k = 0; k < 2; ++k) {
+        out->refill_lanes[k] = A.shape[k].refill;
+        out->steps_per_check[k] = A.shape[k].steps;
+        out->blocks_per_cu[k] = A.shape[k].blocks_per_cu;
+        out->ms[k] = A.shape[k].ms;
+    }
+    out->fixed = A.shape_fixed ? 1u : 0u;
+    return SNKPT_OK;
+}
+
+snkpt_status snkpt_set_trace_tuning(snkpt_ctx* ctx, const snkpt_trace_tuning* t) {
+    if (!ctx) return SNKPT_ERR_INVALID_ARG;
+    Accel& A = ctx->accel;
+    if (!t) {
+        A.shape_fixed = false;  // back to the autotuner (at the next build)
+        return SNKPT_OK;
+    }
+    for (int k = 0; k < 2; ++k)
+        if (t->refill_lanes[k] < 1 || t->refill_lanes[k] > 64 || (t->steps_per_check[k] != 3 &&
+            t->steps_per_check[k] != 4) || t->blocks_per_cu[k] < 1 || t->blocks_per_cu[k] > 64)
+            return fail(ctx, SNKPT_ERR_INVALID_ARG,
+                        "snkpt_set_trace_tuning: refill_lanes 1..64, steps_per_check 3 or 4, blocks_per_cu 1..64");
+    ++ctx->generation;
+    for (int k = 0; k < 2; ++k) A.shape[k] = TraceShape{t->refill_lanes[k], t->steps_per_check[k], t->blocks_per_cu[k], 0.0f};
+    A.shape_fixed = true;
+    return SNKPT_OK;
 }
 
 snkpt_status snkpt_get_accel_info(const snkpt_ctx* ctx, snkpt_accel_info* out) {

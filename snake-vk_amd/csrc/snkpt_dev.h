@@ -350,9 +350,11 @@ __device__ __forceinline__ bool trav_steps(const uint4* __restrict__ nodes, cons
     return false;
 }
 
-template <bool ANY, bool COUNT, int STACK, int REFILL, class Load, class Done>
-__device__ __forceinline__ void refill_trace(uint32_t n, uint32_t wave, uint32_t nwaves, uint2* stk, int stride,
-                                             Load load, Done done) {
+// refill_trace with the refill threshold and the steps per check chosen at run
+// time / compile time (STEPS); refill_trace below fixes them.
+template <bool ANY, bool COUNT, int STACK, int STEPS, class Load, class Done>
+__device__ __forceinline__ void refill_trace_tuned(uint32_t n, uint32_t wave, uint32_t nwaves, uint2* stk, int stride,
+                                                   uint32_t refill, Load load, Done done) {
     const uint32_t lane = __lane_id();
     const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const uint32_t band = nwaves * 64u;
@@ -371,7 +373,7 @@ __device__ __forceinline__ void refill_trace(uint32_t n, uint32_t wave, uint32_t
         const uint64_t idle = __ballot(!active);
         const uint32_t nidle = (uint32_t)__popcll(idle);
         if (nidle == 64u && cursor >= end) break;
-        if (cursor < end && (nidle >= (uint32_t)REFILL || nidle == 64u)) {
+        if (cursor < end && (nidle >= refill || nidle == 64u)) {
             if (!active) {
                 const uint32_t k = cursor + (uint32_t)__popcll(idle & below);
                 if (k < end) {
@@ -385,11 +387,17 @@ __device__ __forceinline__ void refill_trace(uint32_t n, uint32_t wave, uint32_t
             }
             cursor += nidle;
         }
-        if (active && trav_steps<ANY, COUNT, STACK, kStepsPerRefill>(nodes, tris, st, stk, stride, nv, tv)) {
+        if (active && trav_steps<ANY, COUNT, STACK, STEPS>(nodes, tris, st, stk, stride, nv, tv)) {
             active = false;
             done(id, st, nv, tv);
         }
     }
+}
+
+template <bool ANY, bool COUNT, int STACK, int REFILL, class Load, class Done>
+__device__ __forceinline__ void refill_trace(uint32_t n, uint32_t wave, uint32_t nwaves, uint2* stk, int stride,
+                                             Load load, Done done) {
+    refill_trace_tuned<ANY, COUNT, STACK, kStepsPerRefill>(n, wave, nwaves, stk, stride, (uint32_t)REFILL, load, done);
 }
 
 // Barycentrics (u, v) of a leaf record hit; bary = (1-u-v, u, v).

@@ -45,8 +45,19 @@ struct DeviceBuffer {
 };
 
 // Acceleration structure built by build_accel().
+// Launch shape of the persistent traversal kernel (refill_trace): refill once
+// `refill` lanes of a wave are idle, `steps` traversal steps between two
+// refill checks (3 or 4), `blocks_per_cu` resident 128-lane blocks per CU.
+// refill == 0: not set (the defaults apply).
+struct TraceShape {
+    uint32_t refill = 0, steps = 4, blocks_per_cu = 16;
+    float ms = 0.0f;  // the autotuner's measurement
+};
+
 struct Accel {
     bool built = false;
+    TraceShape shape[2];      // [closest hit, any hit]
+    bool shape_fixed = false; // set by the caller: the autotuner leaves it alone
     uint32_t num_tris = 0;
     uint32_t num_nodes = 0;
     uint32_t depth = 0;
@@ -166,6 +177,9 @@ snkpt_status trace_contacts(snkpt_ctx* ctx, const float4* org_tmin, const float4
 // taa.hip
 snkpt_status taa_resolve(snkpt_ctx* ctx, const void* current, void* out, const uint16_t* velocity, uint32_t w,
                          uint32_t h, uint32_t fmt, uint32_t reset, hipStream_t st);
+
+// trace.hip: launch shape of the traversal kernel for the built structure
+snkpt_status autotune_trace(snkpt_ctx* ctx, hipStream_t st);
 
 // cache.hip
 snkpt_status cache_reset(snkpt_ctx* ctx, uint32_t seed, const float* lo, const float* hi, hipStream_t st);

@@ -57,6 +57,7 @@ EXPORTS = [
     "snkpt_render_gbuffer", "snkpt_taa_resolve", "snkpt_trace_contacts", "snkpt_set_prev_transforms",
     "snkpt_import_memory_fd", "snkpt_release_memory", "snkpt_import_semaphore_fd", "snkpt_semaphore_wait",
     "snkpt_semaphore_signal", "snkpt_release_semaphore", "snkpt_cache_infer_noalloc", "snkpt_get_generation",
+    "snkpt_get_trace_tuning", "snkpt_set_trace_tuning",
 ]
 CACHE_SAMPLE_FLOATS = 16
 CACHE_MLP_PARAMS = 4 * 4096 + 16 * 64
@@ -138,6 +139,17 @@ class BatchTuning(C.Structure):
                 "ms_per_frame": round(self.ms_per_frame, 4)}
 
 
+class TraceTuning(C.Structure):
+    _fields_ = [("refill_lanes", C.c_uint32 * 2), ("steps_per_check", C.c_uint32 * 2),
+                ("blocks_per_cu", C.c_uint32 * 2), ("ms", C.c_float * 2), ("fixed", C.c_uint32),
+                ("reserved", C.c_uint32 * 3)]
+
+    def as_dict(self):
+        return {k: [getattr(self, k)[0], getattr(self, k)[1]] for k in ("refill_lanes", "steps_per_check",
+                                                                         "blocks_per_cu")} | \
+            {"ms": [round(self.ms[0], 4), round(self.ms[1], 4)], "fixed": int(self.fixed)}
+
+
 class Scene(C.Structure):
     _fields_ = [("pos", C.POINTER(C.c_float)), ("nrm", C.POINTER(C.c_float)), ("uv", C.POINTER(C.c_float)),
                 ("tan", C.POINTER(C.c_float)), ("n_verts", C.c_uint32), ("idx", C.POINTER(C.c_uint32)),
@@ -191,6 +203,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "snkpt_cache_infer": (I32, [VP, VP, U32, VP, VP]),
         "snkpt_cache_infer_noalloc": (I32, [VP, VP, U32, VP, VP]),
         "snkpt_get_generation": (I32, [VP, VP]),
+        "snkpt_get_trace_tuning": (I32, [VP, VP]),
+        "snkpt_set_trace_tuning": (I32, [VP, VP]),
         "snkpt_cache_train": (I32, [VP, VP, U32, C.c_float, C.c_int, P(C.c_float), VP]),
         "snkpt_cache_params": (I32, [VP, U32, VP, SZ, P(SZ)]),
         "snkpt_cache_set_params": (I32, [VP, U32, VP, SZ]),
@@ -561,6 +575,15 @@ class Context:
         rc = self.lib.snkpt_taa_resolve(self._h, _ptr(current), _ptr(out), _ptr(velocity), w, h, fmt, int(reset),
                                         _stream(stream))
         self._check(rc, "snkpt_taa_resolve")
+
+    def trace_tuning(self) -> TraceTuning:
+        t = TraceTuning()
+        self._check(self.lib.snkpt_get_trace_tuning(self._h, C.byref(t)), "snkpt_get_trace_tuning")
+        return t
+
+    def set_trace_tuning(self, t: TraceTuning | None):
+        self._check(self.lib.snkpt_set_trace_tuning(self._h, None if t is None else C.byref(t)),
+                    "snkpt_set_trace_tuning")
 
     def generation(self) -> int:
         """The context's state generation (changes whenever a captured frame

@@ -290,3 +290,58 @@ def test_ray_counts_around_band_edges(gpu):
             g = gpu_trace(ctx, rays[:n], any_hit=any_hit)
             for k, name in enumerate(["t", "inst", "prim", "u", "v", "nodes visited", "tris visited"]):
                 assert_bits_equal(g[k], c[k][:n], f"n={n} {name}")
+
+
+def test_autotuned_launch_shapes_keep_every_hit(gpu):
+    """snkpt_build_accel's autotuner picks a launch shape per kind; every shape
+    (refill threshold, steps per check, resident blocks) gives the same hits
+    bit for bit, closest and any hit, as the oracle's traversal."""
+    import torch
+
+    import raysets
+    import snkpt
+    scene = snkpt.SceneData(2)
+    ctx = snkpt.Context(0)
+    ctx.load_scene(scene)
+    ctx.build_accel()
+    t0 = ctx.trace_tuning()
+    print("autotuned:", t0.as_dict(), flush=True)
+    assert not t0.fixed and t0.ms[0] > 0 and t0.ms[1] > 0
+    assert t0.refill_lanes[0] in (4, 8, 16) and t0.steps_per_check[0] in (3, 4)
+    org, dr = raysets.random_rays(200_000, scene_lo(ctx), scene_hi(ctx), seed=8)
+    o, d = torch.from_numpy(org).cuda(), torch.from_numpy(dr).cuda()
+    ref = None
+    for refill, steps, bpc in ((8, 4, 16), (1, 3, 3), (64, 4, 1), (16, 3, 12)):
+        t = snkpt.TraceTuning()
+        t.refill_lanes[:] = [refill, refill]
+        t.steps_per_check[:] = [steps, steps]
+        t.blocks_per_cu[:] = [bpc, bpc]
+        ctx.set_trace_tuning(t)
+        assert ctx.trace_tuning().fixed == 1
+        got = [[x.cpu().numpy() for x in ctx.trace(o, d, any_hit=a)[:3]] for a in (False, True)]
+        if ref is None:
+            ref = got
+            continue
+        for a in range(2):
+            for k in range(3):
+                assert_bits_equal(got[a][k], ref[a][k], f"shape {(refill, steps, bpc)} any={a} field {k}")
+    c = O.Bvh(scene_world(scene)).trace(np.concatenate([org, dr], 1))
+    assert_bits_equal(ref[0][0], c[0], "t vs oracle")
+    assert_bits_equal(ref[0][2], c[2], "prim vs oracle")
+    bad = snkpt.TraceTuning()
+    bad.refill_lanes[:] = [0, 8]
+    bad.steps_per_check[:] = [4, 4]
+    bad.blocks_per_cu[:] = [16, 16]
+    with pytest.raises(snkpt.SnkptError):
+        ctx.set_trace_tuning(bad)
+    ctx.set_trace_tuning(None)
+    ctx.build_accel()
+    assert ctx.trace_tuning().fixed == 0
+
+
+def scene_lo(ctx):
+    return list(ctx.accel_info().scene_lo)
+
+
+def scene_hi(ctx):
+    return list(ctx.accel_info().scene_hi)
