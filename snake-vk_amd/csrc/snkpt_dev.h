@@ -61,6 +61,11 @@ struct RayPre {
 __device__ __forceinline__ float sel3(float x, float y, float z, int k) { return k == 0 ? x : (k == 1 ? y : z); }
 __device__ __forceinline__ float ubyte_f(uint32_t w, int b) { return (float)((w >> (8 * b)) & 0xFFu); }
 
+// 0xFF in byte i for every set bit i of a 4-bit mask.
+__device__ __forceinline__ uint32_t byte_mask4(uint32_t m4) {
+    return ((m4 & 1u) | (m4 & 2u) << 7 | (m4 & 4u) << 14 | (m4 & 8u) << 21) * 0xFFu;
+}
+
 // 1/d; an axis with |d| < 1e-20 is "parallel" and gets NaN: every fused slab
 // term of that axis is then NaN, which fminf/fmaxf drop, and the axis is
 // instead tested exactly (origin inside the child's decoded [lo, hi]).  A huge
@@ -266,17 +271,33 @@ __device__ __forceinline__ bool trav_step(const uint4* __restrict__ nodes, const
         uint32_t fy0 = ny ? w2.z : w4.x, fy1 = ny ? w2.w : w4.y;
         uint32_t nz0 = nz ? w4.z : w3.x, nz1 = nz ? w4.w : w3.y;
         uint32_t fz0 = nz ? w3.x : w4.z, fz1 = nz ? w3.y : w4.w;
-        // parallel axes: exact origin-in-slab test on the decoded planes
-        uint32_t pmiss = 0;
-        if (p.par()) pmiss = parallel_miss(p, w0, w2, w3, w4);
-        uint32_t int_hits = 0, tri_hits = 0;
+        // meta bytes of slots 0-3 and 4-7; a parallel-axis miss clears the slot's
+        // byte, so it reads as empty below
+        uint32_t m0 = w1.z, m1 = w1.w;
+        if (p.par()) {
+            const uint32_t pmiss = parallel_miss(p, w0, w2, w3, w4);
+            m0 &= ~byte_mask4(pmiss & 15u);
+            m1 &= ~byte_mask4(pmiss >> 4);
+        }
+        // One hit word for both kinds of child, four slots at a time (Ylitie et al.
+        // 2017, sec. 4): an internal child's meta byte is 0x20 | (24 + s), the only
+        // bytes with bits 4 and 3 both set (leaf offsets stay below 24), and XOR-ing
+        // its low three bits with the octant turns its bit index into 24 + (s ^ oct),
+        // the visiting-order bit of the node group.  A leaf child keeps its index,
+        // the offset of its first triangle, and its unary count.  An empty slot's
+        // byte is 0 and sets nothing.
+        const uint32_t oct4 = __builtin_amdgcn_perm(0u, p.oct(), 0u);  // octant in every byte
+        const uint32_t in0 = (m0 & (m0 << 1)) & 0x10101010u, in1 = (m1 & (m1 << 1)) & 0x10101010u;
+        // (in >> 1) - (in >> 4): 0x07 in the bytes of internal children, no borrow between bytes
+        const uint32_t bi0 = (m0 ^ (oct4 & ((in0 >> 1) - (in0 >> 4)))) & 0x1F1F1F1Fu;
+        const uint32_t bi1 = (m1 ^ (oct4 & ((in1 >> 1) - (in1 >> 4)))) & 0x1F1F1F1Fu;
+        uint32_t hits = 0;
 #pragma unroll
         for (int s2 = 0; s2 < 8; ++s2) {
             const int b = s2 & 3;
             uint32_t nxw = s2 < 4 ? nx0 : nx1, fxw = s2 < 4 ? fx0 : fx1;
             uint32_t nyw = s2 < 4 ? ny0 : ny1, fyw = s2 < 4 ? fy0 : fy1;
             uint32_t nzw = s2 < 4 ? nz0 : nz1, fzw = s2 < 4 ? fz0 : fz1;
-            uint32_t metaw = s2 < 4 ? w1.z : w1.w;
             float tnx = fmaf(ubyte_f(nxw, b), sx, ax);
             float tny = fmaf(ubyte_f(nyw, b), sy, ay);
             float tnz = fmaf(ubyte_f(nzw, b), sz, az);
@@ -285,14 +306,10 @@ __device__ __forceinline__ bool trav_step(const uint4* __restrict__ nodes, const
             float tfz = fmaf(ubyte_f(fzw, b), sz, az);
             float tn = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), tmin);
             float tf = fminf(fminf(fminf(tfx, tfy), tfz), st.res.t) * kBoxRobust;
-            uint32_t meta = (metaw >> (8 * b)) & 0xFFu;
-            if (meta != 0u && tn <= tf && !((pmiss >> s2) & 1u)) {
-                if ((imask >> s2) & 1u)
-                    int_hits |= 1u << (24 + (s2 ^ p.oct()));
-                else
-                    tri_hits |= (meta >> 5) << (meta & 31u);
-            }
+            const uint32_t bits = (((s2 < 4 ? m0 : m1) >> (8 * b + 5)) & 7u) << (((s2 < 4 ? bi0 : bi1) >> (8 * b)) & 31u);
+            hits |= tn <= tf ? bits : 0u;
         }
+        const uint32_t int_hits = hits & 0xFF000000u, tri_hits = hits & 0x00FFFFFFu;
         st.ng_base = w1.x;
         st.ng_hits = int_hits | imask;
         st.tg_base = w1.y;

@@ -562,6 +562,15 @@ class Context:
                     "snkpt_import_semaphore_fd")
         return h
 
+    def semaphore_wait(self, sem, value: int = 0, stream=None):
+        self._check(self.lib.snkpt_semaphore_wait(self._h, sem, value, _stream(stream)), "snkpt_semaphore_wait")
+
+    def semaphore_signal(self, sem, value: int = 0, stream=None):
+        self._check(self.lib.snkpt_semaphore_signal(self._h, sem, value, _stream(stream)), "snkpt_semaphore_signal")
+
+    def release_semaphore(self, sem):
+        self._check(self.lib.snkpt_release_semaphore(self._h, sem), "snkpt_release_semaphore")
+
     def set_prev_transforms(self, mat4s):
         import numpy as np
         m = np.ascontiguousarray(mat4s, np.float32).reshape(-1)
