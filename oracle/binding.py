@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# SNKPT_ORACLE_LIB selects another build of the same sources (the sanitizer
+# build of `make -C oracle sanitize`)
+LIB_PATH = os.environ.get("SNKPT_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 
