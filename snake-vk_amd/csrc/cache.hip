@@ -26,7 +26,7 @@ namespace snkpt {
 struct CacheState {
     bool ready = false;
     GridGeom geom{};
-    uint32_t step = 0;
+    DeviceBuffer adam;        // AdamDev: the optimiser's step counter and the step's factors
     DeviceBuffer grid32, grid16, grid_m, grid_v, grid_grad;  // grid_grad: int64 fixed point
     DeviceBuffer mlp32, mlp16, mlp16t, mlp_m, mlp_v, mlp_gradf, grid_gradf;
     // exponential moving averages of the weights (fp32 and the fp16 copies queries read)
@@ -35,6 +35,7 @@ struct CacheState {
     uint32_t cap = 0;         // samples the inference work buffers hold
     uint32_t tcap = 0;        // samples the training work buffers hold
     uint32_t last_n = 0;
+    uint32_t t_max = 0;       // most training samples the last frame's paths may emit
     // render integration
     DeviceBuffer q_samples, q_meta, train_vtx, train_meta, t_samples, counts;
     DeviceBuffer tcounts, tstats;  // the training paths' own counters and ray statistics
@@ -338,15 +339,21 @@ __device__ __forceinline__ void back_dw(const _Float16* g, const _Float16* x, fl
 // Training step, one wave per 64 samples: forward with stored activations,
 // relative-L2 loss, backward to the weights (partials per wave) and to the 32
 // grid features of each sample.
+// The batch holds min(n, *n_dev) samples (n alone without n_dev); the loss is
+// the mean over them of the three channels' relative squared errors, scaled by
+// kLossScale for the FP16 backward.
 __global__ __launch_bounds__(64) void k_mlp_train(const _Float16* __restrict__ x0, const float* __restrict__ samples,
-                                                  uint32_t n, const _Float16* __restrict__ W,
-                                                  const _Float16* __restrict__ WT, float grad_scale,
+                                                  uint32_t n, const uint32_t* __restrict__ n_dev,
+                                                  const _Float16* __restrict__ W, const _Float16* __restrict__ WT,
                                                   float* __restrict__ partial, float* __restrict__ dx0,
                                                   float* __restrict__ loss_part) {
     __shared__ __attribute__((aligned(16))) _Float16 act[5][64 * kRow];
     __shared__ __attribute__((aligned(16))) _Float16 gb[2][64 * kRow];
     const int lane = threadIdx.x;
     const uint32_t base = blockIdx.x * 64;
+    if (n_dev) n = min(n, *n_dev);
+    if (base >= n) return;  // the whole wave is past the batch
+    const float grad_scale = kLossScale / (3.0f * (float)n);
     load_x0_tile(x0, base, n, act[0], lane);
     __syncthreads();
     for (int l = 0; l < kHidden; ++l) {
@@ -455,9 +462,11 @@ constexpr uint32_t kPrivEntries = priv_base(kPrivLevels - 1) + priv_cells(kPrivL
 // the sums do not depend on the order of the atomics).  One thread per sample
 // walks the levels in turn: at any moment the threads of the grid spread
 // their atomics over several levels.
-__global__ void k_grid_bwd(const float* __restrict__ samples, const float* __restrict__ dx0, uint32_t n, GridGeom G,
-                           unsigned long long* __restrict__ grad, unsigned long long* __restrict__ priv) {
+__global__ void k_grid_bwd(const float* __restrict__ samples, const float* __restrict__ dx0, uint32_t n,
+                           const uint32_t* __restrict__ n_dev, GridGeom G, unsigned long long* __restrict__ grad,
+                           unsigned long long* __restrict__ priv) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n_dev) n = min(n, *n_dev);
     if (i >= n) return;
     unsigned long long* mine = priv + (size_t)(blockIdx.x % kPrivCopies) * kPrivEntries * 2;
     float p[3];
@@ -497,20 +506,58 @@ __global__ void k_grid_priv_fold(unsigned long long* __restrict__ priv, unsigned
 }
 
 struct AdamP {
-    float lr_t;  // lr * sqrt(1 - b2^t) / (1 - b1^t)
     float b1, b2, eps, inv_scale;
-    float ema;   // weight-average decay of this step (0: the average is set to the weights)
     int apply;
 };
+
+// The optimiser's state that changes from step to step lives on the device, so
+// that the replays of a captured frame keep counting steps: k_adam_advance
+// moves the counter and sets the step's bias-corrected rate and weight-average
+// decay before the Adam kernels read them.
+struct AdamDev {
+    uint32_t t;     // steps taken
+    uint32_t live;  // this step has samples (0: the Adam kernels leave everything as it is)
+    float lr_t;     // lr * sqrt(1 - b2^t) / (1 - b1^t)
+    float ema;      // weight-average decay of this step (0: the average is set to the weights)
+};
+
+__global__ void k_adam_reset(AdamDev* a) {
+    if (threadIdx.x == 0) *a = AdamDev{0u, 0u, 0.0f, 0.0f};
+}
+
+__global__ void k_adam_advance(AdamDev* a, uint32_t n, const uint32_t* __restrict__ n_dev, float lr, float ema,
+                               float b1, float b2, int apply) {
+    if (threadIdx.x != 0) return;
+    if (n_dev) n = min(n, *n_dev);
+    AdamDev d = *a;
+    d.live = n > 0u ? 1u : 0u;
+    if (d.live && apply) d.t += 1u;
+    const uint32_t t = d.t ? d.t : 1u;
+    d.lr_t = (float)((double)lr * sqrt(1.0 - pow((double)b2, (double)t)) / (1.0 - pow((double)b1, (double)t)));
+    // decay ramp (1 + t) / (10 + t) so early averages are not dominated by the initial weights
+    d.ema = fminf(ema, (1.0f + (float)t) / (10.0f + (float)t));
+    *a = d;
+}
+
+// Split of the per-wave gradient partials of n samples into reduction groups.
+constexpr uint32_t kRedGroups = 64;
+__host__ __device__ __forceinline__ void red_shape(uint32_t n, uint32_t& chunks, uint32_t& groups, uint32_t& per) {
+    chunks = (n + 63) / 64;
+    groups = chunks < kRedGroups ? chunks : kRedGroups;
+    per = groups ? (chunks + groups - 1) / groups : 0u;
+}
 
 // Deterministic two-stage sum of the per-wave gradient partials: group y of
 // the grid sums chunks [y * per, (y + 1) * per) in order, then k_mlp_adam sums
 // the groups in order.
-constexpr uint32_t kRedGroups = 64;
-__global__ void k_mlp_reduce(const float* __restrict__ partial, uint32_t nchunks, uint32_t per,
+__global__ void k_mlp_reduce(const float* __restrict__ partial, uint32_t n, const uint32_t* __restrict__ n_dev,
                              float* __restrict__ red) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= kMlpParams) return;
+    if (n_dev) n = min(n, *n_dev);
+    uint32_t nchunks, groups, per;
+    red_shape(n, nchunks, groups, per);
+    if (blockIdx.y >= groups) return;
     uint32_t c0 = blockIdx.y * per, c1 = min(nchunks, c0 + per);
     float g = 0.0f;
     for (uint32_t c = c0; c < c1; ++c) g += partial[(size_t)c * kMlpParams + p];
@@ -519,25 +566,30 @@ __global__ void k_mlp_reduce(const float* __restrict__ partial, uint32_t nchunks
 
 // Adam on the reduced gradient.  Keeps fp32 masters, the fp16 weights and the
 // fp16 transposed weights in step.
-__global__ void k_mlp_adam(const float* __restrict__ red, uint32_t ngroups, float* __restrict__ w32,
-                           float* __restrict__ m, float* __restrict__ v, _Float16* __restrict__ w16,
-                           _Float16* __restrict__ wt16, float* __restrict__ gout, float* __restrict__ e32,
-                           _Float16* __restrict__ e16, AdamP A) {
+__global__ void k_mlp_adam(const float* __restrict__ red, uint32_t n, const uint32_t* __restrict__ n_dev,
+                           float* __restrict__ w32, float* __restrict__ m, float* __restrict__ v,
+                           _Float16* __restrict__ w16, _Float16* __restrict__ wt16, float* __restrict__ gout,
+                           float* __restrict__ e32, _Float16* __restrict__ e16, AdamP A,
+                           const AdamDev* __restrict__ ad) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= kMlpParams) return;
+    if (p >= kMlpParams || !ad->live) return;
+    if (n_dev) n = min(n, *n_dev);
+    uint32_t nchunks, ngroups, per;
+    red_shape(n, nchunks, ngroups, per);
     float g = 0.0f;
     for (uint32_t c = 0; c < ngroups; ++c) g += red[(size_t)c * kMlpParams + p];
     g *= A.inv_scale;
     gout[p] = g;
     if (!A.apply) return;
+    const float lr_t = ad->lr_t, ema = ad->ema;
     float mm = A.b1 * m[p] + (1.0f - A.b1) * g;
     float vv = A.b2 * v[p] + (1.0f - A.b2) * g * g;
     m[p] = mm;
     v[p] = vv;
-    float w = w32[p] - A.lr_t * mm / (sqrtf(vv) + A.eps);
+    float w = w32[p] - lr_t * mm / (sqrtf(vv) + A.eps);
     w32[p] = w;
     w16[p] = (_Float16)w;
-    const float e = A.ema * e32[p] + (1.0f - A.ema) * w;
+    const float e = ema * e32[p] + (1.0f - ema) * w;
     e32[p] = e;
     e16[p] = (_Float16)e;
     int layer = p / kLayerParams, r = p % kLayerParams;
@@ -550,22 +602,24 @@ __global__ void k_mlp_adam(const float* __restrict__ red, uint32_t ngroups, floa
 
 __global__ void k_grid_adam(unsigned long long* __restrict__ grad, float* __restrict__ w32, float* __restrict__ m,
                             float* __restrict__ v, __half* __restrict__ w16, float* __restrict__ gout,
-                            float* __restrict__ e32, __half* __restrict__ e16, size_t n, AdamP A) {
+                            float* __restrict__ e32, __half* __restrict__ e16, size_t n, AdamP A,
+                            const AdamDev* __restrict__ ad) {
     size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+    if (p >= n || !ad->live) return;
     long long gi = (long long)grad[p];
     grad[p] = 0ull;
     float g = (float)((double)gi / kFixScale) * A.inv_scale;
     if (gout) gout[p] = g;
     if (!A.apply) return;
+    const float lr_t = ad->lr_t, ema = ad->ema;
     float mm = A.b1 * m[p] + (1.0f - A.b1) * g;
     float vv = A.b2 * v[p] + (1.0f - A.b2) * g * g;
     m[p] = mm;
     v[p] = vv;
-    float w = w32[p] - A.lr_t * mm / (sqrtf(vv) + A.eps);
+    float w = w32[p] - lr_t * mm / (sqrtf(vv) + A.eps);
     w32[p] = w;
     w16[p] = __float2half(w);
-    const float e = A.ema * e32[p] + (1.0f - A.ema) * w;
+    const float e = ema * e32[p] + (1.0f - ema) * w;
     e32[p] = e;
     e16[p] = __float2half(e);
 }
@@ -712,8 +766,9 @@ snkpt_status cache_reset(snkpt_ctx* ctx, uint32_t seed, const float* lo, const f
     HIPC(C.mlp_e32.reserve(kMlpParams * 4));
     HIPC(C.mlp_e16.reserve(kMlpParams * 2));
     HIPC(sync_ema(C, st));
+    HIPC(C.adam.reserve(sizeof(AdamDev)));
+    k_adam_reset<<<1, 64, 0, st>>>(C.adam.as<AdamDev>());
     HIPC(hipStreamSynchronize(st));
-    C.step = 0;
     C.ready = true;
     return SNKPT_OK;
 }
@@ -809,51 +864,51 @@ snkpt_status cache_infer_noalloc(snkpt_ctx* ctx, const float* samples, uint32_t 
 
 snkpt_status cache_train(snkpt_ctx* ctx, const float* samples, uint32_t n, float lr, int apply, float* loss_out,
                          hipStream_t st) {
-    return cache_train_ema(ctx, samples, n, lr, apply, 0.0f, loss_out, st);
+    return cache_train_run(ctx, samples, n, nullptr, lr, apply, 0.0f, loss_out, st);
 }
 
-// One optimisation step; `ema` is the decay of the weight averages the render
-// queries read (0 sets them to the new weights).
-snkpt_status cache_train_ema(snkpt_ctx* ctx, const float* samples, uint32_t n, float lr, int apply, float ema,
-                             float* loss_out, hipStream_t st) {
+// One optimisation step on min(n, *n_dev) samples (n without n_dev): launch
+// sizes follow n, the kernels read the batch size on the device, so a frame's
+// step needs no readback and can be captured.  `ema` is the decay of the
+// weight averages the render queries read (0 sets them to the new weights).
+snkpt_status cache_train_run(snkpt_ctx* ctx, const float* samples, uint32_t n, const uint32_t* n_dev, float lr,
+                             int apply, float ema, float* loss_out, hipStream_t st) {
     CacheState* Cp = ctx->cache;
     if (!Cp || !Cp->ready) return fail(ctx, SNKPT_ERR_NOT_BUILT, "cache_train: call snkpt_cache_reset first");
     CacheState& C = *Cp;
     if (n == 0) return SNKPT_OK;
+    if (loss_out && n_dev) return fail(ctx, SNKPT_ERR_INVALID_ARG, "cache_train: the loss needs a host batch size");
     snkpt_status e = ensure_work(ctx, C, n, true);
     if (e != SNKPT_OK) return e;
+    const float b1 = 0.9f, b2 = 0.99f;
+    AdamDev* ad = C.adam.as<AdamDev>();
+    k_adam_advance<<<1, 64, 0, st>>>(ad, n, n_dev, lr, ema, b1, b2, apply);
     const uint32_t chunks = (n + 63) / 64;
-    k_encode<<<enc_blocks(n), kEncBlock, 0, st>>>(samples, n, nullptr, C.geom, C.grid16.as<__half2>(),
+    k_encode<<<enc_blocks(n), kEncBlock, 0, st>>>(samples, n, n_dev, C.geom, C.grid16.as<__half2>(),
                                                C.x0.as<_Float16>());
-    const float grad_scale = kLossScale / (3.0f * (float)n);
-    k_mlp_train<<<chunks, 64, 0, st>>>(C.x0.as<_Float16>(), samples, n, C.mlp16.as<_Float16>(),
-                                       C.mlp16t.as<_Float16>(), grad_scale, C.partial.as<float>(), C.dx0.as<float>(),
+    k_mlp_train<<<chunks, 64, 0, st>>>(C.x0.as<_Float16>(), samples, n, n_dev, C.mlp16.as<_Float16>(),
+                                       C.mlp16t.as<_Float16>(), C.partial.as<float>(), C.dx0.as<float>(),
                                        C.loss_part.as<float>());
-    k_grid_bwd<<<grid_for(n, 128), 128, 0, st>>>(samples, C.dx0.as<float>(), n, C.geom,
+    k_grid_bwd<<<grid_for(n, 128), 128, 0, st>>>(samples, C.dx0.as<float>(), n, n_dev, C.geom,
                                                  C.grid_grad.as<unsigned long long>(),
                                                  C.grid_priv.as<unsigned long long>());
     k_grid_priv_fold<<<grid_for(kPrivEntries * 2, 256), 256, 0, st>>>(C.grid_priv.as<unsigned long long>(),
                                                                       C.grid_grad.as<unsigned long long>());
-    if (apply) C.step += 1;
-    const float b1 = 0.9f, b2 = 0.99f;
-    const uint32_t t = C.step ? C.step : 1u;
     AdamP A;
     A.b1 = b1;
     A.b2 = b2;
     A.eps = 1e-15f;
-    A.lr_t = (float)((double)lr * std::sqrt(1.0 - std::pow((double)b2, (double)t)) / (1.0 - std::pow((double)b1, (double)t)));
     A.inv_scale = 1.0f / kLossScale;
     A.apply = apply;
-    // decay ramp (1 + t) / (10 + t) so early averages are not dominated by the initial weights
-    A.ema = std::min(ema, (1.0f + (float)t) / (10.0f + (float)t));
-    const uint32_t groups = std::min(kRedGroups, chunks), per = (chunks + groups - 1) / groups;
-    k_mlp_reduce<<<dim3((kMlpParams + 255) / 256, groups), 256, 0, st>>>(C.partial.as<float>(), chunks, per,
+    uint32_t rchunks, groups, per;
+    red_shape(n, rchunks, groups, per);
+    k_mlp_reduce<<<dim3((kMlpParams + 255) / 256, groups), 256, 0, st>>>(C.partial.as<float>(), n, n_dev,
                                                                          C.mlp_red.as<float>());
-    k_mlp_adam<<<(kMlpParams + 255) / 256, 256, 0, st>>>(C.mlp_red.as<float>(), groups, C.mlp32.as<float>(),
+    k_mlp_adam<<<(kMlpParams + 255) / 256, 256, 0, st>>>(C.mlp_red.as<float>(), n, n_dev, C.mlp32.as<float>(),
                                                         C.mlp_m.as<float>(), C.mlp_v.as<float>(),
                                                         C.mlp16.as<_Float16>(), C.mlp16t.as<_Float16>(),
                                                         C.mlp_gradf.as<float>(), C.mlp_e32.as<float>(),
-                                                        C.mlp_e16.as<_Float16>(), A);
+                                                        C.mlp_e16.as<_Float16>(), A, ad);
     // grid: the fixed-point gradient is consumed (and cleared) every call; without
     // Adam it is kept as floats so it can be read back
     float* gout = nullptr;
@@ -863,7 +918,7 @@ snkpt_status cache_train_ema(snkpt_ctx* ctx, const float* samples, uint32_t n, f
     }
     k_grid_adam<<<(uint32_t)((kGridParams + 255) / 256), 256, 0, st>>>(
         C.grid_grad.as<unsigned long long>(), C.grid32.as<float>(), C.grid_m.as<float>(), C.grid_v.as<float>(),
-        C.grid16.as<__half>(), gout, C.grid_e32.as<float>(), C.grid_e16.as<__half>(), kGridParams, A);
+        C.grid16.as<__half>(), gout, C.grid_e32.as<float>(), C.grid_e16.as<__half>(), kGridParams, A, ad);
     C.last_n = n;
     if (loss_out) {
         std::vector<float> lp(chunks);
@@ -986,7 +1041,8 @@ __global__ void k_fill4(float4* p, uint32_t n) {
 // recorded vertex v + 1): v's NEE and the MIS-weighted emission its
 // continuation hits are direct light, everything later is indirect.
 // Every vertex is a candidate and is kept with one probability p for the
-// whole frame (p = min(1, budget / all recorded vertices)), decided by a hash
+// whole frame (p = min(1, budget / all recorded vertices); the samples past
+// the budget, the last few in path order, are dropped), decided by a hash
 // of (seed, path, vertex): selection never depends on how a path continues,
 // which would bias the targets (picking one vertex per path favours short
 // paths).  Samples are emitted in (path, vertex) order through a scan, so the
@@ -1059,14 +1115,14 @@ __global__ void k_train_emit(const float* __restrict__ rec, const uint32_t* __re
                              const uint32_t* __restrict__ cnt, uint32_t budget, const uint32_t* __restrict__ off,
                              float* __restrict__ out, unsigned long long* __restrict__ stat) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j == 0) atomicAdd(stat, (unsigned long long)cnt[1]);
+    if (j == 0) atomicAdd(stat, (unsigned long long)min(cnt[1], budget));
     if (j >= n) return;
     const uint64_t thr = keep_threshold(cnt, budget);
     const uint32_t k = nrec[j];
     const float4 L = rad[j];
     uint32_t slot = off[j];
     float smp[16];
-    for (uint32_t v = 0; v < k; ++v) {
+    for (uint32_t v = 0; v < k && slot < budget; ++v) {
         if (!keep_vertex(seed, j, v, thr) || !train_sample(rec, max_rec, j, k, v, L, indirect, smp)) continue;
         float4* o = (float4*)(out + (size_t)slot * 16);
 #pragma unroll
@@ -1159,6 +1215,7 @@ snkpt_status cache_train_paths(snkpt_ctx* ctx, const CamDev& cd, uint32_t w, uin
     scan_exclusive(pcount, off, NT, sums, qc + 1, st);
     k_train_emit<<<grid_for(NT, 256), 256, 0, st>>>(C.train_vtx.as<float>(), nrec, depth, trad, NT, kseed, indirect,
                                                      qc, budget, off, C.t_samples.as<float>(), stats + 4);
+    C.t_max = budget;
     return SNKPT_OK;
 }
 
@@ -1167,15 +1224,13 @@ static uint32_t train_steps(const snkpt_settings* s) {
     return s->cache_train_steps == 0 ? 1u : std::min(s->cache_train_steps, 16u);
 }
 
-// The frame's optimiser steps on the samples cache_train_paths emitted (reads their count).
+// The frame's optimiser steps on the samples cache_train_paths emitted: their
+// count stays on the device (tcounts[1]; at most t_max of them are emitted).
 snkpt_status cache_train_step(snkpt_ctx* ctx, const snkpt_settings* s, hipStream_t st) {
     CacheState& C = *ctx->cache;
-    uint32_t nt = 0;
-    HIPC(hipMemcpyAsync(&nt, C.tcounts.as<uint32_t>() + 1, 4, hipMemcpyDeviceToHost, st));
-    HIPC(hipStreamSynchronize(st));
-    if (nt == 0) return SNKPT_OK;
     for (uint32_t k = 0; k < train_steps(s); ++k) {
-        snkpt_status e = cache_train_ema(ctx, C.t_samples.as<float>(), nt, s->cache_lr > 0.0f ? s->cache_lr : 1e-2f, 1,
+        snkpt_status e = cache_train_run(ctx, C.t_samples.as<float>(), C.t_max, C.tcounts.as<uint32_t>() + 1,
+                                         s->cache_lr > 0.0f ? s->cache_lr : 1e-2f, 1,
                                          std::min(std::max(s->cache_ema, 0.0f), 0.9999f), nullptr, st);
         if (e != SNKPT_OK) return e;
     }
@@ -1183,31 +1238,17 @@ snkpt_status cache_train_step(snkpt_ctx* ctx, const snkpt_settings* s, hipStream
 }
 
 // Optimiser steps of several contexts whose training paths were traced
-// (cache_train_paths): one readback of all their sample counts, then each
-// context's step in order on `st`.  The paths' ray statistics (tstats) are
-// added into `stats` first.
+// (cache_train_paths), each context's steps in order on `st`; the paths' ray
+// statistics (tstats) are added into `stats` first.  No readback.
 snkpt_status cache_train_steps(snkpt_ctx* const* ctxs, uint32_t n, const snkpt_settings* s,
                                unsigned long long* stats, hipStream_t st) {
-    std::vector<uint32_t> nts(n, 0u);
     for (uint32_t k = 0; k < n; ++k) {
-        snkpt_ctx* ctx = ctxs[k];
-        CacheState& C = *ctx->cache;
-        if (stats) k_add_u64<<<1, 64, 0, st>>>(stats, C.tstats.as<unsigned long long>(), 8);
-        HIPC(hipMemcpyAsync(&nts[k], C.tcounts.as<uint32_t>() + 1, 4, hipMemcpyDeviceToHost, st));
-    }
-    if (n) {
-        snkpt_ctx* ctx = ctxs[0];
-        HIPC(hipStreamSynchronize(st));
-    }
-    for (uint32_t k = 0; k < n; ++k) {
-        if (nts[k] == 0) continue;
         CacheState& C = *ctxs[k]->cache;
-        for (uint32_t j = 0; j < train_steps(s); ++j) {
-            snkpt_status e = cache_train_ema(ctxs[k], C.t_samples.as<float>(), nts[k],
-                                             s->cache_lr > 0.0f ? s->cache_lr : 1e-2f, 1,
-                                             std::min(std::max(s->cache_ema, 0.0f), 0.9999f), nullptr, st);
-            if (e != SNKPT_OK) return e;
-        }
+        if (stats) k_add_u64<<<1, 64, 0, st>>>(stats, C.tstats.as<unsigned long long>(), 8);
+    }
+    for (uint32_t k = 0; k < n; ++k) {
+        snkpt_status e = cache_train_step(ctxs[k], s, st);
+        if (e != SNKPT_OK) return e;
     }
     return SNKPT_OK;
 }
@@ -1221,6 +1262,28 @@ snkpt_status cache_train_paths_forked(snkpt_ctx* ctx, const CamDev& cd, uint32_t
     HIPC(C.tstats.reserve(64));
     k_clear_u64<<<1, 64, 0, side>>>(C.tstats.as<unsigned long long>(), 8);
     return cache_train_paths(ctx, cd, w, h, s, initstate, n_paths, C.tstats.as<unsigned long long>(), side);
+}
+
+// True when a cache-mode frame of w x h with these settings would allocate
+// nothing: the cache exists and every buffer the frame uses (queries, work
+// buffers, training paths, samples, side stream) already has its size, as after
+// an uncaptured frame with the same size, depth and training paths.
+bool cache_capture_ready(snkpt_ctx* ctx, const snkpt_settings* s, uint32_t w, uint32_t h) {
+    const CacheState* Cp = ctx->cache;
+    if (!Cp || !Cp->ready) return false;
+    const CacheState& C = *Cp;
+    const uint32_t n = w * h;
+    const uint32_t cap = ((n + 63) / 64) * 64;
+    if (C.rcap < n || C.cap < cap || C.counts.bytes < 64) return false;
+    if (s->flags & SNKPT_FLAG_NO_TRAIN) return true;
+    const uint32_t depth = s->max_depth ? s->max_depth : 1u;
+    const uint32_t NT = std::min<uint32_t>(s->cache_train_paths ? s->cache_train_paths : 65536u, n);
+    const size_t u32_bytes = ((size_t)NT * 4 + 255) & ~(size_t)255;
+    const size_t sums_bytes = (((size_t)(NT + 4095) / 4096) * 4 + 255) & ~(size_t)255;
+    return C.side != nullptr && C.tstats.bytes >= 64 && C.tcounts.bytes >= 64 &&
+           C.train_vtx.bytes >= (size_t)NT * depth * kRecFloats * 4 &&
+           C.train_meta.bytes >= 3 * u32_bytes + sums_bytes + (size_t)NT * 16 &&
+           C.t_samples.bytes >= (size_t)NT * depth * 64 && C.tcap >= ((NT + 63) / 64) * 64 && C.cap >= NT;
 }
 
 snkpt_status cache_pass(snkpt_ctx* ctx, const CamDev& cd, const snkpt_settings* s, uint64_t initstate, uint32_t pass,

@@ -567,13 +567,17 @@ snkpt_status render(snkpt_ctx* ctx, const snkpt_camera* cam, const snkpt_setting
     if (cap != hipStreamCaptureStatusNone) {
         // a captured frame may launch kernels only: no allocation, no readback,
         // no host-side state that a replay would not repeat
-        if (s->mode != SNKPT_MODE_PT || !(s->flags & SNKPT_FLAG_NO_STATS))
+        if ((s->mode != SNKPT_MODE_PT && s->mode != SNKPT_MODE_CACHE) || !(s->flags & SNKPT_FLAG_NO_STATS))
             return fail(ctx, SNKPT_ERR_INVALID_ARG,
-                        "snkpt_render: under stream capture only SNKPT_MODE_PT with SNKPT_FLAG_NO_STATS");
+                        "snkpt_render: under stream capture only SNKPT_MODE_PT or SNKPT_MODE_CACHE with "
+                        "SNKPT_FLAG_NO_STATS");
         if (!ctx->render || ctx->render->w != w || ctx->render->h != h || ctx->render->counts.ptr == nullptr ||
             ctx->render->stats.ptr == nullptr)
             return fail(ctx, SNKPT_ERR_INVALID_ARG,
                         "snkpt_render: render one uncaptured frame of this size (the last one) before capturing");
+        if (s->mode == SNKPT_MODE_CACHE && !cache_capture_ready(ctx, s, w, h))
+            return fail(ctx, SNKPT_ERR_INVALID_ARG,
+                        "snkpt_render: render one uncaptured cache-mode frame with these settings before capturing");
     }
     snkpt_status e = begin_frame(ctx, w, h, st);
     if (e != SNKPT_OK) return e;

@@ -181,6 +181,8 @@ snkpt_status whole_paths(snkpt_ctx* ctx, const CamDev& cam, uint32_t w, uint32_t
                          const ShadeHooks* hooks, float4* rad, unsigned long long* stats, hipStream_t st);
 // cache.hip: cache query and training for callers that trace the render paths themselves (the batch)
 snkpt_status cache_ensure(snkpt_ctx* ctx, const snkpt_settings* s, hipStream_t st);
+// a cache-mode frame with these settings would allocate nothing (it may be captured)
+bool cache_capture_ready(snkpt_ctx* ctx, const snkpt_settings* s, uint32_t w, uint32_t h);
 snkpt_status cache_query(snkpt_ctx* ctx, const float* samples, uint32_t n, float4* y, bool averaged,
                          hipStream_t st);
 // Training paths of a context enqueued on `side` (statistics kept per context),

@@ -190,8 +190,9 @@ snkpt_status cache_scratch_init(int device);
 snkpt_status cache_infer_noalloc(snkpt_ctx* ctx, const float* samples, uint32_t n, float* out, hipStream_t st);
 snkpt_status cache_train(snkpt_ctx* ctx, const float* samples, uint32_t n, float lr, int apply, float* loss_out,
                          hipStream_t st);
-snkpt_status cache_train_ema(snkpt_ctx* ctx, const float* samples, uint32_t n, float lr, int apply, float ema,
-                             float* loss_out, hipStream_t st);
+// one optimiser step on min(n, *n_dev) samples (n alone when n_dev is null)
+snkpt_status cache_train_run(snkpt_ctx* ctx, const float* samples, uint32_t n, const uint32_t* n_dev, float lr,
+                             int apply, float ema, float* loss_out, hipStream_t st);
 snkpt_status cache_params(snkpt_ctx* ctx, uint32_t which, void* dst, size_t bytes, size_t* out_bytes, hipStream_t st);
 snkpt_status cache_set_params(snkpt_ctx* ctx, uint32_t which, const void* src, size_t bytes, hipStream_t st);
 

@@ -161,3 +161,85 @@ def test_generation_advances_on_every_state_change(gpu):
     sc, _ = floor_light_scene()
     step("upload_mesh", lambda: ctx.upload_mesh(*sc.arrays()[:2]))
     step("cache_reset", lambda: ctx.cache_reset(1, [-1, 0, -1], [1, 2, 1]))
+
+
+@pytest.mark.parametrize("form", ["lo_q2", "indirect_mis_q1"])
+def test_captured_cache_frame_replays_match_eager_frames(gpu, form):
+    """A cache-mode frame (render, queries, training paths on the side stream,
+    optimiser step) recorded by stream capture: the training batch size and the
+    Adam step counter live on the device, so each replay trains one more step
+    exactly as an uncaptured frame with the same settings does.  Two contexts
+    with the same cache seed run the same frames, one eagerly and one as
+    replays, and give the same bits after every frame."""
+    import torch
+
+    import snkpt
+    _say(f"capture: cache-mode graph replays, form {form}")
+    scene = snkpt.SceneData(1)
+    w, h = 96, 64
+    cam = scene.camera(0, w, h)
+    kw = dict(mode=snkpt.MODE_CACHE, seed=5, cache_train_paths=2048, cache_lr=2.5e-3, cache_ema=0.0,
+              flags=snkpt.FLAG_NO_STATS)
+    if form == "lo_q2":
+        kw["cache_query_depth"] = 2
+    else:
+        kw["cache_query_depth"] = 1
+        kw["flags"] |= snkpt.FLAG_CACHE_INDIRECT | snkpt.FLAG_CACHE_MIS
+    eager, capt = make_ctx(scene), make_ctx(scene)
+    a = torch.zeros(h, w, 4, device="cuda")
+    b = torch.zeros_like(a)
+    for ctx, o in ((eager, a), (capt, b)):
+        ctx.render(cam, _settings(frame=0, **kw), o, w, h)  # uncaptured: sizes the buffers, first step
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    s1 = _settings(frame=1, **kw)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        capt.render(cam, s1, b, w, h)
+    torch.cuda.synchronize()
+    outs = []
+    for k in range(4):
+        eager.render(cam, s1, a, w, h)
+        b.zero_()
+        g.replay()
+        torch.cuda.synchronize()
+        ok = torch.equal(a, b)
+        _say(f"capture: cache replay {k} equal={ok}")
+        assert ok
+        assert torch.isfinite(b).all()
+        outs.append(b.clone())
+    # every replay trained the network: the same frame's image keeps changing
+    assert not torch.equal(outs[0], outs[-1])
+    # both contexts' networks are the same after the same number of steps
+    wa = eager.cache_params(0)
+    wb = capt.cache_params(0)
+    assert np.array_equal(wa, wb)
+
+
+def test_capture_refuses_an_unsized_cache_frame(gpu):
+    import torch
+
+    import snkpt
+    _say("capture: cache-mode refusals")
+    scene = snkpt.SceneData(1)
+    ctx = make_ctx(scene)
+    w, h = 64, 48
+    cam = scene.camera(0, w, h)
+    kw = dict(mode=snkpt.MODE_CACHE, seed=5, cache_train_paths=1024, flags=snkpt.FLAG_NO_STATS)
+    out = torch.zeros(h, w, 4, device="cuda")
+    g = torch.cuda.CUDAGraph()
+    with pytest.raises(snkpt.SnkptError):  # no uncaptured cache frame yet
+        with torch.cuda.graph(g):
+            ctx.render(cam, _settings(max_depth=4, **kw), out, w, h)
+    torch.cuda.synchronize()
+    ctx.render(cam, _settings(max_depth=4, **kw), out, w, h)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with pytest.raises(snkpt.SnkptError):  # deeper paths need larger training buffers
+        with torch.cuda.graph(g):
+            ctx.render(cam, _settings(max_depth=8, **kw), out, w, h)
+    torch.cuda.synchronize()
+    again = torch.zeros_like(out)
+    ctx.render(cam, _settings(max_depth=4, **kw), again, w, h)
+    torch.cuda.synchronize()
+    assert torch.isfinite(again).all()
