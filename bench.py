@@ -436,17 +436,21 @@ def batched_config5(stream, frames=10, warmup=2):
         c.cache_reset(1, stream=stream)
     cache_ms = timed(lambda f: b.render(cache_settings(f), use_graph=False, stream=stream))
     cst = b.frame_stats(stream)
+    cache_graph_ms = timed(lambda f: b.render(cache_settings(f), use_graph=True, stream=stream))
     log(f"bench: config 5 ({pixels} px / frame): eager {eager_ms:.2f} ms, graph {graph_ms:.2f} ms, "
-        f"autotuned graph {tuned_ms:.2f} ms, one view per call {single_ms:.2f} ms, cache form {cache_ms:.2f} ms")
+        f"autotuned graph {tuned_ms:.2f} ms, one view per call {single_ms:.2f} ms, cache form {cache_ms:.2f} ms "
+        f"eager / {cache_graph_ms:.2f} ms graph")
     res = {"workload": "config 5: 16 seeded room scenes, 10k-2M triangles log-uniform, 64 views of 128^2-1024^2 "
                        "log-uniform, plain PT depth 6, RR from bounce 3, 1 spp",
            "triangles_min_max": [min(tris), max(tris)], "pixels_per_frame": pixels, "rays_per_frame": rays,
            "eager_ms": round(eager_ms, 3), "graph_ms": round(graph_ms, 3), "autotuned_graph_ms": round(tuned_ms, 3),
            "one_view_per_call_ms": round(single_ms, 3), "fps": round(1e3 / tuned_ms, 2),
-           "cache_form_ms": round(cache_ms, 3), "cache_form_queries": int(cst.cache_queries),
+           "cache_form_ms": round(cache_ms, 3), "cache_form_graph_ms": round(cache_graph_ms, 3),
+           "cache_form_queries": int(cst.cache_queries),
            "cache_form_train_samples": int(cst.cache_train_samples),
            "cache_form": "NEE + indirect from each scene's own network at the primary hit, one training step per "
-                         "scene per frame (65,536 training paths split by pixels), eager",
+                         "scene per frame (65,536 training paths split by pixels); eager and as a replayed "
+                         "hipGraph (autotuned launch shapes)",
            "mrays_s": round(rays / (tuned_ms * 1e-3) / 1e6, 1), "tuning": tuning.as_dict(),
            "autotune_s": round(tune_s, 2)}
     b.close()

@@ -320,10 +320,13 @@ snkpt_status snkpt_release_semaphore(snkpt_ctx* ctx, snkpt_semaphore* sem);
  * RT::RecordRenderCmdBuf (Raytracing.cpp:227-267): renders one frame of
  * settings->spp samples per pixel into out_rgba (device, w*h RGBA, row-major,
  * row 0 at the top).  With SNKPT_FLAG_ACCUMULATE the output is the running mean
- * over all accumulated frames.  On a stream under capture only SNKPT_MODE_PT
- * frames with SNKPT_FLAG_NO_STATS are accepted, at the width and height of the
- * last uncaptured frame (SNKPT_ERR_INVALID_ARG otherwise); a replay renders the
- * captured camera, seed and frame index again, and is valid until the context's
+ * over all accumulated frames.  On a stream under capture only SNKPT_MODE_PT and
+ * SNKPT_MODE_CACHE frames with SNKPT_FLAG_NO_STATS are accepted, at the width
+ * and height of the last uncaptured frame (a cache-mode frame also needs an
+ * uncaptured cache-mode frame with the same depth and training paths first;
+ * SNKPT_ERR_INVALID_ARG otherwise); a replay renders the captured camera, seed
+ * and frame index again (a cache-mode replay also takes one more optimiser
+ * step, as an uncaptured frame does), and is valid until the context's
  * generation (snkpt_get_generation) changes: a larger or differently shaped
  * frame, any scene-input call, snkpt_build_accel, snkpt_cache_reset or
  * snkpt_destroy invalidates it. */
@@ -507,10 +510,15 @@ snkpt_status snkpt_batch_add_scene(snkpt_batch* batch, snkpt_ctx* scene, uint32_
 snkpt_status snkpt_batch_add_view(snkpt_batch* batch, uint32_t scene, const snkpt_camera* cam, uint32_t w,
                                   uint32_t h, uint32_t seed, void* out_rgba_device, uint32_t out_format,
                                   uint32_t* out_view);
-/* One frame of every view.  Uses mode (PT only), max_depth, rr_start, frame
- * and the ACCUMULATE / RESET flags of `s`.  use_graph: capture the frame on the
- * first call (or after the views, tuning, depth or stream change) and replay
- * it afterwards; needs a created (non-null) stream. */
+/* One frame of every view.  Uses mode (SNKPT_MODE_PT or SNKPT_MODE_CACHE),
+ * max_depth, rr_start, frame, the ACCUMULATE / RESET flags and the cache
+ * fields and flags of `s`.  use_graph: capture the frame on the first call (or
+ * after the views, tuning, depth, cache settings or stream change) and replay
+ * it afterwards (the frame index and flags are read from device memory on every
+ * replay); needs a created (non-null) stream.  In the cache form each scene's
+ * own network answers its views' queries and takes its optimiser steps on
+ * training paths through its first view; a captured cache-form frame runs its
+ * training paths as a second graph on a side stream beside the render. */
 snkpt_status snkpt_batch_render(snkpt_batch* batch, const snkpt_settings* s, int use_graph, snkpt_stream stream);
 /* Times whole frames of this batch over traversal block size, persistent grid
  * sizes and queue capacity, keeps the fastest (images do not depend on it). */

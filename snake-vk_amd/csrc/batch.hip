@@ -220,13 +220,71 @@ __global__ __launch_bounds__(kBBlock) void k_bq_scatter(const float* __restrict_
     }
 }
 
-// rad[path] += throughput * max(cache output, 0) for the n sorted queries.
+__global__ void k_bq_clear(uint32_t* __restrict__ hist, uint32_t n) {
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) hist[k] = 0u;
+}
+
+// Segments of the sorted queries: scene k's hist[k] queries start at off[k]
+// (each segment rounded up to 64 rows, so every wave of the query kernels
+// serves one scene); off[nscenes] = all rows.  One block of kMaxHistScenes threads.
+__global__ __launch_bounds__(kMaxHistScenes) void k_bq_layout(const uint32_t* __restrict__ hist, uint32_t nscenes,
+                                                             uint32_t* __restrict__ off,
+                                                             uint32_t* __restrict__ cursor) {
+    __shared__ uint32_t v[kMaxHistScenes];
+    const uint32_t k = threadIdx.x;
+    const uint32_t mine = k < nscenes ? ((hist[k] + 63u) & ~63u) : 0u;
+    v[k] = mine;
+    __syncthreads();
+    for (uint32_t d = 1; d < kMaxHistScenes; d <<= 1) {  // inclusive scan (Hillis-Steele)
+        const uint32_t add = k >= d ? v[k - d] : 0u;
+        __syncthreads();
+        v[k] += add;
+        __syncthreads();
+    }
+    if (k < nscenes) {
+        off[k] = v[k] - mine;
+        cursor[k] = v[k] - mine;
+    }
+    if (k == nscenes - 1) off[nscenes] = v[k];
+}
+
+// The padding rows of every segment: zero samples (finite inputs for the
+// encoder) whose records name no path.
+__global__ void k_bq_pad(const uint32_t* __restrict__ hist, const uint32_t* __restrict__ off,
+                         float* __restrict__ s_out, float4* __restrict__ m_out) {
+    const uint32_t k = blockIdx.x;
+    const uint32_t row = off[k] + hist[k] + threadIdx.x;
+    if (row >= off[k + 1]) return;
+    float4* dst = (float4*)(s_out + (size_t)row * 16);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) dst[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    m_out[row] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0xFFFFFFFFu));
+}
+
+// Training-path streams of each trained scene for this frame (its first view's
+// seed, the frame index from device memory): what the eager path computes on the host.
+__global__ void k_btrain_seeds(const uint2* __restrict__ cfg, uint32_t n, const FrameD* __restrict__ fr,
+                               TrainSeedD* __restrict__ out) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint64_t is = pass_state(cfg[k].x, fr->frame, 0);
+    TrainSeedD t;
+    t.path_state = is ^ kTrainPathsTag;
+    t.kseed = is ^ kTrainTargetTag;
+    t.start = (uint32_t)((is >> 17) % cfg[k].y);
+    t.pad[0] = t.pad[1] = t.pad[2] = 0u;
+    out[k] = t;
+}
+
+// rad[path] += throughput * max(cache output, 0) for the sorted queries (rows
+// of the segments; padding rows name no path).
 __global__ void k_bq_apply(const float4* __restrict__ meta, const float4* __restrict__ y, uint32_t n,
-                           float4* __restrict__ rad) {
+                           const uint32_t* __restrict__ n_dev, float4* __restrict__ rad) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= min(n, *n_dev)) return;
     const float4 m = meta[i], v = y[i];
     const uint32_t path = __float_as_uint(m.w);
+    if (path == 0xFFFFFFFFu) return;
     const float4 r = rad[path];
     rad[path] = make_float4(r.x + m.x * fmaxf(v.x, 0.0f), r.y + m.y * fmaxf(v.y, 0.0f), r.z + m.z * fmaxf(v.z, 0.0f),
                             r.w);
@@ -313,12 +371,20 @@ struct snkpt_batch {
     DeviceBuffer h_t, h_inst, h_prim, h_u, h_v;
     DeviceBuffer s_org, s_dir, s_contrib;
     DeviceBuffer p_thr, p_rng, p_bounce, path_job, rad, accum, counts, stats;
-    // cache form: queries, their per-scene sorting, outputs
-    DeviceBuffer q_samples, q_meta, s_samples, s_meta, q_y, hist;
+    // cache form: queries, their per-scene sorting (64-row segments), encodings, outputs;
+    // hist = scene counts | segment offsets | scatter cursors; the scenes' networks;
+    // the trained scenes' seed and stride, and their training streams of the frame
+    DeviceBuffer q_samples, q_meta, s_samples, s_meta, q_x0, q_y, hist, refs, train_cfg, train_seed;
     uint32_t qcap = 0;
+    uint64_t cache_key = 0;  // settings and tables the cache form's buffers were prepared for
+    std::vector<uint32_t> train_scenes, train_views, train_paths;
     snkpt::FrameD frame_host{};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // cache form: the training paths (replayed on a side stream beside the main
+    // graph) and the optimiser steps (after the join) as graphs of their own
+    hipGraph_t graph_train = nullptr, graph_steps = nullptr;
+    hipGraphExec_t exec_train = nullptr, exec_steps = nullptr;
     uint64_t graph_key = 0;
     hipStream_t graph_stream = nullptr;
     // cache form: side streams on which the scenes' training paths run during the render
@@ -334,10 +400,14 @@ struct snkpt_batch {
         if (fork_ev) (void)hipEventDestroy(fork_ev);
     }
     void drop_graph() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        exec = nullptr;
-        graph = nullptr;
+        for (hipGraphExec_t* e : {&exec, &exec_train, &exec_steps}) {
+            if (*e) (void)hipGraphExecDestroy(*e);
+            *e = nullptr;
+        }
+        for (hipGraph_t* g : {&graph, &graph_train, &graph_steps}) {
+            if (*g) (void)hipGraphDestroy(*g);
+            *g = nullptr;
+        }
         graph_key = 0;
     }
 };
@@ -539,32 +609,149 @@ __global__ void k_bstat_add(const uint32_t* c, uint32_t i, unsigned long long* s
     if (threadIdx.x == 0) *stat += c[i];
 }
 
-// One frame of every view in the cache form: the batched paths end in a cache
-// query at vertex Q (or take NEE there and query the indirect light); the
-// queries are grouped by scene and each scene's own network answers them; then
-// every scene takes one training step from full paths through its first view
-// (cache_train_paths split over the scenes in proportion to their pixels).
-// Eager only: the grouping and the training read counts back to the host.
-snkpt_status record_cache_frame(snkpt_batch* B, const snkpt_settings* s, hipStream_t st) {
+// Everything a cache-form frame with these settings needs before it is
+// recorded (eagerly, outside any capture): each scene's cache, the query and
+// segment buffers, the scenes' networks as a device table, each trained
+// scene's training buffers and seed/stride record, the side streams.  Redone
+// only when the settings or the tables change.
+uint64_t cache_key(const snkpt_batch* B, const snkpt_settings* s) {
+    uint64_t k = 1469598103934665603ull;
+    auto mix = [&](uint64_t v) { k = (k ^ v) * 1099511628211ull; };
+    mix(s->max_depth);
+    mix(s->cache_train_paths);
+    mix(s->flags & (SNKPT_FLAG_NO_TRAIN | SNKPT_FLAG_CACHE_INDIRECT | SNKPT_FLAG_CACHE_MIS));
+    mix(s->cache_ema > 0.0f ? 1u : 0u);
+    mix(B->cap);
+    for (snkpt_ctx* c : B->scenes) mix(c->generation);
+    return k | 1ull;
+}
+
+snkpt_status prepare_cache_frame(snkpt_batch* B, const snkpt_settings* s, hipStream_t st) {
     const uint32_t nsc = (uint32_t)B->scenes.size();
     if (nsc > kMaxHistScenes) return bfail(B, SNKPT_ERR_CAPACITY, "batch cache form: more than 1024 scenes");
+    const uint64_t key = cache_key(B, s);
+    if (key == B->cache_key) return SNKPT_OK;
+    B->drop_graph();
     for (snkpt_ctx* c : B->scenes) {
         snkpt_status e = cache_ensure(c, s, st);
         if (e != SNKPT_OK) return bfail(B, e, std::string("batch cache form: ") + c->last_error);
     }
-    const uint32_t depth = s->max_depth ? s->max_depth : 1u;
-    const uint32_t Q = s->cache_query_depth ? s->cache_query_depth : 2u;
     const uint32_t cap = B->cap;
+    const size_t rows = (size_t)cap + 64u * nsc;  // sorted queries with each scene's segment padded to 64 rows
     if (cap > B->qcap) {
         BHIP(B->q_samples.reserve((size_t)cap * 64));
         BHIP(B->q_meta.reserve((size_t)cap * 16));
-        BHIP(B->s_samples.reserve((size_t)cap * 64));
-        BHIP(B->s_meta.reserve((size_t)cap * 16));
-        BHIP(B->q_y.reserve((size_t)cap * 16));
         B->qcap = cap;
     }
-    BHIP(B->hist.reserve((size_t)nsc * 8));
+    BHIP(B->s_samples.reserve(rows * 64));
+    BHIP(B->s_meta.reserve(rows * 16));
+    BHIP(B->q_x0.reserve(rows * 128));
+    BHIP(B->q_y.reserve(rows * 16));
+    BHIP(B->hist.reserve(((size_t)3 * nsc + 1) * 4));
+    std::vector<CacheRefD> refs(nsc);
     const bool avg = s->cache_ema > 0.0f;
+    for (uint32_t k = 0; k < nsc; ++k) {
+        snkpt_status e = cache_ref(B->scenes[k], avg, &refs[k]);
+        if (e != SNKPT_OK) return bfail(B, e, "batch cache form: " + B->scenes[k]->last_error);
+    }
+    BHIP(B->refs.reserve((size_t)nsc * sizeof(CacheRefD)));
+    BHIP(hipMemcpyAsync(B->refs.ptr, refs.data(), (size_t)nsc * sizeof(CacheRefD), hipMemcpyHostToDevice, st));
+    // training: full paths through each scene's first view, budget split by the scenes' pixels
+    B->train_scenes.clear();
+    B->train_views.clear();
+    B->train_paths.clear();
+    std::vector<uint2> cfg;
+    if (!(s->flags & SNKPT_FLAG_NO_TRAIN)) {
+        if (!B->fork_ev) {
+            for (int i = 0; i < snkpt_batch::kSide; ++i) {
+                BHIP(hipStreamCreateWithFlags(&B->side[i], hipStreamNonBlocking));
+                BHIP(hipEventCreateWithFlags(&B->join_ev[i], hipEventDisableTiming));
+            }
+            BHIP(hipEventCreateWithFlags(&B->fork_ev, hipEventDisableTiming));
+        }
+        std::vector<uint64_t> px(nsc, 0);
+        std::vector<int64_t> first(nsc, -1);
+        uint64_t total = 0;
+        for (size_t v = 0; v < B->views.size(); ++v) {
+            const auto& V = B->views[v];
+            px[V.scene] += (uint64_t)V.w * V.h;
+            total += (uint64_t)V.w * V.h;
+            if (first[V.scene] < 0) first[V.scene] = (int64_t)v;
+        }
+        const uint64_t budget = s->cache_train_paths ? s->cache_train_paths : 65536u;
+        for (uint32_t k = 0; k < nsc; ++k) {
+            if (first[k] < 0) continue;
+            const auto& V = B->views[(size_t)first[k]];
+            const uint64_t npx = (uint64_t)V.w * V.h;
+            uint64_t nt = std::max<uint64_t>(1024u, budget * px[k] / std::max<uint64_t>(total, 1));
+            nt = std::min<uint64_t>(nt, npx);
+            snkpt_status e = cache_prepare_train(B->scenes[k], s, V.w, V.h, (uint32_t)nt, st);
+            if (e != SNKPT_OK) return bfail(B, e, "batch cache training: " + B->scenes[k]->last_error);
+            B->train_scenes.push_back(k);
+            B->train_views.push_back((uint32_t)first[k]);
+            B->train_paths.push_back((uint32_t)nt);
+            cfg.push_back(make_uint2(V.seed, std::max<uint32_t>((uint32_t)(npx / nt), 1u)));
+        }
+    }
+    if (!cfg.empty()) {
+        BHIP(B->train_cfg.reserve(cfg.size() * sizeof(uint2)));
+        BHIP(B->train_seed.reserve(cfg.size() * sizeof(TrainSeedD)));
+        BHIP(hipMemcpyAsync(B->train_cfg.ptr, cfg.data(), cfg.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
+    }
+    BHIP(hipStreamSynchronize(st));
+    B->cache_key = key;
+    return SNKPT_OK;
+}
+
+bool cache_trains(const snkpt_batch* B, const snkpt_settings* s) {
+    return !(s->flags & SNKPT_FLAG_NO_TRAIN) && !B->train_scenes.empty();
+}
+
+// Cache form, training half: each trained scene's training-path streams for
+// this frame, from the frame index in device memory ...
+void record_train_seeds(snkpt_batch* B, hipStream_t st) {
+    const uint32_t nt = (uint32_t)B->train_scenes.size();
+    k_btrain_seeds<<<(nt + 63) / 64, 64, 0, st>>>(B->train_cfg.as<uint2>(), nt, B->d_frame.as<FrameD>(),
+                                                 B->train_seed.as<TrainSeedD>());
+}
+
+// ... then full paths through each trained scene's first view (they need no
+// network); scene j's go to streams[j % n], which must follow the seeds.
+snkpt_status record_train_paths(snkpt_batch* B, const snkpt_settings* s, hipStream_t* streams, int n) {
+    const uint32_t nt = (uint32_t)B->train_scenes.size();
+    TrainSeedD* tseed = B->train_seed.as<TrainSeedD>();
+    for (uint32_t j = 0; j < nt; ++j) {
+        snkpt_ctx* c = B->scenes[B->train_scenes[j]];
+        const auto& V = B->views[B->train_views[j]];
+        const CamDev cd = make_cam(&V.cam);
+        snkpt_status e = cache_train_paths_forked(c, cd, V.w, V.h, s, pass_state(V.seed, s->frame, 0),
+                                                  B->train_paths[j], streams[j % (uint32_t)n], tseed + j);
+        if (e != SNKPT_OK) return bfail(B, e, std::string("batch cache training: ") + c->last_error);
+    }
+    return SNKPT_OK;
+}
+
+// Cache form, optimiser steps of every trained scene (after its training paths).
+snkpt_status record_cache_steps(snkpt_batch* B, const snkpt_settings* s, hipStream_t st) {
+    std::vector<snkpt_ctx*> trained;
+    for (uint32_t k : B->train_scenes) trained.push_back(B->scenes[k]);
+    snkpt_status e = cache_train_steps(trained.data(), (uint32_t)trained.size(), s,
+                                       B->stats.as<unsigned long long>(), st);
+    if (e != SNKPT_OK) return bfail(B, e, "batch cache training step failed");
+    hipError_t le = hipGetLastError();
+    if (le != hipSuccess) return bfail(B, SNKPT_ERR_HIP, std::string("batch launch: ") + hipGetErrorString(le));
+    return SNKPT_OK;
+}
+
+// Cache form, render half: the batched paths end in a cache query at vertex Q
+// (or take NEE there and query the indirect light); the queries are grouped by
+// scene on the device (64-row segments) and each scene's own network answers
+// its segment.
+snkpt_status record_cache_render(snkpt_batch* B, const snkpt_settings* s, hipStream_t st) {
+    const uint32_t nsc = (uint32_t)B->scenes.size();
+    const uint32_t depth = s->max_depth ? s->max_depth : 1u;
+    const uint32_t Q = s->cache_query_depth ? s->cache_query_depth : 2u;
+    const uint32_t rows = B->cap + 64u * nsc;
     // with depth >= Q the vertex before the query keeps its continuation; with depth < Q this is plain PT.
     // The MIS probe of an indirect query vertex is one more bounce.
     const bool mis = (s->flags & SNKPT_FLAG_CACHE_INDIRECT) && (s->flags & SNKPT_FLAG_CACHE_MIS);
@@ -588,45 +775,11 @@ snkpt_status record_cache_frame(snkpt_batch* B, const snkpt_settings* s, hipStre
     const uint32_t tgrid = (uint32_t)B->cus * std::max(1u, B->tuning.trace_blocks_per_cu);
     const uint32_t sgrid = (uint32_t)B->cus * std::max(1u, B->tuning.shade_blocks_per_cu);
     const uint32_t ggrid = (uint32_t)B->cus * 8u;
+    uint32_t* d_hist = B->hist.as<uint32_t>();
+    uint32_t* d_off = d_hist + nsc;  // nsc + 1 entries
+    uint32_t* d_cursor = d_off + nsc + 1;
+    const CacheRefD* refs = B->refs.as<CacheRefD>();
     k_bzero<<<1, 64, 0, st>>>(stats, 8);
-    // training: full paths through each scene's first view, traced on side
-    // streams (round robin) while the render below runs; they need no network
-    const bool train = !(s->flags & SNKPT_FLAG_NO_TRAIN);
-    std::vector<snkpt_ctx*> trained;
-    if (train) {
-        if (!B->fork_ev) {
-            for (int i = 0; i < snkpt_batch::kSide; ++i) {
-                BHIP(hipStreamCreateWithFlags(&B->side[i], hipStreamNonBlocking));
-                BHIP(hipEventCreateWithFlags(&B->join_ev[i], hipEventDisableTiming));
-            }
-            BHIP(hipEventCreateWithFlags(&B->fork_ev, hipEventDisableTiming));
-        }
-        BHIP(hipEventRecord(B->fork_ev, st));
-        for (int i = 0; i < snkpt_batch::kSide; ++i) BHIP(hipStreamWaitEvent(B->side[i], B->fork_ev, 0));
-        std::vector<uint64_t> px(nsc, 0);
-        std::vector<int64_t> first(nsc, -1);
-        uint64_t total = 0;
-        for (size_t v = 0; v < B->views.size(); ++v) {
-            const auto& V = B->views[v];
-            px[V.scene] += (uint64_t)V.w * V.h;
-            total += (uint64_t)V.w * V.h;
-            if (first[V.scene] < 0) first[V.scene] = (int64_t)v;
-        }
-        const uint64_t budget = s->cache_train_paths ? s->cache_train_paths : 65536u;
-        for (uint32_t k = 0; k < nsc; ++k) {
-            if (first[k] < 0) continue;
-            const auto& V = B->views[(size_t)first[k]];
-            uint64_t nt = std::max<uint64_t>(1024u, budget * px[k] / std::max<uint64_t>(total, 1));
-            nt = std::min<uint64_t>(nt, (uint64_t)V.w * V.h);
-            const CamDev cd = make_cam(&V.cam);
-            snkpt_status e = cache_train_paths_forked(B->scenes[k], cd, V.w, V.h, s, pass_state(V.seed, s->frame, 0),
-                                                      (uint32_t)nt, B->side[trained.size() % snkpt_batch::kSide]);
-            if (e != SNKPT_OK) return bfail(B, e, std::string("batch cache training: ") + B->scenes[k]->last_error);
-            trained.push_back(B->scenes[k]);
-        }
-        for (int i = 0; i < snkpt_batch::kSide; ++i) BHIP(hipEventRecord(B->join_ev[i], B->side[i]));
-    }
-    std::vector<uint32_t> hist(nsc), off(nsc);
     for (const auto& g : B->groups) {
         k_bgenerate<<<ggrid, kBBlock, 0, st>>>(B->d_jobs.as<JobD>(), g.j0, g.j1, g.total, B->d_frame.as<FrameD>(),
                                                B->q_org[0].as<float4>(), B->q_dir[0].as<float4>(),
@@ -653,51 +806,67 @@ snkpt_status record_cache_frame(snkpt_batch* B, const snkpt_settings* s, hipStre
         }
         k_bbounce_end<<<1, 64, 0, st>>>(counts, stats);
         k_bstat_add<<<1, 64, 0, st>>>(counts, 3, stats + 3);
-        // group the queries by scene and query each scene's network
-        uint32_t nq = 0;
-        BHIP(hipMemcpyAsync(&nq, counts + 3, 4, hipMemcpyDeviceToHost, st));
-        BHIP(hipStreamSynchronize(st));
-        if (nq > 0) {
-            uint32_t* d_hist = B->hist.as<uint32_t>();
-            uint32_t* d_cursor = d_hist + nsc;
-            std::fill(hist.begin(), hist.end(), 0u);
-            BHIP(hipMemcpyAsync(d_hist, hist.data(), (size_t)nsc * 4, hipMemcpyHostToDevice, st));
-            k_bq_hist<<<grid_for(nq, kBBlock), kBBlock, 0, st>>>(B->q_meta.as<float4>(), counts + 3,
-                                                                 B->d_jobs.as<JobD>(), B->path_job.as<uint32_t>(),
-                                                                 nsc, d_hist);
-            BHIP(hipMemcpyAsync(hist.data(), d_hist, (size_t)nsc * 4, hipMemcpyDeviceToHost, st));
-            BHIP(hipStreamSynchronize(st));
-            uint32_t run = 0;
-            for (uint32_t k = 0; k < nsc; ++k) {
-                off[k] = run;
-                run += hist[k];
-            }
-            BHIP(hipMemcpyAsync(d_cursor, off.data(), (size_t)nsc * 4, hipMemcpyHostToDevice, st));
-            k_bq_scatter<<<grid_for(nq, kBBlock), kBBlock, 0, st>>>(B->q_samples.as<float>(), B->q_meta.as<float4>(),
-                                                                    counts + 3, B->d_jobs.as<JobD>(),
-                                                                    B->path_job.as<uint32_t>(), nsc, d_cursor,
-                                                                    B->s_samples.as<float>(), B->s_meta.as<float4>());
-            for (uint32_t k = 0; k < nsc; ++k) {
-                if (!hist[k]) continue;
-                snkpt_status e = cache_query(B->scenes[k], B->s_samples.as<float>() + (size_t)off[k] * 16, hist[k],
-                                             B->q_y.as<float4>() + off[k], avg, st);
-                if (e != SNKPT_OK) return bfail(B, e, std::string("batch cache query: ") + B->scenes[k]->last_error);
-            }
-            k_bq_apply<<<grid_for(nq, 256), 256, 0, st>>>(B->s_meta.as<float4>(), B->q_y.as<float4>(), nq,
-                                                          B->rad.as<float4>());
-        }
+        // group the queries by scene and query each scene's network, all on the device
+        k_bq_clear<<<1, 256, 0, st>>>(d_hist, nsc);
+        k_bq_hist<<<grid_for(g.total, kBBlock), kBBlock, 0, st>>>(B->q_meta.as<float4>(), counts + 3,
+                                                                  B->d_jobs.as<JobD>(), B->path_job.as<uint32_t>(),
+                                                                  nsc, d_hist);
+        k_bq_layout<<<1, kMaxHistScenes, 0, st>>>(d_hist, nsc, d_off, d_cursor);
+        k_bq_pad<<<nsc, 64, 0, st>>>(d_hist, d_off, B->s_samples.as<float>(), B->s_meta.as<float4>());
+        k_bq_scatter<<<grid_for(g.total, kBBlock), kBBlock, 0, st>>>(B->q_samples.as<float>(), B->q_meta.as<float4>(),
+                                                                     counts + 3, B->d_jobs.as<JobD>(),
+                                                                     B->path_job.as<uint32_t>(), nsc, d_cursor,
+                                                                     B->s_samples.as<float>(), B->s_meta.as<float4>());
+        cache_query_many(refs, d_off, nsc, B->s_samples.as<float>(), rows, B->q_x0.as<_Float16>(),
+                         B->q_y.as<float4>(), st);
+        k_bq_apply<<<grid_for(rows, 256), 256, 0, st>>>(B->s_meta.as<float4>(), B->q_y.as<float4>(), rows,
+                                                        d_off + nsc, B->rad.as<float4>());
         k_bresolve<<<ggrid, kBBlock, 0, st>>>(B->d_jobs.as<JobD>(), B->path_job.as<uint32_t>(),
                                               B->rad.as<float4>(), B->accum.as<float4>(), g.total,
                                               B->d_frame.as<FrameD>());
     }
-    if (train) {
-        // join the training paths, then one step per scene in scene order
-        for (int i = 0; i < snkpt_batch::kSide; ++i) BHIP(hipStreamWaitEvent(st, B->join_ev[i], 0));
-        snkpt_status e = cache_train_steps(trained.data(), (uint32_t)trained.size(), s, stats, st);
-        if (e != SNKPT_OK) return bfail(B, e, "batch cache training step failed");
-    }
     hipError_t le = hipGetLastError();
     if (le != hipSuccess) return bfail(B, SNKPT_ERR_HIP, std::string("batch launch: ") + hipGetErrorString(le));
+    return SNKPT_OK;
+}
+
+// One frame of every view in the cache form, eagerly: the training paths go
+// round robin onto the side streams while the render half runs on `st`; the
+// optimiser steps follow the join.  Launches only (prepare_cache_frame
+// allocated everything; every count stays on the device).
+snkpt_status record_cache_frame(snkpt_batch* B, const snkpt_settings* s, hipStream_t st) {
+    const bool train = cache_trains(B, s);
+    snkpt_status e;
+    if (train) {
+        // the seeds go on `st` before the fork, so every side stream sees them
+        record_train_seeds(B, st);
+        BHIP(hipEventRecord(B->fork_ev, st));
+        for (int i = 0; i < snkpt_batch::kSide; ++i) BHIP(hipStreamWaitEvent(B->side[i], B->fork_ev, 0));
+        if ((e = record_train_paths(B, s, B->side, snkpt_batch::kSide)) != SNKPT_OK) return e;
+        for (int i = 0; i < snkpt_batch::kSide; ++i) BHIP(hipEventRecord(B->join_ev[i], B->side[i]));
+    }
+    if ((e = record_cache_render(B, s, st)) != SNKPT_OK) return e;
+    if (train) {
+        for (int i = 0; i < snkpt_batch::kSide; ++i) BHIP(hipStreamWaitEvent(st, B->join_ev[i], 0));
+        return record_cache_steps(B, s, st);
+    }
+    return SNKPT_OK;
+}
+
+// Records what `rec` enqueues on `st` into a graph and instantiates it.
+template <class Rec>
+snkpt_status capture(snkpt_batch* B, hipStream_t st, hipGraph_t* graph, hipGraphExec_t* exec, Rec rec) {
+    BHIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    snkpt_status re = rec();
+    hipGraph_t g = nullptr;
+    hipError_t ce = hipStreamEndCapture(st, &g);
+    if (re != SNKPT_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return re;
+    }
+    if (ce != hipSuccess) return bfail(B, SNKPT_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+    *graph = g;
+    BHIP(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
     return SNKPT_OK;
 }
 
@@ -713,6 +882,19 @@ uint64_t graph_key(const snkpt_batch* B, const snkpt_settings* s, hipStream_t st
     mix((uint64_t)(uintptr_t)st);
     mix(B->views.size());
     mix(B->scenes.size());
+    mix(s->mode);
+    if (s->mode == SNKPT_MODE_CACHE) {  // values the cache form's launches take as arguments
+        mix(s->flags & (SNKPT_FLAG_NO_TRAIN | SNKPT_FLAG_CACHE_INDIRECT | SNKPT_FLAG_CACHE_MIS));
+        mix(s->cache_query_depth);
+        mix(s->cache_train_paths);
+        mix(s->cache_train_steps);
+        uint32_t bits;
+        std::memcpy(&bits, &s->cache_lr, 4);
+        mix(bits);
+        std::memcpy(&bits, &s->cache_ema, 4);
+        mix(bits);
+        mix(B->cache_key);
+    }
     return k | 1ull;
 }
 
@@ -720,8 +902,6 @@ snkpt_status batch_render(snkpt_batch* B, const snkpt_settings* s, int use_graph
     if (!s) return bfail(B, SNKPT_ERR_INVALID_ARG, "batch_render: settings is NULL");
     if (s->mode != SNKPT_MODE_PT && s->mode != SNKPT_MODE_CACHE)
         return bfail(B, SNKPT_ERR_INVALID_ARG, "batch_render: SNKPT_MODE_PT or SNKPT_MODE_CACHE");
-    if (s->mode == SNKPT_MODE_CACHE && use_graph)
-        return bfail(B, SNKPT_ERR_INVALID_ARG, "batch_render: the cache form runs eagerly (use_graph = 0)");
     if (s->spp > 1) return bfail(B, SNKPT_ERR_INVALID_ARG, "batch_render: one sample per pixel per frame");
     if (use_graph && st == nullptr)
         return bfail(B, SNKPT_ERR_INVALID_ARG, "batch_render: graph capture needs a created (non-null) stream");
@@ -738,25 +918,44 @@ snkpt_status batch_render(snkpt_batch* B, const snkpt_settings* s, int use_graph
     B->frame_host.accumulate = (s->flags & SNKPT_FLAG_ACCUMULATE) ? 1u : 0u;
     B->frame_host.reset = (s->flags & SNKPT_FLAG_RESET) ? 1u : 0u;
     BHIP(hipMemcpyAsync(B->d_frame.ptr, &B->frame_host, sizeof(FrameD), hipMemcpyHostToDevice, st));
-    if (s->mode == SNKPT_MODE_CACHE) return record_cache_frame(B, s, st);
-    if (!use_graph) return record_frame(B, s, st);
+    const bool cache = s->mode == SNKPT_MODE_CACHE;
+    if (cache && (e = prepare_cache_frame(B, s, st)) != SNKPT_OK) return e;
+    if (!use_graph) return cache ? record_cache_frame(B, s, st) : record_frame(B, s, st);
+    const bool train = cache && cache_trains(B, s);
     const uint64_t key = graph_key(B, s, st);
     if (!B->exec || key != B->graph_key) {
         B->drop_graph();
-        BHIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        snkpt_status re = record_frame(B, s, st);
-        hipGraph_t g = nullptr;
-        hipError_t ce = hipStreamEndCapture(st, &g);
-        if (re != SNKPT_OK) {
-            if (g) (void)hipGraphDestroy(g);
-            return re;
+        // HIP replays a graph's branches one after another, so the training paths,
+        // which overlap the render when run eagerly, get a graph of their own that
+        // is replayed on a side stream beside the main one
+        if (train) {
+            if ((e = capture(B, B->side[0], &B->graph_train, &B->exec_train,
+                             [&] {
+                                 record_train_seeds(B, B->side[0]);
+                                 return record_train_paths(B, s, B->side, 1);
+                             })) != SNKPT_OK)
+                return e;
+            if ((e = capture(B, st, &B->graph_steps, &B->exec_steps,
+                             [&] { return record_cache_steps(B, s, st); })) != SNKPT_OK)
+                return e;
         }
-        if (ce != hipSuccess) return bfail(B, SNKPT_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
-        B->graph = g;
-        BHIP(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+        if ((e = capture(B, st, &B->graph, &B->exec, [&] {
+                 return cache ? record_cache_render(B, s, st) : record_frame(B, s, st);
+             })) != SNKPT_OK)
+            return e;
         B->graph_key = key;
     }
+    if (train) {
+        BHIP(hipEventRecord(B->fork_ev, st));
+        BHIP(hipStreamWaitEvent(B->side[0], B->fork_ev, 0));
+        BHIP(hipGraphLaunch(B->exec_train, B->side[0]));
+        BHIP(hipEventRecord(B->join_ev[0], B->side[0]));
+    }
     BHIP(hipGraphLaunch(B->exec, st));
+    if (train) {
+        BHIP(hipStreamWaitEvent(st, B->join_ev[0], 0));
+        BHIP(hipGraphLaunch(B->exec_steps, st));
+    }
     return SNKPT_OK;
 }
 

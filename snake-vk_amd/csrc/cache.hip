@@ -132,13 +132,12 @@ __device__ __forceinline__ float extra_feature(const float* s, int k) {
 // consecutive samples and writes input features 32 + 2l, 33 + 2l as well, so
 // the 16 waves of a block fill the 128 B input rows of their 64 samples.
 constexpr int kEncBlock = 64 * kGridLevels;
-__global__ __launch_bounds__(kEncBlock) void k_encode(const float* __restrict__ samples, uint32_t n,
-                                                     const uint32_t* __restrict__ n_dev, GridGeom G,
-                                                     const __half2* __restrict__ grid, _Float16* __restrict__ x0) {
+// Rows [64 blk, 64 blk + 64) of x0, for blocks of kEncBlock threads.
+__device__ __forceinline__ void encode_block(const float* __restrict__ samples, uint32_t n, const GridGeom& G,
+                                             const __half2* __restrict__ grid, _Float16* __restrict__ x0) {
     // the 64 input rows of the block are assembled in LDS and written out as
     // whole 128 B rows (4 B stores at a 128 B stride would be 64 partial lines)
     __shared__ half2v_t tile[64][33];
-    if (n_dev) n = *n_dev;
     const int l = threadIdx.x >> 6, r = threadIdx.x & 63;
     const uint32_t i = blockIdx.x * 64 + r;
     if (i < n) {
@@ -163,7 +162,41 @@ __global__ __launch_bounds__(kEncBlock) void k_encode(const float* __restrict__ 
         if (gi < n) *(half2v_t*)(x0 + (size_t)64 * gi + 2 * col) = tile[row][col];
     }
 }
+
+__global__ __launch_bounds__(kEncBlock) void k_encode(const float* __restrict__ samples, uint32_t n,
+                                                     const uint32_t* __restrict__ n_dev, GridGeom G,
+                                                     const __half2* __restrict__ grid, _Float16* __restrict__ x0) {
+    if (n_dev) n = min(n, *n_dev);
+    if (blockIdx.x * 64 >= n) return;  // the whole block is past the end
+    encode_block(samples, n, G, grid, x0);
+}
 inline uint32_t enc_blocks(uint32_t n) { return n == 0 ? 1u : (n + 63) / 64; }
+
+// Network of the 64-row-aligned segment holding `row`: segments [off[k], off[k+1])
+// of the rows sorted by scene, each a multiple of 64 rows.
+__device__ __forceinline__ uint32_t segment_of(const uint32_t* __restrict__ off, uint32_t nseg, uint32_t row) {
+    uint32_t lo = 0, hi = nseg;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= row)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Encoding of queries grouped by network: rows [off[k], off[k+1]) use network
+// k's grid; off[nseg] is the number of rows (a multiple of 64, at most n).
+__global__ __launch_bounds__(kEncBlock) void k_encode_many(const float* __restrict__ samples, uint32_t n,
+                                                          const uint32_t* __restrict__ off, uint32_t nseg,
+                                                          const CacheRefD* __restrict__ refs,
+                                                          _Float16* __restrict__ x0) {
+    n = min(n, off[nseg]);
+    if (blockIdx.x * 64 >= n) return;
+    const CacheRefD& R = refs[segment_of(off, nseg, blockIdx.x * 64)];
+    encode_block(samples, n, R.geom, R.grid, x0);
+}
 
 // ------------------------------------------------------------ MLP core ----
 // act: [64][kRow] halves in LDS.  W: [out][64] row-major fp16 in global memory.
@@ -242,16 +275,11 @@ __device__ __forceinline__ void layer_hidden(const _Float16* in, const _Float16*
         }
 }
 
-// Inference: y[q] = MLP(x0[q]) for 64 samples per wave.
-__global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const _Float16* __restrict__ x0, uint32_t n,
-                                                                const uint32_t* __restrict__ n_dev,
-                                                                const _Float16* __restrict__ W,
-                                                                float4* __restrict__ y) {
+// 64 rows per wave from `base` through the network W (rows at or past n read as zero).
+__device__ __forceinline__ void infer_block(const _Float16* __restrict__ x0, uint32_t n, uint32_t base,
+                                            const _Float16* __restrict__ W, float4* __restrict__ y) {
     __shared__ __attribute__((aligned(16))) _Float16 buf[kInferWaves][64 * kRow];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (n_dev) n = *n_dev;
-    if (blockIdx.x * kInferWaves * 64 >= n) return;  // the whole block is past the end
-    const uint32_t base = (blockIdx.x * kInferWaves + wv) * 64;  // waves past n run on zero rows
     _Float16* a0 = buf[wv];
     load_x0_tile(x0, base, n, a0, lane);
     __syncthreads();
@@ -278,6 +306,31 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const _Float16* 
             if (q < n) y[q] = make_float4(acc[m][0], acc[m][1], acc[m][2], 0.0f);
         }
     }
+}
+
+// Inference: y[q] = MLP(x0[q]) for 64 samples per wave.
+__global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(const _Float16* __restrict__ x0, uint32_t n,
+                                                                const uint32_t* __restrict__ n_dev,
+                                                                const _Float16* __restrict__ W,
+                                                                float4* __restrict__ y) {
+    if (n_dev) n = min(n, *n_dev);
+    if (blockIdx.x * kInferWaves * 64 >= n) return;  // the whole block is past the end
+    // waves past n run on zero rows
+    infer_block(x0, n, (blockIdx.x * kInferWaves + (threadIdx.x >> 6)) * 64, W, y);
+}
+
+// Inference of queries grouped by network (segments as for k_encode_many): each
+// wave's 64 rows lie in one segment and go through that network.
+__global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer_many(const _Float16* __restrict__ x0, uint32_t n,
+                                                                     const uint32_t* __restrict__ off, uint32_t nseg,
+                                                                     const CacheRefD* __restrict__ refs,
+                                                                     float4* __restrict__ y) {
+    n = min(n, off[nseg]);
+    if (blockIdx.x * kInferWaves * 64 >= n) return;
+    const uint32_t base = (blockIdx.x * kInferWaves + (threadIdx.x >> 6)) * 64;
+    // a wave past the rows runs on zero rows through network 0 and stores nothing
+    const uint32_t k = base < n ? segment_of(off, nseg, base) : 0u;
+    infer_block(x0, n, base, refs[k].mlp, y);
 }
 
 // dX tile (64 samples x 64 inputs) = dZ (64 x K) * W (K x 64), via W^T rows.
@@ -1096,10 +1149,12 @@ __global__ void k_train_total(const uint32_t* __restrict__ nrec, uint32_t n, uin
 }
 
 __global__ void k_train_count(const float* __restrict__ rec, const uint32_t* __restrict__ nrec, uint32_t max_rec,
-                              const float4* __restrict__ rad, uint32_t n, uint64_t seed, uint32_t indirect,
+                              const float4* __restrict__ rad, uint32_t n, uint64_t seed,
+                              const TrainSeedD* __restrict__ tseed, uint32_t indirect,
                               const uint32_t* __restrict__ cnt, uint32_t budget, uint32_t* __restrict__ count) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
+    if (tseed) seed = tseed->kseed;
     const uint64_t thr = keep_threshold(cnt, budget);
     const uint32_t k = nrec[j];
     const float4 L = rad[j];
@@ -1111,12 +1166,14 @@ __global__ void k_train_count(const float* __restrict__ rec, const uint32_t* __r
 }
 
 __global__ void k_train_emit(const float* __restrict__ rec, const uint32_t* __restrict__ nrec, uint32_t max_rec,
-                             const float4* __restrict__ rad, uint32_t n, uint64_t seed, uint32_t indirect,
+                             const float4* __restrict__ rad, uint32_t n, uint64_t seed,
+                             const TrainSeedD* __restrict__ tseed, uint32_t indirect,
                              const uint32_t* __restrict__ cnt, uint32_t budget, const uint32_t* __restrict__ off,
                              float* __restrict__ out, unsigned long long* __restrict__ stat) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j == 0) atomicAdd(stat, (unsigned long long)min(cnt[1], budget));
     if (j >= n) return;
+    if (tseed) seed = tseed->kseed;
     const uint64_t thr = keep_threshold(cnt, budget);
     const uint32_t k = nrec[j];
     const float4 L = rad[j];
@@ -1130,8 +1187,6 @@ __global__ void k_train_emit(const float* __restrict__ rec, const uint32_t* __re
         ++slot;
     }
 }
-
-constexpr uint64_t kTrainPaths = 0x545241494eull;
 
 }  // namespace
 
@@ -1167,27 +1222,68 @@ __global__ void k_clear_u64(unsigned long long* p, uint32_t n) {
 // subset of a w x h view of camera cd record their vertices; the thinned
 // vertices become one optimisation step (Adam, weight averages of decay cache_ema).
 
+// Buffers of a frame's training half for NT paths of `depth` vertices.
+// train_meta: nrec | per-path sample counts | offsets (u32 each) | scan sums | radiance (float4)
+static size_t train_u32_bytes(uint32_t NT) { return ((size_t)NT * 4 + 255) & ~(size_t)255; }
+static size_t train_sums_bytes(uint32_t NT) { return (((size_t)(NT + 4095) / 4096) * 4 + 255) & ~(size_t)255; }
+static snkpt_status train_reserve(snkpt_ctx* ctx, CacheState& C, uint32_t NT, uint32_t depth) {
+    HIPC(C.tcounts.reserve(64));
+    HIPC(C.tstats.reserve(64));
+    if (NT == 0) return SNKPT_OK;
+    HIPC(C.train_vtx.reserve((size_t)NT * depth * kRecFloats * 4));
+    HIPC(C.train_meta.reserve(3 * train_u32_bytes(NT) + train_sums_bytes(NT) + (size_t)NT * 16));
+    HIPC(C.t_samples.reserve((size_t)NT * depth * 64));
+    return SNKPT_OK;
+}
+
+snkpt_status cache_prepare_train(snkpt_ctx* ctx, const snkpt_settings* s, uint32_t w, uint32_t h, uint32_t n_paths,
+                                 hipStream_t st) {
+    snkpt_status e = cache_ensure(ctx, s, st);
+    if (e != SNKPT_OK) return e;
+    CacheState& C = *ctx->cache;
+    const uint32_t NT = std::min<uint32_t>(n_paths, w * h);
+    if ((e = train_reserve(ctx, C, NT, s->max_depth ? s->max_depth : 1u)) != SNKPT_OK) return e;
+    return NT ? ensure_work(ctx, C, NT, true) : SNKPT_OK;
+}
+
+snkpt_status cache_ref(snkpt_ctx* ctx, bool averaged, CacheRefD* out) {
+    CacheState* Cp = ctx->cache;
+    if (!Cp || !Cp->ready) return fail(ctx, SNKPT_ERR_NOT_BUILT, "cache_ref: no cache");
+    out->geom = Cp->geom;
+    out->grid = (averaged ? Cp->grid_e16 : Cp->grid16).as<__half2>();
+    out->mlp = (averaged ? Cp->mlp_e16 : Cp->mlp16).as<_Float16>();
+    return SNKPT_OK;
+}
+
+void cache_query_many(const CacheRefD* refs, const uint32_t* off, uint32_t nseg, const float* samples,
+                      uint32_t rows_max, _Float16* x0, float4* y, hipStream_t st) {
+    k_encode_many<<<enc_blocks(rows_max), kEncBlock, 0, st>>>(samples, rows_max, off, nseg, refs, x0);
+    const uint32_t waves = (rows_max + 63) / 64;
+    k_mlp_infer_many<<<(waves + kInferWaves - 1) / kInferWaves, 64 * kInferWaves, 0, st>>>(x0, rows_max, off, nseg,
+                                                                                         refs, y);
+}
+
 // Training paths of a frame and their samples (no network involved, no host
 // synchronisation): full-length unbiased paths through a strided pixel subset,
 // recorded vertices thinned and emitted in path order into t_samples; the
-// sample count lands in tcounts[1].
+// sample count lands in tcounts[1].  With `tseed` the paths' random streams and
+// first pixel are read from the device (initstate only sets the stride's phase
+// otherwise).
 snkpt_status cache_train_paths(snkpt_ctx* ctx, const CamDev& cd, uint32_t w, uint32_t h, const snkpt_settings* s,
-                               uint64_t initstate, uint32_t n_paths, unsigned long long* stats, hipStream_t st) {
+                               uint64_t initstate, uint32_t n_paths, unsigned long long* stats, hipStream_t st,
+                               const TrainSeedD* tseed) {
     CacheState& C = *ctx->cache;
     const uint32_t n = w * h;
     const uint32_t depth = s->max_depth ? s->max_depth : 1u;
     const uint32_t NT = std::min<uint32_t>(n_paths, n);
-    HIPC(C.tcounts.reserve(64));
+    snkpt_status re = train_reserve(ctx, C, NT, depth);
+    if (re != SNKPT_OK) return re;
     uint32_t* qc = C.tcounts.as<uint32_t>();
     k_train_clear<<<1, 64, 0, st>>>(qc);
     if (NT == 0) return SNKPT_OK;
-    HIPC(C.train_vtx.reserve((size_t)NT * depth * kRecFloats * 4));
-    // train_meta: nrec | per-path sample counts | offsets (u32 each) | scan sums | radiance (float4)
-    const size_t u32_bytes = ((size_t)NT * 4 + 255) & ~(size_t)255;
-    const size_t sums_bytes = (((size_t)(NT + 4095) / 4096) * 4 + 255) & ~(size_t)255;
+    const size_t u32_bytes = train_u32_bytes(NT);
+    const size_t sums_bytes = train_sums_bytes(NT);
     const size_t trad_off = 3 * u32_bytes + sums_bytes;
-    HIPC(C.train_meta.reserve(trad_off + (size_t)NT * 16));
-    HIPC(C.t_samples.reserve((size_t)NT * depth * 64));
     const uint32_t indirect = (s->flags & SNKPT_FLAG_CACHE_INDIRECT) ? 1u : 0u;
     uint32_t* nrec = C.train_meta.as<uint32_t>();
     float4* trad = (float4*)(C.train_meta.as<uint8_t>() + trad_off);
@@ -1200,21 +1296,22 @@ snkpt_status cache_train_paths(snkpt_ctx* ctx, const CamDev& cd, uint32_t w, uin
     T.nrec = nrec;
     T.max_rec = depth;
     T.rad = trad;
+    T.tseed = tseed;
     // the training paths are few (latency-bound per bounce as a wavefront): one thread runs each whole path
-    snkpt_status e = whole_paths(ctx, cd, w, h, s, initstate ^ kTrainPaths, NT, start, stride, &T, trad, stats, st);
+    snkpt_status e = whole_paths(ctx, cd, w, h, s, initstate ^ kTrainPathsTag, NT, start, stride, &T, trad, stats, st);
     if (e != SNKPT_OK) return e;
     uint8_t* tm = C.train_meta.as<uint8_t>();
     uint32_t* pcount = (uint32_t*)(tm + u32_bytes);
     uint32_t* off = (uint32_t*)(tm + 2 * u32_bytes);
     uint32_t* sums = (uint32_t*)(tm + 3 * u32_bytes);
-    const uint64_t kseed = initstate ^ 0x7461726765ull;
+    const uint64_t kseed = initstate ^ kTrainTargetTag;
     const uint32_t budget = NT;  // training samples per frame
     k_train_total<<<grid_for(NT, 256), 256, 0, st>>>(nrec, NT, qc);
-    k_train_count<<<grid_for(NT, 256), 256, 0, st>>>(C.train_vtx.as<float>(), nrec, depth, trad, NT, kseed,
+    k_train_count<<<grid_for(NT, 256), 256, 0, st>>>(C.train_vtx.as<float>(), nrec, depth, trad, NT, kseed, tseed,
                                                       indirect, qc, budget, pcount);
     scan_exclusive(pcount, off, NT, sums, qc + 1, st);
-    k_train_emit<<<grid_for(NT, 256), 256, 0, st>>>(C.train_vtx.as<float>(), nrec, depth, trad, NT, kseed, indirect,
-                                                     qc, budget, off, C.t_samples.as<float>(), stats + 4);
+    k_train_emit<<<grid_for(NT, 256), 256, 0, st>>>(C.train_vtx.as<float>(), nrec, depth, trad, NT, kseed, tseed,
+                                                     indirect, qc, budget, off, C.t_samples.as<float>(), stats + 4);
     C.t_max = budget;
     return SNKPT_OK;
 }
@@ -1257,11 +1354,11 @@ snkpt_status cache_train_steps(snkpt_ctx* const* ctxs, uint32_t n, const snkpt_s
 // their statistics go to the context's own tstats (cleared first).
 snkpt_status cache_train_paths_forked(snkpt_ctx* ctx, const CamDev& cd, uint32_t w, uint32_t h,
                                       const snkpt_settings* s, uint64_t initstate, uint32_t n_paths,
-                                      hipStream_t side) {
+                                      hipStream_t side, const TrainSeedD* tseed) {
     CacheState& C = *ctx->cache;
     HIPC(C.tstats.reserve(64));
     k_clear_u64<<<1, 64, 0, side>>>(C.tstats.as<unsigned long long>(), 8);
-    return cache_train_paths(ctx, cd, w, h, s, initstate, n_paths, C.tstats.as<unsigned long long>(), side);
+    return cache_train_paths(ctx, cd, w, h, s, initstate, n_paths, C.tstats.as<unsigned long long>(), side, tseed);
 }
 
 // True when a cache-mode frame of w x h with these settings would allocate
@@ -1320,7 +1417,7 @@ snkpt_status cache_pass(snkpt_ctx* ctx, const CamDev& cd, const snkpt_settings* 
         k_clear_u64<<<1, 64, 0, C.side>>>(C.tstats.as<unsigned long long>(), 8);
         if ((e = cache_train_paths(ctx, cd, R.w, R.h, s, initstate,
                                    s->cache_train_paths ? s->cache_train_paths : 65536u,
-                                   C.tstats.as<unsigned long long>(), C.side)) != SNKPT_OK)
+                                   C.tstats.as<unsigned long long>(), C.side, nullptr)) != SNKPT_OK)
             return e;
         HIPC(hipEventRecord(C.join_ev, C.side));
     }

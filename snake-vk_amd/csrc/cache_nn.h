@@ -14,6 +14,7 @@
 //   loss        relative L2: (y - t)^2 / (sg(y)^2 + 0.01), mean over samples and channels
 #pragma once
 
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -43,6 +44,15 @@ struct GridGeom {
     float inv_ext;            // 1 / largest extent of the scene box
     uint32_t res[kGridLevels];
     uint32_t dense[kGridLevels];  // 1: dense indexing at this level
+};
+
+// One context's network as the multi-network query kernels read it: grid
+// geometry, FP16 grid features and FP16 MLP weights (the live weights or their
+// averages).
+struct CacheRefD {
+    GridGeom geom;
+    const __half2* grid;
+    const _Float16* mlp;
 };
 
 __host__ __device__ __forceinline__ uint32_t grid_index(uint32_t x, uint32_t y, uint32_t z, uint32_t res,

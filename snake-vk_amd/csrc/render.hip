@@ -165,6 +165,10 @@ __global__ __launch_bounds__(kMegaBlock) void k_pt_mega(SceneDev S, CamDev cam, 
     __shared__ uint2 stk[STACK * kMegaBlock];
     __shared__ uint32_t red[2];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (KIND == 2 && H.tseed) {  // a replayed frame's training paths: streams of this frame
+        initstate = H.tseed->path_state;
+        start = H.tseed->start;
+    }
     if (threadIdx.x == 0) red[0] = red[1] = 0u;
     __syncthreads();
     uint32_t n_ext = 0, n_sh = 0;

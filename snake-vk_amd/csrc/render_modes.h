@@ -1,6 +1,7 @@
 // Wavefront state shared by the path-trace, ReSTIR and cache passes.
 #pragma once
 
+#include "cache_nn.h"
 #include "render_common.h"
 
 namespace snkpt {
@@ -135,6 +136,16 @@ struct ShadeHooks {
     uint32_t* nrec = nullptr;    // recorded vertices per path
     uint32_t max_rec = 0;
     float4* rad = nullptr;       // radiance target (defaults to the frame radiance)
+    const struct TrainSeedD* tseed = nullptr;  // training paths: RNG state and first pixel read on the device
+};
+
+// Random-stream state of a frame's training paths, computed on the device from
+// the frame index when the frame is a replayed graph (batched cache form).
+struct TrainSeedD {
+    uint64_t path_state;  // RNG state of the paths (the frame's pass state ^ the training-path tag)
+    uint64_t kseed;       // seed of the vertex selection (the frame's pass state ^ the target tag)
+    uint32_t start;       // first pixel of the strided subset
+    uint32_t pad[3];
 };
 constexpr int kRecFloats = 20;  // 16 sample floats (target = radiance so far) + throughput rgb + pad
 
@@ -181,6 +192,18 @@ snkpt_status whole_paths(snkpt_ctx* ctx, const CamDev& cam, uint32_t w, uint32_t
                          const ShadeHooks* hooks, float4* rad, unsigned long long* stats, hipStream_t st);
 // cache.hip: cache query and training for callers that trace the render paths themselves (the batch)
 snkpt_status cache_ensure(snkpt_ctx* ctx, const snkpt_settings* s, hipStream_t st);
+// the context's network for the multi-network queries (weight averages with `averaged`)
+snkpt_status cache_ref(snkpt_ctx* ctx, bool averaged, CacheRefD* out);
+// queries grouped by network: rows [off[k], off[k+1]) (64-row multiples, off[nseg]
+// rows in all, at most rows_max) through network refs[k]; x0 holds rows_max x 64 halves
+void cache_query_many(const CacheRefD* refs, const uint32_t* off, uint32_t nseg, const float* samples,
+                      uint32_t rows_max, _Float16* x0, float4* y, hipStream_t st);
+// every buffer of a frame's training half for n_paths paths through a w x h view
+snkpt_status cache_prepare_train(snkpt_ctx* ctx, const snkpt_settings* s, uint32_t w, uint32_t h, uint32_t n_paths,
+                                 hipStream_t st);
+// tags of the training paths' random streams (TrainSeedD)
+constexpr uint64_t kTrainPathsTag = 0x545241494eull;
+constexpr uint64_t kTrainTargetTag = 0x7461726765ull;
 // a cache-mode frame with these settings would allocate nothing (it may be captured)
 bool cache_capture_ready(snkpt_ctx* ctx, const snkpt_settings* s, uint32_t w, uint32_t h);
 snkpt_status cache_query(snkpt_ctx* ctx, const float* samples, uint32_t n, float4* y, bool averaged,
@@ -190,7 +213,7 @@ snkpt_status cache_query(snkpt_ctx* ctx, const float* samples, uint32_t n, float
 // each one's optimiser step on `st` (their statistics added into `stats`).
 snkpt_status cache_train_paths_forked(snkpt_ctx* ctx, const CamDev& cd, uint32_t w, uint32_t h,
                                       const snkpt_settings* s, uint64_t initstate, uint32_t n_paths,
-                                      hipStream_t side);
+                                      hipStream_t side, const TrainSeedD* tseed);
 snkpt_status cache_train_steps(snkpt_ctx* const* ctxs, uint32_t n, const snkpt_settings* s,
                                unsigned long long* stats, hipStream_t st);
 // Bounces 0.. of one path per pixel: camera rays through the wavefront, or,

@@ -226,6 +226,44 @@ def test_batch_cache_form_equals_single_view_cache_renders(gpu, form):
         assert np.array_equal(ca.cache_params(0), cb.cache_params(0))
 
 
+@pytest.mark.parametrize("form", ["full", "indirect_mis"])
+def test_batch_cache_form_graph_replays_match_eager_frames(gpu, form):
+    """The cache form recorded once as a hipGraph (queries grouped by scene on
+    the device, training batch sizes and Adam steps on the device, training
+    streams from the frame index in device memory) and replayed: every view and
+    every scene's network equal an eager batch over twin contexts after each
+    frame, bit for bit, with several views per scene and accumulation on."""
+    import torch
+
+    import snkpt
+    _say(f"batch cache form: graph replays against eager frames, form {form}")
+    _, ctx_e, b_e, outs_e, _ = _setup()
+    _, ctx_g, b_g, outs_g, _ = _setup()
+    for c in ctx_e + ctx_g:
+        c.cache_reset(9)
+    indirect = form != "full"
+    fl = snkpt.FLAG_ACCUMULATE | ((snkpt.FLAG_CACHE_INDIRECT | snkpt.FLAG_CACHE_MIS) if indirect else 0)
+    kw = dict(mode=snkpt.MODE_CACHE, max_depth=5, rr_start=3, cache_train_paths=8192, cache_lr=2.5e-3,
+              cache_ema=0.0, cache_query_depth=1 if indirect else 2)
+    stream = torch.cuda.Stream()
+    for frame in range(4):
+        f = fl | (snkpt.FLAG_RESET if frame == 0 else 0)
+        b_e.render(_settings(frame=frame, flags=f, **kw), use_graph=False, stream=stream)
+        b_g.render(_settings(frame=frame, flags=f, **kw), use_graph=True, stream=stream)
+        stream.synchronize()
+        ge, gg = _grab(outs_e), _grab(outs_g)
+        for k in range(len(VIEWS)):
+            assert np.array_equal(ge[k], gg[k]), (frame, k)
+        _say(f"batch cache form: frame {frame} equal")
+    assert not np.array_equal(ge[0], np.zeros_like(ge[0]))
+    for ce, cg in zip(ctx_e, ctx_g):
+        assert np.array_equal(ce.cache_params(0), cg.cache_params(0))
+    st = b_g.frame_stats(stream)
+    assert st.cache_queries > 0 and st.cache_train_samples > 0
+    b_e.close()
+    b_g.close()
+
+
 def test_batch_cache_form_frozen_many_views(gpu):
     """Several views per scene, networks frozen: every batched view equals the
     single-context cache render of that view."""
