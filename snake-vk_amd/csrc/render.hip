@@ -64,23 +64,43 @@ __global__ void k_bounce_end(uint32_t* c, unsigned long long* stats) {
     }
 }
 
+// Pixel of queue slot i when a whole w x h frame is enqueued in 8 x 8 tiles:
+// bands of 8 rows, each cut into tiles of 8 columns (the last band and the last
+// tile of a band may be narrower), pixels row by row within a tile.  A
+// bijection of [0, w h); 64 consecutive slots (one traversal chunk) are one tile.
+__device__ __forceinline__ uint32_t tile_pixel(uint32_t i, uint32_t w, uint32_t h) {
+    const uint32_t b = min(i / (8u * w), (h - 1u) / 8u);
+    const uint32_t k = i - 8u * w * b;
+    const uint32_t bh = min(8u, h - 8u * b);
+    const uint32_t t = k / (8u * bh);
+    const uint32_t j = k - 8u * bh * t;
+    const uint32_t tw = min(8u, w - 8u * t);
+    return (8u * b + j / tw) * w + 8u * t + j % tw;
+}
+
+// Camera rays of n paths.  Path p goes through pixel (start + p stride) mod
+// (w h) and owns RNG stream p; with `tiled` (a whole frame: start 0, stride 1)
+// queue slot i holds path tile_pixel(i), so the rays of one traversal chunk,
+// and the queries and shadow rays their paths append, come from one 8 x 8
+// tile.  Every path's work is the same in either order.
 __global__ void k_generate(CamDev cam, uint32_t w, uint32_t h, uint32_t n, uint32_t start, uint32_t stride,
-                           uint64_t initstate, float4* __restrict__ q_org, float4* __restrict__ q_dir,
+                           uint32_t tiled, uint64_t initstate, float4* __restrict__ q_org, float4* __restrict__ q_dir,
                            uint32_t* __restrict__ q_path, PathState ps, uint32_t* __restrict__ counts, uint32_t cur,
                            uint32_t* __restrict__ nrec) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) counts[cur] = n;
     if (i >= n) return;
-    uint32_t px = (uint32_t)(((uint64_t)start + (uint64_t)i * stride) % ((uint64_t)w * h));
-    if (nrec) nrec[i] = 0;
+    const uint32_t p = tiled ? tile_pixel(i, w, h) : i;
+    uint32_t px = (uint32_t)(((uint64_t)start + (uint64_t)p * stride) % ((uint64_t)w * h));
+    if (nrec) nrec[p] = 0;
     V3 d = camera_dir(cam, px % w, px / w, w, h);
     q_org[i] = make_float4(cam.pos.x, cam.pos.y, cam.pos.z, 0.0f);
     q_dir[i] = make_float4(d.x, d.y, d.z, 1e30f);
-    q_path[i] = i;
-    Rng r = rng_seed(initstate, i);
-    ps.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    ps.rng[i] = make_uint2((uint32_t)r.state, (uint32_t)(r.state >> 32));
-    ps.bounce[i] = 0;
+    q_path[i] = p;
+    Rng r = rng_seed(initstate, p);
+    ps.thr[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    ps.rng[p] = make_uint2((uint32_t)r.state, (uint32_t)(r.state >> 32));
+    ps.bounce[p] = 0;
 }
 
 // One path vertex of the unbiased path tracer over the queue of one scene.
@@ -447,10 +467,11 @@ snkpt_status pt_bounces(snkpt_ctx* ctx, const snkpt_settings* s, int cur, uint32
 }
 
 snkpt_status generate_camera(snkpt_ctx* ctx, const CamDev& cam, uint64_t initstate, hipStream_t st, uint32_t n_paths,
-                             uint32_t start, uint32_t stride, uint32_t* nrec_clear) {
+                             uint32_t start, uint32_t stride, uint32_t* nrec_clear, bool tiled) {
     RenderState& R = *ctx->render;
     const uint32_t n = n_paths ? n_paths : R.w * R.h;
-    k_generate<<<nblk(n, kBlock), kBlock, 0, st>>>(cam, R.w, R.h, n, start, stride, initstate,
+    const uint32_t tl = (tiled && n == R.w * R.h && start == 0 && stride == 1) ? 1u : 0u;
+    k_generate<<<nblk(n, kBlock), kBlock, 0, st>>>(cam, R.w, R.h, n, start, stride, tl, initstate,
                                                    R.q[0].org.as<float4>(), R.q[0].dir.as<float4>(),
                                                    R.q[0].path.as<uint32_t>(), paths_of(R), R.counts.as<uint32_t>(), 0,
                                                    nrec_clear);
@@ -461,7 +482,8 @@ snkpt_status primary_bounces(snkpt_ctx* ctx, const CamDev& cam, const snkpt_sett
                              hipStream_t st, const ShadeHooks* hooks, uint32_t max_depth_override) {
     RenderState& R = *ctx->render;
     if (!R.use_gb) {
-        generate_camera(ctx, cam, initstate, st);
+        // the queue stages read every path's state by path index: tiles are free
+        generate_camera(ctx, cam, initstate, st, 0, 0, 1, nullptr, true);
         return pt_bounces(ctx, s, 0, 0, st, hooks, max_depth_override);
     }
     const uint32_t n = R.w * R.h;

@@ -221,9 +221,12 @@ snkpt_status cache_train_steps(snkpt_ctx* const* ctxs, uint32_t n, const snkpt_s
 snkpt_status primary_bounces(snkpt_ctx* ctx, const CamDev& cam, const snkpt_settings* s, uint64_t initstate,
                              hipStream_t st, const ShadeHooks* hooks = nullptr, uint32_t max_depth_override = 0);
 // Camera rays for paths j < n_paths through pixels (start + j * stride) mod (w h).
+// With `tiled` and a whole frame the queue holds the paths in 8 x 8 pixel tiles
+// (queue slot != path); without it slot j holds path j, as per-pixel kernels
+// that read the primary hits by pixel need.
 snkpt_status generate_camera(snkpt_ctx* ctx, const CamDev& cam, uint64_t initstate, hipStream_t st,
                              uint32_t n_paths = 0, uint32_t start = 0, uint32_t stride = 1,
-                             uint32_t* nrec_clear = nullptr);
+                             uint32_t* nrec_clear = nullptr, bool tiled = false);
 inline uint32_t grid_for(uint32_t n, uint32_t b) { return n == 0 ? 1 : (n + b - 1) / b; }
 
 // restir.hip / cache.hip
