@@ -58,6 +58,54 @@ constexpr int kWTRow4 = 32;  // W4^T row length (16 outputs padded to the MFMA K
 constexpr int kMlpTParams = kHidden * kLayerParams + kWidth * kWTRow4;  // 18432
 constexpr double kFixScale = 4294967296.0;  // 2^32 fixed point for grid gradients
 
+// The optimiser's state that changes from step to step lives on the device, so
+// that the replays of a captured frame keep counting steps: k_adam_advance
+// moves the counter and sets the step's bias-corrected rate and weight-average
+// decay before the Adam kernels read them.
+struct AdamDev {
+    uint32_t t;     // steps taken
+    uint32_t live;  // this step has samples (0: the Adam kernels leave everything as it is)
+    float lr_t;     // lr * sqrt(1 - b2^t) / (1 - b1^t)
+    float ema;      // weight-average decay of this step (0: the average is set to the weights)
+    uint32_t gmax;  // bits of the step's largest |dL/dfeature| on the hashed levels (sets their fixed point)
+    uint32_t pad[3];
+};
+
+__global__ void k_adam_reset(AdamDev* a) {
+    if (threadIdx.x == 0) *a = AdamDev{0u, 0u, 0.0f, 0.0f, 0u, {0u, 0u, 0u}};
+}
+
+// Fixed point of the hashed levels' gradients: both features of a corner go
+// into ONE 64-bit atomic as two 32-bit halves (f0 * 2^32 + f1; the integer sum
+// of such words is H * 2^32 + L with H, L the sums of the halves, which
+// decode exactly while each stays in int32).  The scale puts the step's largest
+// |dL/dfeature| at 2^22, so each contribution has |v| <= 2^22 and an entry
+// takes 512 same-signed largest contributions before a half could overflow (a
+// hashed entry gets a few per step).  Resolution: 2^-22 of the largest.
+__host__ __device__ __forceinline__ float packed_scale(uint32_t gmax_bits) {
+    const float gm = __uint_as_float(gmax_bits);
+    if (!(gm > 0.0f)) return 0.0f;
+    int ex;
+    (void)frexpf(gm, &ex);  // gm < 2^ex
+    return ldexpf(1.0f, min(max(22 - ex, -126), 126));
+}
+
+__global__ void k_adam_advance(AdamDev* a, uint32_t n, const uint32_t* __restrict__ n_dev, float lr, float ema,
+                               float b1, float b2, int apply) {
+    if (threadIdx.x != 0) return;
+    if (n_dev) n = min(n, *n_dev);
+    AdamDev d = *a;
+    d.live = n > 0u ? 1u : 0u;
+    if (d.live && apply) d.t += 1u;
+    const uint32_t t = d.t ? d.t : 1u;
+    d.lr_t = (float)((double)lr * sqrt(1.0 - pow((double)b2, (double)t)) / (1.0 - pow((double)b1, (double)t)));
+    // decay ramp (1 + t) / (10 + t) so early averages are not dominated by the initial weights
+    d.ema = fminf(ema, (1.0f + (float)t) / (10.0f + (float)t));
+    d.gmax = 0u;
+    *a = d;
+}
+
+
 __device__ __forceinline__ f32x4_t mfma16(half8_t a, half8_t b, f32x4_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
@@ -399,7 +447,8 @@ __global__ __launch_bounds__(64) void k_mlp_train(const _Float16* __restrict__ x
                                                   uint32_t n, const uint32_t* __restrict__ n_dev,
                                                   const _Float16* __restrict__ W, const _Float16* __restrict__ WT,
                                                   float* __restrict__ partial, float* __restrict__ dx0,
-                                                  float* __restrict__ loss_part) {
+                                                  float* __restrict__ loss_part, uint32_t hashed_levels,
+                                                  AdamDev* __restrict__ ad) {
     __shared__ __attribute__((aligned(16))) _Float16 act[5][64 * kRow];
     __shared__ __attribute__((aligned(16))) _Float16 gb[2][64 * kRow];
     const int lane = threadIdx.x;
@@ -480,6 +529,7 @@ __global__ __launch_bounds__(64) void k_mlp_train(const _Float16* __restrict__ x
             __syncthreads();
         } else {
             // dX0 for the 32 grid features
+            float gmax = 0.0f;
             f32x4_t acc[4][2];
             back_dx<2, 2>(gb[gi], WT + 0 * kLayerParams, kWidth, acc, lane);
 #pragma unroll
@@ -490,8 +540,14 @@ __global__ __launch_bounds__(64) void k_mlp_train(const _Float16* __restrict__ x
                     for (int e = 0; e < 4; ++e) {
                         int row = m * 16 + 4 * (lane >> 4) + e, col = nn * 16 + (lane & 15);
                         uint32_t q = base + row;
-                        if (q < n) dx0[(size_t)32 * q + col] = acc[m][nn][e];
+                        if (q < n) {
+                            dx0[(size_t)32 * q + col] = acc[m][nn][e];
+                            if ((hashed_levels >> (col >> 1)) & 1u) gmax = fmaxf(gmax, fabsf(acc[m][nn][e]));
+                        }
                     }
+            // largest |dL/dfeature| on the hashed levels: the fixed point of their gradient words
+            for (int off = 32; off >= 1; off >>= 1) gmax = fmaxf(gmax, __shfl_xor(gmax, off, 64));
+            if (lane == 0 && gmax > 0.0f) atomicMax(&ad->gmax, __float_as_uint(gmax));
         }
     }
 }
@@ -517,26 +573,38 @@ constexpr uint32_t kPrivEntries = priv_base(kPrivLevels - 1) + priv_cells(kPrivL
 // their atomics over several levels.
 __global__ void k_grid_bwd(const float* __restrict__ samples, const float* __restrict__ dx0, uint32_t n,
                            const uint32_t* __restrict__ n_dev, GridGeom G, unsigned long long* __restrict__ grad,
-                           unsigned long long* __restrict__ priv) {
+                           unsigned long long* __restrict__ priv, const AdamDev* __restrict__ ad) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (n_dev) n = min(n, *n_dev);
     if (i >= n) return;
     unsigned long long* mine = priv + (size_t)(blockIdx.x % kPrivCopies) * kPrivEntries * 2;
+    const float ps = packed_scale(ad->gmax);
     float p[3];
     norm_pos(samples + (size_t)kSampleFloats * i, G, p);
     for (int l = 0; l < kGridLevels; ++l) {
         const float g0 = dx0[(size_t)32 * i + 2 * l], g1 = dx0[(size_t)32 * i + 2 * l + 1];
         if (g0 == 0.0f && g1 == 0.0f) continue;
         const Cell c = cell_of(p, (float)G.res[l]);
-        for (int k = 0; k < 8; ++k) {
-            float w = corner_weight(c, k);
-            const uint32_t idx = corner_index(c, k, G, l);
-            unsigned long long* dst =
-                l < kPrivLevels ? mine + (size_t)(priv_base(l) + idx) * 2 : grad + ((size_t)l * kGridT + idx) * 2;
-            long long v0 = (long long)llrint((double)(w * g0) * kFixScale);
-            long long v1 = (long long)llrint((double)(w * g1) * kFixScale);
-            if (v0) atomicAdd(&dst[0], (unsigned long long)v0);
-            if (v1) atomicAdd(&dst[1], (unsigned long long)v1);
+        if (G.dense[l]) {
+            // dense levels: many samples share an entry; one 64-bit word per feature
+            for (int k = 0; k < 8; ++k) {
+                float w = corner_weight(c, k);
+                const uint32_t idx = corner_index(c, k, G, l);
+                unsigned long long* dst = l < kPrivLevels ? mine + (size_t)(priv_base(l) + idx) * 2
+                                                          : grad + ((size_t)l * kGridT + idx) * 2;
+                long long v0 = (long long)llrint((double)(w * g0) * kFixScale);
+                long long v1 = (long long)llrint((double)(w * g1) * kFixScale);
+                if (v0) atomicAdd(&dst[0], (unsigned long long)v0);
+                if (v1) atomicAdd(&dst[1], (unsigned long long)v1);
+            }
+        } else {
+            for (int k = 0; k < 8; ++k) {
+                float w = corner_weight(c, k);
+                const uint32_t idx = corner_index(c, k, G, l);
+                const long long h = llrintf(w * g0 * ps), lo = llrintf(w * g1 * ps);
+                const long long v = h * 4294967296ll + lo;
+                if (v) atomicAdd(&grad[((size_t)l * kGridT + idx) * 2], (unsigned long long)v);
+            }
         }
     }
 }
@@ -562,35 +630,6 @@ struct AdamP {
     float b1, b2, eps, inv_scale;
     int apply;
 };
-
-// The optimiser's state that changes from step to step lives on the device, so
-// that the replays of a captured frame keep counting steps: k_adam_advance
-// moves the counter and sets the step's bias-corrected rate and weight-average
-// decay before the Adam kernels read them.
-struct AdamDev {
-    uint32_t t;     // steps taken
-    uint32_t live;  // this step has samples (0: the Adam kernels leave everything as it is)
-    float lr_t;     // lr * sqrt(1 - b2^t) / (1 - b1^t)
-    float ema;      // weight-average decay of this step (0: the average is set to the weights)
-};
-
-__global__ void k_adam_reset(AdamDev* a) {
-    if (threadIdx.x == 0) *a = AdamDev{0u, 0u, 0.0f, 0.0f};
-}
-
-__global__ void k_adam_advance(AdamDev* a, uint32_t n, const uint32_t* __restrict__ n_dev, float lr, float ema,
-                               float b1, float b2, int apply) {
-    if (threadIdx.x != 0) return;
-    if (n_dev) n = min(n, *n_dev);
-    AdamDev d = *a;
-    d.live = n > 0u ? 1u : 0u;
-    if (d.live && apply) d.t += 1u;
-    const uint32_t t = d.t ? d.t : 1u;
-    d.lr_t = (float)((double)lr * sqrt(1.0 - pow((double)b2, (double)t)) / (1.0 - pow((double)b1, (double)t)));
-    // decay ramp (1 + t) / (10 + t) so early averages are not dominated by the initial weights
-    d.ema = fminf(ema, (1.0f + (float)t) / (10.0f + (float)t));
-    *a = d;
-}
 
 // Split of the per-wave gradient partials of n samples into reduction groups.
 constexpr uint32_t kRedGroups = 64;
@@ -653,28 +692,56 @@ __global__ void k_mlp_adam(const float* __restrict__ red, uint32_t n, const uint
         wt16[kHidden * kLayerParams + i * kWTRow4 + o] = (_Float16)w;
 }
 
-__global__ void k_grid_adam(unsigned long long* __restrict__ grad, float* __restrict__ w32, float* __restrict__ m,
-                            float* __restrict__ v, __half* __restrict__ w16, float* __restrict__ gout,
-                            float* __restrict__ e32, __half* __restrict__ e16, size_t n, AdamP A,
+// Adam on the grid, one thread per entry (both features): decodes the
+// entry's fixed-point gradient (a 64-bit word per feature on dense levels, one
+// packed word on hashed levels), clears it, and steps the entry.  Entries
+// without gradient this step are left as they are (lazy / sparse Adam: their
+// moments are not decayed), so a step reads 8-16 B for most untouched entries.
+__global__ void k_grid_adam(unsigned long long* __restrict__ grad, float2* __restrict__ w32, float2* __restrict__ m,
+                            float2* __restrict__ v, __half2* __restrict__ w16, float2* __restrict__ gout,
+                            float2* __restrict__ e32, __half2* __restrict__ e16, GridGeom G, AdamP A,
                             const AdamDev* __restrict__ ad) {
-    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n || !ad->live) return;
-    long long gi = (long long)grad[p];
-    grad[p] = 0ull;
-    float g = (float)((double)gi / kFixScale) * A.inv_scale;
-    if (gout) gout[p] = g;
-    if (!A.apply) return;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)kGridLevels * kGridT || !ad->live) return;
+    const uint32_t l = (uint32_t)(e >> kGridLog2T);
+    float g0, g1;
+    if (G.dense[l]) {
+        const long long a = (long long)grad[2 * e], b = (long long)grad[2 * e + 1];
+        if ((a | b) == 0 && !gout) return;
+        if (a) grad[2 * e] = 0ull;
+        if (b) grad[2 * e + 1] = 0ull;
+        g0 = (float)((double)a / kFixScale);
+        g1 = (float)((double)b / kFixScale);
+    } else {
+        const long long w = (long long)grad[2 * e];
+        if (w == 0 && !gout) return;
+        if (w) grad[2 * e] = 0ull;
+        const long long lo = (long long)(int32_t)(uint32_t)((unsigned long long)w & 0xFFFFFFFFull);
+        const long long hi = (w - lo) >> 32;
+        const float inv = 1.0f / packed_scale(ad->gmax);  // a power of two: exact
+        g0 = (float)hi * inv;
+        g1 = (float)lo * inv;
+    }
+    g0 *= A.inv_scale;
+    g1 *= A.inv_scale;
+    if (gout) gout[e] = make_float2(g0, g1);
+    if (!A.apply || (g0 == 0.0f && g1 == 0.0f)) return;
     const float lr_t = ad->lr_t, ema = ad->ema;
-    float mm = A.b1 * m[p] + (1.0f - A.b1) * g;
-    float vv = A.b2 * v[p] + (1.0f - A.b2) * g * g;
-    m[p] = mm;
-    v[p] = vv;
-    float w = w32[p] - lr_t * mm / (sqrtf(vv) + A.eps);
-    w32[p] = w;
-    w16[p] = __float2half(w);
-    const float e = ema * e32[p] + (1.0f - ema) * w;
-    e32[p] = e;
-    e16[p] = __float2half(e);
+    float2 mm = m[e], vv = v[e], w = w32[e], ea = e32[e];
+    mm.x = A.b1 * mm.x + (1.0f - A.b1) * g0;
+    mm.y = A.b1 * mm.y + (1.0f - A.b1) * g1;
+    vv.x = A.b2 * vv.x + (1.0f - A.b2) * g0 * g0;
+    vv.y = A.b2 * vv.y + (1.0f - A.b2) * g1 * g1;
+    m[e] = mm;
+    v[e] = vv;
+    w.x = w.x - lr_t * mm.x / (sqrtf(vv.x) + A.eps);
+    w.y = w.y - lr_t * mm.y / (sqrtf(vv.y) + A.eps);
+    w32[e] = w;
+    w16[e] = __floats2half2_rn(w.x, w.y);
+    ea.x = ema * ea.x + (1.0f - ema) * w.x;
+    ea.y = ema * ea.y + (1.0f - ema) * w.y;
+    e32[e] = ea;
+    e16[e] = __floats2half2_rn(ea.x, ea.y);
 }
 
 __global__ void k_grid_init(float* __restrict__ w32, __half* __restrict__ w16, float* __restrict__ m,
@@ -939,12 +1006,15 @@ snkpt_status cache_train_run(snkpt_ctx* ctx, const float* samples, uint32_t n, c
     const uint32_t chunks = (n + 63) / 64;
     k_encode<<<enc_blocks(n), kEncBlock, 0, st>>>(samples, n, n_dev, C.geom, C.grid16.as<__half2>(),
                                                C.x0.as<_Float16>());
+    uint32_t hashed = 0;
+    for (int l = 0; l < kGridLevels; ++l)
+        if (!C.geom.dense[l]) hashed |= 1u << l;
     k_mlp_train<<<chunks, 64, 0, st>>>(C.x0.as<_Float16>(), samples, n, n_dev, C.mlp16.as<_Float16>(),
                                        C.mlp16t.as<_Float16>(), C.partial.as<float>(), C.dx0.as<float>(),
-                                       C.loss_part.as<float>());
+                                       C.loss_part.as<float>(), hashed, ad);
     k_grid_bwd<<<grid_for(n, 128), 128, 0, st>>>(samples, C.dx0.as<float>(), n, n_dev, C.geom,
                                                  C.grid_grad.as<unsigned long long>(),
-                                                 C.grid_priv.as<unsigned long long>());
+                                                 C.grid_priv.as<unsigned long long>(), ad);
     k_grid_priv_fold<<<grid_for(kPrivEntries * 2, 256), 256, 0, st>>>(C.grid_priv.as<unsigned long long>(),
                                                                       C.grid_grad.as<unsigned long long>());
     AdamP A;
@@ -964,14 +1034,15 @@ snkpt_status cache_train_run(snkpt_ctx* ctx, const float* samples, uint32_t n, c
                                                         C.mlp_e16.as<_Float16>(), A, ad);
     // grid: the fixed-point gradient is consumed (and cleared) every call; without
     // Adam it is kept as floats so it can be read back
-    float* gout = nullptr;
+    float2* gout = nullptr;
     if (!apply) {
         HIPC(C.grid_gradf.reserve(kGridParams * 4));
-        gout = C.grid_gradf.as<float>();
+        gout = C.grid_gradf.as<float2>();
     }
-    k_grid_adam<<<(uint32_t)((kGridParams + 255) / 256), 256, 0, st>>>(
-        C.grid_grad.as<unsigned long long>(), C.grid32.as<float>(), C.grid_m.as<float>(), C.grid_v.as<float>(),
-        C.grid16.as<__half>(), gout, C.grid_e32.as<float>(), C.grid_e16.as<__half>(), kGridParams, A, ad);
+    const size_t entries = (size_t)kGridLevels * kGridT;
+    k_grid_adam<<<(uint32_t)((entries + 255) / 256), 256, 0, st>>>(
+        C.grid_grad.as<unsigned long long>(), C.grid32.as<float2>(), C.grid_m.as<float2>(), C.grid_v.as<float2>(),
+        C.grid16.as<__half2>(), gout, C.grid_e32.as<float2>(), C.grid_e16.as<__half2>(), C.geom, A, ad);
     C.last_n = n;
     if (loss_out) {
         std::vector<float> lp(chunks);
