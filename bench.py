@@ -465,7 +465,7 @@ def time_to_rmse(stream, ref_spp=4096, max_frames=600, warm_frames=256):
     oracle, plain PT against the MLP cache started cold: a freshly initialised
     network, every frame timed (render, training paths, optimiser steps), the
     error recorded from the first frame on.  PT's frames count equally; the
-    cache's frames count (f + 1)^2 in the mean (SNKPT_FLAG_ACCUMULATE with
+    cache's frames count (f + 1)^3 in the mean (SNKPT_FLAG_ACCUMULATE with
     accum_weight), so the still-learning network's early frames fade out
     instead of being discarded.  Thresholds 0.1, 0.05, 0.03 (one PT frame is at
     0.43).  The warm row (network trained for `warm_frames` frames before, not
@@ -534,14 +534,19 @@ def time_to_rmse(stream, ref_spp=4096, max_frames=600, warm_frames=256):
 
         pt = race(snkpt.MODE_PT)
         log(f"bench: PT time-to-RMSE rel {pt['ms_to_rel_rmse']}")
-        quad = lambda f: float(f + 1) ** 2  # noqa: E731
+        cube = lambda f: float(f + 1) ** 3  # noqa: E731
         ind_mis = snkpt.FLAG_CACHE_INDIRECT | snkpt.FLAG_CACHE_MIS
-        forms = {"cache_q1_indirect_mis": dict(cache_query_depth=1, flags=ind_mis, weight=quad, cache_ema=0.0),
-                 "cache_q1_indirect_mis_lr2.5e-3": dict(cache_query_depth=1, flags=ind_mis, weight=quad,
-                                                        cache_ema=0.0, cache_lr=2.5e-3),
-                 "cache_q3_full": dict(cache_query_depth=3, weight=quad, cache_ema=0.0),
+        forms = {"cache_q1_indirect_mis": dict(cache_query_depth=1, flags=ind_mis, weight=cube, cache_ema=0.0,
+                                               cache_lr=5e-3, cache_train_paths=32768),
+                 "cache_q1_indirect_mis_2steps": dict(cache_query_depth=1, flags=ind_mis, weight=cube,
+                                                      cache_ema=0.0, cache_lr=2.5e-3, cache_train_paths=32768,
+                                                      cache_train_steps=2),
+                 "cache_q1_indirect_mis_65k": dict(cache_query_depth=1, flags=ind_mis, weight=cube, cache_ema=0.0,
+                                                   cache_lr=3.5e-3),
+                 "cache_q3_full": dict(cache_query_depth=3, weight=cube, cache_ema=0.0, cache_lr=5e-3,
+                                       cache_train_paths=32768),
                  "warm_not_a_result_q1_indirect_mis": dict(cache_query_depth=1, flags=ind_mis, warm=warm_frames,
-                                                           cache_ema=0.0)}
+                                                           cache_ema=0.0, cache_lr=5e-3, cache_train_paths=32768)}
         caches = {}
         for name, kw in forms.items():
             caches[name] = race(snkpt.MODE_CACHE, **kw)
@@ -553,11 +558,13 @@ def time_to_rmse(stream, ref_spp=4096, max_frames=600, warm_frames=256):
             ratio[k] = round(v / t, 3) if t else None
         c["speedup_vs_pt"] = ratio
     ctx.close()
-    return {"workload": "config 4: room at 1920x1080, 1 spp frames, depth 8, RR from bounce 3, 65,536 training "
-                        "paths/frame, Adam (lr 1e-2 unless named), cold start, every frame timed. cache_q1_indirect_mis:"
-                        " NEE + a traced BRDF sample under the power heuristic at the primary hit, the cache answers "
-                        "its indirect light; cache_q3_full: the cache answers L_o at path vertex 3 (after 2 bounces); "
-                        "cache frames weighted (f + 1)^2 in the running mean, PT frames equally",
+    return {"workload": "config 4: room at 1920x1080, 1 spp frames, depth 8, RR from bounce 3, cold start, every "
+                        "frame timed (render, training paths, optimiser steps). cache_q1_indirect_mis: NEE + a traced "
+                        "BRDF sample under the power heuristic at the primary hit, the cache answers its indirect "
+                        "light, 32,768 training paths and one Adam step (lr 5e-3) per frame; _2steps: two steps per "
+                        "frame at lr 2.5e-3; _65k: 65,536 training paths, lr 3.5e-3; cache_q3_full: the cache answers "
+                        "L_o at path vertex 3 (after 2 bounces); cache frames weighted (f + 1)^3 in the running mean "
+                        "(the still-learning network's early frames fade out), PT frames equally",
             "oracle": {"spp": ref_spp, "mean_radiance": round(ref_mean, 6),
                        "rel_noise_est": round(pt["first_frame_rel_rmse"] / ref_spp ** 0.5, 5)},
             "pt": pt, **caches}
