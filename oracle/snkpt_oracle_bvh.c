@@ -264,7 +264,42 @@ typedef struct {
     float* cost;
     uint32_t* height;
     uint32_t* leafify;
+    float* dpc;     /* C(x, i), i = 1..8, of the BVH8 collapse (8 per internal node) */
+    uint32_t* dpk;  /* split choices, 3 bits per i */
 } sah_ctx;
+
+/* SAH cost of a binary subtree spread over i slots of a BVH8 node (Ylitie, Karras
+   & Laine 2017, sec. 3.1), for the leaves the SAH pass fixed:
+     leaf (a triangle or a leaf cluster of N):  C(x, i) = c_tri N A(x)
+     internal x:  D(x, j) = min_{1 <= k < j} C(l, k) + C(r, j - k)
+                  C(x, 1) = c_node A(x) + D(x, 8);  C(x, i) = min(C(x, i - 1), D(x, i))
+   k ascending, strict comparisons (accel.hip collapse_costs makes the same choices);
+   K: the k of D(x, 8) in bits 0-2, for i >= 2 the k of D(x, i) or 0 when C(x, i) = C(x, i - 1) */
+static void collapse_costs(const float Cl[8], const float Cr[8], float area, float C[8], uint32_t* K) {
+    float D[9];
+    uint32_t Kd[9];
+    for (int j = 2; j <= 8; ++j) {
+        D[j] = INFINITY;
+        Kd[j] = 1u;
+        for (int k = 1; k < j; ++k) {
+            float v = Cl[k - 1] + Cr[j - k - 1];
+            if (v < D[j]) {
+                D[j] = v;
+                Kd[j] = (uint32_t)k;
+            }
+        }
+    }
+    C[0] = COST_NODE * area + D[8];
+    *K = Kd[8];
+    for (int i = 2; i <= 8; ++i) {
+        if (D[i] < C[i - 2]) {
+            C[i - 1] = D[i];
+            *K |= Kd[i] << (3 * (i - 1));
+        } else {
+            C[i - 1] = C[i - 2];
+        }
+    }
+}
 
 static const float* node_box(const float* aabb, uint32_t n_int, uint32_t tagged) {
     uint32_t k = (tagged & LEAF_TAG) ? n_int + (tagged & ~LEAF_TAG) : tagged;
@@ -306,6 +341,19 @@ static void sah_post(sah_ctx* s, uint32_t node) {
     s->cost[node] = cost;
     s->leafify[node] = lf;
     s->height[node] = (hh[0] > hh[1] ? hh[0] : hh[1]) + 1;
+    float Cl[8], Cr[8], C[8];
+    for (int i = 0; i < 8; ++i) {
+        Cl[i] = (c[0] & LEAF_TAG) ? cc[0] : s->dpc[8 * (size_t)c[0] + i];
+        Cr[i] = (c[1] & LEAF_TAG) ? cc[1] : s->dpc[8 * (size_t)c[1] + i];
+    }
+    uint32_t K = 0;
+    if (lf) {
+        for (int i = 0; i < 8; ++i) C[i] = cost;
+    } else {
+        collapse_costs(Cl, Cr, area, C, &K);
+    }
+    for (int i = 0; i < 8; ++i) s->dpc[8 * (size_t)node + i] = C[i];
+    s->dpk[node] = K;
 }
 
 static int quant_exp(float lo, float hi) {
@@ -346,6 +394,8 @@ int o_build(const float* world, uint32_t n, o_bvh* B) {
     B->nodes = (uint32_t*)calloc(20 * (size_t)(n + 1), sizeof(uint32_t));
     B->leaf = (float*)calloc(12 * (size_t)n, sizeof(float));
     float* cost = (float*)calloc(n_int ? n_int : 1, sizeof(float));
+    float* dpc = (float*)calloc(8 * (size_t)(n_int ? n_int : 1), sizeof(float));
+    uint32_t* dpk = (uint32_t*)calloc(n_int ? n_int : 1, sizeof(uint32_t));
     uint32_t* height = (uint32_t*)calloc(n_int ? n_int : 1, sizeof(uint32_t));
     uint64_t* ak = (uint64_t*)malloc(sizeof(uint64_t) * n);
 
@@ -398,7 +448,7 @@ int o_build(const float* world, uint32_t n, o_bvh* B) {
         }
     }
     if (n >= 2) {
-        sah_ctx s = {n_int, B->child, B->count, B->aabb, cost, height, B->leafify};
+        sah_ctx s = {n_int, B->child, B->count, B->aabb, cost, height, B->leafify, dpc, dpk};
         sah_post(&s, 0);
         B->binary_depth = height[0];
     } else {
@@ -415,24 +465,42 @@ int o_build(const float* world, uint32_t n, o_bvh* B) {
         uint32_t S[8];
         int ns = 0;
         if (EXPANDABLE(b)) {
+            /* the SAH collapse: the 8 slots split between b's children as D(b, 8)
+               chose; then the first candidate the costs open (slot count i > 1 after
+               dropping the i whose cost equals i - 1's) is replaced by its left child
+               and its right child appended */
+            uint32_t I[8], done = 0;
+            uint32_t kb = dpk[b] & 7u;
             S[0] = B->child[2 * b];
             S[1] = B->child[2 * b + 1];
+            I[0] = kb;
+            I[1] = 8u - kb;
             ns = 2;
-            while (ns < 8) {
-                int best = -1;
-                float best_area = -1.0f;
-                for (int k2 = 0; k2 < ns; ++k2) {
-                    if (!EXPANDABLE(S[k2])) continue;
-                    float a = area6(node_box(B->aabb, n_int, S[k2]));
-                    if (a > best_area) {
-                        best_area = a;
-                        best = k2;
+            for (;;) {
+                int cur = -1;
+                for (int k2 = 0; k2 < ns; ++k2)
+                    if (!((done >> k2) & 1u)) {
+                        cur = k2;
+                        break;
                     }
+                if (cur < 0) break;
+                uint32_t x = S[cur], i = I[cur];
+                if (!EXPANDABLE(x)) {
+                    done |= 1u << cur;
+                    continue;
                 }
-                if (best < 0) break;
-                uint32_t e = S[best];
-                S[best] = B->child[2 * e];
-                S[ns++] = B->child[2 * e + 1];
+                uint32_t kx = dpk[x];
+                while (i > 1u && ((kx >> (3u * (i - 1u))) & 7u) == 0u) --i;
+                if (i == 1u) {
+                    done |= 1u << cur;
+                    continue;
+                }
+                uint32_t kk = (kx >> (3u * (i - 1u))) & 7u;
+                S[cur] = B->child[2 * x];
+                I[cur] = kk;
+                S[ns] = B->child[2 * x + 1];
+                I[ns] = i - kk;
+                ++ns;
             }
         } else {
             S[0] = b;
@@ -534,6 +602,8 @@ int o_build(const float* world, uint32_t n, o_bvh* B) {
     free(work);
     free(ak);
     free(cost);
+    free(dpc);
+    free(dpk);
     free(height);
     return 0;
 }

@@ -430,7 +430,48 @@ struct BinUp {
     float* cost;
     uint32_t* height;
     uint32_t* leafify;
+    float* dpc;     // C(x, i), i = 1..8, of the BVH8 collapse
+    uint32_t* dpk;  // its split choices, 3 bits per i
 };
+
+// SAH cost of a binary subtree spread over i slots of a BVH8 node (Ylitie,
+// Karras & Laine 2017, sec. 3.1), for the leaves the SAH pass fixed:
+//   leaf (a triangle or a leaf cluster of N):  C(x, i) = c_tri N A(x)
+//   internal x:  D(x, j) = min_{1 <= k < j} C(l, k) + C(r, j - k)   (x opened into j slots)
+//                C(x, 1) = c_node A(x) + D(x, 8)                    (x becomes a BVH8 node)
+//                C(x, i) = min(C(x, i - 1), D(x, i))  for i >= 2
+// k ascending, strict comparisons: the CPU twin makes the same choices.
+// dpk holds the k of D(x, 8) for i = 1 and, for i >= 2, the k of D(x, i) or 0
+// when C(x, i) = C(x, i - 1).
+__device__ __forceinline__ void collapse_costs(const float Cl[8], const float Cr[8], float area, float C[8],
+                                               uint32_t& K) {
+    float D[9];
+    uint32_t Kd[9];
+#pragma unroll
+    for (int j = 2; j <= 8; ++j) {
+        D[j] = __builtin_inff();
+        Kd[j] = 1u;
+#pragma unroll
+        for (int k = 1; k < j; ++k) {
+            const float v = Cl[k - 1] + Cr[j - k - 1];
+            if (v < D[j]) {
+                D[j] = v;
+                Kd[j] = (uint32_t)k;
+            }
+        }
+    }
+    C[0] = kCostNode * area + D[8];
+    K = Kd[8];
+#pragma unroll
+    for (int i = 2; i <= 8; ++i) {
+        if (D[i] < C[i - 2]) {
+            C[i - 1] = D[i];
+            K |= Kd[i] << (3 * (i - 1));
+        } else {
+            C[i - 1] = C[i - 2];
+        }
+    }
+}
 
 // Box, SAH cost, height and leaf decision of internal node `node` from its two
 // children (both already final).
@@ -471,6 +512,24 @@ __device__ __forceinline__ void bin_node(const BinUp& U, uint32_t node) {
     U.cost[node] = c;
     U.height[node] = (hl > hr ? hl : hr) + 1u;
     U.leafify[node] = lf;
+    // collapse costs: a triangle leaf's are its SAH cost cl / cr, an internal
+    // child's were written when it was made
+    float Cl[8], Cr[8], C[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        Cl[i] = (c0 & kLeafTag) ? cl : U.dpc[8 * (size_t)c0 + i];
+        Cr[i] = (c1 & kLeafTag) ? cr : U.dpc[8 * (size_t)c1 + i];
+    }
+    uint32_t K = 0;
+    if (lf) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) C[i] = c;
+    } else {
+        collapse_costs(Cl, Cr, area, C, K);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) U.dpc[8 * (size_t)node + i] = C[i];
+    U.dpk[node] = K;
 }
 
 // One level of the bottom-up pass per launch.  Each live node (every leaf in
@@ -516,6 +575,7 @@ struct BinView {
     const uint32_t* count;
     const float* aabb;
     const uint32_t* leafify;
+    const uint32_t* dpk;
     __device__ const float* box(uint32_t tagged) const {
         uint32_t k = (tagged & kLeafTag) ? n_int + (tagged & ~kLeafTag) : tagged;
         return aabb + 6 * (size_t)k;
@@ -572,40 +632,62 @@ __global__ __launch_bounds__(128) void k_collapse_expand(BinView bv, const uint3
         S[0] = b;
         ns = 1;
         if (ex) {
-            // the greedy expansion: repeatedly open the expandable candidate of
-            // largest area (first one on ties) until 8 candidates or none left
+            // the SAH collapse: the 8 slots are split between b's children as
+            // D(b, 8) chose; then the first candidate that the costs open (its
+            // slot count i > 1 after dropping the i whose cost equals i - 1's)
+            // is replaced by its left child and its right child appended, until
+            // every candidate is a leaf, a BVH8 node of its own (i = 1) or done
+            const uint32_t kb = bv.dpk[b] & 7u;
+            uint32_t I[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) I[k] = 0u;
             uint32_t l = bv.child[2 * b], r = bv.child[2 * b + 1];
             bool el, er;
             child_info(bv, l, el, A[0], X[0], Y[0], Z[0]);
             child_info(bv, r, er, A[1], X[1], Y[1], Z[1]);
             S[0] = l;
             S[1] = r;
+            I[0] = kb;
+            I[1] = 8u - kb;
             em = (el ? 1u : 0u) | (er ? 2u : 0u);
             ns = 2;
-            while (ns < 8) {
-                int best = -1;
-                float best_area = -1.0f;
+            uint32_t done = 0;  // bit k: candidate k stays as it is
+            while (true) {
+                int cur = -1;
 #pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    if (k < ns && ((em >> k) & 1u) && A[k] > best_area) {
-                        best_area = A[k];
-                        best = k;
-                    }
-                }
-                if (best < 0) break;
-                uint32_t e = 0;
+                for (int k = 7; k >= 0; --k)
+                    if (k < ns && !((done >> k) & 1u)) cur = k;
+                if (cur < 0) break;
+                uint32_t x = 0, i = 0;
+                bool xe = false;
 #pragma unroll
                 for (int k = 0; k < 8; ++k)
-                    if (k == best) e = S[k];
-                const uint32_t cl = bv.child[2 * e], cr = bv.child[2 * e + 1];
+                    if (k == cur) {
+                        x = S[k];
+                        i = I[k];
+                        xe = (em >> k) & 1u;
+                    }
+                if (!xe) {
+                    done |= 1u << cur;
+                    continue;
+                }
+                const uint32_t kx = bv.dpk[x];
+                while (i > 1u && ((kx >> (3u * (i - 1u))) & 7u) == 0u) --i;
+                if (i == 1u) {
+                    done |= 1u << cur;
+                    continue;
+                }
+                const uint32_t kk = (kx >> (3u * (i - 1u))) & 7u;
+                const uint32_t cl = bv.child[2 * x], cr = bv.child[2 * x + 1];
                 bool xl, xr;
                 float al, ar, lx, ly, lz, rx, ry, rz;
                 child_info(bv, cl, xl, al, lx, ly, lz);
                 child_info(bv, cr, xr, ar, rx, ry, rz);
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
-                    if (k == best) {
+                    if (k == cur) {
                         S[k] = cl;
+                        I[k] = kk;
                         A[k] = al;
                         X[k] = lx;
                         Y[k] = ly;
@@ -614,6 +696,7 @@ __global__ __launch_bounds__(128) void k_collapse_expand(BinView bv, const uint3
                     }
                     if (k == ns) {
                         S[k] = cr;
+                        I[k] = i - kk;
                         A[k] = ar;
                         X[k] = rx;
                         Y[k] = ry;
@@ -849,6 +932,8 @@ snkpt_status build_accel(snkpt_ctx* ctx, hipStream_t st) {
     HIP_CHECK(A.bin_child.reserve(nn * 8));
     HIP_CHECK(A.bin_aabb.reserve((2 * nn) * 24));
     HIP_CHECK(A.bin_leafify.reserve(nn * 4));
+    HIP_CHECK(A.bin_dpc.reserve(nn * 32));
+    HIP_CHECK(A.bin_dpk.reserve(nn * 4));
 
     DeviceBuffer &keys2 = A.keys2, &ids2 = A.ids2, &hist = A.hist, &hist_scan = A.hist_scan, &sums = A.sums,
                  &bounds = A.bounds, &first = A.first, &count = A.count, &pint = A.pint, &pleaf = A.pleaf,
@@ -917,7 +1002,8 @@ snkpt_status build_accel(snkpt_ctx* ctx, hipStream_t st) {
             k_fill_u32<<<1, 256, 0, st>>>(up_cnt.as<uint32_t>(), kMaxBinLevels + 1, 0u);
             BinUp U{n, n_int, A.bin_child.as<uint32_t>(), count.as<uint32_t>(), pint.as<uint32_t>(),
                     pleaf.as<uint32_t>(), flags.as<uint32_t>(), A.bin_aabb.as<float>(), cost.as<float>(),
-                    height.as<uint32_t>(), A.bin_leafify.as<uint32_t>()};
+                    height.as<uint32_t>(), A.bin_leafify.as<uint32_t>(), A.bin_dpc.as<float>(),
+                    A.bin_dpk.as<uint32_t>()};
             // the live set shrinks level by level: smaller grids (grid-stride
             // loops, so any grid is correct) keep the late, nearly empty launches cheap
             auto grid = [&](uint32_t l) { return std::max(16u, std::min(blocks(n, kUpBlock) >> l, cus * 8u)); };
@@ -930,7 +1016,7 @@ snkpt_status build_accel(snkpt_ctx* ctx, hipStream_t st) {
         }
         // ---- collapse, one level per iteration
         BinView bv{n, n_int, A.bin_child.as<uint32_t>(), first.as<uint32_t>(), count.as<uint32_t>(),
-                   A.bin_aabb.as<float>(), A.bin_leafify.as<uint32_t>()};
+                   A.bin_aabb.as<float>(), A.bin_leafify.as<uint32_t>(), A.bin_dpk.as<uint32_t>()};
         uint32_t root = n >= 2 ? 0u : kLeafTag;
         HIP_CHECK(hipMemcpyAsync(work.ptr, &root, 4, hipMemcpyHostToDevice, st));
         uint32_t level_start = 0, level_count = 1, tri_start = 0, depth = 0;

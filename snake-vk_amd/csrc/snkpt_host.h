@@ -71,6 +71,7 @@ struct Accel {
     DeviceBuffer bin_child;   // internal binary nodes: left, right (tagged)
     DeviceBuffer bin_aabb;    // internal then leaf AABBs, 6 floats each
     DeviceBuffer bin_leafify; // u32 per internal node
+    DeviceBuffer bin_dpc, bin_dpk;  // collapse costs (8 floats) and split choices (8 x 3 bits) per internal node
     DeviceBuffer inst_tri_offset;  // per instance: first global triangle id (n_inst + 1 entries)
     // build scratch, kept between builds so a per-frame rebuild allocates nothing
     DeviceBuffer keys2, ids2, hist, hist_scan, sums, bounds, first, count, pint, pleaf, flags, up_live, up_cnt, cost,
