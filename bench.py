@@ -536,17 +536,19 @@ def time_to_rmse(stream, ref_spp=4096, max_frames=600, warm_frames=256):
         log(f"bench: PT time-to-RMSE rel {pt['ms_to_rel_rmse']}")
         cube = lambda f: float(f + 1) ** 3  # noqa: E731
         ind_mis = snkpt.FLAG_CACHE_INDIRECT | snkpt.FLAG_CACHE_MIS
-        forms = {"cache_q1_indirect_mis": dict(cache_query_depth=1, flags=ind_mis, weight=cube, cache_ema=0.0,
-                                               cache_lr=5e-3, cache_train_paths=32768),
-                 "cache_q1_indirect_mis_2steps": dict(cache_query_depth=1, flags=ind_mis, weight=cube,
+        atq = snkpt.FLAG_CACHE_TRAIN_AT_QUERY
+        forms = {"cache_q1_indirect_mis": dict(cache_query_depth=1, flags=ind_mis | atq, weight=cube, cache_ema=0.0,
+                                               cache_lr=2.5e-3, cache_train_paths=32768),
+                 "cache_q1_indirect_mis_2steps": dict(cache_query_depth=1, flags=ind_mis | atq, weight=cube,
                                                       cache_ema=0.0, cache_lr=2.5e-3, cache_train_paths=32768,
                                                       cache_train_steps=2),
-                 "cache_q1_indirect_mis_65k": dict(cache_query_depth=1, flags=ind_mis, weight=cube, cache_ema=0.0,
-                                                   cache_lr=3.5e-3),
+                 "cache_q1_indirect_mis_all_vertices": dict(cache_query_depth=1, flags=ind_mis, weight=cube,
+                                                            cache_ema=0.0, cache_lr=5e-3, cache_train_paths=32768),
                  "cache_q3_full": dict(cache_query_depth=3, weight=cube, cache_ema=0.0, cache_lr=5e-3,
                                        cache_train_paths=32768),
-                 "warm_not_a_result_q1_indirect_mis": dict(cache_query_depth=1, flags=ind_mis, warm=warm_frames,
-                                                           cache_ema=0.0, cache_lr=5e-3, cache_train_paths=32768)}
+                 "warm_not_a_result_q1_indirect_mis": dict(cache_query_depth=1, flags=ind_mis | atq,
+                                                           warm=warm_frames, cache_ema=0.0, cache_lr=2.5e-3,
+                                                           cache_train_paths=32768)}
         caches = {}
         for name, kw in forms.items():
             caches[name] = race(snkpt.MODE_CACHE, **kw)
@@ -561,8 +563,9 @@ def time_to_rmse(stream, ref_spp=4096, max_frames=600, warm_frames=256):
     return {"workload": "config 4: room at 1920x1080, 1 spp frames, depth 8, RR from bounce 3, cold start, every "
                         "frame timed (render, training paths, optimiser steps). cache_q1_indirect_mis: NEE + a traced "
                         "BRDF sample under the power heuristic at the primary hit, the cache answers its indirect "
-                        "light, 32,768 training paths and one Adam step (lr 5e-3) per frame; _2steps: two steps per "
-                        "frame at lr 2.5e-3; _65k: 65,536 training paths, lr 3.5e-3; cache_q3_full: the cache answers "
+                        "light; it trains on the primary vertices of 32,768 training paths (where it is asked), one "
+                        "Adam step (lr 2.5e-3) per frame; _2steps: two steps per frame; _all_vertices: training "
+                        "samples from every vertex of the training paths, lr 5e-3; cache_q3_full: the cache answers "
                         "L_o at path vertex 3 (after 2 bounces); cache frames weighted (f + 1)^3 in the running mean "
                         "(the still-learning network's early frames fade out), PT frames equally",
             "oracle": {"spp": ref_spp, "mean_radiance": round(ref_mean, 6),

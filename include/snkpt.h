@@ -127,6 +127,8 @@ typedef struct snkpt_pointlight {
                                         1/r^2 outliers of shading points next to an emitter (snkpt_render,
                                         snkpt_batch_render) */
 #define SNKPT_FLAG_NO_BOUNCE 128u   /* SNKPT_MODE_REFERENCE_COMPAT: direct light only (no BounceDiffuse term) */
+#define SNKPT_FLAG_CACHE_TRAIN_AT_QUERY 256u /* cache mode: the training samples are the training paths' vertices at
+                                                the query depth only (where the cache is asked), not every vertex */
 #define SNKPT_FLAG_NO_STATS 64u      /* snkpt_render returns without reading the frame's ray counts back
                                         (snkpt_get_frame_stats keeps the previous values) and without
                                         waiting for the GPU.  With it a SNKPT_MODE_PT frame can be recorded

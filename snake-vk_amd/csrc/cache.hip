@@ -1211,9 +1211,12 @@ __device__ __forceinline__ uint64_t keep_threshold(const uint32_t* __restrict__ 
     return (uint64_t)(((double)budget / (double)total) * 4294967296.0);
 }
 
-__global__ void k_train_total(const uint32_t* __restrict__ nrec, uint32_t n, uint32_t* __restrict__ cnt) {
+// vsel: the one vertex index that is a candidate (the query vertex), or ~0u for all
+__global__ void k_train_total(const uint32_t* __restrict__ nrec, uint32_t n, uint32_t vsel,
+                              uint32_t* __restrict__ cnt) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t k = j < n ? nrec[j] : 0u;
+    if (vsel != ~0u) k = k > vsel ? 1u : 0u;
     // wave sum, one atomic per wave
     for (int o = 32; o > 0; o >>= 1) k += __shfl_xor(k, o, 64);
     if ((threadIdx.x & 63) == 0 && k) atomicAdd(&cnt[2], k);
@@ -1221,7 +1224,7 @@ __global__ void k_train_total(const uint32_t* __restrict__ nrec, uint32_t n, uin
 
 __global__ void k_train_count(const float* __restrict__ rec, const uint32_t* __restrict__ nrec, uint32_t max_rec,
                               const float4* __restrict__ rad, uint32_t n, uint64_t seed,
-                              const TrainSeedD* __restrict__ tseed, uint32_t indirect,
+                              const TrainSeedD* __restrict__ tseed, uint32_t indirect, uint32_t vsel,
                               const uint32_t* __restrict__ cnt, uint32_t budget, uint32_t* __restrict__ count) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
@@ -1232,13 +1235,15 @@ __global__ void k_train_count(const float* __restrict__ rec, const uint32_t* __r
     uint32_t c = 0;
     float smp[16];
     for (uint32_t v = 0; v < k; ++v)
-        if (keep_vertex(seed, j, v, thr) && train_sample(rec, max_rec, j, k, v, L, indirect, smp)) ++c;
+        if ((vsel == ~0u || v == vsel) && keep_vertex(seed, j, v, thr) &&
+            train_sample(rec, max_rec, j, k, v, L, indirect, smp))
+            ++c;
     count[j] = c;
 }
 
 __global__ void k_train_emit(const float* __restrict__ rec, const uint32_t* __restrict__ nrec, uint32_t max_rec,
                              const float4* __restrict__ rad, uint32_t n, uint64_t seed,
-                             const TrainSeedD* __restrict__ tseed, uint32_t indirect,
+                             const TrainSeedD* __restrict__ tseed, uint32_t indirect, uint32_t vsel,
                              const uint32_t* __restrict__ cnt, uint32_t budget, const uint32_t* __restrict__ off,
                              float* __restrict__ out, unsigned long long* __restrict__ stat) {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1251,7 +1256,9 @@ __global__ void k_train_emit(const float* __restrict__ rec, const uint32_t* __re
     uint32_t slot = off[j];
     float smp[16];
     for (uint32_t v = 0; v < k && slot < budget; ++v) {
-        if (!keep_vertex(seed, j, v, thr) || !train_sample(rec, max_rec, j, k, v, L, indirect, smp)) continue;
+        if ((vsel != ~0u && v != vsel) || !keep_vertex(seed, j, v, thr) ||
+            !train_sample(rec, max_rec, j, k, v, L, indirect, smp))
+            continue;
         float4* o = (float4*)(out + (size_t)slot * 16);
 #pragma unroll
         for (int c = 0; c < 4; ++c) o[c] = make_float4(smp[4 * c], smp[4 * c + 1], smp[4 * c + 2], smp[4 * c + 3]);
@@ -1377,12 +1384,16 @@ snkpt_status cache_train_paths(snkpt_ctx* ctx, const CamDev& cd, uint32_t w, uin
     uint32_t* sums = (uint32_t*)(tm + 3 * u32_bytes);
     const uint64_t kseed = initstate ^ kTrainTargetTag;
     const uint32_t budget = NT;  // training samples per frame
-    k_train_total<<<grid_for(NT, 256), 256, 0, st>>>(nrec, NT, qc);
+    // with SNKPT_FLAG_CACHE_TRAIN_AT_QUERY only the vertex the cache answers is a candidate
+    const uint32_t vsel = (s->flags & SNKPT_FLAG_CACHE_TRAIN_AT_QUERY)
+                              ? (s->cache_query_depth ? s->cache_query_depth : 2u) - 1u
+                              : ~0u;
+    k_train_total<<<grid_for(NT, 256), 256, 0, st>>>(nrec, NT, vsel, qc);
     k_train_count<<<grid_for(NT, 256), 256, 0, st>>>(C.train_vtx.as<float>(), nrec, depth, trad, NT, kseed, tseed,
-                                                      indirect, qc, budget, pcount);
+                                                      indirect, vsel, qc, budget, pcount);
     scan_exclusive(pcount, off, NT, sums, qc + 1, st);
     k_train_emit<<<grid_for(NT, 256), 256, 0, st>>>(C.train_vtx.as<float>(), nrec, depth, trad, NT, kseed, tseed,
-                                                     indirect, qc, budget, off, C.t_samples.as<float>(), stats + 4);
+                                                     indirect, vsel, qc, budget, off, C.t_samples.as<float>(), stats + 4);
     C.t_max = budget;
     return SNKPT_OK;
 }

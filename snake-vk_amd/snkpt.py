@@ -41,6 +41,7 @@ FLAG_CACHE_MIS = 32  # with FLAG_CACHE_INDIRECT: NEE + a traced BRDF probe under
 FLAG_WHOLE_PATHS = 16
 FLAG_NO_STATS = 64  # no end-of-frame ray-count readback: PT frames become capturable into a HIP graph
 FLAG_NO_BOUNCE = 128  # reference-compat mode: the reference's ReSTIR direct light only, no BounceDiffuse
+FLAG_CACHE_TRAIN_AT_QUERY = 256  # cache mode: train only on the training paths' vertices at the query depth
 FORMAT_RGBA32F = 0
 FORMAT_RGBA16F = 1
 

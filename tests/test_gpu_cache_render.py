@@ -200,6 +200,32 @@ def test_primary_queries_record_the_hit(gpu):
     assert np.allclose(meta[:, :3], 1.0)
 
 
+def test_training_at_the_query_vertex_only(gpu):
+    """SNKPT_FLAG_CACHE_TRAIN_AT_QUERY with Q = 1 on a Lambertian floor: the
+    training samples are exactly the training paths' primary hits, one per path
+    (every camera ray of this view hits the floor): points of the floor seen
+    along camera rays, direction back to the eye, the floor's normal and albedo."""
+    import snkpt
+    sc, verts = floor_light_scene(albedo=0.4)
+    ctx = snkpt.Context(0)
+    sc.load_into(ctx)
+    w = h = 32
+    eye = np.array([0.2, 1.0, 0.1])
+    cam = look_at_camera(eye, [0.3, 0.0, -0.2], fov=30.0, w=w, h=h)
+    ctx.cache_reset(2)
+    _render(ctx, cam, w, h, mode=snkpt.MODE_CACHE, max_depth=4, cache_query_depth=1, seed=8,
+            cache_train_paths=w * h, flags=snkpt.FLAG_CACHE_INDIRECT | snkpt.FLAG_CACHE_TRAIN_AT_QUERY)
+    n = int(ctx.frame_stats().cache_train_samples)
+    assert n == w * h, n
+    s = ctx.cache_params(7).reshape(-1, 16)[:n]
+    assert np.allclose(s[:, 1], 0.0, atol=1e-4)
+    to_eye = eye[None, :] - s[:, 0:3]
+    to_eye /= np.linalg.norm(to_eye, axis=1, keepdims=True)
+    assert np.allclose(s[:, 3:6], to_eye, atol=1e-3)
+    assert np.allclose(s[:, 6:9], [0, 1, 0], atol=1e-6)
+    assert np.allclose(s[:, 10:13], 0.4)
+
+
 def _wall_scene(albedo_floor=0.5, albedo_wall=0.7):
     sc, verts = floor_light_scene(albedo=albedo_floor)
     # a Lambertian wall at x = 1.2 facing -x, below the emitter's height
