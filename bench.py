@@ -498,7 +498,7 @@ def time_to_rmse(stream, ref_spp=4096, max_frames=600, warm_frames=256):
         stream.synchronize()
         rel_thr = [0.1, 0.05, 0.03]
 
-        def race(mode, warm=0, weight=None, **kw):
+        def race(mode, warm=0, weight=None, early_steps=None, **kw):
             if mode == snkpt.MODE_CACHE:
                 ctx.cache_reset(1, stream=stream)
                 for f in range(warm):
@@ -510,6 +510,8 @@ def time_to_rmse(stream, ref_spp=4096, max_frames=600, warm_frames=256):
                 fl = snkpt.FLAG_ACCUMULATE | (snkpt.FLAG_RESET if f == 0 else 0)
                 kw2 = dict(kw)
                 fl |= kw2.pop("flags", 0)
+                if early_steps and f < early_steps[0]:  # (frames, optimiser steps per frame) at the start
+                    kw2["cache_train_steps"] = early_steps[1]
                 st = snkpt.default_settings(mode=mode, max_depth=depth, rr_start=rr, seed=scene.s.seed, frame=f,
                                             flags=fl, accum_weight=weight(f) if weight else 1.0, **kw2)
                 a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -538,10 +540,12 @@ def time_to_rmse(stream, ref_spp=4096, max_frames=600, warm_frames=256):
         ind_mis = snkpt.FLAG_CACHE_INDIRECT | snkpt.FLAG_CACHE_MIS
         atq = snkpt.FLAG_CACHE_TRAIN_AT_QUERY
         forms = {"cache_q1_indirect_mis": dict(cache_query_depth=1, flags=ind_mis | atq, weight=cube, cache_ema=0.0,
-                                               cache_lr=2.5e-3, cache_train_paths=32768),
-                 "cache_q1_indirect_mis_2steps": dict(cache_query_depth=1, flags=ind_mis | atq, weight=cube,
-                                                      cache_ema=0.0, cache_lr=2.5e-3, cache_train_paths=32768,
-                                                      cache_train_steps=2),
+                                               cache_lr=2.5e-3, cache_train_paths=32768, early_steps=(16, 2)),
+                 "cache_q1_indirect_mis_1step": dict(cache_query_depth=1, flags=ind_mis | atq, weight=cube,
+                                                     cache_ema=0.0, cache_lr=2.5e-3, cache_train_paths=32768),
+                 "cache_q1_indirect_nee_only": dict(cache_query_depth=1, flags=snkpt.FLAG_CACHE_INDIRECT | atq,
+                                                    weight=cube, cache_ema=0.0, cache_lr=2.5e-3,
+                                                    cache_train_paths=32768),
                  "cache_q1_indirect_mis_all_vertices": dict(cache_query_depth=1, flags=ind_mis, weight=cube,
                                                             cache_ema=0.0, cache_lr=5e-3, cache_train_paths=32768),
                  "cache_q3_full": dict(cache_query_depth=3, weight=cube, cache_ema=0.0, cache_lr=5e-3,
@@ -563,9 +567,11 @@ def time_to_rmse(stream, ref_spp=4096, max_frames=600, warm_frames=256):
     return {"workload": "config 4: room at 1920x1080, 1 spp frames, depth 8, RR from bounce 3, cold start, every "
                         "frame timed (render, training paths, optimiser steps). cache_q1_indirect_mis: NEE + a traced "
                         "BRDF sample under the power heuristic at the primary hit, the cache answers its indirect "
-                        "light; it trains on the primary vertices of 32,768 training paths (where it is asked), one "
-                        "Adam step (lr 2.5e-3) per frame; _2steps: two steps per frame; _all_vertices: training "
-                        "samples from every vertex of the training paths, lr 5e-3; cache_q3_full: the cache answers "
+                        "light; it trains on the primary vertices of 32,768 training paths (where it is asked), two "
+                        "Adam steps (lr 2.5e-3) per frame for the first 16 frames, then one; _1step: one step per "
+                        "frame throughout; _nee_only: NEE alone at the primary hit (no BRDF probe); _all_vertices: "
+                        "training samples from every vertex of the training paths, lr 5e-3; cache_q3_full: the cache "
+                        "answers "
                         "L_o at path vertex 3 (after 2 bounces); cache frames weighted (f + 1)^3 in the running mean "
                         "(the still-learning network's early frames fade out), PT frames equally",
             "oracle": {"spp": ref_spp, "mean_radiance": round(ref_mean, 6),
