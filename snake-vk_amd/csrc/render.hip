@@ -404,7 +404,8 @@ snkpt_status launch_shadow(snkpt_ctx* ctx, RenderState& R, float4* rad, hipStrea
     uint32_t need = A.depth > 0 ? A.depth - 1 : 0;
     const uint32_t cus = ctx->props.multiProcessorCount > 0 ? (uint32_t)ctx->props.multiProcessorCount : 256u;
     // persistent grid: as many blocks as the LDS of the stack size lets stay resident
-    const uint32_t resident = need <= 10 ? 16u : need <= 12 ? 13u : need <= 16 ? 10u : need <= 32 ? 5u : 2u;
+    // (an even count: two waves per block spread evenly over the four SIMDs)
+    const uint32_t resident = need <= 10 ? 16u : need <= 12 ? 12u : need <= 16 ? 10u : need <= 32 ? 4u : 2u;
     uint32_t grid = std::min(nblk(n_max ? n_max : R.w * R.h, kShadowBlock), cus * resident);
     if (need <= 10)
         k_shadow<10><<<grid, kShadowBlock, 0, st>>>(A.nodes.as<uint4>(), A.leaf_tris.as<float4>(), shadow_of(R),

@@ -103,8 +103,14 @@ constexpr uint32_t kRefill = 8;
 inline uint32_t refill_blocks_per_cu(uint32_t need) {
     return need <= 10 ? 16u : need <= 12 ? 13u : need <= 16 ? 10u : need <= 32 ? 5u : 2u;
 }
+// Blocks per CU that give every SIMD the same number of waves (two waves per
+// block over four SIMDs: an even count): 13 blocks put 7 waves on two SIMDs and
+// 6 on the others, and ran the 12-entry variant 18 % slower than 12 blocks.
+inline uint32_t balanced_blocks(uint32_t b) { return b > 1u ? (b & ~1u) : 1u; }
 constexpr uint32_t kStepsDefault = 4;
-TraceShape default_shape(uint32_t need) { return TraceShape{kRefill, kStepsDefault, refill_blocks_per_cu(need), 0.0f}; }
+TraceShape default_shape(uint32_t need) {
+    return TraceShape{kRefill, kStepsDefault, balanced_blocks(refill_blocks_per_cu(need)), 0.0f};
+}
 
 template <bool ANY, bool COUNT, int STEPS>
 void launch_stack_steps(uint32_t stack, uint32_t g, uint32_t refill, hipStream_t st, const float4* o,
@@ -139,16 +145,19 @@ void launch_stack(const TraceShape& sh, uint32_t stack, uint32_t grid, uint32_t 
         launch_stack_steps<ANY, COUNT, 4>(stack, g, sh.refill, st, o, d, n, cnt, nodes, tris, h, nv, tv);
 }
 
-// Rays of the autotuner's sample, like a path tracer's bounce rays: from a
-// uniformly chosen triangle (leaf record), a uniform point on it lifted 1e-4 of
-// the scene extent along its normal, on a random side, cosine-distributed
-// around that normal (PCG32 per ray).
+// Rays of the autotuner's sample, like a path tracer's bounce rays: ray i
+// leaves leaf record k = i ntris / n (leaf records are in BVH order, so
+// consecutive rays start on neighbouring surfaces, as the bounce rays of
+// neighbouring pixels do; rays from scattered triangles favoured shapes with
+// fewer resident blocks that were 3 % slower on real bounce rays), from a
+// uniform point on it lifted 1e-4 of the scene extent along its normal, on a
+// random side, cosine-distributed around that normal (PCG32 per ray).
 __global__ void k_tune_rays(const float4* __restrict__ tris, uint32_t ntris, float4* __restrict__ org,
                             float4* __restrict__ dir, uint32_t n, float eps) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Rng r = rng_seed(0x74756E65ull, i);
-    const uint32_t k = min((uint32_t)(r.uniform() * (float)ntris), ntris - 1u);
+    const uint32_t k = min((uint32_t)(((uint64_t)i * ntris) / n), ntris - 1u);
     const float4 a = tris[(size_t)k * kTriWords], b = tris[(size_t)k * kTriWords + 1],
                  c = tris[(size_t)k * kTriWords + 2];
     const V3 v0{a.x, a.y, a.z}, v1{a.w, b.x, b.y}, v2{b.z, b.w, c.x};
@@ -173,7 +182,9 @@ std::map<std::tuple<int, uint32_t, uint32_t>, std::pair<TraceShape, TraceShape>>
 
 // Picks the launch shape of the closest- and any-hit traversal for the built
 // acceleration structure: refill threshold 4 / 8 / 16 idle lanes x 3 / 4 steps
-// per refill check x all / 3/4 / 1/2 of the resident blocks, each timed on 2^20
+// per refill check x every even block count per CU from the resident maximum
+// down to half of it (C2 bounce rays, 12-entry stack: 12 blocks 5.03 Grays/s,
+// 10: 4.48, 8: 4.87; 13: 4.15, 11: 4.74, 9: 4.77), each timed on 2^20
 // bounce-like rays leaving the scene's triangles (best of 3 launches after one
 // warm-up).  The
 // shape changes which lane traces which ray, never a ray's traversal, so hits do
@@ -222,7 +233,8 @@ snkpt_status autotune_trace(snkpt_ctx* ctx, hipStream_t st) {
         best[kind].ms = 1e30f;
         for (uint32_t refill : {4u, 8u, 16u})
             for (uint32_t steps : {3u, 4u})
-                for (uint32_t bpc : {bmax, std::max(1u, bmax * 3 / 4), std::max(1u, bmax / 2)}) {
+                for (uint32_t bpc = balanced_blocks(bmax); bpc >= std::max(1u, bmax / 2) && bpc >= 1u;
+                     bpc = bpc > 2u ? bpc - 2u : 0u) {
                     const TraceShape sh{refill, steps, bpc, 0.0f};
                     float ms = 1e30f;
                     for (int rep = 0; rep < 4; ++rep) {
