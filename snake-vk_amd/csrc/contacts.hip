@@ -15,7 +15,6 @@ namespace snkpt {
 namespace {
 
 constexpr int kBlock = 128;
-constexpr int kRefill = 16;
 
 struct ContactScene {
     const InstD* inst;
@@ -26,14 +25,21 @@ struct ContactScene {
     const float* nrm;
 };
 
+// Waves per SIMD the stacks' LDS leaves room for (8-byte entries, 128 lanes a block,
+// 160 KiB a CU): 8 for 10 entries (16 blocks), 6 for 12 (13 blocks, 12 launched).  The
+// register budget is held to that count: what spills is the contact record of the
+// completion handler, run once per ray, not the traversal loop.
 template <int STACK>
-__global__ __launch_bounds__(kBlock) void k_contacts(const float4* __restrict__ org_tmin,
+constexpr int contact_waves() { return STACK <= 10 ? 8 : STACK <= 12 ? 6 : 1; }
+
+template <int STACK>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(contact_waves<STACK>(), 8))) void k_contacts(const float4* __restrict__ org_tmin,
                                                      const float4* __restrict__ dir_tmax, uint32_t n,
                                                      const uint4* __restrict__ nodes, const float4* __restrict__ tris,
                                                      ContactScene C, float timestep, float* __restrict__ t_out,
                                                      float4* __restrict__ n_out, float4* __restrict__ p_out,
                                                      float4* __restrict__ v_out, uint32_t* __restrict__ inst_out,
-                                                     uint32_t* __restrict__ prim_out) {
+                                                     uint32_t* __restrict__ prim_out, uint32_t refill) {
     __shared__ uint2 stk[STACK * kBlock];
     const uint32_t nwaves = gridDim.x * (kBlock / 64);
     const uint32_t wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
@@ -89,7 +95,8 @@ __global__ __launch_bounds__(kBlock) void k_contacts(const float4* __restrict__ 
         if (inst_out) inst_out[id] = ii;
         if (prim_out) prim_out[id] = prim;
     };
-    refill_trace<true, false, STACK, kRefill>(n, wave, nwaves, &stk[threadIdx.x], kBlock, load, done);
+    refill_trace_tuned<true, false, STACK, kStepsPerRefill>(n, wave, nwaves, &stk[threadIdx.x], kBlock, refill, load,
+                                                             done);
 }
 
 __global__ void k_contacts_empty(uint32_t n, float* t_out, float4* n_out, float4* p_out, float4* v_out,
@@ -122,22 +129,31 @@ snkpt_status trace_contacts(snkpt_ctx* ctx, const float4* org_tmin, const float4
         const uint32_t need = A.depth > 0 ? A.depth - 1 : 0;
         if (need > 64) return fail(ctx, SNKPT_ERR_CAPACITY, "trace_contacts: BVH8 deeper than the 64-entry stack");
         const uint32_t cus = ctx->props.multiProcessorCount > 0 ? (uint32_t)ctx->props.multiProcessorCount : 256u;
-        const uint32_t grid = std::min((n + kBlock - 1) / kBlock, cus * (need <= 16 ? 10u : need <= 32 ? 5u : 2u));
+        // resident blocks per CU by the stacks' LDS, an even count (two waves per block over four SIMDs)
+        const uint32_t per_cu = need <= 10 ? 16u : need <= 12 ? 12u : need <= 16 ? 10u : need <= 32 ? 4u : 2u;
+        const uint32_t grid = std::min((n + kBlock - 1) / kBlock, cus * per_cu);
+        // the any-hit refill threshold the build's autotuner picked
+        const uint32_t refill = A.shape[1].refill ? A.shape[1].refill : 16u;
         const ContactScene C{ctx->d_inst.as<InstD>(), ctx->d_inst_xf.as<float4>(),
                              ctx->d_prev_xf.ptr ? ctx->d_prev_xf.as<float4>() : nullptr, ctx->d_idx.as<uint32_t>(),
                              ctx->d_pos.as<float>(), ctx->d_nrm.as<float>()};
         if (!C.prev_xf) return fail(ctx, SNKPT_ERR_NOT_BUILT, "trace_contacts: no previous-frame transforms");
         const uint4* nodes = A.nodes.as<uint4>();
         const float4* tris = A.leaf_tris.as<float4>();
-        if (need <= 16)
-            k_contacts<16><<<grid, kBlock, 0, st>>>(org_tmin, dir_tmax, n, nodes, tris, C, timestep, out.t, nrm, pos,
-                                                    vel, out.inst, out.prim);
+#define SNKPT_CONTACTS(S)                                                                                     \
+    k_contacts<S><<<grid, kBlock, 0, st>>>(org_tmin, dir_tmax, n, nodes, tris, C, timestep, out.t, nrm, pos, vel, \
+                                           out.inst, out.prim, refill)
+        if (need <= 10)
+            SNKPT_CONTACTS(10);
+        else if (need <= 12)
+            SNKPT_CONTACTS(12);
+        else if (need <= 16)
+            SNKPT_CONTACTS(16);
         else if (need <= 32)
-            k_contacts<32><<<grid, kBlock, 0, st>>>(org_tmin, dir_tmax, n, nodes, tris, C, timestep, out.t, nrm, pos,
-                                                    vel, out.inst, out.prim);
+            SNKPT_CONTACTS(32);
         else
-            k_contacts<64><<<grid, kBlock, 0, st>>>(org_tmin, dir_tmax, n, nodes, tris, C, timestep, out.t, nrm, pos,
-                                                    vel, out.inst, out.prim);
+            SNKPT_CONTACTS(64);
+#undef SNKPT_CONTACTS
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(ctx, e, "trace_contacts launch");
