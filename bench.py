@@ -167,6 +167,7 @@ def frame_rates(stream, frames=10, warmup=3, only=None):
     modes = {"pt": snkpt.MODE_PT, "restir": snkpt.MODE_RESTIR, "cache": snkpt.MODE_CACHE,
              "cache_indirect": snkpt.MODE_CACHE, "reference_compat": snkpt.MODE_REFERENCE_COMPAT}
     res = {}
+    oracles = {}  # (config, camera frame) -> FP64 oracle image
     for cfg, label in FRAME_RUNS:
         if only is not None and f"c{cfg}_{label}" != only:
             continue
@@ -191,10 +192,12 @@ def frame_rates(stream, frames=10, warmup=3, only=None):
                     s.num_temporal_resamples, s.num_spatial_resamples = 1, 5
                 return s
 
-            def frame(f):
+            def frame_cam(f):
                 # config 3 renders its seeded 64-frame camera orbit (temporal reuse under motion)
-                c = scene.camera(f, w, h) if cfg == 3 else cam
-                ctx.render(c, settings(f), out, w, h, stream=stream)
+                return scene.camera(f, w, h) if cfg == 3 else cam
+
+            def frame(f):
+                ctx.render(frame_cam(f), settings(f), out, w, h, stream=stream)
             # warm-up: at least `warmup` frames and ~30 ms of frames (the clocks leave
             # the idle state the scene set-up let them drop to); config 3's frames
             # stay within its 64-frame orbit
@@ -211,6 +214,30 @@ def frame_rates(stream, frames=10, warmup=3, only=None):
                 frame(nw + k)
                 b.record(stream)
             stream.synchronize()
+            # the last timed frame's error against the converged FP64 oracle of its camera
+            # (plain PT of the same depth: the cache's and the compat estimator's bias shows)
+            last = nw + frames - 1
+            okey = (cfg, last if cfg == 3 else 0)
+            spp = 1024 if w * h <= 65536 else 256
+            if okey not in oracles:
+                acc = torch.zeros(h * w * 3, dtype=torch.float64, device="cuda")
+                os_ = snkpt.default_settings(max_depth=scene.s.max_depth, rr_start=scene.s.rr_start,
+                                             seed=scene.s.seed ^ 0x0AC1E)
+                for k in range(spp // 64):
+                    os_.frame = k
+                    ctx.oracle_accumulate(frame_cam(last), os_, acc, w, h, 64, stream=stream)
+                oracles[okey] = (acc / spp).view(h, w, 3)
+            ref = oracles[okey]
+            err = ((out[..., :3].double() - ref) ** 2).mean().sqrt()
+            frame_err = {"oracle_spp": spp, "frame_rel_rmse": round(float(err / ref.mean()), 4)}
+            if mode != snkpt.MODE_PT:
+                # a plain PT frame of the same camera beside it (1 spp, same depth)
+                pt_out = torch.zeros_like(out)
+                ps = snkpt.default_settings(max_depth=scene.s.max_depth, rr_start=scene.s.rr_start,
+                                            seed=scene.s.seed, frame=last)
+                ctx.render(frame_cam(last), ps, pt_out, w, h, stream=stream)
+                perr = ((pt_out[..., :3].double() - ref) ** 2).mean().sqrt()
+                frame_err["pt_frame_rel_rmse"] = round(float(perr / ref.mean()), 4)
             gms = None
             if label == "pt":
                 # the same frame recorded into a HIP graph (no end-of-frame readback)
@@ -254,6 +281,7 @@ def frame_rates(stream, frames=10, warmup=3, only=None):
         res[key]["bytes_per_ray"] = {k: round(v, 1) for k, v in B.items()}
         res[key]["algorithmic_bytes_per_frame"] = int(alg)
         res[key]["hbm_frac_algorithmic"] = round(alg / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)
+        res[key]["error_vs_fp64_oracle"] = frame_err
         if mode == snkpt.MODE_RESTIR:
             res[key]["camera"] = "seeded 64-frame orbit" if cfg == 3 else "static"
             res[key]["history_reused_frac"] = round(int(st.restir_history_reused) / (w * h), 4)

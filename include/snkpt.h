@@ -473,7 +473,8 @@ typedef struct snkpt_scene {
 } snkpt_scene;
 snkpt_status snkpt_scene_generate(uint32_t config, uint32_t variant, snkpt_scene** out);
 void snkpt_scene_free(snkpt_scene* scene);
-/* Camera of frame f of the config's camera path (config 3: seeded orbit). */
+/* Camera of frame f of the config's camera path (config 3: seeded 64-frame orbit arc
+ * in front of the room's open side). */
 snkpt_status snkpt_scene_camera(uint32_t config, uint32_t variant, uint32_t frame, uint32_t w,
                                 uint32_t h, snkpt_camera* out);
 /* upload_mesh + set_instances + set_materials + set_lights for a generated scene. */

@@ -491,9 +491,12 @@ snkpt_status scene_camera(uint32_t config, uint32_t variant, uint32_t frame, uin
             make_camera({0, 15, 75}, {0, 3, 0}, 90.0, aspect, frame, out);
             return SNKPT_OK;
         case 3: {
-            // 64-frame orbit of radius 3.4 at y = 1 with a small seeded jitter
+            // 64-frame orbit arc of radius 3.4 at y = 1 around the room's open side
+            // (+z): the angle swings +-0.6 rad about the config 1 view, with a small
+            // seeded jitter, so every frame looks into the lit room (the walls'
+            // outsides are unlit)
             Pcg32 rng(config_seed(3), 200 + (uint64_t)frame);
-            double th = 2.0 * M_PI * (double)(frame % 64) / 64.0 + rng.uniform(-0.01, 0.01);
+            double th = 0.6 * std::sin(2.0 * M_PI * (double)(frame % 64) / 64.0) + rng.uniform(-0.01, 0.01);
             make_camera({3.4 * std::sin(th), 1, 3.4 * std::cos(th)}, {0, 1, 0}, 90.0, aspect, frame, out);
             return SNKPT_OK;
         }

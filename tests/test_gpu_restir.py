@@ -134,11 +134,15 @@ def _predicted_reuse(bvh, arrs, cam_prev, cam, w, h, mirror=False):
 
 
 def test_restir_orbit_reuses_reprojected_history(gpu):
-    """Config 3's seeded 64-frame camera orbit: every frame's temporal pass
-    accepts the history of the pixels the previous camera saw (count within
-    0.5 % of the CPU prediction, which a mirrored reprojection misses by far), and
-    the frames stay equal to the converged image of their own camera in
-    expectation (summed over the orbit) with less error than plain PT."""
+    """Config 3's seeded 64-frame camera orbit (an arc in front of the room's
+    open side, every frame looking into the lit room): every frame's temporal
+    pass accepts the history of the pixels the previous camera saw (count
+    within 0.5 % of the CPU prediction, which a mirrored reprojection misses by
+    far), and the frames stay equal to the converged image of their own camera
+    in expectation (summed over 47 frames of the orbit) with less error than
+    plain PT.  Single frames' sums scatter by ~3.5 % (pixels next to the 10k
+    emitters are heavy-tailed, in the oracle too), hence 47 frames and a
+    4096-spp oracle."""
     import torch
 
     import snkpt
@@ -147,7 +151,7 @@ def test_restir_orbit_reuses_reprojected_history(gpu):
     arrs = O.scene_arrays(scene)
     bvh = O.Bvh(O.flatten(*arrs[:4]))
     w, h = 64, 48
-    nf = 24
+    nf = 48
     out = torch.zeros(h, w, 4, device="cuda")
     num = den = 0.0
     tiles_d = np.zeros((6, 8, 3))
@@ -171,7 +175,7 @@ def test_restir_orbit_reuses_reprojected_history(gpu):
         assert abs(reused - want) <= 0.005 * valid + 2, (f, reused, want)
         miss_mirror += want - wrong
         n_valid += valid
-        ref = _oracle(ref_ctx, cam, w, h, 1024, 1)
+        ref = _oracle(ref_ctx, cam, w, h, 4096, 1)
         pt = _frames(ref_ctx, cam, w, h, 1, mode=snkpt.MODE_PT, max_depth=1, seed=77 + f)[0]
         num += img.sum()
         den += ref.sum()
