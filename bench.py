@@ -214,6 +214,7 @@ def frame_rates(stream, frames=10, warmup=3, only=None):
                 frame(nw + k)
                 b.record(stream)
             stream.synchronize()
+            st = ctx.frame_stats()  # the last timed frame's ray counts (before the frames below)
             # the last timed frame's error against the converged FP64 oracle of its camera
             # (plain PT of the same depth: the cache's and the compat estimator's bias shows)
             last = nw + frames - 1
@@ -259,7 +260,6 @@ def frame_rates(stream, frames=10, warmup=3, only=None):
                 gms = sum(a.elapsed_time(b) for a, b in gev) / frames
                 del g
         ms = sum(a.elapsed_time(b) for a, b in evs) / frames
-        st = ctx.frame_stats()
         key = f"c{cfg}_{label}"
         res[key] = {"fps": round(1e3 / ms, 2), "ms_per_frame": round(ms, 3), "res": f"{w}x{h}",
                     "depth": scene.s.max_depth, "rays_per_frame": int(st.camera_rays + st.extension_rays +

@@ -105,10 +105,15 @@ snkpt_status build_emitters(snkpt_ctx* ctx, hipStream_t st) {
         for (uint32_t l : large) prob[l] = 1.0f, alias[l] = l;
         for (uint32_t s : small) prob[s] = 1.0f, alias[s] = s;
     }
-    std::vector<uint32_t> alias_packed(2 * (size_t)(ne ? ne : 1));
-    for (uint32_t i = 0; i < (ne ? ne : 1); ++i) {
-        std::memcpy(&alias_packed[2 * (size_t)i], &prob[i], 4);
-        alias_packed[2 * (size_t)i + 1] = alias[i];
+    std::vector<float> rec(16 * (size_t)(ne ? ne : 1), 0.0f);
+    for (uint32_t i = 0; i < ne; ++i) {
+        float* r = &rec[16 * (size_t)i];
+        for (int k = 0; k < 3; ++k)
+            for (int a = 0; a < 3; ++a) r[4 * k + a] = tri[12 * (size_t)i + 4 * k + a];
+        r[3] = prob[i];
+        std::memcpy(&r[7], &alias[i], 4);
+        r[11] = pdf[i];
+        for (int a = 0; a < 4; ++a) r[12 + a] = info[4 * (size_t)i + a];
     }
     auto up = [&](DeviceBuffer& b, const void* src, size_t bytes) -> snkpt_status {
         hipError_t e = b.reserve(bytes ? bytes : 16);
@@ -186,10 +191,7 @@ snkpt_status build_emitters(snkpt_ctx* ctx, hipStream_t st) {
     E.q_tri = ne == 0 ? (E.n_analytic ? 0.0f : 1.0f) : (E.n_analytic ? 0.5f : 1.0f);
     snkpt_status s;
     if ((s = up(E.analytic, al.data(), al.size() * 4)) != SNKPT_OK) return s;
-    if ((s = up(E.tri, tri.data(), tri.size() * 4)) != SNKPT_OK) return s;
-    if ((s = up(E.info, info.data(), info.size() * 4)) != SNKPT_OK) return s;
-    if ((s = up(E.alias, alias_packed.data(), alias_packed.size() * 4)) != SNKPT_OK) return s;
-    if ((s = up(E.pdf, pdf.data(), pdf.size() * 4)) != SNKPT_OK) return s;
+    if ((s = up(E.rec, rec.data(), rec.size() * 4)) != SNKPT_OK) return s;
     if ((s = up(E.gid_to_emitter, gid_to_e.data(), gid_to_e.size() * 4)) != SNKPT_OK) return s;
     if ((s = up(E.ref_pl, rp.data(), rp.size() * 4)) != SNKPT_OK) return s;
     if ((s = up(E.em_list, list.data(), list.size() * 4)) != SNKPT_OK) return s;

@@ -325,10 +325,7 @@ SceneDev scene_dev(snkpt_ctx* ctx) {
     S.idx = ctx->d_idx.as<uint32_t>();
     S.nrm = ctx->d_nrm.as<float>();
     S.world = ctx->accel.world_tris.as<float4>();
-    S.emit_tri = ctx->emit.tri.as<float4>();
-    S.emit_info = ctx->emit.info.as<float4>();
-    S.emit_alias = ctx->emit.alias.as<uint2>();
-    S.emit_pdf = ctx->emit.pdf.as<float>();
+    S.emit_rec = ctx->emit.rec.as<float4>();
     S.gid2e = ctx->emit.gid_to_emitter.as<uint32_t>();
     S.n_emit = ctx->emit.count;
     S.nodes = ctx->accel.nodes.as<uint4>();

@@ -25,10 +25,9 @@ struct SceneDev {
     const uint32_t* idx;
     const float* nrm;
     const float4* world;  // 3 float4 per global triangle
-    const float4* emit_tri;
-    const float4* emit_info;  // emission rgb, area
-    const uint2* emit_alias;  // prob bits, alias
-    const float* emit_pdf;
+    // 4 float4 per emitter: v0 | alias probability bits, v1 | alias index bits,
+    // v2 | selection probability, emission rgb | area
+    const float4* emit_rec;
     const uint32_t* gid2e;
     uint32_t n_emit;
     const uint4* nodes;
@@ -120,6 +119,26 @@ struct LightSample {
     bool delta;       // analytic light: BRDF sampling cannot hit it (no MIS)
 };
 
+// Record of the emitter the alias table picks for uniform index e and draw u:
+// e's own record, or its alias's when u >= e's alias probability.
+__device__ __forceinline__ uint32_t emitter_record(const SceneDev& S, uint32_t e, float u, float4& a, float4& b,
+                                                   float4& c, float4& info) {
+    const float4* r = S.emit_rec + 4 * (size_t)e;
+    a = r[0];
+    b = r[1];
+    c = r[2];
+    info = r[3];
+    if (u >= a.w) {
+        e = __float_as_uint(b.w);
+        r = S.emit_rec + 4 * (size_t)e;
+        a = r[0];
+        b = r[1];
+        c = r[2];
+        info = r[3];
+    }
+    return e;
+}
+
 // Power-proportional emitter choice (alias table), uniform point on it.
 __device__ __forceinline__ LightSample sample_light(const SceneDev& S, V3 p, float u0, float u1, float u2,
                                                     float u3) {
@@ -129,10 +148,8 @@ __device__ __forceinline__ LightSample sample_light(const SceneDev& S, V3 p, flo
     if (S.n_emit == 0) return ls;
     uint32_t e = (uint32_t)(u0 * (float)S.n_emit);
     if (e >= S.n_emit) e = S.n_emit - 1;
-    uint2 al = S.emit_alias[e];
-    if (u1 >= __uint_as_float(al.x)) e = al.y;
-    const float4* t = S.emit_tri + 3 * (size_t)e;
-    float4 a = t[0], b = t[1], c = t[2];
+    float4 a, b, c, info;
+    emitter_record(S, e, u1, a, b, c, info);
     float su = sqrtf(u2);
     float w1 = su * (1.0f - u3), w2 = su * u3, w0 = 1.0f - su;
     V3 A{a.x, a.y, a.z}, B{b.x, b.y, b.z}, C{c.x, c.y, c.z};
@@ -147,9 +164,8 @@ __device__ __forceinline__ LightSample sample_light(const SceneDev& S, V3 p, flo
     if (!(nll > 0.0f)) return ls;
     float cosl = fabsf(dot(nl, ls.wi)) / nll;  // two-sided emitters (RayClosestHit.rchit:119-123)
     if (cosl <= 0.0f) return ls;
-    float4 info = S.emit_info[e];
     ls.le = V3{info.x, info.y, info.z};
-    ls.pdf = S.emit_pdf[e] * d2 / (info.w * cosl);
+    ls.pdf = c.w * d2 / (info.w * cosl);
     ls.dist = dist;
     ls.ok = ls.pdf > 0.0f;
     return ls;
@@ -231,7 +247,8 @@ __device__ __forceinline__ float light_pdf(const SceneDev& S, uint32_t gid, V3 n
     if (e == 0xFFFFFFFFu) return 0.0f;
     float cosl = fabsf(dot(ng_hit, dir));
     if (cosl <= 0.0f) return 0.0f;
-    return S.q_tri * (S.emit_pdf[e] * t * t / (S.emit_info[e].w * cosl));
+    const float4* r = S.emit_rec + 4 * (size_t)e;
+    return S.q_tri * (r[2].w * t * t / (r[3].w * cosl));
 }
 
 // Wave-level queue append: every lane calls it (convergent); lanes with

@@ -80,11 +80,11 @@ struct Res {
     float b1, b2, W, M, wsum;
 };
 
-// Unshadowed contribution of emitter sample (e, b1, b2) at surface g.
-__device__ __forceinline__ float target(const SceneDev& S, const GSurf& g, uint32_t e, float b1, float b2, V3* c_out,
-                                        V3* wi_out, float* dist_out) {
-    const float4* t = S.emit_tri + 3 * (size_t)e;
-    float4 a = t[0], b = t[1], c = t[2];
+// Unshadowed contribution of emitter sample (e, b1, b2) at surface g, from the
+// emitter's record a | b | c | info (4 float4).
+__device__ __forceinline__ float target_rec(const GSurf& g, const float4& a, const float4& b, const float4& c,
+                                            const float4& info, float b1, float b2, V3* c_out, V3* wi_out,
+                                            float* dist_out) {
     V3 A{a.x, a.y, a.z}, B{b.x, b.y, b.z}, C{c.x, c.y, c.z};
     float b0 = 1.0f - b1 - b2;
     V3 q = A * b0 + B * b1 + C * b2;
@@ -99,13 +99,17 @@ __device__ __forceinline__ float target(const SceneDev& S, const GSurf& g, uint3
     float nll = sqrtf(dot(nl, nl));
     if (!(nll > 0.0f)) return 0.0f;
     float cosl = fabsf(dot(nl, wi)) / nll;
-    float4 info = S.emit_info[e];
     V3 f = brdf_eval(g.m, g.ns, g.wo, wi);
     V3 cc = mul(f, V3{info.x, info.y, info.z}) * (cs * cosl / d2);
     if (c_out) *c_out = cc;
     if (wi_out) *wi_out = wi;
     if (dist_out) *dist_out = dist;
     return luminance(cc);
+}
+__device__ __forceinline__ float target(const SceneDev& S, const GSurf& g, uint32_t e, float b1, float b2, V3* c_out,
+                                        V3* wi_out, float* dist_out) {
+    const float4* r = S.emit_rec + 4 * (size_t)e;
+    return target_rec(g, r[0], r[1], r[2], r[3], b1, b2, c_out, wi_out, dist_out);
 }
 
 // Unshadowed contribution of analytic light k (point, spot or directional) at
@@ -237,12 +241,12 @@ __global__ __launch_bounds__(kBlock) void k_primary(SceneDev S, GView G, Queue i
                         float uw = rr.uniform();
                         uint32_t e = (uint32_t)(u0 * (float)S.n_emit);
                         if (e >= S.n_emit) e = S.n_emit - 1;
-                        uint2 al = S.emit_alias[e];
-                        if (u1 >= __uint_as_float(al.x)) e = al.y;
+                        float4 ra, rb, rc, ri;
+                        e = emitter_record(S, e, u1, ra, rb, rc, ri);
                         float su = sqrtf(u2);
                         float b1 = su * (1.0f - u3), b2 = su * u3;
-                        float p = target(S, g, e, b1, b2, nullptr, nullptr, nullptr);
-                        float src = S.emit_pdf[e] / S.emit_info[e].w;  // area measure
+                        float p = target_rec(g, ra, rb, rc, ri, b1, b2, nullptr, nullptr, nullptr);
+                        float src = rc.w / ri.w;  // area measure
                         if (res_update(r, e, b1, b2, p / src, uw)) ph = p;
                     }
                 }

@@ -82,10 +82,11 @@ struct Accel {
 struct Emitters {
     uint32_t count = 0;
     double total_power = 0;
-    DeviceBuffer tri;    // float4 x 3 per emitter: v0 | v1 | v2 (w unused), world space
-    DeviceBuffer info;   // float4: emission rgb, area
-    DeviceBuffer alias;  // alias table: float prob, u32 alias
-    DeviceBuffer pdf;    // per emitter: selection probability
+    // one 64 B record per emitter (4 float4, world space): v0 | alias probability
+    // bits, v1 | alias index bits, v2 | selection probability, emission rgb | area.
+    // A ReSTIR candidate or NEE sample reads the record its uniform draw lands on
+    // and, when the alias redirects, the alias's record: one or two 64 B reads.
+    DeviceBuffer rec;
     DeviceBuffer gid_to_emitter;  // per global triangle: emitter index or ~0u
     DeviceBuffer analytic;        // directional / point / spot lights, 4 float4 each
     uint32_t n_analytic = 0;
