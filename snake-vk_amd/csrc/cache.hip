@@ -283,6 +283,17 @@ __device__ __forceinline__ void load_x0_tile(const _Float16* __restrict__ x0, ui
 
 constexpr int kInferWaves = 4;
 
+// ReLU of FP16 values as a signed 16-bit max with 0 (a negative half has its
+// sign bit set, so it reads as a negative integer), two halves per instruction:
+// equal to rounding max(x, 0) to FP16 (-0 becomes +0).
+typedef _Float16 half4v_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ half4v_t relu_h4(half4v_t h) {
+    typedef short short4_t __attribute__((ext_vector_type(4)));
+    short4_t v = __builtin_bit_cast(short4_t, h);
+    v = v > (short)0 ? v : (short)0;
+    return __builtin_bit_cast(half4v_t, v);
+}
+
 // Hidden layer: out = relu(in * W^T), 64 samples x 64 outputs, computed as
 // out^T = W * in^T (A = weight rows, B = activation rows): a lane's four
 // accumulators are four consecutive outputs of one sample and leave as one 8 B
@@ -315,9 +326,7 @@ __device__ __forceinline__ void layer_hidden(const _Float16* in, const _Float16*
     for (int n = 0; n < 4; ++n)
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
-            half4_t o;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) o[e] = (_Float16)fmaxf(acc[n][m][e], 0.0f);
+            const half4_t o = relu_h4(__builtin_convertvector(acc[n][m], half4_t));
             const int sample = m * 16 + (lane & 15), col = n * 16 + 4 * (lane >> 4);
             *(half4_t*)(out + sample * kRow + col) = o;
         }
