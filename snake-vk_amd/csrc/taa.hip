@@ -29,6 +29,7 @@ void taa_state_free(TaaState* t) { delete t; }
 namespace {
 
 constexpr int kTile = 16;
+constexpr uint32_t kXcds = 8;  // MI355X: 8 accelerator dies, each with its own L2
 
 __device__ __forceinline__ float h2f(uint16_t b) {
     _Float16 x;
@@ -127,8 +128,16 @@ __global__ __launch_bounds__(kTile* kTile) void k_taa_resolve(const void* __rest
                                                                uint32_t* __restrict__ pvel_out, uint32_t w,
                                                                uint32_t h, uint32_t reset) {
     __shared__ float tile[kTile + 2][kTile + 2][3];
+    // The hardware deals consecutive blocks to the XCDs in turn; tile t of the
+    // row-major tile order goes to block (t % per) * kXcds + t / per, so each XCD
+    // walks a contiguous run of tiles and the halo columns and bilinear taps a
+    // tile shares with its neighbours are still in that XCD's L2.
+    const uint32_t nbx = (w + kTile - 1) / kTile, ntiles = nbx * ((h + kTile - 1) / kTile);
+    const uint32_t per = (ntiles + kXcds - 1) / kXcds;
+    const uint32_t tile_id = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    if (tile_id >= ntiles) return;
     const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;
-    const int bx = blockIdx.x * kTile, by = blockIdx.y * kTile;
+    const int bx = (int)(tile_id % nbx) * kTile, by = (int)(tile_id / nbx) * kTile;
     // tile + halo, edge-clamped texel coordinates, colours max(c, 0)
     for (int k = threadIdx.x; k < (kTile + 2) * (kTile + 2); k += kTile * kTile) {
         const int lx = k % (kTile + 2), ly = k / (kTile + 2);
@@ -236,7 +245,8 @@ snkpt_status taa_resolve(snkpt_ctx* ctx, const void* current, void* out, const u
     }
     const uint32_t rs = (reset || !T.valid) ? 1u : 0u;
     const int nxt = T.cur ^ 1;
-    dim3 grid((w + kTile - 1) / kTile, (h + kTile - 1) / kTile);
+    const uint32_t ntiles = ((w + kTile - 1) / kTile) * ((h + kTile - 1) / kTile);
+    const uint32_t grid = ((ntiles + kXcds - 1) / kXcds) * kXcds;
     if (fmt == SNKPT_FORMAT_RGBA32F)
         k_taa_resolve<SNKPT_FORMAT_RGBA32F><<<grid, kTile * kTile, 0, st>>>(
             src, out, (const uint32_t*)velocity, T.hist[T.cur].ptr, T.vel[T.cur].as<uint32_t>(), T.hist[nxt].ptr,
