@@ -448,6 +448,24 @@ __global__ __launch_bounds__(kBlock) void k_temporal(SceneDev S, GView G, GView 
 constexpr uint32_t kSpatTile = 16;
 static_assert(kSpatTile * kSpatTile == kBlock, "one pixel per thread");
 
+// The hardware deals consecutive blocks to the 8 XCDs in turn, each with its own
+// L2.  Tile t of the row-major tile order runs as block (t % per) * 8 + t / per,
+// so each XCD walks a contiguous run of tiles and the neighbour rows and columns
+// that tiles near each other read stay in that XCD's L2.
+constexpr uint32_t kXcds = 8;
+inline uint32_t xcd_tile_blocks(uint32_t w, uint32_t h) {
+    const uint32_t n = ((w + kSpatTile - 1) / kSpatTile) * ((h + kSpatTile - 1) / kSpatTile);
+    return ((n + kXcds - 1) / kXcds) * kXcds;
+}
+__device__ __forceinline__ bool xcd_tile_origin(uint32_t w, uint32_t h, uint32_t& x0, uint32_t& y0) {
+    const uint32_t nbx = (w + kSpatTile - 1) / kSpatTile, n = nbx * ((h + kSpatTile - 1) / kSpatTile);
+    const uint32_t per = (n + kXcds - 1) / kXcds;
+    const uint32_t t = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    x0 = (t % nbx) * kSpatTile;
+    y0 = (t / nbx) * kSpatTile;
+    return t < n;
+}
+
 __device__ __forceinline__ bool neighbour_ok(const GView& G, uint32_t q, const GSurf& g) {
     // the reuse test needs the validity, shading normal and distance only (3 of 5 words)
     const float4 p = G.p[q], n = G.n[q], wv = G.w[q];
@@ -459,8 +477,10 @@ __global__ __launch_bounds__(kBlock) void k_spatial(SceneDev S, GView G, const f
                                                     const float* __restrict__ m_in, float4* __restrict__ r_out,
                                                     float* __restrict__ m_out, uint32_t w, uint32_t h, uint32_t k,
                                                     float radius, uint64_t initstate) {
-    const uint32_t x = blockIdx.x * kSpatTile + (threadIdx.x % kSpatTile);
-    const uint32_t y = blockIdx.y * kSpatTile + (threadIdx.x / kSpatTile);
+    uint32_t x0, y0;
+    if (!xcd_tile_origin(w, h, x0, y0)) return;
+    const uint32_t x = x0 + (threadIdx.x % kSpatTile);
+    const uint32_t y = y0 + (threadIdx.x / kSpatTile);
     if (x >= w || y >= h) return;
     const uint32_t i = y * w + x;
     Res r0 = load_res(r_in, m_in, i);
@@ -538,8 +558,10 @@ constexpr uint64_t kPhaseSpatA = 0x53504141ull;
 __global__ __launch_bounds__(kBlock) void k_spatial_analytic(SceneDev S, GView G, const float4* __restrict__ a_in,
                                                              float4* __restrict__ a_out, uint32_t w, uint32_t h,
                                                              uint32_t k, float radius, uint64_t initstate) {
-    const uint32_t x = blockIdx.x * kSpatTile + (threadIdx.x % kSpatTile);
-    const uint32_t y = blockIdx.y * kSpatTile + (threadIdx.x / kSpatTile);
+    uint32_t x0, y0;
+    if (!xcd_tile_origin(w, h, x0, y0)) return;
+    const uint32_t x = x0 + (threadIdx.x % kSpatTile);
+    const uint32_t y = y0 + (threadIdx.x / kSpatTile);
     if (x >= w || y >= h) return;
     const uint32_t i = y * w + x;
     const GSurf g = load_g(G, i);
@@ -1412,7 +1434,7 @@ snkpt_status restir_pass(snkpt_ctx* ctx, const snkpt_camera* cam, const CamDev& 
         m_sp_in = X.m_tmp.as<float>();
     }
     const uint32_t k = s->num_spatial_resamples > 0 ? (uint32_t)s->num_spatial_resamples : 0u;
-    k_spatial<<<dim3((w + kSpatTile - 1) / kSpatTile, (h + kSpatTile - 1) / kSpatTile), kBlock, 0, st>>>(
+    k_spatial<<<xcd_tile_blocks(w, h), kBlock, 0, st>>>(
         S, G, r_sp_in, m_sp_in, X.r_fin.as<float4>(),
                                                       X.m_fin.as<float>(), w, h, k,
                                                       s->restir_radius > 0.0f ? s->restir_radius : 30.0f, initstate);
@@ -1423,8 +1445,7 @@ snkpt_status restir_pass(snkpt_ctx* ctx, const snkpt_camera* cam, const CamDev& 
     if (S.n_alights > 0) {
         // the shadow queue is reused for the analytic lights of the primary hits
         k_restir_stats2<<<1, 64, 0, st>>>(counts, nullptr);
-        k_spatial_analytic<<<dim3((w + kSpatTile - 1) / kSpatTile, (h + kSpatTile - 1) / kSpatTile), kBlock, 0,
-                             st>>>(S, G, a_sp_in, X.a_fin.as<float4>(), w, h, k,
+        k_spatial_analytic<<<xcd_tile_blocks(w, h), kBlock, 0, st>>>(S, G, a_sp_in, X.a_fin.as<float4>(), w, h, k,
                                    s->restir_radius > 0.0f ? s->restir_radius : 30.0f, initstate);
         k_rshade_analytic<<<grid_for(n, kBlock), kBlock, 0, st>>>(S, G, X.a_fin.as<float4>(), shadow_of(R), counts,
                                                                   n);
