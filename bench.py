@@ -13,6 +13,7 @@ Prints one JSON line.
 from __future__ import annotations
 
 import argparse
+import math
 import json
 import os
 import sys
@@ -199,14 +200,18 @@ def frame_rates(stream, frames=10, warmup=3, only=None):
             def frame(f):
                 ctx.render(frame_cam(f), settings(f), out, w, h, stream=stream)
             # warm-up: at least `warmup` frames and ~30 ms of frames (the clocks leave
-            # the idle state the scene set-up let them drop to); config 3's frames
-            # stay within its 64-frame orbit
-            t_warm, nw = time.perf_counter(), 0
-            while nw < warmup or (time.perf_counter() - t_warm < 0.03 and nw < 64 - frames - 4):
-                frame(nw)
-                nw += 1
-                if nw >= warmup:
-                    stream.synchronize()
+            # the idle state the scene set-up let them drop to): the second frame is
+            # timed and the count rounded up to a power of two, so runs agree on it
+            # (config 3's frames stay within its 64-frame orbit)
+            frame(0)
+            stream.synchronize()
+            t1 = time.perf_counter()
+            frame(1)
+            stream.synchronize()
+            f_ms = max(1e-3, (time.perf_counter() - t1) * 1e3)
+            nw = max(warmup, min(1 << max(1, math.ceil(math.log2(30.0 / f_ms))), 64 - frames - 4))
+            for f in range(2, nw):
+                frame(f)
             evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                    for _ in range(frames)]
             for k, (a, b) in enumerate(evs):
@@ -219,7 +224,7 @@ def frame_rates(stream, frames=10, warmup=3, only=None):
             # (plain PT of the same depth: the cache's and the compat estimator's bias shows)
             last = nw + frames - 1
             okey = (cfg, last if cfg == 3 else 0)
-            spp = 1024 if w * h <= 65536 else 256
+            spp = 1024
             if okey not in oracles:
                 acc = torch.zeros(h * w * 3, dtype=torch.float64, device="cuda")
                 os_ = snkpt.default_settings(max_depth=scene.s.max_depth, rr_start=scene.s.rr_start,
@@ -230,7 +235,9 @@ def frame_rates(stream, frames=10, warmup=3, only=None):
                 oracles[okey] = (acc / spp).view(h, w, 3)
             ref = oracles[okey]
             err = ((out[..., :3].double() - ref) ** 2).mean().sqrt()
-            frame_err = {"oracle_spp": spp, "frame_rel_rmse": round(float(err / ref.mean()), 4)}
+            mae = (out[..., :3].double() - ref).abs().mean()
+            frame_err = {"oracle_spp": spp, "frame_rel_rmse": round(float(err / ref.mean()), 4),
+                         "frame_rel_mae": round(float(mae / ref.mean()), 4)}
             if mode != snkpt.MODE_PT:
                 # a plain PT frame of the same camera beside it (1 spp, same depth)
                 pt_out = torch.zeros_like(out)
@@ -239,6 +246,7 @@ def frame_rates(stream, frames=10, warmup=3, only=None):
                 ctx.render(frame_cam(last), ps, pt_out, w, h, stream=stream)
                 perr = ((pt_out[..., :3].double() - ref) ** 2).mean().sqrt()
                 frame_err["pt_frame_rel_rmse"] = round(float(perr / ref.mean()), 4)
+                frame_err["pt_frame_rel_mae"] = round(float((pt_out[..., :3].double() - ref).abs().mean() / ref.mean()), 4)
             gms = None
             if label == "pt":
                 # the same frame recorded into a HIP graph (no end-of-frame readback)
