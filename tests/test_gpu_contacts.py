@@ -73,6 +73,42 @@ def test_contacts_match_oracle_any_hit_and_restatement(gpu):
     assert not nw[~hit].any() and not vel[~hit].any() and (inst.view(np.uint32)[~hit] == 0xFFFFFFFF).all()
 
 
+def test_contacts_on_the_deep_bvh_match_oracle(gpu):
+    """The 1.02M-triangle scene (BVH8 depth 12: the 12-entry stack variant, held
+    to 80 VGPRs): particle rays over its volume, first hit bit for bit against the
+    oracle's any-hit traversal, contact record against the restatement."""
+    import snkpt
+    scene = snkpt.SceneData(2)
+    ctx = snkpt.Context(0)
+    ctx.load_scene(scene)
+    ctx.build_accel()
+    assert 11 <= ctx.accel_info().depth <= 13
+    arrs = O.scene_arrays(scene)
+    b = O.Bvh(O.flatten(arrs[0], arrs[1], arrs[2], arrs[3]))
+    n = 200000
+    rng = np.random.default_rng(11)
+    p = np.stack([rng.uniform(-50, 50, n), rng.uniform(0.0, 12.0, n), rng.uniform(-50, 50, n)], 1)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = rng.uniform(0.01, 0.08, n)
+    rays = np.zeros((n, 8), F)
+    rays[:, :3] = p - d * r[:, None]
+    rays[:, 4:7] = d
+    rays[:, 7] = rng.uniform(0.0, 30.0, n) / 120.0 + 2 * r
+    dt = 1.0 / 120.0
+    t, nw, cp, vel, inst, prim = _trace(ctx, rays, dt)
+    ct, ci, cpr, cu, cv, _, _ = b.trace(rays, any_hit=True)
+    hit = ct >= 0
+    _say(f"deep BVH: {int(hit.sum())} of {n} collision rays hit")
+    assert hit.sum() > 1000
+    assert np.array_equal(t.view(np.uint32), ct.view(np.uint32))
+    assert np.array_equal(inst.view(np.uint32), ci) and np.array_equal(prim.view(np.uint32), cpr)
+    kn, kp, kv = K.contacts(arrs, arrs[3], rays, ct, ci, cpr, cu, cv, dt)
+    assert np.allclose(nw[:, :3], kn, atol=2e-6)
+    assert np.allclose(cp[:, :3], kp, atol=1e-4)  # world coordinates up to 50 m
+    assert not nw[~hit].any() and not vel[~hit].any()
+
+
 def test_moving_instance_velocity(gpu):
     """The previous frame had instance k's transform 0.02 lower along y: its
     contacts move at (0, 0.02 / dt, 0), everything else is at rest."""
