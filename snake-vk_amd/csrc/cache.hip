@@ -715,6 +715,12 @@ __global__ void k_grid_adam(unsigned long long* __restrict__ grad, float2* __res
     const uint32_t l = (uint32_t)(e >> kGridLog2T);
     float g0, g1;
     if (G.dense[l]) {
+        // a dense level uses its first (res + 1)^3 slots only
+        const uint32_t side = G.res[l] + 1u;
+        if ((uint32_t)(e & (kGridT - 1u)) >= side * side * side) {
+            if (gout) gout[e] = make_float2(0.0f, 0.0f);
+            return;
+        }
         const long long a = (long long)grad[2 * e], b = (long long)grad[2 * e + 1];
         if ((a | b) == 0 && !gout) return;
         if (a) grad[2 * e] = 0ull;
