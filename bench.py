@@ -755,18 +755,20 @@ def main():
         step()
     stream.synchronize()
     torch.cuda.synchronize()
-    # timed region: K steps, per-launch events on the launch stream
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    # timed region: K steps back to back, HIP events on the launch stream at its two
+    # ends (an event pair around every launch would add its own gap to each step);
+    # the average launch duration includes the gaps between launches
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for a, b in evs:
-        a.record(stream)
+    ev0.record(stream)
+    for _ in range(args.steps):
         step()
-        b.record(stream)
+    ev1.record(stream)
     stream.synchronize()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    launch_ms = sum(a.elapsed_time(b) for a, b in evs) / args.steps
+    launch_ms = ev0.elapsed_time(ev1) / args.steps
     ms_per_step = wall * 1e3 / args.steps
     value = n / (ms_per_step * 1e-3) / 1e6
     secondary_hit = float((t >= 0).float().mean())
