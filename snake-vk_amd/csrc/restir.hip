@@ -473,6 +473,7 @@ __device__ __forceinline__ bool neighbour_ok(const GView& G, uint32_t q, const G
     return p.w > 0.0f && !(dot(ns, g.ns) < 0.9f) && !(fabsf(wv.w - g.t) > 0.1f * g.t);
 }
 
+template <uint32_t KMAX>
 __global__ __launch_bounds__(kBlock) void k_spatial(SceneDev S, GView G, const float4* __restrict__ r_in,
                                                     const float* __restrict__ m_in, float4* __restrict__ r_out,
                                                     float* __restrict__ m_out, uint32_t w, uint32_t h, uint32_t k,
@@ -491,10 +492,10 @@ __global__ __launch_bounds__(kBlock) void k_spatial(SceneDev S, GView G, const f
     }
     Rng rr = rng_seed(initstate ^ kPhaseSpat, i);
     // inputs: this pixel's reservoir and up to k accepted neighbours
-    uint32_t px[kMaxSpatial + 1];
+    uint32_t px[KMAX + 1];
     px[0] = i;
     int ni = 1;
-    if (k > kMaxSpatial) k = kMaxSpatial;
+    if (k > KMAX) k = KMAX;
     for (uint32_t j = 0; j < k; ++j) {
         float ur = rr.uniform(), ua = rr.uniform();
         float rad = radius * sqrtf(ur), ang = 2.0f * kPi * ua;
@@ -505,11 +506,11 @@ __global__ __launch_bounds__(kBlock) void k_spatial(SceneDev S, GView G, const f
         px[ni++] = q;
     }
     // the inputs' samples and their uniforms (drawn in input order)
-    float Ms[kMaxSpatial + 1], uw[kMaxSpatial + 1], Wl[kMaxSpatial + 1], b1[kMaxSpatial + 1], b2[kMaxSpatial + 1];
-    uint32_t el[kMaxSpatial + 1];
-    float den[kMaxSpatial + 1], p_self[kMaxSpatial + 1], p_l[kMaxSpatial + 1];
+    float Ms[KMAX + 1], uw[KMAX + 1], Wl[KMAX + 1], b1[KMAX + 1], b2[KMAX + 1];
+    uint32_t el[KMAX + 1];
+    float den[KMAX + 1], p_self[KMAX + 1], p_l[KMAX + 1];
 #pragma unroll
-    for (int l = 0; l < (int)kMaxSpatial + 1; ++l) {
+    for (int l = 0; l < (int)KMAX + 1; ++l) {
         if (l >= ni) break;
         const Res rl = load_res(r_in, m_in, px[l]);
         Ms[l] = rl.M;
@@ -522,11 +523,11 @@ __global__ __launch_bounds__(kBlock) void k_spatial(SceneDev S, GView G, const f
     }
     // balance heuristic over the inputs' targets, each on its own surface
 #pragma unroll
-    for (int j = 0; j < (int)kMaxSpatial + 1; ++j) {
+    for (int j = 0; j < (int)KMAX + 1; ++j) {
         if (j >= ni) break;
         const GSurf gj = j == 0 ? g : load_g(G, px[j]);
 #pragma unroll
-        for (int l = 0; l < (int)kMaxSpatial + 1; ++l) {
+        for (int l = 0; l < (int)KMAX + 1; ++l) {
             if (l >= ni) break;
             if (el[l] == 0xFFFFFFFFu) continue;
             const float pj = target(S, gj, el[l], b1[l], b2[l], nullptr, nullptr, nullptr);
@@ -539,7 +540,7 @@ __global__ __launch_bounds__(kBlock) void k_spatial(SceneDev S, GView G, const f
     float ph = 0.0f;
     s.M = 0.0f;
 #pragma unroll
-    for (int l = 0; l < (int)kMaxSpatial + 1; ++l) {
+    for (int l = 0; l < (int)KMAX + 1; ++l) {
         if (l >= ni) break;
         s.M += Ms[l];
         if (el[l] == 0xFFFFFFFFu) continue;
@@ -1434,10 +1435,16 @@ snkpt_status restir_pass(snkpt_ctx* ctx, const snkpt_camera* cam, const CamDev& 
         m_sp_in = X.m_tmp.as<float>();
     }
     const uint32_t k = s->num_spatial_resamples > 0 ? (uint32_t)s->num_spatial_resamples : 0u;
-    k_spatial<<<xcd_tile_blocks(w, h), kBlock, 0, st>>>(
-        S, G, r_sp_in, m_sp_in, X.r_fin.as<float4>(),
-                                                      X.m_fin.as<float>(), w, h, k,
-                                                      s->restir_radius > 0.0f ? s->restir_radius : 30.0f, initstate);
+    // the register arrays hold one slot per input: the 5-neighbour variant (the
+    // reference's configurations) keeps fewer live than the 8-neighbour one
+    if (k <= 5)
+        k_spatial<5><<<xcd_tile_blocks(w, h), kBlock, 0, st>>>(
+            S, G, r_sp_in, m_sp_in, X.r_fin.as<float4>(), X.m_fin.as<float>(), w, h, k,
+            s->restir_radius > 0.0f ? s->restir_radius : 30.0f, initstate);
+    else
+        k_spatial<kMaxSpatial><<<xcd_tile_blocks(w, h), kBlock, 0, st>>>(
+            S, G, r_sp_in, m_sp_in, X.r_fin.as<float4>(), X.m_fin.as<float>(), w, h, k,
+            s->restir_radius > 0.0f ? s->restir_radius : 30.0f, initstate);
     k_rshade<<<grid_for(n, kBlock), kBlock, 0, st>>>(S, G, X.r_fin.as<float4>(), X.m_fin.as<float>(), shadow_of(R),
                                                      counts, n, stats);
     if ((e = launch_shadow(ctx, R, R.rad.as<float4>(), st)) != SNKPT_OK) return e;

@@ -50,7 +50,7 @@ def _frames(ctx, cam, w, h, nframes, **kw):
     return np.stack(imgs)
 
 
-@pytest.mark.parametrize("temporal,spatial", [(0, 0), (1, 0), (1, 5)])
+@pytest.mark.parametrize("temporal,spatial", [(0, 0), (1, 0), (1, 5), (1, 8)])
 def test_restir_equals_pt_in_expectation(gpu, temporal, spatial):
     import snkpt
     scene, ctx = _setup(3)
