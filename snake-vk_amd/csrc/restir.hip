@@ -235,20 +235,61 @@ __global__ __launch_bounds__(kBlock) void k_primary(SceneDev S, GView G, Queue i
                 g.eps = s.eps;
                 Rng rr = rng_seed(initstate ^ kPhaseCand, i);
                 float ph = 0.0f;
-                if (S.n_emit > 0) {
+                if (S.n_emit > 0 && n_cand > 0) {
+                    // Candidate c + 1's uniforms and the record its uniform draw lands on
+                    // are fetched while candidate c is weighed (the same draws in the
+                    // same order), so the record's L2 latency overlaps the work.
+                    auto draw = [&](float& a0, float& a1, float& a2, float& a3, float& aw, uint32_t& ei) {
+                        a0 = rr.uniform();
+                        a1 = rr.uniform();
+                        a2 = rr.uniform();
+                        a3 = rr.uniform();
+                        aw = rr.uniform();
+                        ei = (uint32_t)(a0 * (float)S.n_emit);
+                        if (ei >= S.n_emit) ei = S.n_emit - 1;
+                    };
+                    float u0, u1, u2, u3, uw;
+                    uint32_t e;
+                    draw(u0, u1, u2, u3, uw, e);
+                    const float4* rp = S.emit_rec + 4 * (size_t)e;
+                    float4 ra = rp[0], rb = rp[1], rc = rp[2], ri = rp[3];
                     for (uint32_t c = 0; c < n_cand; ++c) {
-                        float u0 = rr.uniform(), u1 = rr.uniform(), u2 = rr.uniform(), u3 = rr.uniform();
-                        float uw = rr.uniform();
-                        uint32_t e = (uint32_t)(u0 * (float)S.n_emit);
-                        if (e >= S.n_emit) e = S.n_emit - 1;
-                        float4 ra, rb, rc, ri;
-                        e = emitter_record(S, e, u1, ra, rb, rc, ri);
+                        float v0 = 0.0f, v1 = 0.0f, v2 = 0.0f, v3 = 0.0f, vw = 0.0f;
+                        uint32_t en = 0;
+                        float4 na = ra, nb = rb, nc = rc, ni = ri;
+                        if (c + 1 < n_cand) {
+                            draw(v0, v1, v2, v3, vw, en);
+                            const float4* np = S.emit_rec + 4 * (size_t)en;
+                            na = np[0];
+                            nb = np[1];
+                            nc = np[2];
+                            ni = np[3];
+                        }
+                        if (u1 >= ra.w) {  // the alias's record
+                            e = __float_as_uint(rb.w);
+                            const float4* ap = S.emit_rec + 4 * (size_t)e;
+                            ra = ap[0];
+                            rb = ap[1];
+                            rc = ap[2];
+                            ri = ap[3];
+                        }
                         float su = sqrtf(u2);
                         float b1 = su * (1.0f - u3), b2 = su * u3;
                         float p = target_rec(g, ra, rb, rc, ri, b1, b2, nullptr, nullptr, nullptr);
                         float src = rc.w / ri.w;  // area measure
                         if (res_update(r, e, b1, b2, p / src, uw)) ph = p;
+                        u0 = v0;
+                        u1 = v1;
+                        u2 = v2;
+                        u3 = v3;
+                        uw = vw;
+                        e = en;
+                        ra = na;
+                        rb = nb;
+                        rc = nc;
+                        ri = ni;
                     }
+                    (void)u0;
                 }
                 r.M = (float)n_cand;
                 r.W = ph > 0.0f ? r.wsum / (r.M * ph) : 0.0f;
