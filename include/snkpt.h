@@ -161,7 +161,12 @@ typedef struct snkpt_settings {
     float accum_weight;             /* SNKPT_FLAG_ACCUMULATE (snkpt_render): weight of this frame's samples in the
                                        running mean (0 = 1); rising weights let a still-learning cache's early frames
                                        fade out of the mean */
-    uint32_t reserved[2];
+    uint32_t tail_bounce;           /* SNKPT_MODE_PT (and the path-traced bounces of ReSTIR and G-buffer frames):
+                                       bounce from which the paths still alive run one thread per path instead of one
+                                       launch per bounce; same radiance bit for bit.  0 = chosen from the context's
+                                       previous frame (the first bounce whose queue of paths fits twice the resident
+                                       lanes); >= max_depth = never */
+    uint32_t reserved[1];
 } snkpt_settings;
 
 /* Ray and hit streams for snkpt_trace: structure-of-arrays, device pointers.

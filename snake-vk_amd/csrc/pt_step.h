@@ -193,6 +193,40 @@ __device__ __forceinline__ void shade_step(const SS& ss, const PtParams& P, cons
     vertex_appends<KIND, BLOCK>(o, path, bounce, rng, out, sq, ps, counts, nxt, H, lds_app);
 }
 
+// The rest of one path, traced in place: bounces bounce0..max_depth-1 from the
+// ray r of bounce bounce0, with the path's throughput and random stream at that
+// bounce.  Every contribution is added to L in the wavefront's order (emission,
+// then the unoccluded shadow ray, vertex by vertex).  n_ext counts the rays after
+// the first, n_sh the shadow rays.
+template <int KIND, int STACK>
+__device__ __forceinline__ float4 path_rest(const SceneDev& S, const PtParams& P, const ShadeHooks& H, uint32_t path,
+                                            uint32_t bounce0, Ray r, float4 thr4, Rng rng, uint2* stk, int sstride,
+                                            float4 L, uint32_t& n_ext, uint32_t& n_sh) {
+    uint32_t nv = 0, tv = 0;
+    for (uint32_t bounce = bounce0; bounce < P.max_depth; ++bounce) {
+        if (bounce > bounce0) ++n_ext;
+        TraceResult res = traverse<false, false, STACK>(S.nodes, S.leaf, r, stk, sstride, nv, tv);
+        if (res.rec == 0xFFFFFFFFu) break;
+        const float4 r2 = S.leaf[(size_t)res.rec * kTriWords + 2];
+        float hu, hv;
+        hit_uv(S.leaf, res.rec, r, hu, hv);
+        VertexOut o;
+        path_vertex<KIND>(S, P, H, path, bounce, V3{r.dx, r.dy, r.dz}, res.t, __float_as_uint(r2.y),
+                          __float_as_uint(r2.z), hu, hv, thr4, rng, L, o);
+        if (o.emit) L = make_float4(L.x + o.c_emit.x, L.y + o.c_emit.y, L.z + o.c_emit.z, L.w);
+        if (o.sh) {
+            ++n_sh;
+            Ray sr{o.sh_o.x, o.sh_o.y, o.sh_o.z, 0.0f, o.sh_d.x, o.sh_d.y, o.sh_d.z, o.sh_t};
+            TraceResult vis = traverse<true, false, STACK>(S.nodes, S.leaf, sr, stk, sstride, nv, tv);
+            if (vis.rec == 0xFFFFFFFFu) L = make_float4(L.x + o.sh_c.x, L.y + o.sh_c.y, L.z + o.sh_c.z, L.w);
+        }
+        if (!o.ext) break;
+        r = Ray{o.ext_o.x, o.ext_o.y, o.ext_o.z, 0.0f, o.ext_d.x, o.ext_d.y, o.ext_d.z, 1e30f};
+        thr4 = o.thr4;
+    }
+    return L;
+}
+
 // A whole path per thread (no queues): camera ray of path i through pixel
 // (start + i * stride) mod (w h), closest hits and shadow rays traced in place.
 // Same vertex logic, random stream and order of radiance additions as the
@@ -207,32 +241,9 @@ __device__ __forceinline__ float4 whole_path(const SceneDev& S, const CamDev& ca
     const uint32_t px = (uint32_t)(((uint64_t)start + (uint64_t)i * stride) % ((uint64_t)w * h));
     if (KIND == 2) H.nrec[i] = 0;
     const V3 d0 = camera_dir(cam, px % w, px / w, w, h);
-    Ray r{cam.pos.x, cam.pos.y, cam.pos.z, 0.0f, d0.x, d0.y, d0.z, 1e30f};
-    Rng rng = rng_seed(initstate, i);
-    float4 thr4 = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    uint32_t nv = 0, tv = 0;
-    for (uint32_t bounce = 0; bounce < P.max_depth; ++bounce) {
-        if (bounce > 0) ++n_ext;
-        TraceResult res = traverse<false, false, STACK>(S.nodes, S.leaf, r, stk, sstride, nv, tv);
-        if (res.rec == 0xFFFFFFFFu) break;
-        const float4 r2 = S.leaf[(size_t)res.rec * kTriWords + 2];
-        float hu, hv;
-        hit_uv(S.leaf, res.rec, r, hu, hv);
-        VertexOut o;
-        path_vertex<KIND>(S, P, H, i, bounce, V3{r.dx, r.dy, r.dz}, res.t, __float_as_uint(r2.y),
-                          __float_as_uint(r2.z), hu, hv, thr4, rng, L, o);
-        if (o.emit) L = make_float4(L.x + o.c_emit.x, L.y + o.c_emit.y, L.z + o.c_emit.z, L.w);
-        if (o.sh) {
-            ++n_sh;
-            Ray sr{o.sh_o.x, o.sh_o.y, o.sh_o.z, 0.0f, o.sh_d.x, o.sh_d.y, o.sh_d.z, o.sh_t};
-            TraceResult vis = traverse<true, false, STACK>(S.nodes, S.leaf, sr, stk, sstride, nv, tv);
-            if (vis.rec == 0xFFFFFFFFu) L = make_float4(L.x + o.sh_c.x, L.y + o.sh_c.y, L.z + o.sh_c.z, L.w);
-        }
-        if (!o.ext) break;
-        r = Ray{o.ext_o.x, o.ext_o.y, o.ext_o.z, 0.0f, o.ext_d.x, o.ext_d.y, o.ext_d.z, 1e30f};
-        thr4 = o.thr4;
-    }
-    return L;
+    const Ray r{cam.pos.x, cam.pos.y, cam.pos.z, 0.0f, d0.x, d0.y, d0.z, 1e30f};
+    return path_rest<KIND, STACK>(S, P, H, i, 0u, r, make_float4(1.0f, 1.0f, 1.0f, 0.0f), rng_seed(initstate, i),
+                                  stk, sstride, L, n_ext, n_sh);
 }
 
 // Lanes a persistent traversal wave waits for before refilling idle lanes.

@@ -47,12 +47,15 @@ __global__ void k_set_counts(uint32_t* c, uint32_t i0, uint32_t v0, uint32_t i1,
 // Start of a bounce: count the extension rays queued for it (count_ext) and
 // the shadow rays traced in the previous bounce (count_shadow), then clear the
 // queues this bounce fills.  Counting from the queue sizes keeps per-wave
-// statistics atomics out of the shade kernel.
+// statistics atomics out of the shade kernel.  The queue length of bounce
+// b < kQueueHist goes to stats[kQueueHistAt + b] (the per-thread tail's choice).
+constexpr uint32_t kQueueHist = 16, kQueueHistAt = 16;
 __global__ void k_bounce_begin(uint32_t* c, uint32_t cur, uint32_t nxt, unsigned long long* stats,
-                               uint32_t count_ext, uint32_t count_shadow) {
+                               uint32_t count_ext, uint32_t count_shadow, uint32_t bounce) {
     if (threadIdx.x == 0) {
         if (count_ext) stats[1] += c[cur];
         if (count_shadow) stats[2] += c[2];
+        if (bounce < kQueueHist) stats[kQueueHistAt + bounce] = c[cur];
         c[nxt] = 0;
         c[2] = 0;
     }
@@ -211,6 +214,47 @@ __global__ __launch_bounds__(kMegaBlock) void k_pt_mega(SceneDev S, CamDev cam, 
     }
 }
 
+// The late bounces of the plain path tracer, one thread per path: every path
+// still queued (queue `in`, its length in *count) runs to its end in place, from
+// its stored throughput, random stream and bounce.  After Russian roulette has
+// thinned the queue, a launch per bounce leaves most lanes idle; here a lane that
+// ends its path takes the next one.  path_rest adds the contributions in the
+// wavefront's order, so the radiance is bit-identical.  The first ray of each
+// entry was counted when it was queued; stats gets the rest.
+template <int STACK>
+__global__ __launch_bounds__(kMegaBlock) void k_pt_tail(SceneDev S, PtParams P, ShadeHooks H, Queue in,
+                                                        const uint32_t* __restrict__ count, PathState ps,
+                                                        float4* __restrict__ rad,
+                                                        unsigned long long* __restrict__ stats) {
+    __shared__ uint2 stk[STACK * kMegaBlock];
+    __shared__ uint32_t red[2];
+    if (threadIdx.x == 0) red[0] = red[1] = 0u;
+    __syncthreads();
+    const uint32_t n = *count;
+    uint32_t n_ext = 0, n_sh = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t path = in.path[i];
+        const float4 o = in.org[i], d = in.dir[i];
+        const uint2 rs = ps.rng[path];
+        const Rng rng{(uint64_t)rs.x | ((uint64_t)rs.y << 32), ((uint64_t)path << 1u) | 1u};
+        rad[path] = path_rest<0, STACK>(S, P, H, path, ps.bounce[path], Ray{o.x, o.y, o.z, o.w, d.x, d.y, d.z, d.w},
+                                        ps.thr[path], rng, &stk[threadIdx.x], kMegaBlock, rad[path], n_ext, n_sh);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        n_ext += __shfl_xor(n_ext, o, 64);
+        n_sh += __shfl_xor(n_sh, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&red[0], n_ext);
+        atomicAdd(&red[1], n_sh);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && (red[0] | red[1])) {
+        atomicAdd(&stats[1], (unsigned long long)red[0]);
+        atomicAdd(&stats[2], (unsigned long long)red[1]);
+    }
+}
+
 // Output of the frame: its mean radiance, or with `accumulate` the weighted
 // running mean (this frame's samples count `weight` each).
 __global__ void k_resolve(const float4* __restrict__ rad, float4* __restrict__ accum, void* __restrict__ out,
@@ -250,6 +294,9 @@ __global__ void k_add_f64(const float4* __restrict__ rad, double* __restrict__ a
 }
 
 inline uint32_t nblk(uint32_t n, uint32_t b) { return n == 0 ? 1 : (n + b - 1) / b; }
+// Resident blocks per CU of the per-thread tail (119 VGPRs: 4 waves per SIMD).
+constexpr uint32_t kTailBlocksPerCu = 8;
+constexpr uint32_t kTailNever = 0xFFFFFFFFu;
 
 // inverse of a 4x4 (row-major) by cofactors, in double
 bool inverse4(const double m[16], double inv[16]) {
@@ -369,10 +416,11 @@ snkpt_status ensure_state(snkpt_ctx* ctx, uint32_t w, uint32_t h, hipStream_t st
         ++ctx->generation;
     }
     HIPR(R.counts.reserve(64));
-    HIPR(R.stats.reserve(64));
+    HIPR(R.stats.reserve(8 * (kQueueHistAt + kQueueHist)));
     if (R.w != w || R.h != h) {
         R.w = w;
         R.h = h;
+        R.tail_auto = 0;
         ++ctx->generation;
         k_fill_f4<<<nblk(n, kBlock), kBlock, 0, st>>>(R.accum.as<float4>(), n, make_float4(0, 0, 0, 0));
         R.accum_frames = 0;
@@ -393,6 +441,33 @@ ShadowQueue shadow_of(RenderState& R) {
 }
 PathState paths_of(RenderState& R) {
     return PathState{R.p_thr.as<float4>(), R.p_rng.as<uint2>(), R.p_bounce.as<uint32_t>()};
+}
+
+// The rest of every path queued in `cur` (up to n_max of them), one thread per
+// path on a persistent grid.
+snkpt_status launch_tail(snkpt_ctx* ctx, RenderState& R, const SceneDev& S, const PtParams& P, const ShadeHooks& H,
+                         int cur, float4* rad, hipStream_t st, uint32_t n_max) {
+    const Accel& A = ctx->accel;
+    if (A.num_tris == 0) return SNKPT_OK;  // no hits, so nothing was queued
+    const uint32_t need = A.depth > 0 ? A.depth - 1 : 0;
+    const uint32_t cus = ctx->props.multiProcessorCount > 0 ? (uint32_t)ctx->props.multiProcessorCount : 256u;
+    const uint32_t grid = std::min(nblk(n_max, kMegaBlock), cus * kTailBlocksPerCu);
+    const uint32_t* count = R.counts.as<uint32_t>() + cur;
+    unsigned long long* stats = R.stats.as<unsigned long long>();
+#define SNKPT_TAIL(STK) \
+    k_pt_tail<STK><<<grid, kMegaBlock, 0, st>>>(S, P, H, queue_of(R, cur), count, paths_of(R), rad, stats)
+    if (need <= 10)
+        SNKPT_TAIL(10);
+    else if (need <= 16)
+        SNKPT_TAIL(16);
+    else if (need <= 32)
+        SNKPT_TAIL(32);
+    else if (need <= 64)
+        SNKPT_TAIL(64);
+    else
+        return fail(ctx, SNKPT_ERR_CAPACITY, "path tail: BVH8 deeper than the 64-entry traversal stack");
+#undef SNKPT_TAIL
+    return SNKPT_OK;
 }
 
 snkpt_status launch_shadow(snkpt_ctx* ctx, RenderState& R, float4* rad, hipStream_t st, uint32_t n_max) {
@@ -436,10 +511,27 @@ snkpt_status pt_bounces(snkpt_ctx* ctx, const snkpt_settings* s, int cur, uint32
     float4* rad = H.rad ? H.rad : R.rad.as<float4>();
     uint32_t* counts = R.counts.as<uint32_t>();
     unsigned long long* stats = R.stats.as<unsigned long long>();
+    // bounce from which the surviving paths run one thread per path (0: never)
+    uint32_t tail_from = 0;
+    if (H.kind == 0) {
+        const uint32_t t = s->tail_bounce ? s->tail_bounce
+                           : R.tail_auto  ? R.tail_auto
+                                          : std::max(1u, P.rr_start);
+        tail_from = t < P.max_depth ? std::max(t, 1u) : 0u;
+    }
+    R.hist_first = first;
+    R.hist_depth = P.max_depth;
+    R.tail_used = 0;
     for (uint32_t b = first; b < P.max_depth; ++b) {
         int nxt = cur ^ 1;
         k_bounce_begin<<<1, 64, 0, st>>>(counts, (uint32_t)cur, (uint32_t)nxt, stats, b > first ? 1u : 0u,
-                                         b > first ? 1u : 0u);
+                                         b > first ? 1u : 0u, b);
+        if (tail_from && b >= tail_from) {
+            snkpt_status e = launch_tail(ctx, R, S, P, H, cur, rad, st, n);
+            if (e != SNKPT_OK) return e;
+            R.tail_used = b;
+            break;
+        }
         snkpt_hits h{R.h_t.as<float>(), R.h_inst.as<uint32_t>(), R.h_prim.as<uint32_t>(), R.h_u.as<float>(),
                      R.h_v.as<float>()};
         snkpt_status e = launch_trace(ctx, R.q[cur].org.as<float4>(), R.q[cur].dir.as<float4>(), n, h, false,
@@ -491,7 +583,7 @@ snkpt_status primary_bounces(snkpt_ctx* ctx, const CamDev& cam, const snkpt_sett
     float4* rad = H.rad ? H.rad : R.rad.as<float4>();
     uint32_t* counts = R.counts.as<uint32_t>();
     unsigned long long* stats = R.stats.as<unsigned long long>();
-    k_bounce_begin<<<1, 64, 0, st>>>(counts, 0u, 1u, stats, 0u, 0u);  // clears queue 1 and the shadow queue
+    k_bounce_begin<<<1, 64, 0, st>>>(counts, 0u, 1u, stats, 0u, 0u, 0u);  // clears queue 1 and the shadow queue
     if (H.kind == 1)
         k_gshade<1><<<nblk(n, kBlock), kBlock, 0, st>>>(S, R.gb, cam, R.w, R.h, P, queue_of(R, 1), shadow_of(R),
                                                         paths_of(R), rad, counts, 1u, initstate, H);
@@ -562,11 +654,32 @@ snkpt_status begin_frame(snkpt_ctx* ctx, uint32_t w, uint32_t h, hipStream_t st)
     return SNKPT_OK;
 }
 
+// The per-thread tail's bounce for the next frame: the first bounce (>= 1) whose
+// queue of paths fits twice the tail's resident lanes.  q: this frame's queue
+// lengths, known for the bounces the wavefront began (up to the tail's start);
+// when the tail started on a longer queue, the next frame tries one bounce later.
+static void tail_feedback(snkpt_ctx* ctx, const unsigned long long* q) {
+    RenderState& R = *ctx->render;
+    if (R.hist_depth == 0) return;  // no path-traced bounces this frame
+    const uint32_t cus = ctx->props.multiProcessorCount > 0 ? (uint32_t)ctx->props.multiProcessorCount : 256u;
+    const unsigned long long fit = 2ull * cus * kTailBlocksPerCu * kMegaBlock;
+    const uint32_t last = std::min(R.tail_used ? R.tail_used : R.hist_depth - 1, kQueueHist - 1);
+    uint32_t pick = R.tail_used && R.tail_used + 1 < R.hist_depth ? R.tail_used + 1 : kTailNever;
+    for (uint32_t b = std::max(R.hist_first, 1u); b <= last; ++b)
+        if (q[b] <= fit) {
+            pick = b;
+            break;
+        }
+    R.tail_auto = pick;
+    R.hist_depth = 0;
+}
+
 static snkpt_status read_stats(snkpt_ctx* ctx, hipStream_t st, uint64_t camera_rays) {
     RenderState& R = *ctx->render;
-    unsigned long long h[8];
+    unsigned long long h[kQueueHistAt + kQueueHist];
     HIPR(hipMemcpyAsync(h, R.stats.ptr, sizeof(h), hipMemcpyDeviceToHost, st));
     HIPR(hipStreamSynchronize(st));
+    tail_feedback(ctx, h + kQueueHistAt);
     ctx->stats.camera_rays = camera_rays;
     ctx->stats.extension_rays = h[1];
     ctx->stats.shadow_rays = h[2];

@@ -164,6 +164,11 @@ struct RenderState {
     DeviceBuffer p_thr, p_rng, p_bounce;
     DeviceBuffer rad, accum, counts, stats;
     uint32_t accum_frames = 0;
+    // per-thread tail of the path tracer: the bounce chosen from the last frame
+    // read back (0 = none yet, 0xFFFFFFFF = never), and what the last bounce
+    // loop did (its first bounce, its depth (0 = none since the last readback),
+    // the bounce its tail started at (0 = no tail))
+    uint32_t tail_auto = 0, hist_first = 0, hist_depth = 0, tail_used = 0;
     bool history_valid = false;
     bool use_gb = false;  // this frame's primary vertices come from `gb`
     GbufDev gb;

@@ -93,7 +93,8 @@ class Settings(C.Structure):
                 ("restir_radius", C.c_float), ("cache_query_depth", C.c_uint32),
                 ("cache_train_paths", C.c_uint32), ("cache_lr", C.c_float), ("flags", C.c_uint32),
                 ("cache_ema", C.c_float), ("cache_train_steps", C.c_uint32),
-                ("accum_weight", C.c_float), ("reserved", C.c_uint32 * 2)]
+                ("accum_weight", C.c_float), ("tail_bounce", C.c_uint32),
+                ("reserved", C.c_uint32 * 1)]
 
 
 class Rays(C.Structure):

@@ -36,6 +36,27 @@ def test_struct_sizes_match_reference_layouts():
     assert C.sizeof(snkpt.Camera) == 240   # CommonUBO, CommonUBO.glsl:1-13 (std140 minus tail pad)
 
 
+def test_settings_layout_matches_header(tmp_path):
+    """The binding's snkpt_settings has the header's field offsets (gcc on
+    include/snkpt.h prints them)."""
+    import shutil
+    import subprocess
+
+    import snkpt
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not found")
+    fields = [f[0] for f in snkpt.Settings._fields_]
+    src = tmp_path / "off.c"
+    src.write_text('#include <stdio.h>\n#include "snkpt.h"\nint main(void) {\n'
+                   + "".join(f'printf("%zu\\n", offsetof(snkpt_settings, {f}));\n' for f in fields)
+                   + 'printf("%zu\\n", sizeof(snkpt_settings));\nreturn 0;\n}\n')
+    exe = tmp_path / "off"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    want = [getattr(snkpt.Settings, f).offset for f in fields] + [C.sizeof(snkpt.Settings)]
+    assert got == want
+
+
 def test_null_arguments_rejected_without_gpu():
     import snkpt
     lib = snkpt.load()

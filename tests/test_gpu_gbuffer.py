@@ -102,6 +102,18 @@ def test_gbuffer_whole_paths_flag_keeps_the_wavefront_image(gpu):
     assert np.array_equal(a, c)
 
 
+def test_gbuffer_tail_bounces_keep_the_wavefront_image(gpu):
+    """The bounces after the G-buffer vertex, run one thread per path from
+    any bounce on: the wavefront's image bit for bit."""
+    w = h = 40
+    scene, ctx, arrs, b, cam, gb = _scene_gbuffer(1, w, h)
+    gbd = _dev(gb)
+    a = _render_gb(ctx, cam, gbd, w, h, max_depth=5, rr_start=3, seed=2, tail_bounce=0xFFFFFFFF)
+    for t in (1, 2, 3, 4):
+        c = _render_gb(ctx, cam, gbd, w, h, max_depth=5, rr_start=3, seed=2, tail_bounce=t)
+        assert np.array_equal(a, c), t
+
+
 @pytest.mark.parametrize("temporal,frames,tol", [(0, 96, 0.05), (1, 384, 0.08)])
 def test_gbuffer_restir_equals_pt_in_expectation(gpu, temporal, frames, tol):
     """Many-light scene over a G-buffer: ReSTIR frames average to the

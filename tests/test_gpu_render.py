@@ -154,6 +154,43 @@ def test_whole_path_kernel_matches_wavefront(gpu, config, w, h):
     assert outs[0][1:] == outs[1][1:]
 
 
+@pytest.mark.parametrize("config,w,h", [(1, 96, 64), (4, 160, 90), (2, 128, 72), (4, 640, 360)])
+def test_per_thread_tail_matches_wavefront(gpu, config, w, h):
+    """From any bounce on, the paths still alive can run one thread per path
+    instead of a launch per bounce: the image and the ray counts stay those of
+    the pure wavefront bit for bit, for every tail bounce and for the bounce the
+    context picks itself from the previous frame's queue lengths."""
+    import torch
+
+    import snkpt
+    never = 0xFFFFFFFF
+    scene = snkpt.SceneData(config)
+    ctx = snkpt.Context(0)
+    ctx.load_scene(scene)
+    ctx.build_accel()
+    cam = scene.camera(0, w, h)
+    md = scene.s.max_depth
+
+    def run(tail, frame):
+        o = torch.zeros(h, w, 4, device="cuda")
+        s = snkpt.default_settings(max_depth=md, rr_start=scene.s.rr_start, seed=5, frame=frame, spp=2,
+                                   tail_bounce=tail)
+        ctx.render(cam, s, o, w, h)
+        st = ctx.frame_stats()
+        return o.cpu().numpy(), st.extension_rays, st.shadow_rays
+
+    ref = run(never, 2)
+    assert ref[1] > 0 and ref[2] > 0
+    for t in range(1, md):
+        got = run(t, 2)
+        assert np.array_equal(ref[0], got[0]), t
+        assert ref[1:] == got[1:], t
+    for f in range(4):
+        want = run(never, f)
+        got = run(0, f)
+        assert np.array_equal(want[0], got[0]) and want[1:] == got[1:], f
+
+
 def test_weighted_accumulation_is_the_weighted_mean(gpu):
     """SNKPT_FLAG_ACCUMULATE with accum_weight w_f: the output is
     sum w_f x_f / sum w_f over the accumulated frames."""
