@@ -456,6 +456,7 @@ def batched_config5(stream, frames=10, warmup=2):
         b.add_view(var, cam, w, h, 0x5000 + view, out)
         views.append((var, w, h, cam))
         outs.append(out)
+    torch.cuda.synchronize()  # the zeroed outputs before the batch's stream writes them
     pixels = sum(w * h for _, w, h, _ in views)
     depth, rr = 6, 3
 
@@ -491,20 +492,22 @@ def batched_config5(stream, frames=10, warmup=2):
     single_ms = timed(one_by_one)
     # the last frame of the smallest, a middle and the largest view (equal to the
     # batch's frame, as its tests check) against a 256-spp FP64 oracle of its camera
+    # (torch work on the oracle's stream: its sums are read after it has written them)
     view_err = {}
     by_size = sorted(range(len(views)), key=lambda k: views[k][1] * views[k][2])
-    for k in (by_size[0], by_size[len(by_size) // 2], by_size[-1]):
-        var, w, h, cam = views[k]
-        acc = torch.zeros(h * w * 3, dtype=torch.float64, device="cuda")
-        os_ = snkpt.default_settings(max_depth=depth, rr_start=rr, seed=0x0AC1E)
-        for j in range(4):
-            os_.frame = j
-            ctxs[var].oracle_accumulate(cam, os_, acc, w, h, 64, stream=stream)
-        ref = (acc / 256).view(h, w, 3)
-        dv = outs[k][..., :3].double() - ref
-        view_err[f"view {k} ({w}x{h})"] = {
-            "frame_rel_rmse": round(float((dv ** 2).mean().sqrt() / ref.mean()), 4),
-            "frame_rel_mae": round(float(dv.abs().mean() / ref.mean()), 4)}
+    with torch.cuda.stream(stream):
+        for k in (by_size[0], by_size[len(by_size) // 2], by_size[-1]):
+            var, w, h, cam = views[k]
+            acc = torch.zeros(h * w * 3, dtype=torch.float64, device="cuda")
+            os_ = snkpt.default_settings(max_depth=depth, rr_start=rr, seed=0x0AC1E)
+            for j in range(4):
+                os_.frame = j
+                ctxs[var].oracle_accumulate(cam, os_, acc, w, h, 64, stream=stream)
+            ref = (acc / 256).view(h, w, 3)
+            dv = outs[k][..., :3].double() - ref
+            view_err[f"view {k} ({w}x{h})"] = {
+                "frame_rel_rmse": round(float((dv ** 2).mean().sqrt() / ref.mean()), 4),
+                "frame_rel_mae": round(float(dv.abs().mean() / ref.mean()), 4)}
 
     def cache_settings(f):
         return snkpt.default_settings(mode=snkpt.MODE_CACHE, max_depth=depth, rr_start=rr, frame=f,

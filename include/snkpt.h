@@ -508,7 +508,9 @@ typedef struct snkpt_batch_tuning {
     uint32_t shade_blocks_per_cu; /* persistent shade grid (256-lane blocks) = CUs x this */
     uint32_t queue_capacity;      /* paths per group of views processed together; 0 = all views at once */
     float ms_per_frame;           /* set by snkpt_batch_autotune: best measured frame time */
-    uint32_t reserved[3];
+    uint32_t tail_bounce;         /* bounce from which the paths still alive run one thread per path (same images
+                                     bit for bit); 0 = the frame's rr_start, 0xFFFFFFFF (or >= max_depth) = never */
+    uint32_t reserved[2];
 } snkpt_batch_tuning;
 snkpt_status snkpt_batch_create(int hip_device, snkpt_batch** out);
 void snkpt_batch_destroy(snkpt_batch* batch);
@@ -529,7 +531,7 @@ snkpt_status snkpt_batch_add_view(snkpt_batch* batch, uint32_t scene, const snkp
  * training paths as a second graph on a side stream beside the render. */
 snkpt_status snkpt_batch_render(snkpt_batch* batch, const snkpt_settings* s, int use_graph, snkpt_stream stream);
 /* Times whole frames of this batch over traversal block size, persistent grid
- * sizes and queue capacity, keeps the fastest (images do not depend on it). */
+ * sizes, queue capacity and tail bounce, keeps the fastest (images do not depend on it). */
 snkpt_status snkpt_batch_autotune(snkpt_batch* batch, const snkpt_settings* s, snkpt_stream stream,
                                   snkpt_batch_tuning* out);
 snkpt_status snkpt_batch_set_tuning(snkpt_batch* batch, const snkpt_batch_tuning* tuning);

@@ -157,6 +157,19 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(STACK <= 10 
     refill_trace<false, false, STACK, kRefillLanes>(n, wave, nwaves, &stk[threadIdx.x], TB, load, done);
 }
 
+// The late bounces of the batch, one thread per path (tail_paths).
+constexpr int kBTailBlock = 128;
+constexpr uint32_t kBTailBlocksPerCu = 8;
+template <int STACK>
+__global__ __launch_bounds__(kBTailBlock) void k_btail(SceneMany ss, PtParams P, Queue in,
+                                                       const uint32_t* __restrict__ count, PathState ps,
+                                                       float4* __restrict__ rad,
+                                                       unsigned long long* __restrict__ stats) {
+    __shared__ uint2 stk[STACK * kBTailBlock];
+    __shared__ uint32_t red[2];
+    tail_paths<STACK, kBTailBlock>(ss, P, ShadeHooks{}, in, *count, ps, rad, &stk[threadIdx.x], stats, red);
+}
+
 template <int KIND>
 __global__ __launch_bounds__(kBBlock) void k_bshade(SceneMany ss, PtParams P, Queue in, HitArrays hit, Queue out,
                                                     ShadowQueue sq, PathState ps, float4* __restrict__ rad,
@@ -360,7 +373,7 @@ struct snkpt_batch {
     std::vector<uint64_t> scene_gen;  // each scene context's generation when the tables were built
     std::vector<View> views;
     std::vector<Group> groups;
-    snkpt_batch_tuning tuning{128, 16, 8, 0, 0.0f, {0, 0, 0}};
+    snkpt_batch_tuning tuning{128, 16, 8, 0, 0.0f, 0, {0, 0}};
     bool tables_dirty = true;
     uint32_t stack = 16;  // traversal stack entries (deepest scene)
     uint32_t cap = 0;     // queue capacity allocated (paths)
@@ -553,6 +566,29 @@ void launch_traces_tb(snkpt_batch* B, const SceneMany& ss, int cur, uint32_t gri
         launch_traces<64, TB>(B, ss, cur, grid, st, shadow);
 }
 
+void launch_btail(snkpt_batch* B, const SceneMany& ss, const PtParams& P, int cur, hipStream_t st) {
+    Queue qin{B->q_org[cur].as<float4>(), B->q_dir[cur].as<float4>(), B->q_path[cur].as<uint32_t>()};
+    PathState ps{B->p_thr.as<float4>(), B->p_rng.as<uint2>(), B->p_bounce.as<uint32_t>()};
+    const uint32_t grid = (uint32_t)B->cus * kBTailBlocksPerCu;
+    const uint32_t* count = B->counts.as<uint32_t>() + cur;
+    float4* rad = B->rad.as<float4>();
+    unsigned long long* stats = B->stats.as<unsigned long long>();
+    if (B->stack <= 10)
+        k_btail<10><<<grid, kBTailBlock, 0, st>>>(ss, P, qin, count, ps, rad, stats);
+    else if (B->stack <= 16)
+        k_btail<16><<<grid, kBTailBlock, 0, st>>>(ss, P, qin, count, ps, rad, stats);
+    else if (B->stack <= 32)
+        k_btail<32><<<grid, kBTailBlock, 0, st>>>(ss, P, qin, count, ps, rad, stats);
+    else
+        k_btail<64><<<grid, kBTailBlock, 0, st>>>(ss, P, qin, count, ps, rad, stats);
+}
+
+// Bounce of the plain batch frame from which the tail runs (0: never).
+uint32_t batch_tail_from(const snkpt_batch_tuning& t, const PtParams& P) {
+    const uint32_t b = t.tail_bounce ? t.tail_bounce : P.rr_start;
+    return b >= 1 && b < P.max_depth ? b : 0u;
+}
+
 // Enqueues one frame of every view (no host synchronisation: capturable).
 snkpt_status record_frame(snkpt_batch* B, const snkpt_settings* s, hipStream_t st) {
     const uint32_t depth = s->max_depth ? s->max_depth : 1u;
@@ -568,6 +604,7 @@ snkpt_status record_frame(snkpt_batch* B, const snkpt_settings* s, hipStream_t s
     const uint32_t tgrid = (uint32_t)B->cus * std::max(1u, B->tuning.trace_blocks_per_cu);
     const uint32_t sgrid = (uint32_t)B->cus * std::max(1u, B->tuning.shade_blocks_per_cu);
     const uint32_t ggrid = (uint32_t)B->cus * 8u;
+    const uint32_t tail = batch_tail_from(B->tuning, P);
     k_bzero<<<1, 64, 0, st>>>(stats, 8);
     for (const auto& g : B->groups) {
         k_bgenerate<<<ggrid, kBBlock, 0, st>>>(B->d_jobs.as<JobD>(), g.j0, g.j1, g.total, B->d_frame.as<FrameD>(),
@@ -578,6 +615,10 @@ snkpt_status record_frame(snkpt_batch* B, const snkpt_settings* s, hipStream_t s
         for (uint32_t b = 0; b < depth; ++b) {
             const int nxt = cur ^ 1;
             k_bbounce<<<1, 64, 0, st>>>(counts, (uint32_t)cur, (uint32_t)nxt, stats, b > 0 ? 1u : 0u);
+            if (tail && b >= tail) {
+                launch_btail(B, ss, P, cur, st);
+                break;
+            }
             if (tb == 64)
                 launch_traces_tb<64>(B, ss, cur, tgrid, st, false);
             else
@@ -879,6 +920,7 @@ uint64_t graph_key(const snkpt_batch* B, const snkpt_settings* s, hipStream_t st
     mix(B->tuning.trace_blocks_per_cu);
     mix(B->tuning.shade_blocks_per_cu);
     mix(B->tuning.queue_capacity);
+    mix(B->tuning.tail_bounce);
     mix((uint64_t)(uintptr_t)st);
     mix(B->views.size());
     mix(B->scenes.size());
@@ -985,7 +1027,7 @@ double time_frames(snkpt_batch* B, const snkpt_settings* s, hipStream_t st, int 
     return ms;
 }
 
-// Coordinate descent over the launch shapes and the queue capacity, timing
+// Coordinate descent over the launch shapes, the queue capacity and the tail bounce, timing
 // whole frames of this batch (results do not depend on the choice).
 snkpt_status batch_autotune(snkpt_batch* B, const snkpt_settings* s, hipStream_t st, snkpt_batch_tuning* out) {
     if (hipSetDevice(B->device) != hipSuccess) return bfail(B, SNKPT_ERR_HIP, "hipSetDevice");
@@ -1028,6 +1070,14 @@ snkpt_status batch_autotune(snkpt_batch* B, const snkpt_settings* s, hipStream_t
             if (c > 0xFFFFFFFFull) continue;
             snkpt_batch_tuning t = best;
             t.queue_capacity = (uint32_t)c;
+            double ms;
+            if ((e = trial(t, ms)) != SNKPT_OK) return e;
+            if (ms < best_ms) best_ms = ms, best = t;
+        }
+        // tail bounce: never, or from each bounce after the camera rays' on
+        for (uint32_t b = 0; b < std::max(s->max_depth, 1u); ++b) {
+            snkpt_batch_tuning t = best;
+            t.tail_bounce = b == 0 ? 0xFFFFFFFFu : b;
             double ms;
             if ((e = trial(t, ms)) != SNKPT_OK) return e;
             if (ms < best_ms) best_ms = ms, best = t;

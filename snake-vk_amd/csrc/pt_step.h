@@ -227,6 +227,44 @@ __device__ __forceinline__ float4 path_rest(const SceneDev& S, const PtParams& P
     return L;
 }
 
+// The late bounces, one thread per path: every path queued in `in` (n of them)
+// runs to its end in place, from its stored throughput, random stream and
+// bounce.  After Russian roulette has thinned the queue, a launch per bounce
+// leaves most lanes idle; here a lane that ends its path takes the next one.
+// path_rest adds the contributions in the wavefront's order, so the radiance is
+// bit-identical.  The first ray of each entry was counted when it was queued;
+// stats[1] / stats[2] get the rest (red: two words of LDS).
+template <int STACK, int BLOCK, class SS>
+__device__ __forceinline__ void tail_paths(const SS& ss, const PtParams& P, const ShadeHooks& H, const Queue& in,
+                                           uint32_t n, const PathState& ps, float4* __restrict__ rad, uint2* stk,
+                                           unsigned long long* __restrict__ stats, uint32_t* red) {
+    if (threadIdx.x == 0) red[0] = red[1] = 0u;
+    __syncthreads();
+    uint32_t n_ext = 0, n_sh = 0;
+    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+        const uint32_t path = in.path[i];
+        const float4 o = in.org[i], d = in.dir[i];
+        const uint2 rs = ps.rng[path];
+        const Rng rng{(uint64_t)rs.x | ((uint64_t)rs.y << 32), ((uint64_t)ss.stream(path) << 1u) | 1u};
+        rad[path] = path_rest<0, STACK>(ss.at(path), P, H, path, ps.bounce[path],
+                                        Ray{o.x, o.y, o.z, o.w, d.x, d.y, d.z, d.w}, ps.thr[path], rng, stk, BLOCK,
+                                        rad[path], n_ext, n_sh);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        n_ext += __shfl_xor(n_ext, o, 64);
+        n_sh += __shfl_xor(n_sh, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&red[0], n_ext);
+        atomicAdd(&red[1], n_sh);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && (red[0] | red[1])) {
+        atomicAdd(&stats[1], (unsigned long long)red[0]);
+        atomicAdd(&stats[2], (unsigned long long)red[1]);
+    }
+}
+
 // A whole path per thread (no queues): camera ray of path i through pixel
 // (start + i * stride) mod (w h), closest hits and shadow rays traced in place.
 // Same vertex logic, random stream and order of radiance additions as the

@@ -214,13 +214,7 @@ __global__ __launch_bounds__(kMegaBlock) void k_pt_mega(SceneDev S, CamDev cam, 
     }
 }
 
-// The late bounces of the plain path tracer, one thread per path: every path
-// still queued (queue `in`, its length in *count) runs to its end in place, from
-// its stored throughput, random stream and bounce.  After Russian roulette has
-// thinned the queue, a launch per bounce leaves most lanes idle; here a lane that
-// ends its path takes the next one.  path_rest adds the contributions in the
-// wavefront's order, so the radiance is bit-identical.  The first ray of each
-// entry was counted when it was queued; stats gets the rest.
+// The late bounces of the plain path tracer, one thread per path (tail_paths).
 template <int STACK>
 __global__ __launch_bounds__(kMegaBlock) void k_pt_tail(SceneDev S, PtParams P, ShadeHooks H, Queue in,
                                                         const uint32_t* __restrict__ count, PathState ps,
@@ -228,31 +222,7 @@ __global__ __launch_bounds__(kMegaBlock) void k_pt_tail(SceneDev S, PtParams P, 
                                                         unsigned long long* __restrict__ stats) {
     __shared__ uint2 stk[STACK * kMegaBlock];
     __shared__ uint32_t red[2];
-    if (threadIdx.x == 0) red[0] = red[1] = 0u;
-    __syncthreads();
-    const uint32_t n = *count;
-    uint32_t n_ext = 0, n_sh = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t path = in.path[i];
-        const float4 o = in.org[i], d = in.dir[i];
-        const uint2 rs = ps.rng[path];
-        const Rng rng{(uint64_t)rs.x | ((uint64_t)rs.y << 32), ((uint64_t)path << 1u) | 1u};
-        rad[path] = path_rest<0, STACK>(S, P, H, path, ps.bounce[path], Ray{o.x, o.y, o.z, o.w, d.x, d.y, d.z, d.w},
-                                        ps.thr[path], rng, &stk[threadIdx.x], kMegaBlock, rad[path], n_ext, n_sh);
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-        n_ext += __shfl_xor(n_ext, o, 64);
-        n_sh += __shfl_xor(n_sh, o, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&red[0], n_ext);
-        atomicAdd(&red[1], n_sh);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0 && (red[0] | red[1])) {
-        atomicAdd(&stats[1], (unsigned long long)red[0]);
-        atomicAdd(&stats[2], (unsigned long long)red[1]);
-    }
+    tail_paths<STACK, kMegaBlock>(SceneOne{S}, P, H, in, *count, ps, rad, &stk[threadIdx.x], stats, red);
 }
 
 // Output of the frame: its mean radiance, or with `accumulate` the weighted

@@ -133,12 +133,12 @@ class Contacts(C.Structure):
 class BatchTuning(C.Structure):
     _fields_ = [("trace_block", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32),
                 ("shade_blocks_per_cu", C.c_uint32), ("queue_capacity", C.c_uint32), ("ms_per_frame", C.c_float),
-                ("reserved", C.c_uint32 * 3)]
+                ("tail_bounce", C.c_uint32), ("reserved", C.c_uint32 * 2)]
 
     def as_dict(self):
         return {"trace_block": self.trace_block, "trace_blocks_per_cu": self.trace_blocks_per_cu,
                 "shade_blocks_per_cu": self.shade_blocks_per_cu, "queue_capacity": self.queue_capacity,
-                "ms_per_frame": round(self.ms_per_frame, 4)}
+                "tail_bounce": self.tail_bounce, "ms_per_frame": round(self.ms_per_frame, 4)}
 
 
 class TraceTuning(C.Structure):

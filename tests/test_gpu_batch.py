@@ -99,6 +99,32 @@ def test_grouping_and_launch_shapes_do_not_change_images(gpu):
             assert np.array_equal(got[k], ref[k]), (tb, tpc, spc, cap, k)
 
 
+def test_tail_bounce_does_not_change_images_or_ray_counts(gpu):
+    """The paths still alive from any bounce on run one thread per path: the
+    images and ray counts of the launch-per-bounce frame, bit for bit."""
+    import snkpt
+    scenes, ctxs, b, outs, cams = _setup()
+    s = _settings(frame=4)
+    t = snkpt.BatchTuning()
+    t.trace_block, t.trace_blocks_per_cu, t.shade_blocks_per_cu, t.tail_bounce = 128, 16, 8, 0xFFFFFFFF
+    b.set_tuning(t)
+    b.render(s)
+    ref = _grab(outs)
+    rs = b.frame_stats()
+    assert rs.extension_rays > 0
+    for tail in range(1, s.max_depth):
+        t.tail_bounce = tail
+        b.set_tuning(t)
+        for o in outs:
+            o.zero_()
+        b.render(s)
+        got = _grab(outs)
+        st = b.frame_stats()
+        for k in range(len(VIEWS)):
+            assert np.array_equal(got[k], ref[k]), (tail, k)
+        assert (st.extension_rays, st.shadow_rays) == (rs.extension_rays, rs.shadow_rays), tail
+
+
 def test_graph_replays_match_eager_frames(gpu):
     import torch
     scenes, ctxs, b, outs, cams = _setup()
