@@ -191,6 +191,31 @@ def test_per_thread_tail_matches_wavefront(gpu, config, w, h):
         assert np.array_equal(want[0], got[0]) and want[1:] == got[1:], f
 
 
+def test_automatic_tail_follows_resolution_and_depth_changes(gpu):
+    """The context's tail bounce comes from its previous frame; frames of other
+    sizes and depths in between (which reset or move it) still give the pure
+    wavefront's image and ray counts."""
+    import torch
+
+    import snkpt
+    scene = snkpt.SceneData(4)
+    ctx = snkpt.Context(0)
+    ctx.load_scene(scene)
+    ctx.build_accel()
+
+    def run(tail, w, h, depth, frame):
+        o = torch.zeros(h, w, 4, device="cuda")
+        s = snkpt.default_settings(max_depth=depth, rr_start=2, seed=9, frame=frame, tail_bounce=tail)
+        ctx.render(scene.camera(0, w, h), s, o, w, h)
+        st = ctx.frame_stats()
+        return o.cpu().numpy(), st.extension_rays, st.shadow_rays
+
+    for f, (w, h, depth) in enumerate([(160, 90, 8), (96, 64, 8), (96, 64, 3), (320, 180, 6), (160, 90, 8)]):
+        got = run(0, w, h, depth, f)
+        want = run(0xFFFFFFFF, w, h, depth, f)
+        assert np.array_equal(got[0], want[0]) and got[1:] == want[1:], (w, h, depth)
+
+
 def test_weighted_accumulation_is_the_weighted_mean(gpu):
     """SNKPT_FLAG_ACCUMULATE with accum_weight w_f: the output is
     sum w_f x_f / sum w_f over the accumulated frames."""
